@@ -1,0 +1,155 @@
+/* tbtkx: C ABI of the MI355X (gfx950) Chebyshev-expansion engine.
+ *
+ * This is the device boundary of the drop-in Solver::ChebyshevExpander. In the
+ * reference the boundary is a build-time object swap between
+ * Lib/src/cuda/ChebyshevSolver.cu and Lib/src/nocuda/ChebyshevSolver.cpp,
+ * both defining the private GPU members of Solver::ChebyshevExpander
+ * (Lib/include/TBTK/Solver/ChebyshevExpander.h:337-389). Here those members
+ * call the functions below instead.
+ *
+ * Conventions
+ *  - Complex numbers are interleaved (re, im) doubles on the host side.
+ *  - Every function returns 0 on success and a nonzero TBTKX_* status
+ *    otherwise; tbtkx_last_error() gives the message. Nothing throws across
+ *    the ABI.
+ *  - A handle owns one HIP stream on one device and all device state (the
+ *    Hamiltonian, recurrence vectors, the lookup table). Calls on one handle
+ *    must not overlap; different handles are independent.
+ *  - Host buffers are owned by the caller.
+ */
+#ifndef TBTKX_H
+#define TBTKX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tbtkx_context *tbtkx_handle;
+
+enum {
+	TBTKX_OK = 0,
+	TBTKX_ERR_ARG = 1,        /* invalid argument */
+	TBTKX_ERR_HIP = 2,        /* a HIP runtime call failed */
+	TBTKX_ERR_STATE = 3,      /* call out of order (e.g. no model set) */
+	TBTKX_ERR_NODEVICE = 4,   /* no usable GPU */
+	TBTKX_ERR_OOM = 5         /* device allocation failed */
+};
+
+/* Green's function types, numbered as Solver::ChebyshevExpander::Type
+ * (Lib/include/TBTK/Solver/ChebyshevExpander.h:206-211). */
+enum {
+	TBTKX_ADVANCED = 0,
+	TBTKX_RETARDED = 1,
+	TBTKX_PRINCIPAL = 2,
+	TBTKX_NONPRINCIPAL = 3
+};
+
+/* Flags for the batched recurrences. */
+enum {
+	TBTKX_H_FP64 = 1 << 0,     /* keep H in FP64 (default: FP32 complex) */
+	TBTKX_DOUBLING = 1 << 1,   /* diagonal moments from the doubling
+	                              identities mu_2n = 2<n|n> - mu_0,
+	                              mu_2n+1 = 2<n+1|n> - mu_1 (M/2 steps) */
+	TBTKX_VEC_FP64 = 1 << 2    /* FP64 recurrence vectors (reference block) */
+};
+
+/* Message of the last failing call on this thread. */
+const char *tbtkx_last_error(void);
+
+/* Number of visible GPUs (replaces GPUResourceManager::createDeviceTable,
+ * Lib/src/cuda/GPUResourceManager.cu:28-46). */
+int tbtkx_device_count(int *count);
+
+/* Device pool: hand out a free device, or wait for one (replaces
+ * GPUResourceManager::allocateDevice / freeDevice,
+ * Lib/src/Utilities/GPUResourceManager.cpp:44-90). */
+int tbtkx_device_acquire(int *device);
+int tbtkx_device_release(int device);
+
+int tbtkx_create(int device, tbtkx_handle *handle);
+int tbtkx_destroy(tbtkx_handle handle);
+
+/* Upload the Hamiltonian as CSR (rows = basis(to), columns = basis(from),
+ * columns ascending in a row) and the scale factor s. The device keeps
+ * H*(1/s) and H*(1/s)*2 in FP64 (the two scalings of
+ * Lib/src/Solver/ChebyshevExpander.cpp:147,168) and FP32 copies for the
+ * batched path. Replaces the per-call H upload of calculateCoefficientsGPU
+ * (Lib/src/cuda/ChebyshevSolver.cu:153-317). */
+int tbtkx_set_csr(tbtkx_handle handle, int64_t num_rows, int64_t nnz,
+                  const uint32_t *rowptr, const uint32_t *col,
+                  const double *val, double scale);
+
+/* Plain FP64 path, one start vector: raw moments
+ * out[k*M + n] = <to_k| T_n(H/s) |from>, 0 <= n < M, bitwise equal to the
+ * reference CPU recurrence. If a to-index repeats, only its last position
+ * receives moments and the others stay zero, as the reference's
+ * coefficientMap does. Replaces calculateCoefficientsGPU(to, from)
+ * (Lib/src/cuda/ChebyshevSolver.cu:74-648; damping stays on the host). */
+int tbtkx_moments(tbtkx_handle handle, int64_t from, int num_to,
+                  const int64_t *to, int num_moments, double *out);
+
+/* Load a host-generated lookup table L[n*E + e] (M x E complex) for the
+ * contraction. Replaces loadLookupTableGPU
+ * (Lib/src/cuda/ChebyshevSolver.cu:672-760). */
+int tbtkx_lut_upload(tbtkx_handle handle, int num_moments, int resolution,
+                     const double *table);
+
+/* Generate the lookup table on the device in FP64: the formula of
+ * generateLookupTable (Lib/src/Solver/ChebyshevExpander.cpp:256-268) over
+ * Range(lo, hi, resolution). */
+int tbtkx_lut_generate(tbtkx_handle handle, int num_moments, int resolution,
+                       double lower, double upper, double scale);
+
+/* Free the device lookup table (destroyLookupTableGPU,
+ * Lib/src/cuda/ChebyshevSolver.cu:762-783). */
+int tbtkx_lut_destroy(tbtkx_handle handle);
+
+/* FP64 contraction of K coefficient vectors coeffs[k*M + n] with the loaded
+ * table into out[k*E + e], for any of the four types, summing n in ascending
+ * order as the reference (Lib/src/Solver/ChebyshevExpander.cpp:324-370).
+ * Replaces generateGreensFunctionGPU (Lib/src/cuda/ChebyshevSolver.cu:786-921),
+ * which supports Retarded only. */
+int tbtkx_contract(tbtkx_handle handle, int num_vectors, const double *coeffs,
+                   int type, double *out);
+
+/* Batched path: diagonal moments mu[b*M + n] = <s_b| T_n(H/s) |s_b> for B
+ * unit start vectors at basis indices sites[b] (LDOS sites), run as blocks
+ * of start vectors through one SpMM recurrence. FP32 complex vectors (unless
+ * TBTKX_VEC_FP64), FP64 accumulation of products and moments. No reference
+ * counterpart: the reference loops calculateCoefficients once per site
+ * (Lib/src/PropertyExtractor/ChebyshevExpander.cpp:661-684). */
+int tbtkx_moments_block(tbtkx_handle handle, int num_sites,
+                        const int64_t *sites, int num_moments, int flags,
+                        double *out);
+
+/* Stochastic trace: mu[n] = sum_r <r| T_n(H/s) |r> over R random-phase
+ * vectors r_i = exp(2*pi*i*u), u = the 53-bit splitmix64 uniform of
+ * (seed, r*N + i). No reference counterpart (the reference's sampleDOS,
+ * Lib/src/PropertyExtractor/PropertyExtractor.cpp:234-290, sums LDOS of
+ * sampled sites). */
+int tbtkx_moments_trace(tbtkx_handle handle, int num_vectors, uint64_t seed,
+                        int num_moments, int flags, double *out);
+
+/* Device time of the last recurrence call: total milliseconds between the
+ * first and last step launch (HIP events on the handle's stream) and the
+ * number of step launches in it. */
+int tbtkx_last_timing(tbtkx_handle handle, double *milliseconds,
+                      int64_t *launches);
+
+/* Engine options (0 = default). TBTKX_OPT_PLAIN_KERNEL: 0 = LDS-staged
+ * row blocks, 1 = one thread per row. TBTKX_OPT_GRAPH: 1 = replay the step
+ * loop as a hipGraph. TBTKX_OPT_BLOCK_WIDTH: start vectors per SpMM block. */
+enum {
+	TBTKX_OPT_PLAIN_KERNEL = 1,
+	TBTKX_OPT_GRAPH = 2,
+	TBTKX_OPT_BLOCK_WIDTH = 3,
+	TBTKX_OPT_XCD_REMAP = 4
+};
+int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,434 @@
+// Batched path: blocks of B start vectors through one SpMM recurrence
+//   Phi_1 = (H/s) Phi_0,   Phi_{n+1} = (2H/s) Phi_n - Phi_{n-1}
+// with Phi stored row-major (N rows x B columns, each row contiguous) in FP32
+// complex (or FP64 with TBTKX_VEC_FP64), H in FP32 (or FP64 with
+// TBTKX_H_FP64), and every product and row sum formed in FP64 and rounded once
+// on store.
+//
+// Moments (diagonal, per column b):
+//  - gather mode: mu_n[b] = Phi_n[site_b][b], one recurrence step per moment;
+//  - doubling mode: per step the dot products D0_n = <Phi_n|Phi_n> and
+//    D1_n = <Phi_{n+1}|Phi_n> are fused into the SpMM pass (FP64 sums), and
+//    mu_0 = D0_0, mu_1 = D1_0, mu_2n = 2 D0_n - mu_0, mu_2n+1 = 2 D1_n - mu_1
+//    (Weisse et al., Rev. Mod. Phys. 78, 275 (2006), eq. 31), so M moments
+//    take ceil(M/2) steps.
+//
+// Work mapping: one 64-lane wavefront per row; lane l owns B/64 consecutive
+// columns (B = 64, 128, 256), so each non-zero streams one contiguous B-wide
+// row of Phi_n (coalesced); column index and value are wave-uniform scalar
+// loads. Each wavefront walks a contiguous range of rows; the workgroup id is
+// remapped so every XCD owns one contiguous slice of the rows.
+//
+// Algorithmic HBM bytes per step (FP32 vectors, FP32 H):
+//   12 Z (value + column) + 4 N (row pointer) + 24 B N (read Phi_n, read
+//   Phi_{n-1}, write Phi_{n+1}, 8 B each per element).
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+
+#include "common.hpp"
+
+namespace tbtkx {
+namespace {
+
+constexpr int kWaves = kThreads / 64;
+constexpr int kBlockWG = 2048;     // workgroups per step launch (at most)
+constexpr int kDotChunk = 128;     // steps whose dot partials are kept
+
+template <typename T> struct C2;
+template <> struct C2<float2> {
+	__device__ static double2 load(const float2 &a) {
+		return make_double2((double)a.x, (double)a.y);
+	}
+	__device__ static float2 store(double2 a) {
+		return make_float2((float)a.x, (float)a.y);
+	}
+};
+template <> struct C2<double2> {
+	__device__ static double2 load(const double2 &a) { return a; }
+	__device__ static double2 store(double2 a) { return a; }
+};
+
+// Initial block: unit vectors at sites[b] (mode 0) or random phases
+// exp(2 pi i u(seed, (col0 + b) N + row)) (mode 1). Columns past the number
+// of live vectors are zero.
+template <typename VT>
+__global__ __launch_bounds__(kThreads) void block_init(
+    VT *__restrict__ X, int64_t N, int B, int live, int mode,
+    const int64_t *__restrict__ sites, uint64_t seed, int64_t col0) {
+	const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+	if (i >= N * B) return;
+	const int64_t row = i / B;
+	const int b = (int)(i - row * B);
+	double2 v = make_double2(0.0, 0.0);
+	if (b < live) {
+		if (mode == 0) {
+			if (sites[b] == row) v.x = 1.0;
+		} else {
+			uint64_t c = (uint64_t)(col0 + b) * (uint64_t)N + (uint64_t)row;
+			uint64_t z = seed * 0x9E3779B97F4A7C15ull + c * 0xD1B54A32D192ED03ull +
+			             0x632BE59BD9B4E019ull;
+			z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+			z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+			z = z ^ (z >> 31);
+			const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+			double sn, cs;
+			sincos(2.0 * M_PI * u, &sn, &cs);
+			v = make_double2(cs, sn);
+		}
+	}
+	X[i] = C2<VT>::store(v);
+}
+
+// One recurrence step on a block. Y holds Phi_{n-1} on entry and Phi_{n+1}
+// on exit (in place, same lane reads and writes a row's columns). With DOTS,
+// the workgroup's sums of conj(X)X and conj(Y)X per column go to
+// part[wg][0..1][b]; in gather mode column b's moment of X goes to
+// mu[b*M + nprev].
+template <typename VT, typename HT, int B, bool FIRST, bool DOTS>
+__global__ __launch_bounds__(kThreads) void spmm_step(
+    const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ col,
+    const HT *__restrict__ v, const VT *__restrict__ X, VT *Y, int64_t N,
+    int64_t rows_per_wave, int nwg, int remap, double2 *__restrict__ part,
+    const int64_t *__restrict__ sites, int live, double2 *__restrict__ mu,
+    int M, int nprev) {
+	constexpr int CPL = B / 64;   // complex values per lane
+	const int wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+	const int wg = remap ? xcd_remap(blockIdx.x, nwg) : (int)blockIdx.x;
+
+	if (!DOTS) {
+		const int g = blockIdx.x * kThreads + threadIdx.x;
+		if (g < live)
+			mu[(int64_t)g * M + nprev] =
+			    C2<VT>::load(X[sites[g] * B + g]);
+	}
+
+	double2 d0[CPL], d1[CPL];
+#pragma unroll
+	for (int j = 0; j < CPL; j++) {
+		d0[j] = make_double2(0.0, 0.0);
+		d1[j] = make_double2(0.0, 0.0);
+	}
+
+	const int64_t rbeg = ((int64_t)wg * kWaves + wave) * rows_per_wave;
+	const int64_t rend = rbeg + rows_per_wave < N ? rbeg + rows_per_wave : N;
+	for (int64_t row = rbeg; row < rend; row++) {
+		const int64_t base = row * B + lane * CPL;
+		double2 acc[CPL];
+#pragma unroll
+		for (int j = 0; j < CPL; j++) {
+			if (FIRST) {
+				acc[j] = make_double2(0.0, 0.0);
+			} else {
+				const double2 p = C2<VT>::load(Y[base + j]);
+				acc[j] = make_double2(-p.x, -p.y);
+			}
+		}
+		const uint32_t kb = rowptr[row], ke = rowptr[row + 1];
+		for (uint32_t k = kb; k < ke; k++) {
+			const double2 a = C2<HT>::load(v[k]);
+			const int64_t cb = (int64_t)col[k] * B + lane * CPL;
+#pragma unroll
+			for (int j = 0; j < CPL; j++) {
+				const double2 x = C2<VT>::load(X[cb + j]);
+				acc[j].x = fma(a.x, x.x, fma(-a.y, x.y, acc[j].x));
+				acc[j].y = fma(a.x, x.y, fma(a.y, x.x, acc[j].y));
+			}
+		}
+#pragma unroll
+		for (int j = 0; j < CPL; j++) {
+			const VT s = C2<VT>::store(acc[j]);
+			Y[base + j] = s;
+			if (DOTS) {
+				const double2 y = C2<VT>::load(s);
+				const double2 x = C2<VT>::load(X[base + j]);
+				d0[j].x = fma(x.x, x.x, fma(x.y, x.y, d0[j].x));
+				// conj(y) * x
+				d1[j].x = fma(y.x, x.x, fma(y.y, x.y, d1[j].x));
+				d1[j].y = fma(y.x, x.y, fma(-y.y, x.x, d1[j].y));
+			}
+		}
+	}
+
+	if (DOTS) {
+		// Sum the four wavefronts' partials in LDS, in wave order.
+		__shared__ double2 red[kWaves][2][B];
+#pragma unroll
+		for (int j = 0; j < CPL; j++) {
+			red[wave][0][lane * CPL + j] = d0[j];
+			red[wave][1][lane * CPL + j] = d1[j];
+		}
+		__syncthreads();
+		for (int i = threadIdx.x; i < 2 * B; i += kThreads) {
+			const int w = i / B, b = i - w * B;
+			double2 s = red[0][w][b];
+#pragma unroll
+			for (int q = 1; q < kWaves; q++) {
+				s.x += red[q][w][b].x;
+				s.y += red[q][w][b].y;
+			}
+			part[(int64_t)wg * 2 * B + i] = s;
+		}
+	}
+}
+
+// D[step][w][b] = sum over workgroups of part[step][wg][w][b], in workgroup
+// order (deterministic).
+__global__ __launch_bounds__(kThreads) void reduce_parts(
+    const double2 *__restrict__ part, int nsteps, int nwg, int B,
+    double2 *__restrict__ D) {
+	const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+	const int64_t per = 2 * (int64_t)B;
+	if (i >= nsteps * per) return;
+	const int64_t step = i / per, c = i - step * per;
+	const double2 *p = part + step * nwg * per + c;
+	double sx = 0, sy = 0;
+	for (int w = 0; w < nwg; w++) {
+		sx += p[(int64_t)w * per].x;
+		sy += p[(int64_t)w * per].y;
+	}
+	D[i] = make_double2(sx, sy);
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kThreads) void gather_block(
+    const VT *__restrict__ X, const int64_t *__restrict__ sites, int live,
+    int B, double2 *__restrict__ mu, int M, int n) {
+	const int g = blockIdx.x * kThreads + threadIdx.x;
+	if (g < live) mu[(int64_t)g * M + n] = C2<VT>::load(X[sites[g] * B + g]);
+}
+
+inline unsigned grid_for(int64_t n) {
+	return (unsigned)((n + kThreads - 1) / kThreads);
+}
+
+struct BlockRun {
+	tbtkx_context *h;
+	int B;             // block width
+	int flags;
+	int nwg;
+	int64_t rows_per_wave;
+	int64_t launches = 0;
+};
+
+template <typename VT, typename HT, int B>
+int launch_step(BlockRun &r, bool first, bool dots, const VT *X, VT *Y,
+                double2 *part, const int64_t *sites, int live, double2 *mu,
+                int M, int nprev) {
+	tbtkx_context *h = r.h;
+	const HT *v = first ? h->v1.as<HT>() : h->v2.as<HT>();
+	if (std::is_same<HT, float2>::value)
+		v = first ? (const HT *)h->v1f.ptr : (const HT *)h->v2f.ptr;
+	const uint32_t *rp = h->rowptr.as<uint32_t>(), *cl = h->col.as<uint32_t>();
+#define TBTKX_STEP(F, D)                                                       \
+	spmm_step<VT, HT, B, F, D><<<r.nwg, kThreads, 0, h->stream>>>(        \
+	    rp, cl, v, X, Y, h->N, r.rows_per_wave, r.nwg, h->xcd_remap, part, \
+	    sites, live, mu, M, nprev)
+	if (first && dots) TBTKX_STEP(true, true);
+	else if (first) TBTKX_STEP(true, false);
+	else if (dots) TBTKX_STEP(false, true);
+	else TBTKX_STEP(false, false);
+#undef TBTKX_STEP
+	r.launches++;
+	return TBTKX_OK;
+}
+
+// Run one block: S steps, moments into mu_dev (gather: live x M; doubling:
+// D buffer of S x 2 x B). Returns the number of steps.
+template <typename VT, typename HT, int B>
+int run_block(BlockRun &r, int live, const int64_t *d_sites, int mode,
+              uint64_t seed, int64_t col0, int M, double2 *mu_dev,
+              double2 *D_dev) {
+	tbtkx_context *h = r.h;
+	const bool dots = (r.flags & TBTKX_DOUBLING) != 0;
+	VT *X = h->blk_x.as<VT>(), *Y = h->blk_y.as<VT>();
+	double2 *part = h->blk_part.as<double2>();
+	block_init<VT><<<grid_for(h->N * B), kThreads, 0, h->stream>>>(
+	    X, h->N, B, live, mode, d_sites, seed, col0);
+	const int S = dots ? (M + 1) / 2 : M - 1;
+	int chunk0 = 0;
+	for (int n = 0; n < S; n++) {
+		// Step n computes Phi_{n+1} from Phi_n (X) and Phi_{n-1} (Y).
+		double2 *p = part + (int64_t)(n - chunk0) * r.nwg * 2 * B;
+		int rc = launch_step<VT, HT, B>(r, n == 0, dots, X, Y, p, d_sites,
+		                                live, mu_dev, M, n);
+		if (rc) return rc;
+		VT *t = X;
+		X = Y;
+		Y = t;
+		if (dots && (n - chunk0 + 1 == kDotChunk || n == S - 1)) {
+			const int ns = n - chunk0 + 1;
+			reduce_parts<<<grid_for((int64_t)ns * 2 * B), kThreads, 0,
+			               h->stream>>>(part, ns, r.nwg, B,
+			                            D_dev + (int64_t)chunk0 * 2 * B);
+			chunk0 = n + 1;
+		}
+	}
+	if (!dots)
+		gather_block<VT><<<grid_for(live), kThreads, 0, h->stream>>>(
+		    X, d_sites, live, B, mu_dev, M, M - 1);
+	TBTKX_CHECK_HIP(hipGetLastError());
+	return TBTKX_OK;
+}
+
+template <typename VT, typename HT, int B>
+int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
+            uint64_t seed, int M, int flags, std::vector<double2> &host_mu) {
+	BlockRun r{h, B, flags, 0, 0};
+	const int64_t waves_needed = (h->N + 15) / 16;   // >= 16 rows per wave
+	int64_t nwg = (waves_needed + kWaves - 1) / kWaves;
+	if (nwg > kBlockWG) nwg = kBlockWG;
+	if (nwg < 1) nwg = 1;
+	r.nwg = (int)nwg;
+	r.rows_per_wave = (h->N + nwg * kWaves - 1) / (nwg * kWaves);
+	const bool dots = (flags & TBTKX_DOUBLING) != 0;
+	const int S = dots ? (M + 1) / 2 : M - 1;
+	int rc;
+	if ((rc = h->blk_x.reserve((size_t)h->N * B * sizeof(VT)))) return rc;
+	if ((rc = h->blk_y.reserve((size_t)h->N * B * sizeof(VT)))) return rc;
+	if (dots &&
+	    (rc = h->blk_part.reserve((size_t)kDotChunk * r.nwg * 2 * B * 16)))
+		return rc;
+	const size_t mu_elems = dots ? (size_t)(S > 0 ? S : 1) * 2 * B : (size_t)B * M;
+	if ((rc = h->blk_mu.reserve(mu_elems * 16))) return rc;
+	if ((rc = h->blk_sites.reserve((size_t)B * 8))) return rc;
+
+	const int nblocks = (total + B - 1) / B;
+	host_mu.assign((size_t)total * M, make_double2(0, 0));
+	std::vector<double2> buf(mu_elems);
+	bool timing = false;
+	for (int blk = 0; blk < nblocks; blk++) {
+		const int live = total - blk * B < B ? total - blk * B : B;
+		if (mode == 0)
+			TBTKX_CHECK_HIP(hipMemcpyAsync(h->blk_sites.ptr, sites + (int64_t)blk * B,
+			                               (size_t)live * 8, hipMemcpyHostToDevice,
+			                               h->stream));
+		if (!timing) {
+			TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
+			timing = true;
+		}
+		rc = run_block<VT, HT, B>(r, live, h->blk_sites.as<int64_t>(), mode, seed,
+		                          (int64_t)blk * B, M, h->blk_mu.as<double2>(),
+		                          h->blk_mu.as<double2>());
+		if (rc) return rc;
+		TBTKX_CHECK_HIP(hipMemcpyAsync(buf.data(), h->blk_mu.ptr, mu_elems * 16,
+		                               hipMemcpyDeviceToHost, h->stream));
+		TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+		for (int b = 0; b < live; b++) {
+			double2 *o = host_mu.data() + (size_t)(blk * B + b) * M;
+			if (!dots) {
+				memcpy(o, buf.data() + (size_t)b * M, (size_t)M * 16);
+				continue;
+			}
+			// D[n][0][b] = <Phi_n|Phi_n>, D[n][1][b] = <Phi_n+1|Phi_n>
+			const double2 mu0 = buf[(size_t)0 * 2 * B + b];
+			const double2 mu1 = M > 1 ? buf[(size_t)0 * 2 * B + B + b]
+			                          : make_double2(0, 0);
+			o[0] = mu0;
+			if (M > 1) o[1] = mu1;
+			for (int n = 1; n < S; n++) {
+				const double2 a = buf[(size_t)n * 2 * B + b];
+				const double2 c = buf[(size_t)n * 2 * B + B + b];
+				if (2 * n < M) o[2 * n] = make_double2(2 * a.x - mu0.x, 2 * a.y - mu0.y);
+				if (2 * n + 1 < M)
+					o[2 * n + 1] = make_double2(2 * c.x - mu1.x, 2 * c.y - mu1.y);
+			}
+		}
+	}
+	TBTKX_CHECK_HIP(hipEventRecord(h->ev_stop, h->stream));
+	TBTKX_CHECK_HIP(hipEventSynchronize(h->ev_stop));
+	float ms = 0;
+	TBTKX_CHECK_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
+	h->last_ms = ms;
+	h->last_launches = r.launches;
+	return TBTKX_OK;
+}
+
+template <typename VT, typename HT>
+int dispatch_width(tbtkx_context *h, int B, int total, const int64_t *sites,
+                   int mode, uint64_t seed, int M, int flags,
+                   std::vector<double2> &mu) {
+	switch (B) {
+	case 64: return run_all<VT, HT, 64>(h, total, sites, mode, seed, M, flags, mu);
+	case 128: return run_all<VT, HT, 128>(h, total, sites, mode, seed, M, flags, mu);
+	default: return run_all<VT, HT, 256>(h, total, sites, mode, seed, M, flags, mu);
+	}
+}
+
+int block_entry(tbtkx_context *h, int total, const int64_t *sites, int mode,
+                uint64_t seed, int M, int flags, std::vector<double2> &mu) {
+	int B = h->block_width;
+	if (B == 0 || B == 32) {
+		B = 256;
+		if (total <= 64) B = 64;
+		else if (total <= 128) B = 128;
+	}
+	int rc = set_device(h);
+	if (rc) return rc;
+	if (flags & TBTKX_VEC_FP64) {
+		if (!(flags & TBTKX_H_FP64)) flags |= TBTKX_H_FP64;
+		return dispatch_width<double2, double2>(h, B, total, sites, mode, seed, M,
+		                                        flags, mu);
+	}
+	if (flags & TBTKX_H_FP64)
+		return dispatch_width<float2, double2>(h, B, total, sites, mode, seed, M,
+		                                       flags, mu);
+	return dispatch_width<float2, float2>(h, B, total, sites, mode, seed, M, flags,
+	                                      mu);
+}
+
+}  // namespace
+}  // namespace tbtkx
+
+using namespace tbtkx;
+
+extern "C" {
+
+int tbtkx_moments_block(tbtkx_handle h, int num_sites, const int64_t *sites,
+                        int M, int flags, double *out) {
+	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_moments_block: null handle");
+	TBTKX_REQUIRE(h->N > 0, TBTKX_ERR_STATE, "tbtkx_moments_block: no model set");
+	TBTKX_REQUIRE(M > 0, TBTKX_ERR_ARG,
+	              "tbtkx_moments_block: numCoefficients has to be larger than 0");
+	TBTKX_REQUIRE(num_sites >= 0 && (num_sites == 0 || (sites && out)),
+	              TBTKX_ERR_ARG, "tbtkx_moments_block: bad site list");
+	TBTKX_REQUIRE((flags & ~(TBTKX_H_FP64 | TBTKX_DOUBLING | TBTKX_VEC_FP64)) == 0,
+	              TBTKX_ERR_ARG, "tbtkx_moments_block: unknown flags");
+	for (int b = 0; b < num_sites; b++)
+		TBTKX_REQUIRE(sites[b] >= 0 && sites[b] < h->N, TBTKX_ERR_ARG,
+		              "tbtkx_moments_block: site outside the basis");
+	if (num_sites == 0) return TBTKX_OK;
+	std::vector<double2> mu;
+	int rc = block_entry(h, num_sites, sites, 0, 0, M, flags, mu);
+	if (rc) return rc;
+	memcpy(out, mu.data(), mu.size() * 16);
+	return TBTKX_OK;
+}
+
+int tbtkx_moments_trace(tbtkx_handle h, int R, uint64_t seed, int M, int flags,
+                        double *out) {
+	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_moments_trace: null handle");
+	TBTKX_REQUIRE(h->N > 0, TBTKX_ERR_STATE, "tbtkx_moments_trace: no model set");
+	TBTKX_REQUIRE(M > 0 && R > 0 && out, TBTKX_ERR_ARG,
+	              "tbtkx_moments_trace: need M > 0, R > 0 and an output array");
+	TBTKX_REQUIRE((flags & ~(TBTKX_H_FP64 | TBTKX_DOUBLING | TBTKX_VEC_FP64)) == 0,
+	              TBTKX_ERR_ARG, "tbtkx_moments_trace: unknown flags");
+	// The random start vectors are never stored, so the moments come from the
+	// doubling identities.
+	std::vector<double2> mu;
+	int rc = block_entry(h, R, nullptr, 1, seed, M, flags | TBTKX_DOUBLING, mu);
+	if (rc) return rc;
+	for (int n = 0; n < M; n++) {
+		double sx = 0, sy = 0;
+		for (int r = 0; r < R; r++) {
+			sx += mu[(size_t)r * M + n].x;
+			sy += mu[(size_t)r * M + n].y;
+		}
+		out[2 * n] = sx;
+		out[2 * n + 1] = sy;
+	}
+	return TBTKX_OK;
+}
+
+}  // extern "C"

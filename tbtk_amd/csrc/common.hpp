@@ -1,0 +1,116 @@
+// Shared definitions of the tbtkx engine (HIP, gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/tbtkx.h"
+
+namespace tbtkx {
+
+// ---------------------------------------------------------------- errors --
+
+void set_error(const std::string &msg);
+
+#define TBTKX_CHECK_HIP(expr)                                                  \
+	do {                                                                   \
+		hipError_t _e = (expr);                                        \
+		if (_e != hipSuccess) {                                        \
+			::tbtkx::set_error(std::string(#expr) + ": " +         \
+			                   hipGetErrorString(_e) + " at " +    \
+			                   __FILE__ + ":" +                    \
+			                   std::to_string(__LINE__));          \
+			return _e == hipErrorOutOfMemory ? TBTKX_ERR_OOM       \
+			                                 : TBTKX_ERR_HIP;      \
+		}                                                              \
+	} while (0)
+
+#define TBTKX_REQUIRE(cond, code, msg)                                         \
+	do {                                                                   \
+		if (!(cond)) {                                                 \
+			::tbtkx::set_error(msg);                               \
+			return (code);                                         \
+		}                                                              \
+	} while (0)
+
+// ------------------------------------------------------------ device mem --
+
+// Owning device buffer; grows on demand, never shrinks until destroyed.
+struct DevBuf {
+	void *ptr = nullptr;
+	size_t bytes = 0;
+	int reserve(size_t want);
+	void release();
+	template <typename T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// ----------------------------------------------------------------- state --
+
+constexpr int kThreads = 256;       // workgroup size of every step kernel
+constexpr int kPlainCap = 1536;     // products staged in LDS per row block
+constexpr int kPlainRows = 256;     // rows per row block (one per thread)
+
+struct GraphCache {
+	hipGraphExec_t exec = nullptr;
+	hipGraph_t graph = nullptr;
+	int64_t key[4] = {-1, -1, -1, -1};
+	void reset();
+};
+
+}  // namespace tbtkx
+
+struct tbtkx_context {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+	double last_ms = 0;
+	int64_t last_launches = 0;
+
+	// Hamiltonian (CSR), scaled by 1/s (v1) and 2/s (v2).
+	int64_t N = 0, Z = 0;
+	double scale = 0;
+	tbtkx::DevBuf rowptr, col, v1, v2, v1f, v2f;
+	// Row blocks for the LDS-staged plain kernel.
+	tbtkx::DevBuf rblk;
+	int nblk = 0;
+
+	// Plain path vectors and moment output.
+	tbtkx::DevBuf vec_a, vec_b, coef, to;
+
+	// Lookup table L[n*E + e], complex double.
+	tbtkx::DevBuf lut;
+	int lut_M = 0, lut_E = 0;
+	tbtkx::DevBuf gf_in, gf_out;
+
+	// Batched path buffers.
+	tbtkx::DevBuf blk_x, blk_y, blk_mu, blk_sites, blk_part;
+
+	// Options.
+	int plain_kernel = 0;
+	int use_graph = 0;
+	int block_width = 0;   // 0 = autotune
+	int xcd_remap = 1;
+
+	tbtkx::GraphCache plain_graph;
+};
+
+namespace tbtkx {
+
+// Map the hardware workgroup id to a logical one so that each of the 8 XCDs
+// (which receive workgroups round-robin) works on one contiguous range of
+// rows and keeps its neighbours' vector entries in its own L2.
+__device__ __forceinline__ int xcd_remap(int hw, int n) {
+	const int xcd = hw & 7;
+	const int local = hw >> 3;
+	const int per = n >> 3;
+	const int rem = n & 7;
+	return xcd * per + (xcd < rem ? xcd : rem) + local;
+}
+
+int set_device(tbtkx_context *h);
+
+}  // namespace tbtkx

@@ -1,0 +1,289 @@
+// Handle lifetime, device pool, Hamiltonian upload and options.
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
+
+#include "common.hpp"
+
+namespace tbtkx {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int DevBuf::reserve(size_t want) {
+	if (want <= bytes && ptr) return TBTKX_OK;
+	release();
+	if (want == 0) want = 256;
+	hipError_t e = hipMalloc(&ptr, want);
+	if (e != hipSuccess) {
+		ptr = nullptr;
+		set_error("hipMalloc(" + std::to_string(want) + " B): " +
+		          hipGetErrorString(e));
+		return e == hipErrorOutOfMemory ? TBTKX_ERR_OOM : TBTKX_ERR_HIP;
+	}
+	bytes = want;
+	return TBTKX_OK;
+}
+
+void DevBuf::release() {
+	if (ptr) (void)hipFree(ptr);
+	ptr = nullptr;
+	bytes = 0;
+}
+
+void GraphCache::reset() {
+	if (exec) (void)hipGraphExecDestroy(exec);
+	if (graph) (void)hipGraphDestroy(graph);
+	exec = nullptr;
+	graph = nullptr;
+	for (auto &k : key) k = -1;
+}
+
+int set_device(tbtkx_context *h) {
+	TBTKX_CHECK_HIP(hipSetDevice(h->device));
+	return TBTKX_OK;
+}
+
+// Device pool: one flag per device, guarded by a mutex. acquire() waits on a
+// condition variable instead of the reference's spin on an omp_lock_t.
+namespace {
+std::mutex g_pool_mutex;
+std::condition_variable g_pool_cv;
+std::vector<char> g_pool_busy;
+bool g_pool_ready = false;
+}  // namespace
+
+static int pool_init() {
+	if (g_pool_ready) return TBTKX_OK;
+	int n = 0;
+	hipError_t e = hipGetDeviceCount(&n);
+	if (e != hipSuccess || n <= 0) {
+		set_error("no HIP device available");
+		return TBTKX_ERR_NODEVICE;
+	}
+	g_pool_busy.assign(n, 0);
+	g_pool_ready = true;
+	return TBTKX_OK;
+}
+
+}  // namespace tbtkx
+
+using namespace tbtkx;
+
+extern "C" {
+
+const char *tbtkx_last_error(void) { return g_last_error.c_str(); }
+
+int tbtkx_device_count(int *count) {
+	TBTKX_REQUIRE(count, TBTKX_ERR_ARG, "tbtkx_device_count: null pointer");
+	int n = 0;
+	hipError_t e = hipGetDeviceCount(&n);
+	if (e != hipSuccess) n = 0;
+	*count = n;
+	return TBTKX_OK;
+}
+
+int tbtkx_device_acquire(int *device) {
+	TBTKX_REQUIRE(device, TBTKX_ERR_ARG, "tbtkx_device_acquire: null pointer");
+	std::unique_lock<std::mutex> lock(g_pool_mutex);
+	int rc = pool_init();
+	if (rc) return rc;
+	for (;;) {
+		for (size_t d = 0; d < g_pool_busy.size(); d++) {
+			if (!g_pool_busy[d]) {
+				g_pool_busy[d] = 1;
+				*device = (int)d;
+				return TBTKX_OK;
+			}
+		}
+		g_pool_cv.wait(lock);
+	}
+}
+
+int tbtkx_device_release(int device) {
+	std::lock_guard<std::mutex> lock(g_pool_mutex);
+	TBTKX_REQUIRE(g_pool_ready && device >= 0 &&
+	                  device < (int)g_pool_busy.size() && g_pool_busy[device],
+	              TBTKX_ERR_ARG, "tbtkx_device_release: device not held");
+	g_pool_busy[device] = 0;
+	g_pool_cv.notify_one();
+	return TBTKX_OK;
+}
+
+int tbtkx_create(int device, tbtkx_handle *handle) {
+	TBTKX_REQUIRE(handle, TBTKX_ERR_ARG, "tbtkx_create: null handle pointer");
+	*handle = nullptr;
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+		set_error("tbtkx_create: no HIP device available");
+		return TBTKX_ERR_NODEVICE;
+	}
+	TBTKX_REQUIRE(device >= 0 && device < n, TBTKX_ERR_ARG,
+	              "tbtkx_create: device " + std::to_string(device) +
+	                  " out of range (" + std::to_string(n) + " devices)");
+	auto *h = new tbtkx_context();
+	h->device = device;
+	hipError_t e = hipSetDevice(device);
+	if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+	if (e == hipSuccess) e = hipEventCreate(&h->ev_start);
+	if (e == hipSuccess) e = hipEventCreate(&h->ev_stop);
+	if (e != hipSuccess) {
+		set_error(std::string("tbtkx_create: ") + hipGetErrorString(e));
+		tbtkx_destroy(h);
+		return TBTKX_ERR_HIP;
+	}
+	*handle = h;
+	return TBTKX_OK;
+}
+
+int tbtkx_destroy(tbtkx_handle h) {
+	if (!h) return TBTKX_OK;
+	(void)hipSetDevice(h->device);
+	if (h->stream) (void)hipStreamSynchronize(h->stream);
+	h->plain_graph.reset();
+	for (DevBuf *b : {&h->rowptr, &h->col, &h->v1, &h->v2, &h->v1f, &h->v2f,
+	                  &h->rblk, &h->vec_a, &h->vec_b, &h->coef, &h->to,
+	                  &h->lut, &h->gf_in, &h->gf_out, &h->blk_x, &h->blk_y,
+	                  &h->blk_mu, &h->blk_sites, &h->blk_part})
+		b->release();
+	if (h->ev_start) (void)hipEventDestroy(h->ev_start);
+	if (h->ev_stop) (void)hipEventDestroy(h->ev_stop);
+	if (h->stream) (void)hipStreamDestroy(h->stream);
+	delete h;
+	return TBTKX_OK;
+}
+
+// Row blocks for the LDS-staged kernel: at most kPlainRows rows and
+// kPlainCap non-zeros each; a row longer than kPlainCap is a block of its own.
+static std::vector<uint32_t> make_row_blocks(int64_t N, const uint32_t *rowptr) {
+	std::vector<uint32_t> b;
+	b.push_back(0);
+	int64_t r = 0;
+	while (r < N) {
+		const int64_t start = r;
+		int64_t nnz = 0;
+		while (r < N && r - start < kPlainRows &&
+		       nnz + (rowptr[r + 1] - rowptr[r]) <= kPlainCap) {
+			nnz += rowptr[r + 1] - rowptr[r];
+			r++;
+		}
+		if (r == start) r++;
+		b.push_back((uint32_t)r);
+	}
+	return b;
+}
+
+// The reference scales the CSR values in place, first by the complex scalar
+// (1/s, 0) and later by (2, 0) (Lib/src/Solver/ChebyshevExpander.cpp:147,168,
+// SparseMatrix::operator*= at Lib/include/Utilities/TBTK/SparseMatrix.h:
+// 1106-1125). The same two complex products are formed here on the host.
+static void scale_complex(const double *in, double *out, int64_t Z, double d) {
+#pragma clang fp contract(off)
+	for (int64_t k = 0; k < Z; k++) {
+		const double a = in[2 * k], b = in[2 * k + 1];
+		out[2 * k] = a * d - b * 0.0;
+		out[2 * k + 1] = a * 0.0 + b * d;
+	}
+}
+
+int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
+                  const uint32_t *col, const double *val, double scale) {
+	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_set_csr: null handle");
+	TBTKX_REQUIRE(N > 0 && N < (int64_t)1 << 31, TBTKX_ERR_ARG,
+	              "tbtkx_set_csr: num_rows must be in [1, 2^31)");
+	TBTKX_REQUIRE(Z >= 0 && Z < ((int64_t)1 << 32), TBTKX_ERR_ARG,
+	              "tbtkx_set_csr: nnz must be in [0, 2^32)");
+	TBTKX_REQUIRE(rowptr && (Z == 0 || (col && val)), TBTKX_ERR_ARG,
+	              "tbtkx_set_csr: null array");
+	TBTKX_REQUIRE(scale > 0, TBTKX_ERR_ARG,
+	              "tbtkx_set_csr: the scale factor has to be larger than 0");
+	TBTKX_REQUIRE(rowptr[0] == 0 && (int64_t)rowptr[N] == Z, TBTKX_ERR_ARG,
+	              "tbtkx_set_csr: rowptr[0] must be 0 and rowptr[N] == nnz");
+	for (int64_t r = 0; r < N; r++)
+		TBTKX_REQUIRE(rowptr[r] <= rowptr[r + 1], TBTKX_ERR_ARG,
+		              "tbtkx_set_csr: rowptr not monotone");
+	for (int64_t k = 0; k < Z; k++)
+		TBTKX_REQUIRE(col[k] < (uint64_t)N, TBTKX_ERR_ARG,
+		              "tbtkx_set_csr: column index out of range at nnz " +
+		                  std::to_string(k));
+	int rc = set_device(h);
+	if (rc) return rc;
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	h->plain_graph.reset();
+
+	std::vector<double> v1(2 * Z), v2(2 * Z);
+	scale_complex(val, v1.data(), Z, 1 / scale);
+	scale_complex(v1.data(), v2.data(), Z, 2);
+	std::vector<float> v1f(2 * Z), v2f(2 * Z);
+	for (int64_t k = 0; k < 2 * Z; k++) {
+		v1f[k] = (float)v1[k];
+		v2f[k] = (float)v2[k];
+	}
+	std::vector<uint32_t> blocks = make_row_blocks(N, rowptr);
+
+	if ((rc = h->rowptr.reserve((N + 1) * 4))) return rc;
+	if ((rc = h->col.reserve(Z * 4))) return rc;
+	if ((rc = h->v1.reserve(Z * 16))) return rc;
+	if ((rc = h->v2.reserve(Z * 16))) return rc;
+	if ((rc = h->v1f.reserve(Z * 8))) return rc;
+	if ((rc = h->v2f.reserve(Z * 8))) return rc;
+	if ((rc = h->rblk.reserve(blocks.size() * 4))) return rc;
+	if ((rc = h->vec_a.reserve(N * 16))) return rc;
+	if ((rc = h->vec_b.reserve(N * 16))) return rc;
+	TBTKX_CHECK_HIP(hipMemcpy(h->rowptr.ptr, rowptr, (N + 1) * 4, hipMemcpyHostToDevice));
+	if (Z) {
+		TBTKX_CHECK_HIP(hipMemcpy(h->col.ptr, col, Z * 4, hipMemcpyHostToDevice));
+		TBTKX_CHECK_HIP(hipMemcpy(h->v1.ptr, v1.data(), Z * 16, hipMemcpyHostToDevice));
+		TBTKX_CHECK_HIP(hipMemcpy(h->v2.ptr, v2.data(), Z * 16, hipMemcpyHostToDevice));
+		TBTKX_CHECK_HIP(hipMemcpy(h->v1f.ptr, v1f.data(), Z * 8, hipMemcpyHostToDevice));
+		TBTKX_CHECK_HIP(hipMemcpy(h->v2f.ptr, v2f.data(), Z * 8, hipMemcpyHostToDevice));
+	}
+	TBTKX_CHECK_HIP(hipMemcpy(h->rblk.ptr, blocks.data(), blocks.size() * 4,
+	                          hipMemcpyHostToDevice));
+	h->N = N;
+	h->Z = Z;
+	h->scale = scale;
+	h->nblk = (int)blocks.size() - 1;
+	return TBTKX_OK;
+}
+
+int tbtkx_last_timing(tbtkx_handle h, double *ms, int64_t *launches) {
+	TBTKX_REQUIRE(h && ms && launches, TBTKX_ERR_ARG,
+	              "tbtkx_last_timing: null pointer");
+	*ms = h->last_ms;
+	*launches = h->last_launches;
+	return TBTKX_OK;
+}
+
+int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
+	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_set_option: null handle");
+	switch (option) {
+	case TBTKX_OPT_PLAIN_KERNEL:
+		TBTKX_REQUIRE(value == 0 || value == 1, TBTKX_ERR_ARG,
+		              "tbtkx_set_option: plain kernel must be 0 or 1");
+		h->plain_kernel = (int)value;
+		h->plain_graph.reset();
+		return TBTKX_OK;
+	case TBTKX_OPT_GRAPH:
+		h->use_graph = value != 0;
+		h->plain_graph.reset();
+		return TBTKX_OK;
+	case TBTKX_OPT_BLOCK_WIDTH:
+		TBTKX_REQUIRE(value == 0 || value == 32 || value == 64 ||
+		                  value == 128 || value == 256,
+		              TBTKX_ERR_ARG,
+		              "tbtkx_set_option: block width must be 0, 32, 64, 128 or 256");
+		h->block_width = (int)value;
+		return TBTKX_OK;
+	case TBTKX_OPT_XCD_REMAP:
+		h->xcd_remap = value != 0;
+		h->plain_graph.reset();
+		return TBTKX_OK;
+	default:
+		set_error("tbtkx_set_option: unknown option " + std::to_string(option));
+		return TBTKX_ERR_ARG;
+	}
+}
+
+}  // extern "C"

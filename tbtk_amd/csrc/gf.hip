@@ -1,0 +1,171 @@
+// Lookup table and FP64 Green's-function contraction.
+//
+// Table (Lib/src/Solver/ChebyshevExpander.cpp:256-268), E~ = E_e/s:
+//   L[n][e] = (1/s) * (-2i/sqrt(1 - E~^2)) * exp(-i n acos E~) / (n == 0 ? 2 : 1)
+// Contraction (Lib/src/Solver/ChebyshevExpander.cpp:324-370), n ascending:
+//   Retarded      G += L*c            Advanced     G += c*conj(L)
+//   Principal     G += -c*Re(L)       NonPrincipal G -= (c*i)*Im(L)
+// The FP64 kernel keeps that order and operation sequence (no fused
+// multiply-add), so with an uploaded host table it reproduces the reference
+// bit for bit; with the device-generated table the difference is that of
+// the device acos/sqrt/sincos against the host libm (a few ulp).
+//
+// Traffic: the table is read once per coefficient vector by a thread per
+// energy (16 B per (n, e)); it is an FP64 GEMV/GEMM of 8 K M E flops.
+#include "common.hpp"
+
+namespace tbtkx {
+namespace {
+
+__global__ __launch_bounds__(kThreads) void lut_generate(
+    double2 *__restrict__ L, int M, int E, double lo, double dx, double s) {
+#pragma clang fp contract(off)
+	const int e = blockIdx.x * kThreads + threadIdx.x;
+	const int n = blockIdx.y;
+	if (e >= E) return;
+	const double En = (lo + e * dx) / s;
+	const double r = sqrt(1 - En * En);
+	// (1/s) * ((-0, -2) / r)
+	const double ar = (-2.0 * 0.0 / r) * (1 / s);
+	const double ai = (-2.0 / r) * (1 / s);
+	double sn, cs;
+	sincos((-1.0 * (double)n) * acos(En), &sn, &cs);
+	const double denom = n == 0 ? 2.0 : 1.0;
+	L[(int64_t)n * E + e] = make_double2((ar * cs - ai * sn) / denom,
+	                                     (ar * sn + ai * cs) / denom);
+}
+
+template <int TYPE>
+__global__ __launch_bounds__(kThreads) void contract_fp64(
+    const double2 *__restrict__ L, const double2 *__restrict__ C,
+    double2 *__restrict__ G, int M, int E) {
+#pragma clang fp contract(off)
+	const int e = blockIdx.x * kThreads + threadIdx.x;
+	const int k = blockIdx.y;
+	if (e >= E) return;
+	const double2 *c = C + (int64_t)k * M;
+	double gr = 0.0, gi = 0.0;
+	for (int n = 0; n < M; n++) {
+		const double2 l = L[(int64_t)n * E + e];
+		const double2 cn = c[n];
+		if (TYPE == TBTKX_RETARDED) {
+			gr = gr + (l.x * cn.x - l.y * cn.y);
+			gi = gi + (l.x * cn.y + l.y * cn.x);
+		} else if (TYPE == TBTKX_ADVANCED) {
+			gr = gr + (cn.x * l.x - cn.y * (-l.y));
+			gi = gi + (cn.x * (-l.y) + cn.y * l.x);
+		} else if (TYPE == TBTKX_PRINCIPAL) {
+			gr = gr + (-cn.x) * l.x;
+			gi = gi + (-cn.y) * l.x;
+		} else {
+			// (c * i) = (cr*0 - ci*1, cr*1 + ci*0), then times Im(L)
+			const double pr = cn.x * 0.0 - cn.y * 1.0;
+			const double pi = cn.x * 1.0 + cn.y * 0.0;
+			gr = gr - pr * l.y;
+			gi = gi - pi * l.y;
+		}
+	}
+	G[(int64_t)k * E + e] = make_double2(gr, gi);
+}
+
+}  // namespace
+}  // namespace tbtkx
+
+using namespace tbtkx;
+
+extern "C" {
+
+int tbtkx_lut_upload(tbtkx_handle h, int M, int E, const double *table) {
+	TBTKX_REQUIRE(h && table, TBTKX_ERR_ARG, "tbtkx_lut_upload: null pointer");
+	TBTKX_REQUIRE(M > 0 && E > 0, TBTKX_ERR_ARG,
+	              "tbtkx_lut_upload: table dimensions must be positive");
+	int rc = set_device(h);
+	if (rc) return rc;
+	if ((rc = h->lut.reserve((size_t)M * E * 16))) return rc;
+	TBTKX_CHECK_HIP(hipMemcpyAsync(h->lut.ptr, table, (size_t)M * E * 16,
+	                               hipMemcpyHostToDevice, h->stream));
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	h->lut_M = M;
+	h->lut_E = E;
+	return TBTKX_OK;
+}
+
+int tbtkx_lut_generate(tbtkx_handle h, int M, int E, double lo, double hi,
+                       double s) {
+	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_lut_generate: null handle");
+	TBTKX_REQUIRE(M > 0, TBTKX_ERR_ARG,
+	              "tbtkx_lut_generate: numCoefficients has to be larger than 0");
+	TBTKX_REQUIRE(E > 1, TBTKX_ERR_ARG,
+	              "tbtkx_lut_generate: the energy resolution has to be larger than 1");
+	TBTKX_REQUIRE(lo < hi, TBTKX_ERR_ARG,
+	              "tbtkx_lut_generate: the lower bound has to be smaller than the upper bound");
+	TBTKX_REQUIRE(s > 0 && lo > -s && hi < s, TBTKX_ERR_ARG,
+	              "tbtkx_lut_generate: the energy window has to lie inside (-s, s)");
+	int rc = set_device(h);
+	if (rc) return rc;
+	if ((rc = h->lut.reserve((size_t)M * E * 16))) return rc;
+	const double dx = (hi - lo) / (E - 1);
+	dim3 grid((E + kThreads - 1) / kThreads, M);
+	lut_generate<<<grid, kThreads, 0, h->stream>>>(h->lut.as<double2>(), M, E,
+	                                               lo, dx, s);
+	TBTKX_CHECK_HIP(hipGetLastError());
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	h->lut_M = M;
+	h->lut_E = E;
+	return TBTKX_OK;
+}
+
+int tbtkx_lut_destroy(tbtkx_handle h) {
+	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_lut_destroy: null handle");
+	int rc = set_device(h);
+	if (rc) return rc;
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	h->lut.release();
+	h->lut_M = h->lut_E = 0;
+	return TBTKX_OK;
+}
+
+int tbtkx_contract(tbtkx_handle h, int K, const double *coeffs, int type,
+                   double *out) {
+	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_contract: null handle");
+	TBTKX_REQUIRE(h->lut_M > 0, TBTKX_ERR_STATE,
+	              "tbtkx_contract: no lookup table loaded");
+	TBTKX_REQUIRE(K >= 0 && (K == 0 || (coeffs && out)), TBTKX_ERR_ARG,
+	              "tbtkx_contract: bad coefficient arrays");
+	TBTKX_REQUIRE(type >= TBTKX_ADVANCED && type <= TBTKX_NONPRINCIPAL,
+	              TBTKX_ERR_ARG, "tbtkx_contract: unknown Green's function type");
+	TBTKX_REQUIRE(K <= 65535, TBTKX_ERR_ARG,
+	              "tbtkx_contract: at most 65535 vectors per call");
+	if (K == 0) return TBTKX_OK;
+	const int M = h->lut_M, E = h->lut_E;
+	int rc = set_device(h);
+	if (rc) return rc;
+	if ((rc = h->gf_in.reserve((size_t)K * M * 16))) return rc;
+	if ((rc = h->gf_out.reserve((size_t)K * E * 16))) return rc;
+	TBTKX_CHECK_HIP(hipMemcpyAsync(h->gf_in.ptr, coeffs, (size_t)K * M * 16,
+	                               hipMemcpyHostToDevice, h->stream));
+	dim3 grid((E + kThreads - 1) / kThreads, K);
+	const double2 *L = h->lut.as<double2>(), *C = h->gf_in.as<double2>();
+	double2 *G = h->gf_out.as<double2>();
+	switch (type) {
+	case TBTKX_RETARDED:
+		contract_fp64<TBTKX_RETARDED><<<grid, kThreads, 0, h->stream>>>(L, C, G, M, E);
+		break;
+	case TBTKX_ADVANCED:
+		contract_fp64<TBTKX_ADVANCED><<<grid, kThreads, 0, h->stream>>>(L, C, G, M, E);
+		break;
+	case TBTKX_PRINCIPAL:
+		contract_fp64<TBTKX_PRINCIPAL><<<grid, kThreads, 0, h->stream>>>(L, C, G, M, E);
+		break;
+	default:
+		contract_fp64<TBTKX_NONPRINCIPAL><<<grid, kThreads, 0, h->stream>>>(L, C, G, M, E);
+		break;
+	}
+	TBTKX_CHECK_HIP(hipGetLastError());
+	TBTKX_CHECK_HIP(hipMemcpyAsync(out, h->gf_out.ptr, (size_t)K * E * 16,
+	                               hipMemcpyDeviceToHost, h->stream));
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	return TBTKX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,277 @@
+// Plain FP64 path: one start vector, the three-term Chebyshev recurrence
+//   j_1 = (H/s) j_0,   j_n = (2H/s) j_{n-1} - j_{n-2},
+// with the moments <to_k|j_n> gathered as the recurrence runs.
+//
+// Arithmetic order is that of the reference CPU recurrence
+// (Lib/src/Solver/ChebyshevExpander.cpp:65-82, 157-186): every row starts
+// from -j_{n-2}[row] (0 for j_1) and adds v_k * x[col_k] for k ascending, each
+// product formed as (vr*xr - vi*xi, vr*xi + vi*xr) with no fused multiply-add.
+// The moments are therefore bitwise equal to the reference's.
+//
+// HBM traffic per step (algorithmic): 20 B per non-zero (16 B value + 4 B
+// column), 4 B row pointer, 16 B read of x, 16 B read and 16 B write of the
+// in-place j_{n-2} -> j_n row: 20 Z + 52 N bytes.
+#include <cstring>
+
+#include "common.hpp"
+
+namespace tbtkx {
+namespace {
+
+// Row-block kernel: the workgroup streams its block's non-zeros with
+// consecutive threads on consecutive non-zeros (coalesced value/column loads
+// and gathers of x), forms the products in registers, stages them in LDS and
+// then lets one thread per row add them up in column order.
+template <bool FIRST>
+__global__ __launch_bounds__(kThreads) void plain_step_lds(
+    const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ col,
+    const double2 *__restrict__ v, const double2 *__restrict__ x,
+    double2 *y, const uint32_t *__restrict__ rblk, int nblk, int remap,
+    const int64_t *__restrict__ to, int K, double2 *__restrict__ coef,
+    int M, int nprev) {
+#pragma clang fp contract(off)
+	__shared__ double2 prod[kPlainCap];
+	const int tid = threadIdx.x;
+
+	// Moments of the previous vector (x = j_{nprev}); x is not written in
+	// this launch, so any workgroup may read it.
+	const int64_t g = (int64_t)blockIdx.x * kThreads + tid;
+	if (g < K) coef[g * M + nprev] = x[to[g]];
+
+	const int b = remap ? xcd_remap(blockIdx.x, nblk) : (int)blockIdx.x;
+	const uint32_t r0 = rblk[b], r1 = rblk[b + 1];
+	const uint32_t k0 = rowptr[r0], k1 = rowptr[r1];
+
+	if (k1 - k0 <= (uint32_t)kPlainCap) {
+		for (uint32_t k = k0 + tid; k < k1; k += kThreads) {
+			const double2 a = v[k];
+			const double2 c = x[col[k]];
+			prod[k - k0] = make_double2(a.x * c.x - a.y * c.y,
+			                            a.x * c.y + a.y * c.x);
+		}
+		__syncthreads();
+		const uint32_t row = r0 + tid;
+		if (row < r1) {
+			double2 acc;
+			if (FIRST) {
+				acc = make_double2(0.0, 0.0);
+			} else {
+				const double2 p = y[row];
+				acc = make_double2(-p.x, -p.y);
+			}
+			const uint32_t e = rowptr[row + 1] - k0;
+			for (uint32_t k = rowptr[row] - k0; k < e; k++) {
+				acc.x = acc.x + prod[k].x;
+				acc.y = acc.y + prod[k].y;
+			}
+			y[row] = acc;
+		}
+		return;
+	}
+
+	// A single row with more than kPlainCap non-zeros: stage it chunk by
+	// chunk and let thread 0 keep the sequential sum.
+	double2 acc = make_double2(0.0, 0.0);
+	if (tid == 0 && !FIRST) {
+		const double2 p = y[r0];
+		acc = make_double2(-p.x, -p.y);
+	}
+	for (uint32_t base = k0; base < k1; base += kPlainCap) {
+		const uint32_t end = base + kPlainCap < k1 ? base + kPlainCap : k1;
+		for (uint32_t k = base + tid; k < end; k += kThreads) {
+			const double2 a = v[k];
+			const double2 c = x[col[k]];
+			prod[k - base] = make_double2(a.x * c.x - a.y * c.y,
+			                              a.x * c.y + a.y * c.x);
+		}
+		__syncthreads();
+		if (tid == 0)
+			for (uint32_t k = 0; k < end - base; k++) {
+				acc.x = acc.x + prod[k].x;
+				acc.y = acc.y + prod[k].y;
+			}
+		__syncthreads();
+	}
+	if (tid == 0) y[r0] = acc;
+}
+
+// One thread per row, loads straight from memory (no LDS).
+template <bool FIRST>
+__global__ __launch_bounds__(kThreads) void plain_step_row(
+    const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ col,
+    const double2 *__restrict__ v, const double2 *__restrict__ x,
+    double2 *y, int64_t N, int nwg, int remap, const int64_t *__restrict__ to,
+    int K, double2 *__restrict__ coef, int M, int nprev) {
+#pragma clang fp contract(off)
+	const int64_t g = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+	if (g < K) coef[g * M + nprev] = x[to[g]];
+	const int b = remap ? xcd_remap(blockIdx.x, nwg) : (int)blockIdx.x;
+	const int64_t row = (int64_t)b * kThreads + threadIdx.x;
+	if (row >= N) return;
+	double2 acc;
+	if (FIRST) {
+		acc = make_double2(0.0, 0.0);
+	} else {
+		const double2 p = y[row];
+		acc = make_double2(-p.x, -p.y);
+	}
+	const uint32_t e = rowptr[row + 1];
+	for (uint32_t k = rowptr[row]; k < e; k++) {
+		const double2 a = v[k];
+		const double2 c = x[col[k]];
+		acc.x = acc.x + (a.x * c.x - a.y * c.y);
+		acc.y = acc.y + (a.x * c.y + a.y * c.x);
+	}
+	y[row] = acc;
+}
+
+__global__ __launch_bounds__(kThreads) void init_unit(double2 *a, int64_t N,
+                                                      int64_t from) {
+	const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+	if (i < N) a[i] = make_double2(i == from ? 1.0 : 0.0, 0.0);
+}
+
+__global__ __launch_bounds__(kThreads) void gather_moments(
+    const double2 *__restrict__ x, const int64_t *__restrict__ to, int K,
+    double2 *__restrict__ coef, int M, int n) {
+	const int64_t g = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+	if (g < K) coef[g * M + n] = x[to[g]];
+}
+
+__global__ __launch_bounds__(kThreads) void zero_c128(double2 *p, int64_t n) {
+	const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+	if (i < n) p[i] = make_double2(0.0, 0.0);
+}
+
+inline unsigned grid_for(int64_t n) {
+	return (unsigned)((n + kThreads - 1) / kThreads);
+}
+
+// Enqueue the step launches n = 1 .. M-1 and the final gather.
+int enqueue_steps(tbtkx_context *h, int K, int M) {
+	const double2 *v1 = h->v1.as<double2>(), *v2 = h->v2.as<double2>();
+	double2 *a = h->vec_a.as<double2>(), *b = h->vec_b.as<double2>();
+	double2 *coef = h->coef.as<double2>();
+	const int64_t *to = h->to.as<int64_t>();
+	const uint32_t *rp = h->rowptr.as<uint32_t>(), *cl = h->col.as<uint32_t>();
+	const int remap = h->xcd_remap;
+	const bool lds = h->plain_kernel == 0;
+	const int nwg_row = (int)grid_for(h->N);
+	const unsigned grid = lds ? (unsigned)h->nblk : (unsigned)nwg_row;
+	// The moment gather rides on the step launches; their grid covers
+	// K <= N threads (to-indices are unique basis indices).
+	for (int n = 1; n < M; n++) {
+		const double2 *x = (n & 1) ? a : b;
+		double2 *y = (n & 1) ? b : a;
+		const double2 *v = n == 1 ? v1 : v2;
+		if (lds) {
+			if (n == 1)
+				plain_step_lds<true><<<grid, kThreads, 0, h->stream>>>(
+				    rp, cl, v, x, y, h->rblk.as<uint32_t>(), h->nblk,
+				    remap, to, K, coef, M, n - 1);
+			else
+				plain_step_lds<false><<<grid, kThreads, 0, h->stream>>>(
+				    rp, cl, v, x, y, h->rblk.as<uint32_t>(), h->nblk,
+				    remap, to, K, coef, M, n - 1);
+		} else {
+			if (n == 1)
+				plain_step_row<true><<<grid, kThreads, 0, h->stream>>>(
+				    rp, cl, v, x, y, h->N, nwg_row, remap, to, K, coef,
+				    M, n - 1);
+			else
+				plain_step_row<false><<<grid, kThreads, 0, h->stream>>>(
+				    rp, cl, v, x, y, h->N, nwg_row, remap, to, K, coef,
+				    M, n - 1);
+		}
+	}
+	const double2 *last = ((M - 1) & 1) ? b : a;
+	gather_moments<<<grid_for(K), kThreads, 0, h->stream>>>(last, to, K, coef,
+	                                                         M, M - 1);
+	TBTKX_CHECK_HIP(hipGetLastError());
+	return TBTKX_OK;
+}
+
+}  // namespace
+}  // namespace tbtkx
+
+using namespace tbtkx;
+
+extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
+                             const int64_t *to, int M, double *out) {
+	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_moments: null handle");
+	TBTKX_REQUIRE(h->N > 0, TBTKX_ERR_STATE, "tbtkx_moments: no model set");
+	TBTKX_REQUIRE(M > 0, TBTKX_ERR_ARG,
+	              "tbtkx_moments: numCoefficients has to be larger than 0");
+	TBTKX_REQUIRE(K >= 0 && (K == 0 || (to && out)), TBTKX_ERR_ARG,
+	              "tbtkx_moments: bad to-index list");
+	TBTKX_REQUIRE(from >= 0 && from < h->N, TBTKX_ERR_ARG,
+	              "tbtkx_moments: from-index outside the basis");
+	if (K == 0) return TBTKX_OK;
+
+	// Unique to-indices; a repeated index keeps only its last position, as
+	// the reference's coefficientMap (ChebyshevExpander.cpp:121-126).
+	std::vector<int64_t> uniq;
+	std::vector<int> slot(K, -1);
+	{
+		std::map<int64_t, int> last;
+		for (int k = 0; k < K; k++) {
+			TBTKX_REQUIRE(to[k] >= 0 && to[k] < h->N, TBTKX_ERR_ARG,
+			              "tbtkx_moments: to-index outside the basis");
+			last[to[k]] = k;
+		}
+		for (auto &kv : last) {
+			slot[kv.second] = (int)uniq.size();
+			uniq.push_back(kv.first);
+		}
+	}
+	const int U = (int)uniq.size();
+	int rc = set_device(h);
+	if (rc) return rc;
+	if ((rc = h->to.reserve((size_t)U * 8))) return rc;
+	if ((rc = h->coef.reserve((size_t)U * M * 16))) return rc;
+	TBTKX_CHECK_HIP(hipMemcpyAsync(h->to.ptr, uniq.data(), (size_t)U * 8,
+	                               hipMemcpyHostToDevice, h->stream));
+	init_unit<<<grid_for(h->N), kThreads, 0, h->stream>>>(h->vec_a.as<double2>(),
+	                                                      h->N, from);
+	TBTKX_CHECK_HIP(hipGetLastError());
+
+	TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
+	if (h->use_graph) {
+		GraphCache &gc = h->plain_graph;
+		const int64_t key[4] = {U, M, h->plain_kernel, h->xcd_remap};
+		if (!gc.exec || memcmp(key, gc.key, sizeof key) != 0) {
+			gc.reset();
+			TBTKX_CHECK_HIP(hipStreamBeginCapture(h->stream,
+			                                      hipStreamCaptureModeThreadLocal));
+			int erc = enqueue_steps(h, U, M);
+			hipError_t ec = hipStreamEndCapture(h->stream, &gc.graph);
+			if (erc) return erc;
+			TBTKX_CHECK_HIP(ec);
+			TBTKX_CHECK_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr,
+			                                    nullptr, 0));
+			memcpy(gc.key, key, sizeof key);
+		}
+		TBTKX_CHECK_HIP(hipGraphLaunch(gc.exec, h->stream));
+	} else {
+		if ((rc = enqueue_steps(h, U, M))) return rc;
+	}
+	TBTKX_CHECK_HIP(hipEventRecord(h->ev_stop, h->stream));
+
+	std::vector<double> dense((size_t)U * M * 2);
+	TBTKX_CHECK_HIP(hipMemcpyAsync(dense.data(), h->coef.ptr, dense.size() * 8,
+	                               hipMemcpyDeviceToHost, h->stream));
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	float ms = 0;
+	TBTKX_CHECK_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
+	h->last_ms = ms;
+	h->last_launches = M > 1 ? M - 1 : 0;
+
+	for (int k = 0; k < K; k++) {
+		double *o = out + (size_t)k * M * 2;
+		if (slot[k] < 0)
+			memset(o, 0, (size_t)M * 16);
+		else
+			memcpy(o, dense.data() + (size_t)slot[k] * M * 2, (size_t)M * 16);
+	}
+	return TBTKX_OK;
+}

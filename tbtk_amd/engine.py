@@ -1,0 +1,171 @@
+"""Python binding of the tbtkx C ABI (include/tbtkx.h) via ctypes.
+
+This is a thin host-side view of the HIP engine used by tests and bench.py;
+the C++ drop-in classes (tbtk_amd/host) call the same ABI. There is no CPU
+fallback: if the native library is missing or no GPU is present the calls
+raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libtbtkx.so")
+
+ADVANCED, RETARDED, PRINCIPAL, NONPRINCIPAL = 0, 1, 2, 3
+H_FP64, DOUBLING, VEC_FP64 = 1, 2, 4
+OPT_PLAIN_KERNEL, OPT_GRAPH, OPT_BLOCK_WIDTH, OPT_XCD_REMAP = 1, 2, 3, 4
+
+# (name, argtypes) of every exported function, in header order.
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I = ctypes.c_int
+_D = ctypes.c_double
+SIGNATURES = {
+    "tbtkx_last_error": [],
+    "tbtkx_device_count": [_P],
+    "tbtkx_device_acquire": [_P],
+    "tbtkx_device_release": [_I],
+    "tbtkx_create": [_I, _P],
+    "tbtkx_destroy": [_P],
+    "tbtkx_set_csr": [_P, _I64, _I64, _P, _P, _P, _D],
+    "tbtkx_moments": [_P, _I64, _I, _P, _I, _P],
+    "tbtkx_lut_upload": [_P, _I, _I, _P],
+    "tbtkx_lut_generate": [_P, _I, _I, _D, _D, _D],
+    "tbtkx_lut_destroy": [_P],
+    "tbtkx_contract": [_P, _I, _P, _I, _P],
+    "tbtkx_moments_block": [_P, _I, _P, _I, _I, _P],
+    "tbtkx_moments_trace": [_P, _I, ctypes.c_uint64, _I, _I, _P],
+    "tbtkx_last_timing": [_P, _P, _P],
+    "tbtkx_set_option": [_P, _I, _I64],
+}
+
+_lib = None
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libtbtkx.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EngineError(
+                f"native engine {LIB_PATH} is missing: run `python -m tbtk_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_char_p if name == "tbtkx_last_error" else ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise EngineError(f"tbtkx error {rc}: {load().tbtkx_last_error().decode()}")
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    _check(load().tbtkx_device_count(ctypes.byref(n)))
+    return n.value
+
+
+class Engine:
+    """One device handle: a HIP stream plus the resident Hamiltonian."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load()
+        h = ctypes.c_void_p()
+        _check(self._lib.tbtkx_create(device, ctypes.byref(h)))
+        self._h = h
+        self.N = 0
+        self.lut_shape = None
+
+    def close(self):
+        if self._h:
+            self._lib.tbtkx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------- model --
+    def set_csr(self, csr, scale: float):
+        rp, col, val = csr
+        rp = np.ascontiguousarray(rp, dtype=np.uint32)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        val = np.ascontiguousarray(val, dtype=np.complex128)
+        _check(self._lib.tbtkx_set_csr(self._h, rp.size - 1, col.size, _p(rp),
+                                       _p(col), _p(val), float(scale)))
+        self.N = rp.size - 1
+
+    def set_option(self, option: int, value: int):
+        _check(self._lib.tbtkx_set_option(self._h, option, int(value)))
+
+    # ------------------------------------------------------------ plain ---
+    def moments(self, frm: int, to, M: int) -> np.ndarray:
+        to = np.ascontiguousarray(np.atleast_1d(to), dtype=np.int64)
+        out = np.zeros((to.size, M), dtype=np.complex128)
+        _check(self._lib.tbtkx_moments(self._h, int(frm), to.size, _p(to), M, _p(out)))
+        return out
+
+    # ---------------------------------------------------------- contraction
+    def lut_upload(self, table: np.ndarray):
+        table = np.ascontiguousarray(table, dtype=np.complex128)
+        _check(self._lib.tbtkx_lut_upload(self._h, table.shape[0], table.shape[1], _p(table)))
+        self.lut_shape = table.shape
+
+    def lut_generate(self, M, E, lo, hi, scale):
+        _check(self._lib.tbtkx_lut_generate(self._h, M, E, lo, hi, scale))
+        self.lut_shape = (M, E)
+
+    def lut_destroy(self):
+        _check(self._lib.tbtkx_lut_destroy(self._h))
+        self.lut_shape = None
+
+    def contract(self, coeffs, gtype: int) -> np.ndarray:
+        c = np.ascontiguousarray(np.atleast_2d(coeffs), dtype=np.complex128)
+        # Without a table the C call reports the error; E = 1 keeps the
+        # output pointer valid for it.
+        E = self.lut_shape[1] if self.lut_shape else 1
+        out = np.zeros((c.shape[0], E), dtype=np.complex128)
+        _check(self._lib.tbtkx_contract(self._h, c.shape[0], _p(c), gtype, _p(out)))
+        return out
+
+    # ---------------------------------------------------------- batched ---
+    def moments_block(self, sites, M: int, flags: int = 0) -> np.ndarray:
+        sites = np.ascontiguousarray(np.atleast_1d(sites), dtype=np.int64)
+        out = np.zeros((sites.size, M), dtype=np.complex128)
+        _check(self._lib.tbtkx_moments_block(self._h, sites.size, _p(sites), M, flags, _p(out)))
+        return out
+
+    def moments_trace(self, R: int, seed: int, M: int, flags: int = 0) -> np.ndarray:
+        out = np.zeros(M, dtype=np.complex128)
+        _check(self._lib.tbtkx_moments_trace(self._h, R, seed, M, flags, _p(out)))
+        return out
+
+    def last_timing(self):
+        ms = ctypes.c_double(0)
+        n = ctypes.c_int64(0)
+        _check(self._lib.tbtkx_last_timing(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,350 @@
+"""Seeded synthetic tight-binding models for the Chebyshev path, built as CSR.
+
+Every generator returns a :class:`LatticeModel` whose amplitudes are exactly
+what the corresponding TBTK model-building loop would add with
+``model << HoppingAmplitude(amp, to, from)`` (the ``+ HC`` partner written out
+explicitly), and whose basis numbering is TBTK's: depth-first over subindex
+values (``Lib/src/Core/HoppingAmplitudeTree.cpp:596-616``), which for the
+rectangular index sets used here is plain lexicographic order of the Index.
+The CSR form follows ``HoppingAmplitudeSet::getSparseMatrix``
+(``Lib/include/Core/TBTK/HoppingAmplitudeSet.h:397-424``): row = basis(to),
+column = basis(from), columns ascending within a row, duplicates summed
+(``Lib/include/Utilities/TBTK/SparseMatrix.h:1309-1420``). The generators
+never emit duplicate (to, from) pairs: each state gets one combined on-site
+amplitude, so the CSR values do not depend on a summation order.
+
+Model idioms followed:
+  * square lattice with spin, Zeeman ``-J*2*(1/2 - s)``: reference ``README.md``
+  * Nambu s-wave superconductor: ``Templates/UnderDevelopment/
+    SelfConsistentSuperconductivityChebyshev/src/main.cpp:188-214`` and its
+    ``DeltaCallback`` (:147-170)
+  * honeycomb 4-site cell: ``Templates/HexagonalLattice/src/main.cpp:58-103``
+Open boundary conditions throughout (the ``if(x+1 < SIZE_X)`` idiom).
+Random on-site disorder is drawn with ``numpy.random.default_rng(seed)``, one
+value per lattice site in lexicographic site order.
+"""
+from __future__ import annotations
+
+import struct
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class LatticeModel:
+    name: str
+    shape: tuple            # extent of each subindex (rectangular index set)
+    rows: np.ndarray        # int64 [nnz]  basis(to)
+    cols: np.ndarray        # int64 [nnz]  basis(from)
+    vals: np.ndarray        # complex128 [nnz]
+    params: dict = field(default_factory=dict)
+    _csr: tuple | None = None
+
+    @property
+    def nsub(self) -> int:
+        return len(self.shape)
+
+    @property
+    def basis_size(self) -> int:
+        return int(np.prod(self.shape))
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rows.size)
+
+    def basis_index(self, index) -> int:
+        """TBTK basis number of a physical Index (lexicographic order)."""
+        b = 0
+        for v, n in zip(index, self.shape):
+            if not 0 <= v < n:
+                raise IndexError(f"index {index} outside {self.shape}")
+            b = b * n + int(v)
+        return b
+
+    def physical_index(self, b: int) -> tuple:
+        out = []
+        for n in reversed(self.shape):
+            out.append(b % n)
+            b //= n
+        return tuple(reversed(out))
+
+    def csr(self):
+        """(rowptr u32[N+1], col u32[Z], val c128[Z]); cached."""
+        if self._csr is None:
+            self._csr = coo_to_csr(self.rows, self.cols, self.vals, self.basis_size)
+        return self._csr
+
+    def csr_c64(self):
+        rp, col, val = self.csr()
+        return rp, col, val.astype(np.complex64)
+
+    def gershgorin_bound(self) -> float:
+        """max_i sum_j |H_ij| in FP64: an upper bound of the spectral radius."""
+        rp, col, val = self.csr()
+        a = np.abs(val)
+        row_sum = np.add.reduceat(a, rp[:-1].astype(np.int64)) if a.size else a
+        # reduceat on empty rows returns the next element; guard them
+        empty = rp[1:] == rp[:-1]
+        row_sum = np.where(empty, 0.0, row_sum)
+        return float(row_sum.max())
+
+
+def coo_to_csr(rows, cols, vals, n):
+    """Sort by (row, col), merge duplicates by summation, build CSR."""
+    rows = np.asarray(rows, dtype=np.int64)
+    cols = np.asarray(cols, dtype=np.int64)
+    vals = np.asarray(vals, dtype=np.complex128)
+    key = rows * n + cols
+    order = np.argsort(key, kind="stable")
+    key = key[order]
+    vals = vals[order]
+    uniq, start = np.unique(key, return_index=True)
+    if uniq.size != key.size:
+        merged = np.add.reduceat(vals, start)
+    else:
+        merged = vals
+    r = uniq // n
+    c = uniq % n
+    counts = np.bincount(r, minlength=n)
+    rowptr = np.zeros(n + 1, dtype=np.uint32)
+    np.cumsum(counts, out=rowptr[1:])
+    return rowptr, c.astype(np.uint32), np.ascontiguousarray(merged)
+
+
+def _grid(*shape):
+    return np.indices(shape).reshape(len(shape), -1)
+
+
+def _bonds(shape, axis, sub_slice=None):
+    """(to_multi, from_multi) pairs for +1 steps along `axis` (open BC)."""
+    idx = _grid(*shape)
+    keep = idx[axis] + 1 < shape[axis]
+    frm = idx[:, keep]
+    to = frm.copy()
+    to[axis] += 1
+    return to, frm
+
+
+def _flat(multi, shape):
+    return np.ravel_multi_index(tuple(multi), shape).astype(np.int64)
+
+
+def square_spin(lx, ly, t=1.0, J=0.5, mu=0.0, disorder=0.0, seed=0):
+    """{x, y, s}: -t nearest-neighbour hopping, Zeeman -J*2*(1/2-s), -mu,
+    on-site disorder U[-disorder, disorder] per site (same for both spins)."""
+    shape = (lx, ly, 2)
+    rows, cols, vals = [], [], []
+    for axis in (0, 1):
+        to, frm = _bonds(shape, axis)
+        a, b = _flat(to, shape), _flat(frm, shape)
+        amp = np.full(a.size, -t, dtype=np.complex128)
+        rows += [a, b]; cols += [b, a]; vals += [amp, np.conj(amp)]
+    site_w = np.zeros((lx, ly))
+    if disorder:
+        site_w = np.random.default_rng(seed).uniform(-disorder, disorder, size=(lx, ly))
+    g = _grid(*shape)
+    s = g[2]
+    diag = -mu + (-J * 2 * (0.5 - s)) + site_w[g[0], g[1]]
+    d = np.arange(np.prod(shape), dtype=np.int64)
+    rows.append(d); cols.append(d); vals.append(diag.astype(np.complex128))
+    return LatticeModel(
+        "square_spin", shape, np.concatenate(rows), np.concatenate(cols),
+        np.concatenate(vals),
+        dict(lx=lx, ly=ly, t=t, J=J, mu=mu, disorder=disorder, seed=seed))
+
+
+def nambu_swave(lx, ly, t=1.0, mu=-1.0, delta=0.3, J=0.5, disorder=0.0, seed=0):
+    """{x, y, s}, s in 0..3: electron block s=0,1 (-t hopping, -mu, Zeeman
+    -J*2*(1/2-s), disorder w), hole block s=2,3 (negated), and the s-wave
+    pair potential of the SC template's DeltaCallback (to={x,y,3-s},
+    from={x,y,s}: conj(D) for s=0, -conj(D) for s=1; + HC)."""
+    shape = (lx, ly, 4)
+    rows, cols, vals = [], [], []
+    site_w = np.zeros((lx, ly))
+    if disorder:
+        site_w = np.random.default_rng(seed).uniform(-disorder, disorder, size=(lx, ly))
+    for axis in (0, 1):
+        to, frm = _bonds(shape, axis)
+        a, b = _flat(to, shape), _flat(frm, shape)
+        orb = frm[2]
+        amp = np.where(orb < 2, -t, t).astype(np.complex128)
+        rows += [a, b]; cols += [b, a]; vals += [amp, np.conj(amp)]
+    g = _grid(*shape)
+    s = g[2]
+    w = site_w[g[0], g[1]]
+    se = np.where(s < 2, s, s - 2)
+    zee = -J * 2 * (0.5 - se)
+    diag = np.where(s < 2, -mu + zee + w, -(-mu + zee + w))
+    d = np.arange(np.prod(shape), dtype=np.int64)
+    rows.append(d); cols.append(d); vals.append(diag.astype(np.complex128))
+    # pair potential
+    site = _grid(lx, ly)
+    for so in (0, 1):
+        frm = np.vstack([site, np.full(site.shape[1], so)])
+        to = np.vstack([site, np.full(site.shape[1], 3 - so)])
+        a, b = _flat(to, shape), _flat(frm, shape)
+        D = complex(delta)
+        amp = np.full(a.size, np.conj(D) if so == 0 else -np.conj(D), dtype=np.complex128)
+        rows += [a, b]; cols += [b, a]; vals += [amp, np.conj(amp)]
+    return LatticeModel(
+        "nambu_swave", shape, np.concatenate(rows), np.concatenate(cols),
+        np.concatenate(vals),
+        dict(lx=lx, ly=ly, t=t, mu=mu, delta=delta, J=J, disorder=disorder, seed=seed))
+
+
+def honeycomb_ribbon(lx, ly, t=2.7, disorder=0.0, seed=0):
+    """{x, y, n}, n in 0..3: the 4-site rectangular cell of the honeycomb
+    lattice (chain 0-1-2-3 inside the cell, 3 -> next cell's 0 along x,
+    1 -> next cell's 0 and 2 -> next cell's 3 along y), -t hopping, on-site
+    disorder U[-disorder, disorder] per atom."""
+    shape = (lx, ly, 4)
+    rows, cols, vals = [], [], []
+    cell = _grid(lx, ly)
+    ncell = cell.shape[1]
+
+    def add(to_m, frm_m):
+        a, b = _flat(to_m, shape), _flat(frm_m, shape)
+        amp = np.full(a.size, -t, dtype=np.complex128)
+        rows.extend([a, b]); cols.extend([b, a]); vals.extend([amp, np.conj(amp)])
+
+    for n0, n1 in ((0, 1), (1, 2), (2, 3)):
+        add(np.vstack([cell, np.full(ncell, n1)]), np.vstack([cell, np.full(ncell, n0)]))
+    kx = cell[0] + 1 < lx
+    c = cell[:, kx]
+    add(np.vstack([c[0] + 1, c[1], np.zeros(c.shape[1], int)]),
+        np.vstack([c, np.full(c.shape[1], 3)]))
+    ky = cell[1] + 1 < ly
+    c = cell[:, ky]
+    add(np.vstack([c[0], c[1] + 1, np.zeros(c.shape[1], int)]),
+        np.vstack([c, np.full(c.shape[1], 1)]))
+    add(np.vstack([c[0], c[1] + 1, np.full(c.shape[1], 3)]),
+        np.vstack([c, np.full(c.shape[1], 2)]))
+    n = lx * ly * 4
+    w = np.zeros(n)
+    if disorder:
+        w = np.random.default_rng(seed).uniform(-disorder, disorder, size=n)
+    d = np.arange(n, dtype=np.int64)
+    rows.append(d); cols.append(d); vals.append(w.astype(np.complex128))
+    return LatticeModel(
+        "honeycomb_ribbon", shape, np.concatenate(rows), np.concatenate(cols),
+        np.concatenate(vals), dict(lx=lx, ly=ly, t=t, disorder=disorder, seed=seed))
+
+
+def square(lx, ly, t=1.0, disorder=0.0, seed=0):
+    """{x, y}: -t nearest-neighbour hopping, disorder U[-disorder, disorder]."""
+    shape = (lx, ly)
+    rows, cols, vals = [], [], []
+    for axis in (0, 1):
+        to, frm = _bonds(shape, axis)
+        a, b = _flat(to, shape), _flat(frm, shape)
+        amp = np.full(a.size, -t, dtype=np.complex128)
+        rows += [a, b]; cols += [b, a]; vals += [amp, np.conj(amp)]
+    n = lx * ly
+    w = np.zeros(n)
+    if disorder:
+        w = np.random.default_rng(seed).uniform(-disorder, disorder, size=n)
+    d = np.arange(n, dtype=np.int64)
+    rows.append(d); cols.append(d); vals.append(w.astype(np.complex128))
+    return LatticeModel(
+        "square", shape, np.concatenate(rows), np.concatenate(cols),
+        np.concatenate(vals), dict(lx=lx, ly=ly, t=t, disorder=disorder, seed=seed))
+
+
+def cubic(lx, ly, lz, t=1.0, disorder=0.0, seed=0):
+    """{x, y, z}: -t nearest-neighbour hopping, disorder U[-disorder, disorder]."""
+    shape = (lx, ly, lz)
+    rows, cols, vals = [], [], []
+    for axis in (0, 1, 2):
+        to, frm = _bonds(shape, axis)
+        a, b = _flat(to, shape), _flat(frm, shape)
+        amp = np.full(a.size, -t, dtype=np.complex128)
+        rows += [a, b]; cols += [b, a]; vals += [amp, np.conj(amp)]
+    n = lx * ly * lz
+    w = np.zeros(n)
+    if disorder:
+        w = np.random.default_rng(seed).uniform(-disorder, disorder, size=n)
+    d = np.arange(n, dtype=np.int64)
+    rows.append(d); cols.append(d); vals.append(w.astype(np.complex128))
+    return LatticeModel(
+        "cubic", shape, np.concatenate(rows), np.concatenate(cols),
+        np.concatenate(vals), dict(lx=lx, ly=ly, lz=lz, t=t, disorder=disorder, seed=seed))
+
+
+def periodic_square(size, t=1.0, mu=-2.0):
+    """The 5x5 periodic square lattice of the reference's known-answer test
+    (``Test/include/TBTK/Test/Solver/ChebyshevExpander.h:184-200``): {x, y},
+    -mu on site, -t to (x+1)%SIZE and (y+1)%SIZE, + HC. For SIZE=2 the
+    periodic partner coincides with the open one and duplicates are summed."""
+    shape = (size, size)
+    rows, cols, vals = [], [], []
+    g = _grid(size, size)
+    frm = _flat(g, shape)
+    rows.append(frm); cols.append(frm); vals.append(np.full(frm.size, -mu, dtype=np.complex128))
+    for axis in (0, 1):
+        to = g.copy()
+        to[axis] = (to[axis] + 1) % size
+        a = _flat(to, shape)
+        amp = np.full(a.size, -t, dtype=np.complex128)
+        rows += [a, frm]; cols += [frm, a]; vals += [amp, np.conj(amp)]
+    return LatticeModel(
+        "periodic_square", shape, np.concatenate(rows), np.concatenate(cols),
+        np.concatenate(vals), dict(size=size, t=t, mu=mu))
+
+
+def ring(size, t=-1.0):
+    """The 10-site ring of the reference's LDOS sum-rule test
+    (``Test/include/TBTK/Test/PropertyExtractor/ChebyshevExpander.h:19-25``):
+    HoppingAmplitude(-1, {n}, {(n+1)%SIZE}) + HC."""
+    shape = (size,)
+    n = np.arange(size, dtype=np.int64)
+    to = n
+    frm = (n + 1) % size
+    amp = np.full(size, t, dtype=np.complex128)
+    return LatticeModel(
+        "ring", shape, np.concatenate([to, frm]), np.concatenate([frm, to]),
+        np.concatenate([amp, np.conj(amp)]), dict(size=size, t=t))
+
+
+def write_amplitudes(model: LatticeModel, path: str) -> None:
+    """Binary HoppingAmplitude list: b"TBHA", int32 version=1, int32 nsub,
+    int64 count, then per amplitude int32 to[nsub], int32 from[nsub],
+    float64 re, float64 im (little endian)."""
+    nsub = model.nsub
+    to = np.stack(np.unravel_index(model.rows, model.shape), axis=1).astype(np.int32)
+    frm = np.stack(np.unravel_index(model.cols, model.shape), axis=1).astype(np.int32)
+    rec = np.dtype([("to", np.int32, (nsub,)), ("frm", np.int32, (nsub,)),
+                    ("re", np.float64), ("im", np.float64)])
+    arr = np.empty(model.nnz, dtype=rec)
+    arr["to"] = to
+    arr["frm"] = frm
+    arr["re"] = model.vals.real
+    arr["im"] = model.vals.imag
+    with open(path, "wb") as f:
+        f.write(b"TBHA")
+        f.write(struct.pack("<iiq", 1, nsub, model.nnz))
+        arr.tofile(f)
+
+
+# --------------------------------------------------------------------------
+# The workloads of the Chebyshev path, sized as used by the bench and tests.
+# --------------------------------------------------------------------------
+
+def config_plain_small():
+    """30x30 with spin, t=1, J=0.5, mu=0 (1,800 states)."""
+    return square_spin(30, 30, t=1.0, J=0.5, mu=0.0)
+
+
+def config_plain_large(seed=2):
+    """1024x1024 with spin, t=1, J=0.5, disorder U[-1,1] (2,097,152 states)."""
+    return square_spin(1024, 1024, t=1.0, J=0.5, mu=0.0, disorder=1.0, seed=seed)
+
+
+def config_nambu(l=512, seed=3):
+    """l x l Nambu s-wave, t=1, mu=-1, Delta=0.3, J=0.5, disorder U[-1/2,1/2]."""
+    return nambu_swave(l, l, t=1.0, mu=-1.0, delta=0.3, J=0.5, disorder=0.5, seed=seed)
+
+
+def config_cubic(l=128, seed=5):
+    """l^3 cubic lattice, t=1, disorder U[-1,1]."""
+    return cubic(l, l, l, t=1.0, disorder=1.0, seed=seed)
