@@ -1,0 +1,287 @@
+#!/usr/bin/env python3
+"""Benchmark of the Chebyshev-expansion path on one MI355X.
+
+Default workload (one "step" = one LDOS evaluation at the lattice centre,
+summed over spin, i.e. two diagonal Green's functions G_ii):
+  2D square lattice 1024 x 1024 with spin (2,097,152 states, 10,477,568
+  non-zeros, complex128 CSR), t = 1, Zeeman J = 0.5, on-site disorder
+  U[-1, 1] (seed 2), scale factor 12, 4000 Chebyshev coefficients,
+  Lorentz damping 1e-6 (the reference's default), 2000 energies on
+  [-11.88, 11.88], plain FP64 path (one start vector per recurrence),
+  Hamiltonian resident in HBM.
+`value` = LDOS sites/s, one site = one diagonal element G_ii to 4000 moments
+contracted to 2000 energies.
+
+The same line carries the batched comparison ("batched FP32 vs plain FP64
+at fixed error"): 512 x 512 Nambu s-wave superconductor (1,048,576 states),
+256 LDOS sites drawn by seed as one block of unit start vectors, 2000
+coefficients, FP32 storage with FP64 accumulation and the doubling identities,
+against the plain FP64 path on the same model; error = max over compared sites
+of max_E |LDOS - LDOS_FP64| / max_E |LDOS_FP64|.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+                       [--skip-cpu-baseline] [--skip-batched]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from tbtk_amd import engine as X  # noqa: E402
+from tbtk_amd import kpm  # noqa: E402
+from tbtk_amd import lattice as L  # noqa: E402
+
+METRIC = "LDOS sites/s + SpMV GB/s, batched FP32 vs plain FP64 at fixed error"
+HBM_PEAK_GBPS = 8000.0
+
+# Default workload
+SCALE = 12.0
+M_PLAIN = 4000
+E_PTS = 2000
+WINDOW = (-0.99 * SCALE, 0.99 * SCALE)
+BROADENING = 1e-6
+CENTRE = (512, 512)
+
+# Batched comparison
+M_BATCH = 2000
+B_SITES = 256
+PLAIN_CMP_SITES = 4
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def plain_ldos(eng, model, site_xy, M):
+    """LDOS at a site summed over spin: per spin one plain FP64 recurrence,
+    Lorentz damping on the host, FP64 NonPrincipal contraction on the GPU."""
+    ldos = np.zeros(E_PTS)
+    kernel_ms = 0.0
+    launches = 0
+    for s in (0, 1):
+        b = model.basis_index((*site_xy, s))
+        c = eng.moments(b, [b], M)
+        ms, n = eng.last_timing()
+        kernel_ms += ms
+        launches += n
+        c = kpm.lorentz(c, BROADENING)
+        ldos += kpm.ldos_from_gf(eng.contract(c, X.NONPRINCIPAL))
+    return ldos, kernel_ms, launches
+
+
+def cpu_baseline(model, patterns, scale, window, E, M_target, sites_per_call):
+    """Time the reference's own CPU path (oracle/_ref/ref_driver, built from
+    the reference sources) on a bounded sample: calculateLDOS at M=20 and
+    M=60, one thread, extrapolated linearly in M to the full moment count.
+    Falls back to the FP64 C restatement (oracle) if the reference build is
+    absent."""
+    from oracle import oracle as O
+    with tempfile.TemporaryDirectory() as td:
+        if O.ref_available():
+            path = os.path.join(td, "model.bin")
+            L.write_amplitudes(model, path)
+            Ms = (20, 60)
+            r = O.run_ref(threads=1, mode="timeldos", model=path, scale=scale, M=Ms[0],
+                          broadening=BROADENING, window=f"{window[0]},{window[1]},{E}",
+                          lookup=1, patterns=patterns, Ms=",".join(map(str, Ms)))
+            rows = [json.loads(x) for x in r.stderr.splitlines() if x.startswith("{")]
+            t = {int(x["M"]): x["seconds"] for x in rows}
+            slope = (t[Ms[1]] - t[Ms[0]]) / (Ms[1] - Ms[0])
+            fixed = t[Ms[0]] - Ms[0] * slope
+            t_full = fixed + M_target * slope
+            return {"value": sites_per_call / t_full, "unit": "LDOS sites/s", "cores": 1,
+                    "kind": "reference",
+                    "sample": (f"reference PropertyExtractor::ChebyshevExpander::calculateLDOS"
+                               f"({patterns}) on the same model, M={Ms[0]} ({t[Ms[0]]:.2f} s) and "
+                               f"M={Ms[1]} ({t[Ms[1]]:.2f} s), 1 thread, extrapolated linearly to "
+                               f"M={M_target}: {t_full:.1f} s per call of {sites_per_call} sites"),
+                    "seconds_per_call": t_full}
+        # restatement fallback: moments only
+        O.build()
+        b = model.basis_index((*CENTRE, 0))
+        Ms = 30
+        t0 = time.perf_counter()
+        O.moments(model.csr(), scale, b, [b], Ms)
+        dt = (time.perf_counter() - t0) / Ms
+        t_full = sites_per_call * dt * M_target
+        return {"value": sites_per_call / t_full, "unit": "LDOS sites/s", "cores": 1,
+                "kind": "port", "sample": f"oracle recurrence, {Ms} steps, extrapolated",
+                "seconds_per_call": t_full}
+
+
+def load_traffic(kernel):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    entry = data.get(kernel)
+    return None if entry is None else entry.get("hbm_bytes_per_launch")
+
+
+def batched_comparison(eng):
+    """Config: 512x512 Nambu, 256 sites, M=2000; batched FP32 vs plain FP64."""
+    m = L.config_nambu(512)
+    csr = m.csr()
+    N, Z = m.basis_size, csr[1].size
+    s = kpm.gershgorin_scale(csr, 1.05)
+    eng.set_csr(csr, s)
+    rng = np.random.default_rng(3)
+    sites = np.sort(rng.choice(N, B_SITES, replace=False))
+    lo, hi = -0.99 * s, 0.99 * s
+    eng.lut_generate(M_BATCH, E_PTS, lo, hi, s)
+    g = kpm.jackson(M_BATCH)
+
+    def contract(mu):
+        return eng.contract(mu * g[None, :], X.NONPRINCIPAL).imag / np.pi
+
+    # warm-up (allocations), then timed
+    eng.moments_block(sites[:64], 16, X.DOUBLING)
+    t0 = time.perf_counter()
+    mu = eng.moments_block(sites, M_BATCH, X.DOUBLING)
+    ms_rec, launches = eng.last_timing()
+    ld_batched = contract(mu)
+    t_batched = time.perf_counter() - t0
+
+    eng.moments(int(sites[0]), [int(sites[0])], 16)
+    t0 = time.perf_counter()
+    ref = []
+    plain_ms = 0.0
+    for b in sites[:PLAIN_CMP_SITES]:
+        c = eng.moments(int(b), [int(b)], M_BATCH)
+        plain_ms += eng.last_timing()[0]
+        ref.append(c[0])
+    ld_plain = contract(np.array(ref))
+    t_plain = (time.perf_counter() - t0) / PLAIN_CMP_SITES
+
+    err = max(np.abs(ld_batched[i] - ld_plain[i]).max() / np.abs(ld_plain[i]).max()
+              for i in range(PLAIN_CMP_SITES))
+    B = min(256, B_SITES)
+    step_bytes = 12 * Z + 4 * N + 24 * B * N
+    per_step = ms_rec * 1e-3 / launches
+    sps_b = B_SITES / t_batched
+    sps_p = 1.0 / t_plain
+    return {
+        "workload": ("nambu s-wave 512x512x4 (1,048,576 states, t=1, mu=-1, Delta=0.3, "
+                     "J=0.5, disorder U[-0.5,0.5] seed 3), 256 seeded LDOS sites, "
+                     f"M={M_BATCH}, E={E_PTS}, Jackson damping, s=Gershgorin*1.05={s:.6f}"),
+        "nnz": int(Z),
+        "sites_per_s": sps_b,
+        "plain_fp64_sites_per_s": sps_p,
+        "speedup_vs_plain_fp64": sps_b / sps_p,
+        "max_ldos_rel_err_vs_fp64": float(err),
+        "error_sites_compared": PLAIN_CMP_SITES,
+        "batched_dtype": "fp32 storage (H and vectors), fp64 accumulate, doubling",
+        "spmm_block_width": B,
+        "spmm_step_ms": per_step * 1e3,
+        "spmm_GBps": step_bytes / per_step / 1e9,
+        "spmm_hbm_frac": step_bytes / per_step / 1e9 / HBM_PEAK_GBPS,
+        "plain_spmv_GBps": (20 * Z + 52 * N) / (plain_ms * 1e-3 / (PLAIN_CMP_SITES * (M_BATCH - 1))) / 1e9,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--skip-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build runs the Chebyshev path on a "
+              "single MI355X (independent sites/vectors shard across GPUs with no "
+              "communication; see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    log("building 1024x1024 spin lattice")
+    model = L.config_plain_large(seed=2)
+    csr = model.csr()
+    N, Z = model.basis_size, csr[1].size
+    eng = X.Engine(0)
+    eng.set_csr(csr, SCALE)
+    kpm.check_window(*WINDOW, SCALE)
+    eng.lut_generate(M_PLAIN, E_PTS, WINDOW[0], WINDOW[1], SCALE)
+
+    for _ in range(args.warmup):
+        plain_ldos(eng, model, CENTRE, M_PLAIN)
+    log(f"timing {args.steps} steps")
+    kernel_ms = 0.0
+    launches = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ldos, kms, n = plain_ldos(eng, model, CENTRE, M_PLAIN)
+        kernel_ms += kms
+        launches += n
+    elapsed = time.perf_counter() - t0
+    sites_per_step = 2
+    value = args.steps * sites_per_step / elapsed
+    step_bytes = 20 * Z + 52 * N
+    avg_kernel_s = kernel_ms * 1e-3 / launches
+    achieved = step_bytes / avg_kernel_s / 1e9
+    dE = (WINDOW[1] - WINDOW[0]) / (E_PTS - 1)
+    checks = {"ldos_integral": float(ldos.sum() * dE),
+              "ldos_integral_expected": 2.0,
+              "ldos_min": float(ldos.min())}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "LDOS sites/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": ("square lattice 1024x1024 with spin, t=1, J=0.5, disorder U[-1,1] "
+                         "seed 2; plain FP64 single start vector; LDOS at (512,512,sum over "
+                         "spin) = 2 sites per step; s=12, M=4000, E=2000 on [-11.88,11.88], "
+                         "Lorentz 1e-6"),
+            "states": int(N), "nnz": int(Z), "moments": M_PLAIN, "energies": E_PTS,
+            "sites_per_step": sites_per_step,
+        },
+        "physical_sites_per_s": value / 2,
+        "spmv_GBps": achieved,
+        "roofline": {
+            "kernel": "plain_step_lds (FP64 CSR SpMV recurrence step)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": load_traffic("plain_step_lds"),
+            "algorithmic_bytes_per_launch": step_bytes,
+            "avg_launch_us": avg_kernel_s * 1e6,
+        },
+        "checks": checks,
+    }
+    if not args.skip_batched:
+        log("batched comparison (512x512 Nambu, 256 sites)")
+        out["batched"] = batched_comparison(eng)
+    eng.close()
+    if not args.skip_cpu_baseline:
+        log("CPU baseline (reference CPU path, bounded sample)")
+        out["cpu_baseline"] = cpu_baseline(model, "512,512,+", SCALE, WINDOW, E_PTS,
+                                           M_PLAIN, sites_per_step)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

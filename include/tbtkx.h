@@ -140,12 +140,18 @@ int tbtkx_last_timing(tbtkx_handle handle, double *milliseconds,
 
 /* Engine options (0 = default). TBTKX_OPT_PLAIN_KERNEL: 0 = LDS-staged
  * row blocks, 1 = one thread per row. TBTKX_OPT_GRAPH: 1 = replay the step
- * loop as a hipGraph. TBTKX_OPT_BLOCK_WIDTH: start vectors per SpMM block. */
+ * loop as a hipGraph. TBTKX_OPT_BLOCK_WIDTH: start vectors per SpMM block
+ * (64, 128, 256). TBTKX_OPT_XCD_REMAP: 1 = give each XCD a contiguous row
+ * slice. TBTKX_OPT_TILE_ROWS: rows per wavefront tile of the SpMM (1-32).
+ * TBTKX_OPT_NT_STREAM: 1 = non-temporal loads/stores for the SpMM rows that
+ * are used once per step (default 1). */
 enum {
 	TBTKX_OPT_PLAIN_KERNEL = 1,
 	TBTKX_OPT_GRAPH = 2,
 	TBTKX_OPT_BLOCK_WIDTH = 3,
-	TBTKX_OPT_XCD_REMAP = 4
+	TBTKX_OPT_XCD_REMAP = 4,
+	TBTKX_OPT_TILE_ROWS = 5,
+	TBTKX_OPT_NT_STREAM = 6
 };
 int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
 

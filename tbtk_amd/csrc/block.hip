@@ -32,7 +32,6 @@ namespace tbtkx {
 namespace {
 
 constexpr int kWaves = kThreads / 64;
-constexpr int kBlockWG = 2048;     // workgroups per step launch (at most)
 constexpr int kDotChunk = 128;     // steps whose dot partials are kept
 
 template <typename T> struct C2;
@@ -47,6 +46,96 @@ template <> struct C2<float2> {
 template <> struct C2<double2> {
 	__device__ static double2 load(const double2 &a) { return a; }
 	__device__ static double2 store(double2 a) { return a; }
+};
+
+// A lane's CPL consecutive complex values of one block row, moved with the
+// widest aligned accesses (16 B): FP32 pairs as float4, FP64 as double2.
+template <typename VT, int CPL> struct Row {
+	__device__ static void load(const VT *p, VT (&o)[CPL]) {
+#pragma unroll
+		for (int j = 0; j < CPL; j++) o[j] = p[j];
+	}
+	__device__ static void store(VT *p, const VT (&o)[CPL]) {
+#pragma unroll
+		for (int j = 0; j < CPL; j++) p[j] = o[j];
+	}
+};
+template <int CPL> struct Row<float2, CPL> {
+	static_assert(CPL == 1 || CPL % 2 == 0, "FP32 rows move in pairs");
+	__device__ static void load(const float2 *p, float2 (&o)[CPL]) {
+		if (CPL == 1) {
+			o[0] = p[0];
+		} else {
+#pragma unroll
+			for (int j = 0; j < CPL; j += 2) {
+				const float4 q = *reinterpret_cast<const float4 *>(p + j);
+				o[j] = make_float2(q.x, q.y);
+				o[j + 1] = make_float2(q.z, q.w);
+			}
+		}
+	}
+	__device__ static void store(float2 *p, const float2 (&o)[CPL]) {
+		if (CPL == 1) {
+			p[0] = o[0];
+		} else {
+#pragma unroll
+			for (int j = 0; j < CPL; j += 2)
+				*reinterpret_cast<float4 *>(p + j) =
+				    make_float4(o[j].x, o[j].y, o[j + 1].x, o[j + 1].y);
+		}
+	}
+};
+
+// The same moves with the non-temporal hint, for the rows that are read or
+// written exactly once per step (Phi_{n-1} in, Phi_{n+1} out): they stream
+// past L2 instead of evicting the Phi_n rows that neighbouring rows gather.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <typename VT, int CPL> struct RowNT;
+template <int CPL> struct RowNT<float2, CPL> {
+	__device__ static void load(const float2 *p, float2 (&o)[CPL]) {
+		if (CPL == 1) {
+			const f32x2 q = __builtin_nontemporal_load(reinterpret_cast<const f32x2 *>(p));
+			o[0] = make_float2(q.x, q.y);
+		} else {
+#pragma unroll
+			for (int j = 0; j < CPL; j += 2) {
+				const f32x4 q =
+				    __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(p + j));
+				o[j] = make_float2(q.x, q.y);
+				o[j + 1] = make_float2(q.z, q.w);
+			}
+		}
+	}
+	__device__ static void store(float2 *p, const float2 (&o)[CPL]) {
+		if (CPL == 1) {
+			f32x2 q = {o[0].x, o[0].y};
+			__builtin_nontemporal_store(q, reinterpret_cast<f32x2 *>(p));
+		} else {
+#pragma unroll
+			for (int j = 0; j < CPL; j += 2) {
+				f32x4 q = {o[j].x, o[j].y, o[j + 1].x, o[j + 1].y};
+				__builtin_nontemporal_store(q, reinterpret_cast<f32x4 *>(p + j));
+			}
+		}
+	}
+};
+template <int CPL> struct RowNT<double2, CPL> {
+	__device__ static void load(const double2 *p, double2 (&o)[CPL]) {
+#pragma unroll
+		for (int j = 0; j < CPL; j++) {
+			const f64x2 q = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(p + j));
+			o[j] = make_double2(q.x, q.y);
+		}
+	}
+	__device__ static void store(double2 *p, const double2 (&o)[CPL]) {
+#pragma unroll
+		for (int j = 0; j < CPL; j++) {
+			f64x2 q = {o[j].x, o[j].y};
+			__builtin_nontemporal_store(q, reinterpret_cast<f64x2 *>(p + j));
+		}
+	}
 };
 
 // Initial block: unit vectors at sites[b] (mode 0) or random phases
@@ -80,22 +169,107 @@ __global__ __launch_bounds__(kThreads) void block_init(
 	X[i] = C2<VT>::store(v);
 }
 
+// Complex value in the accumulation precision AT (float or double).
+template <typename AT> struct CA { AT x, y; };
+template <typename AT, typename T>
+__device__ __forceinline__ CA<AT> to_acc(const T &v) {
+	return CA<AT>{(AT)v.x, (AT)v.y};
+}
+
+// Broadcast lane `idx` (uniform) of a per-lane value to the whole wavefront.
+__device__ __forceinline__ uint32_t bcast(uint32_t v, int idx) {
+	return (uint32_t)__builtin_amdgcn_readlane((int)v, idx);
+}
+__device__ __forceinline__ float bcast(float v, int idx) {
+	return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), idx));
+}
+__device__ __forceinline__ double bcast(double v, int idx) {
+	const long long b = __double_as_longlong(v);
+	const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), idx);
+	const int hi = __builtin_amdgcn_readlane((int)(b >> 32), idx);
+	return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Column indices, weights and the lane's slice of the Phi rows for the
+// non-zeros k0 .. k0+U-1 of a row ending at ke (weights past ke are zero).
+// The tile's CSR entries kb0 .. kb0+63 are held one per lane in wcol/wval
+// (loaded with one coalesced vector load); entries inside that window are
+// broadcast with readlane, others (rows longer than the window) come from
+// scalar loads. The CSR arrays carry kCsrPad >= 64 zero entries of padding,
+// so every read stays in bounds and entries past ke get weight zero.
+template <typename VT, typename HT, typename AT, int CPL, int U>
+__device__ __forceinline__ void gather_chunk(
+    const uint32_t *__restrict__ col, const HT *__restrict__ v,
+    const VT *__restrict__ X, uint32_t kb0, uint32_t wcol, HT wval,
+    uint32_t k0, uint32_t ke, int lane, CA<AT> (&aa)[U], VT (&xv)[U][CPL]) {
+	constexpr int B = CPL * 64;
+	uint32_t cc[U];
+	if (k0 - kb0 + U <= 64) {
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			const int idx = (int)(k0 - kb0) + u;
+			cc[u] = bcast(wcol, idx);
+			const AT live = k0 + u < ke ? (AT)1 : (AT)0;
+			aa[u] = CA<AT>{(AT)bcast(wval.x, idx) * live, (AT)bcast(wval.y, idx) * live};
+		}
+	} else {
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			cc[u] = col[k0 + u];
+			const CA<AT> w = to_acc<AT>(v[k0 + u]);
+			const AT live = k0 + u < ke ? (AT)1 : (AT)0;
+			aa[u] = CA<AT>{w.x * live, w.y * live};
+		}
+	}
+#pragma unroll
+	for (int u = 0; u < U; u++)
+		Row<VT, CPL>::load(X + (int64_t)cc[u] * B + lane * CPL, xv[u]);
+}
+
+template <typename VT, typename AT, int CPL, int U, bool REALH>
+__device__ __forceinline__ void accumulate_chunk(const CA<AT> (&aa)[U],
+                                                 const VT (&xv)[U][CPL],
+                                                 CA<AT> (&acc)[CPL]) {
+#pragma unroll
+	for (int u = 0; u < U; u++) {
+#pragma unroll
+		for (int j = 0; j < CPL; j++) {
+			const CA<AT> x = to_acc<AT>(xv[u][j]);
+			const CA<AT> a = aa[u];
+			if (REALH) {
+				acc[j].x = fma(a.x, x.x, acc[j].x);
+				acc[j].y = fma(a.x, x.y, acc[j].y);
+			} else {
+				acc[j].x = fma(a.x, x.x, fma(-a.y, x.y, acc[j].x));
+				acc[j].y = fma(a.x, x.y, fma(a.y, x.x, acc[j].y));
+			}
+		}
+	}
+}
+
 // One recurrence step on a block. Y holds Phi_{n-1} on entry and Phi_{n+1}
-// on exit (in place, same lane reads and writes a row's columns). With DOTS,
-// the workgroup's sums of conj(X)X and conj(Y)X per column go to
-// part[wg][0..1][b]; in gather mode column b's moment of X goes to
-// mu[b*M + nprev].
-template <typename VT, typename HT, int B, bool FIRST, bool DOTS>
+// on exit (in place, same lane reads and writes a row's columns). Row sums
+// are formed in AT and rounded once to the storage type VT. With DOTS, the
+// workgroup's FP64 sums of conj(X)X and conj(Y)X per column (from the stored
+// values) go to part[wg][0..1][b]; in gather mode column b's moment of X goes
+// to mu[b*M + nprev]. REALH: every value of H is real (2 FMAs per complex
+// product instead of 4).
+template <typename VT, typename HT, typename AT, int B, bool FIRST, bool DOTS,
+          bool REALH>
 __global__ __launch_bounds__(kThreads) void spmm_step(
     const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ col,
     const HT *__restrict__ v, const VT *__restrict__ X, VT *Y, int64_t N,
-    int64_t rows_per_wave, int nwg, int remap, double2 *__restrict__ part,
+    int tile, int nwg, int remap, int nt, double2 *__restrict__ part,
     const int64_t *__restrict__ sites, int live, double2 *__restrict__ mu,
     int M, int nprev) {
 	constexpr int CPL = B / 64;   // complex values per lane
+	// Rows in flight per wavefront: CPL * RPI = 4 complex columns per lane.
+	constexpr int RPI = CPL >= 4 ? 1 : 4 / CPL;
+	// Non-zeros gathered together (bounded by the CSR padding kCsrPad).
+	constexpr int kUnroll = sizeof(VT) == 8 ? 8 : 4;
+	static_assert(kUnroll <= kCsrPad, "CSR padding too short");
 	const int wave = threadIdx.x >> 6;
 	const int lane = threadIdx.x & 63;
-	const int wg = remap ? xcd_remap(blockIdx.x, nwg) : (int)blockIdx.x;
 
 	if (!DOTS) {
 		const int g = blockIdx.x * kThreads + threadIdx.x;
@@ -104,49 +278,113 @@ __global__ __launch_bounds__(kThreads) void spmm_step(
 			    C2<VT>::load(X[sites[g] * B + g]);
 	}
 
-	double2 d0[CPL], d1[CPL];
+	double d0[CPL];
+	double2 d1[CPL];
 #pragma unroll
 	for (int j = 0; j < CPL; j++) {
-		d0[j] = make_double2(0.0, 0.0);
+		d0[j] = 0.0;
 		d1[j] = make_double2(0.0, 0.0);
 	}
 
-	const int64_t rbeg = ((int64_t)wg * kWaves + wave) * rows_per_wave;
-	const int64_t rend = rbeg + rows_per_wave < N ? rbeg + rows_per_wave : N;
-	for (int64_t row = rbeg; row < rend; row++) {
-		const int64_t base = row * B + lane * CPL;
-		double2 acc[CPL];
+	// Persistent grid (nwg, a multiple of 8, is what fits on the chip at
+	// once). The workgroups of one XCD (hardware id mod 8, the round-robin
+	// dispatch order) own one contiguous slice of the rows, and their
+	// wavefronts take tiles of `tile` rows round-robin. At any time an XCD's
+	// resident wavefronts therefore sweep one window of about
+	// (wavefronts x tile) rows, so the Phi_n rows of a row's near neighbours
+	// are still in that XCD's L2 when they are gathered again.
+	const int per = nwg >> 3;
+	int xcd, local;
+	if (remap) {
+		xcd = blockIdx.x & 7;
+		local = blockIdx.x >> 3;
+	} else {
+		xcd = blockIdx.x / per;
+		local = blockIdx.x - xcd * per;
+	}
+	const int64_t s_lo = N * xcd / 8, s_hi = N * (xcd + 1) / 8;
+	const int64_t stride = (int64_t)per * kWaves * tile;
+	for (int64_t t0 = s_lo + ((int64_t)local * kWaves + wave) * tile; t0 < s_hi;
+	     t0 += stride) {
+		// Row pointers of the tile (tile < 64): lane i holds rowptr[t0 + i];
+		// rows read theirs back with readlane (uniform).
+		const uint32_t rp_lane = rowptr[t0 + lane < N ? t0 + lane : N];
+		const int nrows = s_hi - t0 < tile ? (int)(s_hi - t0) : tile;
+		// The tile's CSR window, one entry per lane (coalesced).
+		const uint32_t kb0 = bcast(rp_lane, 0);
+		const uint32_t wcol = col[kb0 + lane];
+		const HT wval = v[kb0 + lane];
+		for (int i0 = 0; i0 < nrows; i0 += RPI) {
+			// RPI rows at once. For each row, kUnroll non-zeros: their
+			// column indices and values are uniform (scalar) loads, and the
+			// gathers of all RPI rows, their Phi_{n-1} rows (and, for the dot
+			// products, their own Phi_n rows) are issued before the first
+			// multiply, so a wavefront keeps RPI*(kUnroll+2) row loads in
+			// flight. The CSR arrays carry kCsrPad zero entries of padding,
+			// so reads past a row's end stay in bounds; those entries get
+			// weight zero. Rows past the tile end are clamped to a valid
+			// row for their loads and not stored.
+			uint32_t kb[RPI], ke[RPI];
+			int64_t base[RPI];
 #pragma unroll
-		for (int j = 0; j < CPL; j++) {
-			if (FIRST) {
-				acc[j] = make_double2(0.0, 0.0);
-			} else {
-				const double2 p = C2<VT>::load(Y[base + j]);
-				acc[j] = make_double2(-p.x, -p.y);
+			for (int r = 0; r < RPI; r++) {
+				const int i = i0 + r < nrows ? i0 + r : nrows - 1;
+				kb[r] = (uint32_t)__builtin_amdgcn_readlane((int)rp_lane, i);
+				ke[r] = (uint32_t)__builtin_amdgcn_readlane((int)rp_lane, i + 1);
+				base[r] = (t0 + i) * B + lane * CPL;
 			}
-		}
-		const uint32_t kb = rowptr[row], ke = rowptr[row + 1];
-		for (uint32_t k = kb; k < ke; k++) {
-			const double2 a = C2<HT>::load(v[k]);
-			const int64_t cb = (int64_t)col[k] * B + lane * CPL;
+			CA<AT> aa[RPI][kUnroll];
+			VT xv[RPI][kUnroll][CPL];
+			VT yo[RPI][CPL], xs[RPI][CPL];
 #pragma unroll
-			for (int j = 0; j < CPL; j++) {
-				const double2 x = C2<VT>::load(X[cb + j]);
-				acc[j].x = fma(a.x, x.x, fma(-a.y, x.y, acc[j].x));
-				acc[j].y = fma(a.x, x.y, fma(a.y, x.x, acc[j].y));
+			for (int r = 0; r < RPI; r++) {
+				if (!FIRST) {
+					if (nt) RowNT<VT, CPL>::load(Y + base[r], yo[r]);
+					else Row<VT, CPL>::load(Y + base[r], yo[r]);
+				}
+				if (DOTS) Row<VT, CPL>::load(X + base[r], xs[r]);
+				gather_chunk<VT, HT, AT, CPL, kUnroll>(col, v, X, kb0, wcol, wval, kb[r],
+				                                       ke[r], lane, aa[r], xv[r]);
 			}
-		}
 #pragma unroll
-		for (int j = 0; j < CPL; j++) {
-			const VT s = C2<VT>::store(acc[j]);
-			Y[base + j] = s;
-			if (DOTS) {
-				const double2 y = C2<VT>::load(s);
-				const double2 x = C2<VT>::load(X[base + j]);
-				d0[j].x = fma(x.x, x.x, fma(x.y, x.y, d0[j].x));
-				// conj(y) * x
-				d1[j].x = fma(y.x, x.x, fma(y.y, x.y, d1[j].x));
-				d1[j].y = fma(y.x, x.y, fma(-y.y, x.x, d1[j].y));
+			for (int r = 0; r < RPI; r++) {
+				CA<AT> acc[CPL];
+#pragma unroll
+				for (int j = 0; j < CPL; j++) {
+					if (FIRST) {
+						acc[j] = CA<AT>{(AT)0, (AT)0};
+					} else {
+						const CA<AT> p = to_acc<AT>(yo[r][j]);
+						acc[j] = CA<AT>{-p.x, -p.y};
+					}
+				}
+				accumulate_chunk<VT, AT, CPL, kUnroll, REALH>(aa[r], xv[r], acc);
+				// Rows longer than kUnroll (none in nearest-neighbour
+				// lattices) continue chunk by chunk.
+				for (uint32_t k0 = kb[r] + kUnroll; k0 < ke[r]; k0 += kUnroll) {
+					gather_chunk<VT, HT, AT, CPL, kUnroll>(col, v, X, kb0, wcol, wval,
+					                                       k0, ke[r], lane, aa[r], xv[r]);
+					accumulate_chunk<VT, AT, CPL, kUnroll, REALH>(aa[r], xv[r], acc);
+				}
+				if (i0 + r >= nrows) continue;
+				VT st[CPL];
+#pragma unroll
+				for (int j = 0; j < CPL; j++)
+					st[j] = C2<VT>::store(
+					    make_double2((double)acc[j].x, (double)acc[j].y));
+				if (nt) RowNT<VT, CPL>::store(Y + base[r], st);
+				else Row<VT, CPL>::store(Y + base[r], st);
+				if (DOTS) {
+#pragma unroll
+					for (int j = 0; j < CPL; j++) {
+						const double2 y = C2<VT>::load(st[j]);
+						const double2 x = C2<VT>::load(xs[r][j]);
+						d0[j] = fma(x.x, x.x, fma(x.y, x.y, d0[j]));
+						// conj(y) * x
+						d1[j].x = fma(y.x, x.x, fma(y.y, x.y, d1[j].x));
+						d1[j].y = fma(y.x, x.y, fma(-y.y, x.x, d1[j].y));
+					}
+				}
 			}
 		}
 	}
@@ -156,7 +394,7 @@ __global__ __launch_bounds__(kThreads) void spmm_step(
 		__shared__ double2 red[kWaves][2][B];
 #pragma unroll
 		for (int j = 0; j < CPL; j++) {
-			red[wave][0][lane * CPL + j] = d0[j];
+			red[wave][0][lane * CPL + j] = make_double2(d0[j], 0.0);
 			red[wave][1][lane * CPL + j] = d1[j];
 		}
 		__syncthreads();
@@ -168,7 +406,7 @@ __global__ __launch_bounds__(kThreads) void spmm_step(
 				s.x += red[q][w][b].x;
 				s.y += red[q][w][b].y;
 			}
-			part[(int64_t)wg * 2 * B + i] = s;
+			part[(int64_t)blockIdx.x * 2 * B + i] = s;
 		}
 	}
 }
@@ -208,7 +446,7 @@ struct BlockRun {
 	int B;             // block width
 	int flags;
 	int nwg;
-	int64_t rows_per_wave;
+	int tile;          // rows per wavefront tile
 	int64_t launches = 0;
 };
 
@@ -221,14 +459,25 @@ int launch_step(BlockRun &r, bool first, bool dots, const VT *X, VT *Y,
 	if (std::is_same<HT, float2>::value)
 		v = first ? (const HT *)h->v1f.ptr : (const HT *)h->v2f.ptr;
 	const uint32_t *rp = h->rowptr.as<uint32_t>(), *cl = h->col.as<uint32_t>();
-#define TBTKX_STEP(F, D)                                                       \
-	spmm_step<VT, HT, B, F, D><<<r.nwg, kThreads, 0, h->stream>>>(        \
-	    rp, cl, v, X, Y, h->N, r.rows_per_wave, r.nwg, h->xcd_remap, part, \
-	    sites, live, mu, M, nprev)
-	if (first && dots) TBTKX_STEP(true, true);
-	else if (first) TBTKX_STEP(true, false);
-	else if (dots) TBTKX_STEP(false, true);
-	else TBTKX_STEP(false, false);
+	// FP32 H and vectors accumulate rows in FP32; anything with FP64 in it
+	// accumulates in FP64.
+	using AT = typename std::conditional<std::is_same<HT, float2>::value &&
+	                                         std::is_same<VT, float2>::value,
+	                                     float, double>::type;
+#define TBTKX_STEP(F, D, R)                                                    \
+	spmm_step<VT, HT, AT, B, F, D, R><<<r.nwg, kThreads, 0, h->stream>>>(  \
+	    rp, cl, v, X, Y, h->N, r.tile, r.nwg, h->xcd_remap, h->nt_stream,   \
+	    part, sites, live, mu, M, nprev)
+#define TBTKX_STEP_R(F, D)                                                     \
+	do {                                                                   \
+		if (h->h_real) TBTKX_STEP(F, D, true);                         \
+		else TBTKX_STEP(F, D, false);                                  \
+	} while (0)
+	if (first && dots) TBTKX_STEP_R(true, true);
+	else if (first) TBTKX_STEP_R(true, false);
+	else if (dots) TBTKX_STEP_R(false, true);
+	else TBTKX_STEP_R(false, false);
+#undef TBTKX_STEP_R
 #undef TBTKX_STEP
 	r.launches++;
 	return TBTKX_OK;
@@ -272,19 +521,49 @@ int run_block(BlockRun &r, int live, const int64_t *d_sites, int mode,
 	return TBTKX_OK;
 }
 
+// Workgroups of the persistent step grid: as many as fit on the device at
+// once for the steady-state kernel variant, a multiple of 8 (one share per
+// XCD), and no more than the row tiles need.
+template <typename VT, typename HT, int B>
+int persistent_grid(tbtkx_context *h, bool dots, int tile, int *nwg) {
+	using AT = typename std::conditional<std::is_same<HT, float2>::value &&
+	                                         std::is_same<VT, float2>::value,
+	                                     float, double>::type;
+	const void *fn;
+	if (dots)
+		fn = h->h_real ? (const void *)spmm_step<VT, HT, AT, B, false, true, true>
+		               : (const void *)spmm_step<VT, HT, AT, B, false, true, false>;
+	else
+		fn = h->h_real ? (const void *)spmm_step<VT, HT, AT, B, false, false, true>
+		               : (const void *)spmm_step<VT, HT, AT, B, false, false, false>;
+	int per_cu = 0, cus = 0;
+	TBTKX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0));
+	TBTKX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+	                                      h->device));
+	int64_t n = (int64_t)(per_cu > 0 ? per_cu : 1) * cus;
+	const int64_t tiles = (h->N + tile - 1) / tile;
+	const int64_t need = (tiles + kWaves - 1) / kWaves;
+	if (n > need) n = need;
+	n = (n + 7) / 8 * 8;
+	if (n < 8) n = 8;
+	*nwg = (int)n;
+	return TBTKX_OK;
+}
+
 template <typename VT, typename HT, int B>
 int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
             uint64_t seed, int M, int flags, std::vector<double2> &host_mu) {
-	BlockRun r{h, B, flags, 0, 0};
-	const int64_t waves_needed = (h->N + 15) / 16;   // >= 16 rows per wave
-	int64_t nwg = (waves_needed + kWaves - 1) / kWaves;
-	if (nwg > kBlockWG) nwg = kBlockWG;
-	if (nwg < 1) nwg = 1;
-	r.nwg = (int)nwg;
-	r.rows_per_wave = (h->N + nwg * kWaves - 1) / (nwg * kWaves);
+	// Tile: rows a wavefront takes per round-robin turn, a multiple of the
+	// rows it keeps in flight (RPI of spmm_step), at most 32.
+	constexpr int kRpi = B >= 256 ? 1 : 256 / B;
+	int tile = h->tile_rows > 0 ? h->tile_rows : 4;
+	tile = (tile + kRpi - 1) / kRpi * kRpi;
+	if (tile > 32) tile = 32;
+	BlockRun r{h, B, flags, 0, tile, 0};
+	int rc = persistent_grid<VT, HT, B>(h, (flags & TBTKX_DOUBLING) != 0, r.tile, &r.nwg);
+	if (rc) return rc;
 	const bool dots = (flags & TBTKX_DOUBLING) != 0;
 	const int S = dots ? (M + 1) / 2 : M - 1;
-	int rc;
 	if ((rc = h->blk_x.reserve((size_t)h->N * B * sizeof(VT)))) return rc;
 	if ((rc = h->blk_y.reserve((size_t)h->N * B * sizeof(VT)))) return rc;
 	if (dots &&

@@ -51,6 +51,7 @@ struct DevBuf {
 // ----------------------------------------------------------------- state --
 
 constexpr int kThreads = 256;       // workgroup size of every step kernel
+constexpr int kCsrPad = 64;         // zero entries after the last non-zero
 constexpr int kPlainCap = 1536;     // products staged in LDS per row block
 constexpr int kPlainRows = 256;     // rows per row block (one per thread)
 
@@ -73,6 +74,7 @@ struct tbtkx_context {
 	// Hamiltonian (CSR), scaled by 1/s (v1) and 2/s (v2).
 	int64_t N = 0, Z = 0;
 	double scale = 0;
+	bool h_real = false;   // every value of H has a zero imaginary part
 	tbtkx::DevBuf rowptr, col, v1, v2, v1f, v2f;
 	// Row blocks for the LDS-staged plain kernel.
 	tbtkx::DevBuf rblk;
@@ -94,6 +96,8 @@ struct tbtkx_context {
 	int use_graph = 0;
 	int block_width = 0;   // 0 = autotune
 	int xcd_remap = 1;
+	int tile_rows = 0;     // 0 = default
+	int nt_stream = 1;     // non-temporal hints on once-used SpMM rows
 
 	tbtkx::GraphCache plain_graph;
 };

@@ -212,10 +212,16 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
 	h->plain_graph.reset();
 
-	std::vector<double> v1(2 * Z), v2(2 * Z);
+	// Device copies carry kCsrPad zero entries (column 0, value 0) after the
+	// last non-zero, so the batched kernels may read a fixed number of
+	// entries past a row's end without leaving the arrays.
+	const int64_t Zp = Z + kCsrPad;
+	std::vector<uint32_t> colp(Zp, 0);
+	if (Z) memcpy(colp.data(), col, Z * 4);
+	std::vector<double> v1(2 * Zp, 0.0), v2(2 * Zp, 0.0);
 	scale_complex(val, v1.data(), Z, 1 / scale);
 	scale_complex(v1.data(), v2.data(), Z, 2);
-	std::vector<float> v1f(2 * Z), v2f(2 * Z);
+	std::vector<float> v1f(2 * Zp, 0.0f), v2f(2 * Zp, 0.0f);
 	for (int64_t k = 0; k < 2 * Z; k++) {
 		v1f[k] = (float)v1[k];
 		v2f[k] = (float)v2[k];
@@ -223,27 +229,28 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 	std::vector<uint32_t> blocks = make_row_blocks(N, rowptr);
 
 	if ((rc = h->rowptr.reserve((N + 1) * 4))) return rc;
-	if ((rc = h->col.reserve(Z * 4))) return rc;
-	if ((rc = h->v1.reserve(Z * 16))) return rc;
-	if ((rc = h->v2.reserve(Z * 16))) return rc;
-	if ((rc = h->v1f.reserve(Z * 8))) return rc;
-	if ((rc = h->v2f.reserve(Z * 8))) return rc;
+	if ((rc = h->col.reserve(Zp * 4))) return rc;
+	if ((rc = h->v1.reserve(Zp * 16))) return rc;
+	if ((rc = h->v2.reserve(Zp * 16))) return rc;
+	if ((rc = h->v1f.reserve(Zp * 8))) return rc;
+	if ((rc = h->v2f.reserve(Zp * 8))) return rc;
 	if ((rc = h->rblk.reserve(blocks.size() * 4))) return rc;
 	if ((rc = h->vec_a.reserve(N * 16))) return rc;
 	if ((rc = h->vec_b.reserve(N * 16))) return rc;
 	TBTKX_CHECK_HIP(hipMemcpy(h->rowptr.ptr, rowptr, (N + 1) * 4, hipMemcpyHostToDevice));
-	if (Z) {
-		TBTKX_CHECK_HIP(hipMemcpy(h->col.ptr, col, Z * 4, hipMemcpyHostToDevice));
-		TBTKX_CHECK_HIP(hipMemcpy(h->v1.ptr, v1.data(), Z * 16, hipMemcpyHostToDevice));
-		TBTKX_CHECK_HIP(hipMemcpy(h->v2.ptr, v2.data(), Z * 16, hipMemcpyHostToDevice));
-		TBTKX_CHECK_HIP(hipMemcpy(h->v1f.ptr, v1f.data(), Z * 8, hipMemcpyHostToDevice));
-		TBTKX_CHECK_HIP(hipMemcpy(h->v2f.ptr, v2f.data(), Z * 8, hipMemcpyHostToDevice));
-	}
+	TBTKX_CHECK_HIP(hipMemcpy(h->col.ptr, colp.data(), Zp * 4, hipMemcpyHostToDevice));
+	TBTKX_CHECK_HIP(hipMemcpy(h->v1.ptr, v1.data(), Zp * 16, hipMemcpyHostToDevice));
+	TBTKX_CHECK_HIP(hipMemcpy(h->v2.ptr, v2.data(), Zp * 16, hipMemcpyHostToDevice));
+	TBTKX_CHECK_HIP(hipMemcpy(h->v1f.ptr, v1f.data(), Zp * 8, hipMemcpyHostToDevice));
+	TBTKX_CHECK_HIP(hipMemcpy(h->v2f.ptr, v2f.data(), Zp * 8, hipMemcpyHostToDevice));
 	TBTKX_CHECK_HIP(hipMemcpy(h->rblk.ptr, blocks.data(), blocks.size() * 4,
 	                          hipMemcpyHostToDevice));
 	h->N = N;
 	h->Z = Z;
 	h->scale = scale;
+	h->h_real = true;
+	for (int64_t k = 0; k < Z && h->h_real; k++)
+		if (val[2 * k + 1] != 0.0) h->h_real = false;
 	h->nblk = (int)blocks.size() - 1;
 	return TBTKX_OK;
 }
@@ -279,6 +286,14 @@ int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
 	case TBTKX_OPT_XCD_REMAP:
 		h->xcd_remap = value != 0;
 		h->plain_graph.reset();
+		return TBTKX_OK;
+	case TBTKX_OPT_TILE_ROWS:
+		TBTKX_REQUIRE(value >= 0 && value <= 32, TBTKX_ERR_ARG,
+		              "tbtkx_set_option: tile rows must be in [1, 32] (0 = default)");
+		h->tile_rows = (int)value;
+		return TBTKX_OK;
+	case TBTKX_OPT_NT_STREAM:
+		h->nt_stream = value != 0;
 		return TBTKX_OK;
 	default:
 		set_error("tbtkx_set_option: unknown option " + std::to_string(option));
