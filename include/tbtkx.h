@@ -114,6 +114,15 @@ int tbtkx_lut_destroy(tbtkx_handle handle);
 int tbtkx_contract(tbtkx_handle handle, int num_vectors, const double *coeffs,
                    int type, double *out);
 
+/* FP64 contraction without a lookup table: the direct evaluation of
+ * generateGreensFunctionCPU (Lib/src/Solver/ChebyshevExpander.cpp:371-432)
+ * over the energies e0 + e*de, e < E, for K vectors of M coefficients. The
+ * reference has no GPU counterpart (its GPU path requires the table,
+ * Lib/include/TBTK/Solver/ChebyshevExpander.h:555-565). */
+int tbtkx_contract_direct(tbtkx_handle handle, int num_vectors, int num_moments,
+                          const double *coeffs, int type, int resolution,
+                          double e0, double de, double scale, double *out);
+
 /* Batched path: diagonal moments mu[b*M + n] = <s_b| T_n(H/s) |s_b> for B
  * unit start vectors at basis indices sites[b] (LDOS sites), run as blocks
  * of start vectors through one SpMM recurrence. FP32 complex vectors (unless

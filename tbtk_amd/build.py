@@ -74,9 +74,27 @@ def build_host(jobs: int = 8) -> str | None:
     return HOSTLIB
 
 
+def build_cpp_tests(jobs: int = 8) -> list:
+    """The C++ test programs of tests/cpp (drop-in API tests), linked against
+    the two libraries with an $ORIGIN-relative rpath."""
+    tdir = os.path.join(ROOT, "tests", "cpp")
+    srcs = sorted(glob.glob(os.path.join(tdir, "*.cpp")))
+    if not srcs:
+        return []
+    bdir = os.path.join(tdir, "bin")
+    os.makedirs(bdir, exist_ok=True)
+    rpath = "-Wl,-rpath,$ORIGIN/../../../tbtk_amd/lib"
+    outs = [os.path.join(bdir, os.path.splitext(os.path.basename(s))[0]) for s in srcs]
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, [["g++", *HOST_FLAGS, s, "-o", o, "-L", LIB, "-ltbtk_host",
+                            "-ltbtkx", rpath] for s, o in zip(srcs, outs)]))
+    return outs
+
+
 def build_all() -> None:
     build_engine()
     build_host()
+    build_cpp_tests()
 
 
 if __name__ == "__main__":

@@ -68,6 +68,74 @@ __global__ __launch_bounds__(kThreads) void contract_fp64(
 	G[(int64_t)k * E + e] = make_double2(gr, gi);
 }
 
+// Direct evaluation without a table, Lib/src/Solver/ChebyshevExpander.cpp:
+// 371-432: per energy E~ = E/s, theta = acos(E~),
+//   Retarded     (c0/2 + sum_n c_n e^{-in theta}) * (-2i/s)
+//   Advanced     (c0/2 + sum_n conj(c_n e^{-in theta})) * (2i/s)
+//   Principal    (sum_n c_n sin(n theta)) * (2/s)
+//   NonPrincipal (c0/2 + sum_n c_n cos(n theta)) * (2i/s)
+// all divided by sqrt(1 - E~^2); energies e0 + e*de as Range::operator[].
+template <int TYPE>
+__global__ __launch_bounds__(kThreads) void contract_direct(
+    const double2 *__restrict__ C, double2 *__restrict__ G, int M, int E,
+    double e0, double de, double s) {
+#pragma clang fp contract(off)
+	const int e = blockIdx.x * kThreads + threadIdx.x;
+	const int k = blockIdx.y;
+	if (e >= E) return;
+	const double2 *c = C + (int64_t)k * M;
+	double gr, gi;
+	if (TYPE == TBTKX_PRINCIPAL) {
+		gr = 0.0;
+		gi = 0.0;
+	} else {
+		gr = c[0].x / 2.;
+		gi = c[0].y / 2.;
+	}
+	const double En = (e0 + e * de) / s;
+	const double th = acos(En);
+	for (int n = 1; n < M; n++) {
+		const double2 cn = c[n];
+		if (TYPE == TBTKX_RETARDED || TYPE == TBTKX_ADVANCED) {
+			double sn, cs;
+			sincos((-1.0 * (double)n) * th, &sn, &cs);
+			const double pr = cn.x * cs - cn.y * sn;
+			const double pi = cn.x * sn + cn.y * cs;
+			gr = gr + pr;
+			gi = TYPE == TBTKX_RETARDED ? gi + pi : gi + (-pi);
+		} else if (TYPE == TBTKX_PRINCIPAL) {
+			const double sn = sin(((double)n) * th);
+			gr = gr + cn.x * sn;
+			gi = gi + cn.y * sn;
+		} else {
+			const double cs = cos(((double)n) * th);
+			gr = gr + cn.x * cs;
+			gi = gi + cn.y * cs;
+		}
+	}
+	double fr, fi;   // the type's prefactor
+	if (TYPE == TBTKX_RETARDED) {
+		fr = -2. * 0.0 / s;
+		fi = -2. / s;
+	} else if (TYPE == TBTKX_PRINCIPAL) {
+		fr = 2. / s;
+		fi = 0.0;
+	} else {
+		fr = 2. * 0.0 / s;
+		fi = 2. / s;
+	}
+	double hr, hi;
+	if (TYPE == TBTKX_PRINCIPAL) {
+		hr = gr * fr;
+		hi = gi * fr;
+	} else {
+		hr = gr * fr - gi * fi;
+		hi = gr * fi + gi * fr;
+	}
+	const double r = sqrt(1 - En * En);
+	G[(int64_t)k * E + e] = make_double2(hr / r, hi / r);
+}
+
 }  // namespace
 }  // namespace tbtkx
 
@@ -159,6 +227,46 @@ int tbtkx_contract(tbtkx_handle h, int K, const double *coeffs, int type,
 		break;
 	default:
 		contract_fp64<TBTKX_NONPRINCIPAL><<<grid, kThreads, 0, h->stream>>>(L, C, G, M, E);
+		break;
+	}
+	TBTKX_CHECK_HIP(hipGetLastError());
+	TBTKX_CHECK_HIP(hipMemcpyAsync(out, h->gf_out.ptr, (size_t)K * E * 16,
+	                               hipMemcpyDeviceToHost, h->stream));
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	return TBTKX_OK;
+}
+
+int tbtkx_contract_direct(tbtkx_handle h, int K, int M, const double *coeffs, int type,
+                          int E, double e0, double de, double s, double *out) {
+	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_contract_direct: null handle");
+	TBTKX_REQUIRE(K >= 0 && K <= 65535 && M > 0 && E > 0 && (K == 0 || (coeffs && out)),
+	              TBTKX_ERR_ARG, "tbtkx_contract_direct: bad sizes or arrays");
+	TBTKX_REQUIRE(type >= TBTKX_ADVANCED && type <= TBTKX_NONPRINCIPAL, TBTKX_ERR_ARG,
+	              "tbtkx_contract_direct: unknown Green's function type");
+	TBTKX_REQUIRE(s > 0 && e0 > -s && e0 + (E - 1) * de < s, TBTKX_ERR_ARG,
+	              "tbtkx_contract_direct: the energy window has to lie inside (-s, s)");
+	if (K == 0) return TBTKX_OK;
+	int rc = set_device(h);
+	if (rc) return rc;
+	if ((rc = h->gf_in.reserve((size_t)K * M * 16))) return rc;
+	if ((rc = h->gf_out.reserve((size_t)K * E * 16))) return rc;
+	TBTKX_CHECK_HIP(hipMemcpyAsync(h->gf_in.ptr, coeffs, (size_t)K * M * 16,
+	                               hipMemcpyHostToDevice, h->stream));
+	dim3 grid((E + kThreads - 1) / kThreads, K);
+	const double2 *C = h->gf_in.as<double2>();
+	double2 *G = h->gf_out.as<double2>();
+	switch (type) {
+	case TBTKX_RETARDED:
+		contract_direct<TBTKX_RETARDED><<<grid, kThreads, 0, h->stream>>>(C, G, M, E, e0, de, s);
+		break;
+	case TBTKX_ADVANCED:
+		contract_direct<TBTKX_ADVANCED><<<grid, kThreads, 0, h->stream>>>(C, G, M, E, e0, de, s);
+		break;
+	case TBTKX_PRINCIPAL:
+		contract_direct<TBTKX_PRINCIPAL><<<grid, kThreads, 0, h->stream>>>(C, G, M, E, e0, de, s);
+		break;
+	default:
+		contract_direct<TBTKX_NONPRINCIPAL><<<grid, kThreads, 0, h->stream>>>(C, G, M, E, e0, de, s);
 		break;
 	}
 	TBTKX_CHECK_HIP(hipGetLastError());

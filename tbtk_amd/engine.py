@@ -37,6 +37,7 @@ SIGNATURES = {
     "tbtkx_lut_generate": [_P, _I, _I, _D, _D, _D],
     "tbtkx_lut_destroy": [_P],
     "tbtkx_contract": [_P, _I, _P, _I, _P],
+    "tbtkx_contract_direct": [_P, _I, _I, _P, _I, _I, _D, _D, _D, _P],
     "tbtkx_moments_block": [_P, _I, _P, _I, _I, _P],
     "tbtkx_moments_trace": [_P, _I, ctypes.c_uint64, _I, _I, _P],
     "tbtkx_last_timing": [_P, _P, _P],
@@ -150,6 +151,14 @@ class Engine:
         E = self.lut_shape[1] if self.lut_shape else 1
         out = np.zeros((c.shape[0], E), dtype=np.complex128)
         _check(self._lib.tbtkx_contract(self._h, c.shape[0], _p(c), gtype, _p(out)))
+        return out
+
+    def contract_direct(self, coeffs, gtype: int, E: int, e0: float, de: float,
+                        scale: float) -> np.ndarray:
+        c = np.ascontiguousarray(np.atleast_2d(coeffs), dtype=np.complex128)
+        out = np.zeros((c.shape[0], E), dtype=np.complex128)
+        _check(self._lib.tbtkx_contract_direct(self._h, c.shape[0], c.shape[1], _p(c), gtype,
+                                               E, e0, de, scale, _p(out)))
         return out
 
     # ---------------------------------------------------------- batched ---

@@ -1,0 +1,111 @@
+/* Drop-in PropertyExtractor::ChebyshevExpander (and the parts of the
+ * PropertyExtractor base it relies on).
+ *
+ * Interface as Lib/include/TBTK/PropertyExtractor/ChebyshevExpander.h:51-178
+ * and Lib/include/TBTK/PropertyExtractor/PropertyExtractor.h: the energy
+ * window is pushed into the solver; calculateGreensFunction (single, pattern
+ * and multi-to forms) and calculateLDOS (Custom and Ranges formats) return
+ * the same containers in the same memory layout. calculateDOS is new (the
+ * reference's base class exits): a stochastic trace of seeded random-phase
+ * vectors. sampleDOS keeps the reference's srand/rand site sampling
+ * (Lib/src/PropertyExtractor/PropertyExtractor.cpp:234-290).
+ *
+ * Differences in execution, not in results: all to-indices of one
+ * from-index are contracted in one GPU call (the reference loops them under
+ * an OpenMP parallel for that also races on the lookup table, F9), and with
+ * Solver::ChebyshevExpander::setUseBlockRecurrence(true) the diagonal
+ * elements of calculateLDOS run as blocks of start vectors.
+ */
+#ifndef TBTK_AMD_PROPERTY_EXTRACTOR_CHEBYSHEV_EXPANDER
+#define TBTK_AMD_PROPERTY_EXTRACTOR_CHEBYSHEV_EXPANDER
+
+#include "TBTK/Property/Property.h"
+#include "TBTK/Solver/ChebyshevExpander.h"
+
+#include <complex>
+#include <vector>
+
+namespace TBTK {
+namespace PropertyExtractor {
+
+class PropertyExtractor {
+public:
+	PropertyExtractor() : energyWindow(-1, 1, 1000) {}
+	virtual ~PropertyExtractor() {}
+
+	virtual void setEnergyWindow(double lowerBound, double upperBound,
+	                             int energyResolution);
+	const Range &getEnergyWindow() const { return energyWindow; }
+
+	/* Base behaviour: not supported (exit 1), as PropertyExtractor.cpp:
+	 * 226-232. */
+	virtual Property::DOS calculateDOS();
+	virtual Property::LDOS calculateLDOS(std::vector<Index> patterns);
+	/* Sum of LDOS over numSamples basis states drawn with rand() after
+	 * srand(seed), scaled by (sample space)/numSamples. */
+	Property::DOS sampleDOS(unsigned int numSamples,
+	                        const std::vector<Index> &patterns = {},
+	                        unsigned int seed = 0);
+
+protected:
+	/* Expand patterns into an IndexTree of concrete indices, keeping
+	 * IDX_SUM_ALL / IDX_SPIN in place when asked (PropertyExtractor.cpp:
+	 * 323-433). */
+	static IndexTree generateIndexTree(const std::vector<Index> &patterns,
+	                                   const HoppingAmplitudeSet &has,
+	                                   bool keepSummationWildcards,
+	                                   bool keepSpinWildcards);
+	virtual const Model &getModelForSampling() const = 0;
+
+private:
+	Range energyWindow;
+};
+
+class ChebyshevExpander : public PropertyExtractor {
+public:
+	explicit ChebyshevExpander(Solver::ChebyshevExpander &solver);
+
+	void setEnergyWindow(double lowerBound, double upperBound,
+	                     int energyResolution) override;
+
+	Property::GreensFunction calculateGreensFunction(
+	    Index to, Index from,
+	    Property::GreensFunction::Type type = Property::GreensFunction::Type::Retarded);
+	Property::GreensFunction calculateGreensFunction(
+	    std::vector<std::vector<Index>> patterns,
+	    Property::GreensFunction::Type type = Property::GreensFunction::Type::Retarded);
+	Property::GreensFunction calculateGreensFunctions(
+	    std::vector<Index> &to, Index from,
+	    Property::GreensFunction::Type type = Property::GreensFunction::Type::Retarded);
+
+	Property::LDOS calculateLDOS(std::vector<Index> patterns) override;
+	Property::LDOS calculateLDOS(Index pattern, Index ranges);
+
+	/* DOS(E) = Im G_NonPrincipal(E)/pi of the trace moments estimated with
+	 * numStochasticVectors random-phase vectors (setStochasticVectors); with
+	 * 0 vectors the trace is exact: the sum of the LDOS of every basis
+	 * state. Integrates to the basis size. */
+	Property::DOS calculateDOS() override;
+	void setStochasticVectors(int numVectors, uint64_t seed) {
+		stochasticVectors = numVectors;
+		stochasticSeed = seed;
+	}
+
+	Solver::ChebyshevExpander &getSolver() { return *solver; }
+
+private:
+	Solver::ChebyshevExpander *solver;
+	int stochasticVectors = 0;
+	uint64_t stochasticSeed = 0;
+
+	const Model &getModelForSampling() const override { return solver->getModel(); }
+	Solver::ChebyshevExpander::Type toSolverType(Property::GreensFunction::Type t) const;
+	/* LDOS for a list of concrete diagonal indices, summed into the blocks
+	 * of `ldos` given by the offsets. */
+	void accumulateLDOS(const std::vector<Index> &indices,
+	                    const std::vector<int> &offsets, Property::LDOS &ldos);
+};
+
+}  // namespace PropertyExtractor
+}  // namespace TBTK
+#endif

@@ -1,0 +1,29 @@
+/* Error convention of the drop-in: print where and why, then exit(1), as
+ * TBTKAssert / TBTKExit do in the reference (Lib/include/Utilities/TBTK/
+ * TBTKMacros.h:55-97), so EXPECT_EXIT-style tests carry over. */
+#ifndef TBTK_AMD_TBTK_MACROS
+#define TBTK_AMD_TBTK_MACROS
+
+#include "TBTK/Streams.h"
+
+#include <cstdlib>
+#include <sstream>
+
+#define TBTKWhere (std::string(__FILE__) + ", " + std::to_string(__LINE__))
+
+#define TBTKExit(function, message, hint)                                     \
+	do {                                                                  \
+		std::stringstream tbtk_ss_;                                   \
+		tbtk_ss_ << "Error in " << function << "\n\t" << message     \
+		         << "\n\tHint: " << hint << "\n\tWhere: "            \
+		         << TBTKWhere << "\n";                               \
+		::TBTK::Streams::err(tbtk_ss_.str());                         \
+		std::exit(1);                                                 \
+	} while (0)
+
+#define TBTKAssert(condition, function, message, hint)                        \
+	do {                                                                  \
+		if (!(condition)) TBTKExit(function, message, hint);          \
+	} while (0)
+
+#endif

@@ -1,0 +1,189 @@
+// Runs the golden-fixture queries through the drop-in C++ API, building each
+// model with the same loops a TBTK user writes (Model << HoppingAmplitude
+// ... + HC), and writes the raw outputs to <outdir>/<name>.bin for
+// comparison with tests/golden/*.npz (written by the compiled reference).
+//
+// Usage: pe_outputs <outdir> [<crand.bin>]
+#include "TBTK/Model.h"
+#include "TBTK/PropertyExtractor/ChebyshevExpander.h"
+#include "TBTK/Solver/ChebyshevExpander.h"
+
+#include <complex>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+using namespace TBTK;
+using std::complex;
+
+static std::string outdir;
+
+template <typename T> static void write(const std::string &name, const std::vector<T> &v) {
+	FILE *f = fopen((outdir + "/" + name + ".bin").c_str(), "wb");
+	fwrite(v.data(), sizeof(T), v.size(), f);
+	fclose(f);
+}
+
+static void gpu(Solver::ChebyshevExpander &s) {
+	s.setVerbose(false);
+	s.setCalculateCoefficientsOnGPU(true);
+	s.setGenerateGreensFunctionsOnGPU(true);
+}
+
+// 30x30 with spin, README idiom: -t hopping, Zeeman -J*2*(1/2 - s).
+static void squareSpin(Model &model, int L, double t, double J) {
+	model.setVerbose(false);
+	for (int x = 0; x < L; x++)
+		for (int y = 0; y < L; y++)
+			for (int s = 0; s < 2; s++) {
+				if (x + 1 < L) model << HoppingAmplitude(-t, {x + 1, y, s}, {x, y, s}) + HC;
+				if (y + 1 < L) model << HoppingAmplitude(-t, {x, y + 1, s}, {x, y, s}) + HC;
+				model << HoppingAmplitude(-J * 2 * (1 / 2. - s), {x, y, s}, {x, y, s});
+			}
+	model.construct();
+}
+
+static void periodicSquare(Model &model, int SIZE, double mu, double t) {
+	model.setVerbose(false);
+	for (int x = 0; x < SIZE; x++)
+		for (int y = 0; y < SIZE; y++) {
+			model << HoppingAmplitude(-mu, {x, y}, {x, y});
+			model << HoppingAmplitude(-t, {(x + 1) % SIZE, y}, {x, y}) + HC;
+			model << HoppingAmplitude(-t, {x, (y + 1) % SIZE}, {x, y}) + HC;
+		}
+	model.construct();
+}
+
+template <typename C> static std::vector<complex<double>> flatten(const C &rows) {
+	std::vector<complex<double>> out;
+	for (const auto &r : rows) out.insert(out.end(), r.begin(), r.end());
+	return out;
+}
+
+int main(int argc, char **argv) {
+	if (argc < 2) {
+		fprintf(stderr, "usage: pe_outputs <outdir> [crand.bin]\n");
+		return 2;
+	}
+	outdir = argv[1];
+
+	{	// square30_spin: coefficients, broadening 0 and 1e-6
+		Model model;
+		squareSpin(model, 30, 1, 0.5);
+		Solver::ChebyshevExpander solver;
+		gpu(solver);
+		solver.setModel(model);
+		solver.setScaleFactor(10);
+		solver.setNumCoefficients(1000);
+		std::vector<Index> to = {{15, 15, 0}, {16, 15, 0}, {15, 15, 1}, {0, 0, 0}, {15, 15, 0}};
+		solver.setBroadening(0);
+		write("square30_spin_coef_b0", flatten(solver.calculateCoefficients(to, {15, 15, 0})));
+		solver.setBroadening(1e-6);
+		write("square30_spin_coef_b1e-06", flatten(solver.calculateCoefficients(to, {15, 15, 0})));
+
+		// LDOS at (15,15,IDX_SUM_ALL) on [-9.99, 9.99] x 1000
+		for (int lookup = 0; lookup < 2; lookup++) {
+			solver.setUseLookupTable(lookup);
+			PropertyExtractor::ChebyshevExpander pe(solver);
+			pe.setEnergyWindow(-9.99, 9.99, 1000);
+			Property::LDOS ldos = pe.calculateLDOS({{15, 15, IDX_SUM_ALL}});
+			write("square30_spin_ldos_lookup" + std::to_string(lookup), ldos.getData());
+		}
+		// Green's functions, 4 types, M = 400, (-9.5, 9.5) x 300, lookup
+		solver.setUseLookupTable(true);
+		solver.setNumCoefficients(400);
+		PropertyExtractor::ChebyshevExpander pe(solver);
+		pe.setEnergyWindow(-9.5, 9.5, 300);
+		const char *names[4] = {"Advanced", "Retarded", "Principal", "NonPrincipal"};
+		const Property::GreensFunction::Type types[4] = {
+		    Property::GreensFunction::Type::Advanced, Property::GreensFunction::Type::Retarded,
+		    Property::GreensFunction::Type::Principal,
+		    Property::GreensFunction::Type::NonPrincipal};
+		for (int t = 0; t < 4; t++) {
+			std::vector<Index> tos = {{15, 15, 0}, {16, 15, 0}};
+			Property::GreensFunction g = pe.calculateGreensFunctions(tos, {15, 15, 0}, types[t]);
+			std::vector<complex<double>> out;
+			for (const Index &ti : tos)
+				for (int e = 0; e < 300; e++) out.push_back(g({ti, {15, 15, 0}}, e));
+			write(std::string("square30_spin_gf_") + names[t], out);
+		}
+		// The pattern form of calculateGreensFunction must give the same.
+		Property::GreensFunction gp = pe.calculateGreensFunction(
+		    {{{15, 15, 0}, {15, 15, 0}}, {{16, 15, 0}, {15, 15, 0}}},
+		    Property::GreensFunction::Type::NonPrincipal);
+		std::vector<complex<double>> out;
+		for (const Index &ti : std::vector<Index>{{15, 15, 0}, {16, 15, 0}})
+			for (int e = 0; e < 300; e++) out.push_back(gp({ti, {15, 15, 0}}, e));
+		write("square30_spin_gfpattern_NonPrincipal", out);
+	}
+
+	{	// periodic5: known-answer model, M = 100, broadening 0
+		Model model;
+		periodicSquare(model, 5, -2, 1);
+		Solver::ChebyshevExpander solver;
+		gpu(solver);
+		solver.setModel(model);
+		solver.setScaleFactor(10);
+		solver.setNumCoefficients(100);
+		solver.setBroadening(0);
+		std::vector<Index> to = {{0, 0}, {1, 0}, {2, 3}};
+		write("periodic5_coef_b0", flatten(solver.calculateCoefficients(to, {0, 0})));
+
+		// sampleDOS: 7 samples, seed 3, M = 200, (-9, 9) x 200
+		solver.setNumCoefficients(200);
+		solver.setBroadening(1e-6);
+		solver.setUseLookupTable(true);
+		PropertyExtractor::ChebyshevExpander pe(solver);
+		pe.setEnergyWindow(-9, 9, 200);
+		write("periodic5_sampledos", pe.sampleDOS(7, {}, 3).getData());
+	}
+
+	{	// ring10 LDOS, windows +-0.9s and +-0.7s, direct contraction
+		Model model;
+		model.setVerbose(false);
+		for (int n = 0; n < 10; n++) model << HoppingAmplitude(-1, {n}, {(n + 1) % 10}) + HC;
+		model.construct();
+		Solver::ChebyshevExpander solver;
+		gpu(solver);
+		solver.setModel(model);
+		solver.setScaleFactor(10);
+		PropertyExtractor::ChebyshevExpander pe(solver);
+		for (double f : {0.9, 0.7}) {
+			pe.setEnergyWindow(-10 * f, 10 * f, 1000);
+			write(std::string("ring10_ldos_") + (f == 0.9 ? "0.9" : "0.7"),
+			      pe.calculateLDOS({{0}}).getData());
+		}
+	}
+
+	if (argc > 2) {	// Green's functions of given coefficients, 4 types
+		std::vector<complex<double>> crand;
+		FILE *f = fopen(argv[2], "rb");
+		double v[2];
+		while (fread(v, 8, 2, f) == 2) crand.push_back(complex<double>(v[0], v[1]));
+		fclose(f);
+		struct Case {
+			const char *name;
+			std::vector<complex<double>> c;
+			double lo, hi;
+			int E;
+		} cases[2] = {{"c123", {1, 2, 3}, -5, 5, 10}, {"crand", crand, -8.7, 9.1, 37}};
+		const char *names[4] = {"Advanced", "Retarded", "Principal", "NonPrincipal"};
+		typedef Solver::ChebyshevExpander::Type T;
+		const T types[4] = {T::Advanced, T::Retarded, T::Principal, T::NonPrincipal};
+		for (const Case &cs : cases)
+			for (int lookup = 0; lookup < 2; lookup++)
+				for (int t = 0; t < 4; t++) {
+					Solver::ChebyshevExpander solver;
+					gpu(solver);
+					solver.setScaleFactor(10);
+					solver.setNumCoefficients((int)cs.c.size());
+					solver.setUseLookupTable(lookup);
+					solver.setEnergyWindow(Range(cs.lo, cs.hi, cs.E));
+					write(std::string("gf_types_") + cs.name + "_" + names[t] + "_lookup" +
+					          std::to_string(lookup),
+					      solver.generateGreensFunction(cs.c, types[t]));
+				}
+	}
+	printf("pe_outputs done\n");
+	return 0;
+}

@@ -1,0 +1,90 @@
+"""GPU tests of the C++ drop-in (TBTK::Solver / PropertyExtractor::
+ChebyshevExpander on the tbtkx engine).
+
+tests/cpp/pe_outputs builds the fixture models with the public API (the
+loops of the reference's README and tests) and writes coefficients, LDOS,
+Green's functions of all four types (multi-to and pattern forms) and a
+sampleDOS; they must equal the reference's outputs in tests/golden bit for
+bit. The direct (no table) contraction runs on the device's acos/sincos, so
+those outputs are checked to 1e-13 relative instead. test_solver Gpu.* holds
+the reference's known-answer and closed-form tests.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import load_fixture
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "bin")
+
+
+@pytest.fixture(scope="module")
+def outputs(tmp_path_factory):
+    d = tmp_path_factory.mktemp("pe")
+    crand = d / "crand.bin"
+    load_fixture("gf_types")["crand_coef"].tofile(crand)
+    r = subprocess.run([os.path.join(BIN, "pe_outputs"), str(d), str(crand)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return d
+
+
+def _read(d, name, dtype):
+    return np.fromfile(d / f"{name}.bin", dtype=dtype)
+
+
+def test_cpp_gpu_cases():
+    r = subprocess.run([os.path.join(BIN, "test_solver"), "Gpu."], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_coefficients_bitexact(outputs):
+    fx = load_fixture("square30_spin")
+    for b in ("0", "1e-06"):
+        got = _read(outputs, f"square30_spin_coef_b{b}", np.complex128).reshape(5, 1000)
+        assert np.array_equal(got, fx[f"coef_b{b}"]), b
+    fx = load_fixture("periodic5")
+    got = _read(outputs, "periodic5_coef_b0", np.complex128).reshape(3, 100)
+    assert np.array_equal(got, fx["coef_b0"])
+
+
+def test_ldos_and_gf_bitexact(outputs):
+    fx = load_fixture("square30_spin_ldos")
+    assert np.array_equal(_read(outputs, "square30_spin_ldos_lookup1", np.float64),
+                          fx["ldos_lookup1"])
+    direct = _read(outputs, "square30_spin_ldos_lookup0", np.float64)
+    ref = fx["ldos_lookup0"]
+    assert np.abs(direct - ref).max() <= 1e-13 * np.abs(ref).max()
+    for t in ("Advanced", "Retarded", "Principal", "NonPrincipal"):
+        got = _read(outputs, f"square30_spin_gf_{t}", np.complex128).reshape(2, 300)
+        assert np.array_equal(got, fx[f"gf_{t}"]), t
+    pat = _read(outputs, "square30_spin_gfpattern_NonPrincipal", np.complex128).reshape(2, 300)
+    assert np.array_equal(pat, fx["gf_NonPrincipal"])
+
+
+def test_gf_types_from_coefficients(outputs):
+    fx = load_fixture("gf_types")
+    for c in ("c123", "crand"):
+        for t in ("Advanced", "Retarded", "Principal", "NonPrincipal"):
+            ref1 = fx[f"{c}_{t}_lookup1"]
+            assert np.array_equal(_read(outputs, f"gf_types_{c}_{t}_lookup1", np.complex128),
+                                  ref1), (c, t)
+            ref0 = fx[f"{c}_{t}_lookup0"]
+            got0 = _read(outputs, f"gf_types_{c}_{t}_lookup0", np.complex128)
+            assert np.abs(got0 - ref0).max() <= 1e-13 * np.abs(ref0).max(), (c, t)
+
+
+def test_ring_ldos_and_sampledos(outputs):
+    fx = load_fixture("ring10_ldos")
+    for f in ("0.9", "0.7"):
+        got = _read(outputs, f"ring10_ldos_{f}", np.float64)
+        ref = fx[f"ldos_{f}"]
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), f
+    fx = load_fixture("periodic5_sampledos")
+    assert np.array_equal(_read(outputs, "periodic5_sampledos", np.float64), fx["dos"])
