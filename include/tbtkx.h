@@ -123,6 +123,20 @@ int tbtkx_contract_direct(tbtkx_handle handle, int num_vectors, int num_moments,
                           const double *coeffs, int type, int resolution,
                           double e0, double de, double scale, double *out);
 
+/* Coefficient vectors coeffs[k*M + n] (K x M complex) for the MFMA
+ * contraction; kept on the device as split-BF16 planes until replaced. */
+int tbtkx_coeffs_upload(tbtkx_handle handle, int num_vectors, int num_moments,
+                        const double *coeffs);
+
+/* Contraction of the uploaded coefficients with the loaded table on MFMA:
+ * split-BF16 three-term products (about 16 significant bits per operand)
+ * with FP32 accumulation, any of the four types. out[k*E + e] (complex, as
+ * doubles) receives the result; with out == NULL it stays on the device
+ * (for timing). The device time is reported by tbtkx_last_timing. No
+ * reference counterpart (the reference contracts one vector at a time,
+ * Lib/src/cuda/ChebyshevSolver.cu:650-670). */
+int tbtkx_contract_mfma(tbtkx_handle handle, int type, double *out);
+
 /* Batched path: diagonal moments mu[b*M + n] = <s_b| T_n(H/s) |s_b> for B
  * unit start vectors at basis indices sites[b] (LDOS sites), run as blocks
  * of start vectors through one SpMM recurrence. FP32 complex vectors (unless

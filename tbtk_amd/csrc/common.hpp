@@ -87,6 +87,10 @@ struct tbtkx_context {
 	tbtkx::DevBuf lut;
 	int lut_M = 0, lut_E = 0;
 	tbtkx::DevBuf gf_in, gf_out;
+	// Split-BF16 planes for the MFMA contraction.
+	tbtkx::DevBuf cplanes, lplanes, gf_mfma_out;
+	int cplanes_K = 0, cplanes_M = 0;
+	int lplanes_valid = 0;   // the table planes match the current table
 
 	// Batched path buffers.
 	tbtkx::DevBuf blk_x, blk_y, blk_mu, blk_sites, blk_part;
@@ -95,7 +99,7 @@ struct tbtkx_context {
 	int plain_kernel = 0;
 	int use_graph = 0;
 	int block_width = 0;   // 0 = autotune
-	int xcd_remap = 1;
+	int xcd_remap = 0;     // measured faster without for both step kernels
 	int tile_rows = 0;     // 0 = default
 	int nt_stream = 1;     // non-temporal hints on once-used SpMM rows
 

@@ -145,7 +145,8 @@ int tbtkx_destroy(tbtkx_handle h) {
 	for (DevBuf *b : {&h->rowptr, &h->col, &h->v1, &h->v2, &h->v1f, &h->v2f,
 	                  &h->rblk, &h->vec_a, &h->vec_b, &h->coef, &h->to,
 	                  &h->lut, &h->gf_in, &h->gf_out, &h->blk_x, &h->blk_y,
-	                  &h->blk_mu, &h->blk_sites, &h->blk_part})
+	                  &h->blk_mu, &h->blk_sites, &h->blk_part, &h->cplanes,
+	                  &h->lplanes, &h->gf_mfma_out})
 		b->release();
 	if (h->ev_start) (void)hipEventDestroy(h->ev_start);
 	if (h->ev_stop) (void)hipEventDestroy(h->ev_stop);

@@ -154,6 +154,7 @@ int tbtkx_lut_upload(tbtkx_handle h, int M, int E, const double *table) {
 	                               hipMemcpyHostToDevice, h->stream));
 	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
 	h->lut_M = M;
+	h->lplanes_valid = 0;
 	h->lut_E = E;
 	return TBTKX_OK;
 }
@@ -179,6 +180,7 @@ int tbtkx_lut_generate(tbtkx_handle h, int M, int E, double lo, double hi,
 	TBTKX_CHECK_HIP(hipGetLastError());
 	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
 	h->lut_M = M;
+	h->lplanes_valid = 0;
 	h->lut_E = E;
 	return TBTKX_OK;
 }
@@ -190,6 +192,7 @@ int tbtkx_lut_destroy(tbtkx_handle h) {
 	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
 	h->lut.release();
 	h->lut_M = h->lut_E = 0;
+	h->lplanes_valid = 0;
 	return TBTKX_OK;
 }
 

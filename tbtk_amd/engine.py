@@ -38,6 +38,8 @@ SIGNATURES = {
     "tbtkx_lut_destroy": [_P],
     "tbtkx_contract": [_P, _I, _P, _I, _P],
     "tbtkx_contract_direct": [_P, _I, _I, _P, _I, _I, _D, _D, _D, _P],
+    "tbtkx_coeffs_upload": [_P, _I, _I, _P],
+    "tbtkx_contract_mfma": [_P, _I, _P],
     "tbtkx_moments_block": [_P, _I, _P, _I, _I, _P],
     "tbtkx_moments_trace": [_P, _I, ctypes.c_uint64, _I, _I, _P],
     "tbtkx_last_timing": [_P, _P, _P],
@@ -159,6 +161,20 @@ class Engine:
         out = np.zeros((c.shape[0], E), dtype=np.complex128)
         _check(self._lib.tbtkx_contract_direct(self._h, c.shape[0], c.shape[1], _p(c), gtype,
                                                E, e0, de, scale, _p(out)))
+        return out
+
+    def coeffs_upload(self, coeffs):
+        c = np.ascontiguousarray(np.atleast_2d(coeffs), dtype=np.complex128)
+        _check(self._lib.tbtkx_coeffs_upload(self._h, c.shape[0], c.shape[1], _p(c)))
+        self.coeffs_shape = c.shape
+
+    def contract_mfma(self, gtype: int, keep_on_device: bool = False):
+        """Split-BF16 MFMA contraction of the uploaded coefficients."""
+        if keep_on_device:
+            _check(self._lib.tbtkx_contract_mfma(self._h, gtype, None))
+            return None
+        out = np.zeros((self.coeffs_shape[0], self.lut_shape[1]), dtype=np.complex128)
+        _check(self._lib.tbtkx_contract_mfma(self._h, gtype, _p(out)))
         return out
 
     # ---------------------------------------------------------- batched ---
