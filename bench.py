@@ -55,6 +55,7 @@ CENTRE = (512, 512)
 # Batched comparison
 M_BATCH = 2000
 B_SITES = 256
+BLOCK_WIDTH = 128     # start vectors per SpMM block (two blocks for 256 sites)
 PLAIN_CMP_SITES = 4
 
 
@@ -136,6 +137,7 @@ def batched_comparison(eng):
     N, Z = m.basis_size, csr[1].size
     s = kpm.gershgorin_scale(csr, 1.05)
     eng.set_csr(csr, s)
+    eng.set_option(X.OPT_BLOCK_WIDTH, BLOCK_WIDTH)
     rng = np.random.default_rng(3)
     sites = np.sort(rng.choice(N, B_SITES, replace=False))
     lo, hi = -0.99 * s, 0.99 * s
@@ -166,7 +168,7 @@ def batched_comparison(eng):
 
     err = max(np.abs(ld_batched[i] - ld_plain[i]).max() / np.abs(ld_plain[i]).max()
               for i in range(PLAIN_CMP_SITES))
-    B = min(256, B_SITES)
+    B = BLOCK_WIDTH
     step_bytes = 12 * Z + 4 * N + 24 * B * N
     per_step = ms_rec * 1e-3 / launches
     sps_b = B_SITES / t_batched
@@ -197,6 +199,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--skip-cpu-baseline", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true",
+                    help="run only the batched comparison (for profiling)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a 1 GiB read + 1 GiB write elementwise kernel (torch) "
+                         "whose known bytes scale the rocprofv3 memory counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the Chebyshev path on a "
@@ -204,6 +211,18 @@ def main():
               "communication; see DESIGN.md). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.add(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
+    if args.skip_plain:
+        # Profiling aid: the batched comparison alone (not a bench line).
+        with X.Engine(0) as eng:
+            print(json.dumps({"metric": METRIC, "batched": batched_comparison(eng)}))
+        return
     log("building 1024x1024 spin lattice")
     model = L.config_plain_large(seed=2)
     csr = model.csr()

@@ -174,7 +174,9 @@ enum {
 	TBTKX_OPT_BLOCK_WIDTH = 3,
 	TBTKX_OPT_XCD_REMAP = 4,
 	TBTKX_OPT_TILE_ROWS = 5,
-	TBTKX_OPT_NT_STREAM = 6
+	TBTKX_OPT_NT_STREAM = 6,
+	TBTKX_OPT_ELL = 7       /* 1 = batched step on the tile-major ELL copy of H
+	                           when no row has more than 8 entries (default) */
 };
 int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
 

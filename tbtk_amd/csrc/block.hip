@@ -411,6 +411,155 @@ __global__ __launch_bounds__(kThreads) void spmm_step(
 	}
 }
 
+// Tile-major ELL form of H for the batched step: tiles of T = 64/W rows,
+// W slots per row (W >= the longest row), stored tile by tile as
+// ecol[tile][row][slot] and the values likewise, so one coalesced load of
+// T*W <= 64 lanes brings a whole tile's column indices (and one its
+// values) without reading row pointers first. Unused slots hold the row's
+// own index with value zero.
+//
+// The wavefront prefetches the next tile's window while it gathers and
+// multiplies the current tile, so a tile costs one exposed round trip to
+// memory (the gathers), hidden by the other resident wavefronts.
+template <typename VT, typename HT, typename AT, int B, bool FIRST, bool DOTS,
+          bool REALH, int W>
+__global__ __launch_bounds__(kThreads) void spmm_step_ell(
+    const uint32_t *__restrict__ ecol, const HT *__restrict__ eval,
+    const VT *__restrict__ X, VT *Y, int64_t N, int nwg, int remap, int nt,
+    double2 *__restrict__ part, const int64_t *__restrict__ sites, int live,
+    double2 *__restrict__ mu, int M, int nprev) {
+	constexpr int CPL = B / 64;
+	constexpr int T = 64 / W;          // rows per tile
+	constexpr int TW = T * W;          // entries per tile (<= 64)
+	constexpr int RPI = CPL >= 4 ? 1 : 4 / CPL;
+	const int wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+
+	if (!DOTS) {
+		const int g = blockIdx.x * kThreads + threadIdx.x;
+		if (g < live)
+			mu[(int64_t)g * M + nprev] = C2<VT>::load(X[sites[g] * B + g]);
+	}
+
+	double d0[CPL];
+	double2 d1[CPL];
+#pragma unroll
+	for (int j = 0; j < CPL; j++) {
+		d0[j] = 0.0;
+		d1[j] = make_double2(0.0, 0.0);
+	}
+
+	// Persistent grid; tile slices per XCD (remap) or per block of
+	// consecutive workgroups, wavefronts round-robin over the slice's tiles.
+	const int per = nwg >> 3;
+	int xcd, local;
+	if (remap) {
+		xcd = blockIdx.x & 7;
+		local = blockIdx.x >> 3;
+	} else {
+		xcd = blockIdx.x / per;
+		local = blockIdx.x - xcd * per;
+	}
+	const int64_t ntiles = (N + T - 1) / T;
+	const int64_t s_lo = ntiles * xcd / 8, s_hi = ntiles * (xcd + 1) / 8;
+	const int64_t stride = (int64_t)per * kWaves;
+	int64_t t = s_lo + (int64_t)local * kWaves + wave;
+
+	uint32_t wcol = 0;
+	HT wval = {};
+	if (t < s_hi && lane < TW) {
+		wcol = ecol[t * TW + lane];
+		wval = eval[t * TW + lane];
+	}
+	for (; t < s_hi; t += stride) {
+		// Prefetch the next tile's window.
+		const int64_t tn = t + stride;
+		uint32_t ncol = 0;
+		HT nval = {};
+		if (tn < s_hi && lane < TW) {
+			ncol = ecol[tn * TW + lane];
+			nval = eval[tn * TW + lane];
+		}
+		const int64_t r0 = t * T;
+		const int nrows = N - r0 < T ? (int)(N - r0) : T;
+		for (int i0 = 0; i0 < nrows; i0 += RPI) {
+			int64_t base[RPI];
+			CA<AT> aa[RPI][W];
+			VT xv[RPI][W][CPL];
+			VT yo[RPI][CPL], xs[RPI][CPL];
+#pragma unroll
+			for (int r = 0; r < RPI; r++) {
+				const int i = i0 + r < nrows ? i0 + r : nrows - 1;
+				base[r] = (r0 + i) * B + lane * CPL;
+				if (!FIRST) {
+					if (nt) RowNT<VT, CPL>::load(Y + base[r], yo[r]);
+					else Row<VT, CPL>::load(Y + base[r], yo[r]);
+				}
+				if (DOTS) Row<VT, CPL>::load(X + base[r], xs[r]);
+#pragma unroll
+				for (int u = 0; u < W; u++) {
+					const uint32_t c = bcast(wcol, i * W + u);
+					aa[r][u] = CA<AT>{(AT)bcast(wval.x, i * W + u), (AT)bcast(wval.y, i * W + u)};
+					Row<VT, CPL>::load(X + (int64_t)c * B + lane * CPL, xv[r][u]);
+				}
+			}
+#pragma unroll
+			for (int r = 0; r < RPI; r++) {
+				CA<AT> acc[CPL];
+#pragma unroll
+				for (int j = 0; j < CPL; j++) {
+					if (FIRST) {
+						acc[j] = CA<AT>{(AT)0, (AT)0};
+					} else {
+						const CA<AT> p = to_acc<AT>(yo[r][j]);
+						acc[j] = CA<AT>{-p.x, -p.y};
+					}
+				}
+				accumulate_chunk<VT, AT, CPL, W, REALH>(aa[r], xv[r], acc);
+				if (i0 + r >= nrows) continue;
+				VT st[CPL];
+#pragma unroll
+				for (int j = 0; j < CPL; j++)
+					st[j] = C2<VT>::store(make_double2((double)acc[j].x, (double)acc[j].y));
+				if (nt) RowNT<VT, CPL>::store(Y + base[r], st);
+				else Row<VT, CPL>::store(Y + base[r], st);
+				if (DOTS) {
+#pragma unroll
+					for (int j = 0; j < CPL; j++) {
+						const double2 y = C2<VT>::load(st[j]);
+						const double2 x = C2<VT>::load(xs[r][j]);
+						d0[j] = fma(x.x, x.x, fma(x.y, x.y, d0[j]));
+						d1[j].x = fma(y.x, x.x, fma(y.y, x.y, d1[j].x));
+						d1[j].y = fma(y.x, x.y, fma(-y.y, x.x, d1[j].y));
+					}
+				}
+			}
+		}
+		wcol = ncol;
+		wval = nval;
+	}
+
+	if (DOTS) {
+		__shared__ double2 red[kWaves][2][B];
+#pragma unroll
+		for (int j = 0; j < CPL; j++) {
+			red[wave][0][lane * CPL + j] = make_double2(d0[j], 0.0);
+			red[wave][1][lane * CPL + j] = d1[j];
+		}
+		__syncthreads();
+		for (int i = threadIdx.x; i < 2 * B; i += kThreads) {
+			const int w = i / B, b = i - w * B;
+			double2 s = red[0][w][b];
+#pragma unroll
+			for (int q = 1; q < kWaves; q++) {
+				s.x += red[q][w][b].x;
+				s.y += red[q][w][b].y;
+			}
+			part[(int64_t)blockIdx.x * 2 * B + i] = s;
+		}
+	}
+}
+
 // D[step][w][b] = sum over workgroups of part[step][wg][w][b], in workgroup
 // order (deterministic).
 __global__ __launch_bounds__(kThreads) void reduce_parts(
@@ -464,6 +613,38 @@ int launch_step(BlockRun &r, bool first, bool dots, const VT *X, VT *Y,
 	using AT = typename std::conditional<std::is_same<HT, float2>::value &&
 	                                         std::is_same<VT, float2>::value,
 	                                     float, double>::type;
+	if (h->ell_W > 0 && h->use_ell) {
+		const uint32_t *ec = h->ell_col.as<uint32_t>();
+		const HT *ev;
+		if (std::is_same<HT, float2>::value)
+			ev = first ? (const HT *)h->ell_v1f.ptr : (const HT *)h->ell_v2f.ptr;
+		else
+			ev = first ? (const HT *)h->ell_v1d.ptr : (const HT *)h->ell_v2d.ptr;
+#define TBTKX_ELL(F, D, R, WW)                                                 \
+	spmm_step_ell<VT, HT, AT, B, F, D, R, WW><<<r.nwg, kThreads, 0, h->stream>>>( \
+	    ec, ev, X, Y, h->N, r.nwg, h->xcd_remap, h->nt_stream, part, sites,    \
+	    live, mu, M, nprev)
+#define TBTKX_ELL_W(F, D, R)                                                   \
+	do {                                                                   \
+		if (h->ell_W == 4) TBTKX_ELL(F, D, R, 4);                      \
+		else if (h->ell_W == 6) TBTKX_ELL(F, D, R, 6);                 \
+		else TBTKX_ELL(F, D, R, 8);                                    \
+	} while (0)
+#define TBTKX_ELL_R(F, D)                                                      \
+	do {                                                                   \
+		if (h->h_real) TBTKX_ELL_W(F, D, true);                        \
+		else TBTKX_ELL_W(F, D, false);                                 \
+	} while (0)
+		if (first && dots) TBTKX_ELL_R(true, true);
+		else if (first) TBTKX_ELL_R(true, false);
+		else if (dots) TBTKX_ELL_R(false, true);
+		else TBTKX_ELL_R(false, false);
+#undef TBTKX_ELL_R
+#undef TBTKX_ELL_W
+#undef TBTKX_ELL
+		r.launches++;
+		return TBTKX_OK;
+	}
 #define TBTKX_STEP(F, D, R)                                                    \
 	spmm_step<VT, HT, AT, B, F, D, R><<<r.nwg, kThreads, 0, h->stream>>>(  \
 	    rp, cl, v, X, Y, h->N, r.tile, r.nwg, h->xcd_remap, h->nt_stream,   \
@@ -525,17 +706,37 @@ int run_block(BlockRun &r, int live, const int64_t *d_sites, int mode,
 // once for the steady-state kernel variant, a multiple of 8 (one share per
 // XCD), and no more than the row tiles need.
 template <typename VT, typename HT, int B>
+int persistent_grid(tbtkx_context *h, bool dots, int tile, int *nwg);
+
+template <typename VT, typename HT, typename AT, int B, bool D, bool R>
+const void *ell_fn(int W) {
+	if (W == 4) return (const void *)spmm_step_ell<VT, HT, AT, B, false, D, R, 4>;
+	if (W == 6) return (const void *)spmm_step_ell<VT, HT, AT, B, false, D, R, 6>;
+	return (const void *)spmm_step_ell<VT, HT, AT, B, false, D, R, 8>;
+}
+
+template <typename VT, typename HT, int B>
 int persistent_grid(tbtkx_context *h, bool dots, int tile, int *nwg) {
 	using AT = typename std::conditional<std::is_same<HT, float2>::value &&
 	                                         std::is_same<VT, float2>::value,
 	                                     float, double>::type;
 	const void *fn;
-	if (dots)
+	if (h->ell_W > 0 && h->use_ell) {
+		// The occupancy of the steady-state ELL variant.
+		tile = 64 / h->ell_W;
+		if (dots)
+			fn = h->h_real ? ell_fn<VT, HT, AT, B, true, true>(h->ell_W)
+			               : ell_fn<VT, HT, AT, B, true, false>(h->ell_W);
+		else
+			fn = h->h_real ? ell_fn<VT, HT, AT, B, false, true>(h->ell_W)
+			               : ell_fn<VT, HT, AT, B, false, false>(h->ell_W);
+	} else if (dots) {
 		fn = h->h_real ? (const void *)spmm_step<VT, HT, AT, B, false, true, true>
 		               : (const void *)spmm_step<VT, HT, AT, B, false, true, false>;
-	else
+	} else {
 		fn = h->h_real ? (const void *)spmm_step<VT, HT, AT, B, false, false, true>
 		               : (const void *)spmm_step<VT, HT, AT, B, false, false, false>;
+	}
 	int per_cu = 0, cus = 0;
 	TBTKX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0));
 	TBTKX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,

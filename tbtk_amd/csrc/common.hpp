@@ -76,6 +76,10 @@ struct tbtkx_context {
 	double scale = 0;
 	bool h_real = false;   // every value of H has a zero imaginary part
 	tbtkx::DevBuf rowptr, col, v1, v2, v1f, v2f;
+	// Tile-major ELL copy of H for the batched step (ell_W = slots per row,
+	// 0 when a row is longer than 8): tiles of 64/ell_W rows.
+	tbtkx::DevBuf ell_col, ell_v1f, ell_v2f, ell_v1d, ell_v2d;
+	int ell_W = 0;
 	// Row blocks for the LDS-staged plain kernel.
 	tbtkx::DevBuf rblk;
 	int nblk = 0;
@@ -102,6 +106,7 @@ struct tbtkx_context {
 	int xcd_remap = 0;     // measured faster without for both step kernels
 	int tile_rows = 0;     // 0 = default
 	int nt_stream = 1;     // non-temporal hints on once-used SpMM rows
+	int use_ell = 1;       // batched step on the ELL copy when it exists
 
 	tbtkx::GraphCache plain_graph;
 };

@@ -146,7 +146,8 @@ int tbtkx_destroy(tbtkx_handle h) {
 	                  &h->rblk, &h->vec_a, &h->vec_b, &h->coef, &h->to,
 	                  &h->lut, &h->gf_in, &h->gf_out, &h->blk_x, &h->blk_y,
 	                  &h->blk_mu, &h->blk_sites, &h->blk_part, &h->cplanes,
-	                  &h->lplanes, &h->gf_mfma_out})
+	                  &h->lplanes, &h->gf_mfma_out, &h->ell_col, &h->ell_v1f,
+	                  &h->ell_v2f, &h->ell_v1d, &h->ell_v2d})
 		b->release();
 	if (h->ev_start) (void)hipEventDestroy(h->ev_start);
 	if (h->ev_stop) (void)hipEventDestroy(h->ev_stop);
@@ -246,6 +247,46 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 	TBTKX_CHECK_HIP(hipMemcpy(h->v2f.ptr, v2f.data(), Zp * 8, hipMemcpyHostToDevice));
 	TBTKX_CHECK_HIP(hipMemcpy(h->rblk.ptr, blocks.data(), blocks.size() * 4,
 	                          hipMemcpyHostToDevice));
+	// Tile-major ELL copy (see spmm_step_ell): W slots per row, tiles of
+	// 64/W rows; padding slots point at the row itself with value zero.
+	uint32_t maxlen = 0;
+	for (int64_t r = 0; r < N; r++)
+		if (rowptr[r + 1] - rowptr[r] > maxlen) maxlen = rowptr[r + 1] - rowptr[r];
+	h->ell_W = maxlen <= 4 ? 4 : maxlen <= 6 ? 6 : maxlen <= 8 ? 8 : 0;
+	if (h->ell_W) {
+		const int W = h->ell_W, T = 64 / W;
+		const int64_t ntiles = (N + T - 1) / T, E = ntiles * T * W;
+		std::vector<uint32_t> ec(E);
+		std::vector<float> e1f(2 * E, 0.0f), e2f(2 * E, 0.0f);
+		std::vector<double> e1d(2 * E, 0.0), e2d(2 * E, 0.0);
+		for (int64_t t = 0; t < ntiles; t++)
+			for (int i = 0; i < T; i++) {
+				const int64_t r = t * T + i;
+				for (int u = 0; u < W; u++) {
+					const int64_t e = (t * T + i) * W + u;
+					ec[e] = (uint32_t)(r < N ? r : 0);
+					if (r >= N || (int64_t)rowptr[r] + u >= (int64_t)rowptr[r + 1]) continue;
+					const int64_t k = rowptr[r] + u;
+					ec[e] = col[k];
+					for (int c = 0; c < 2; c++) {
+						e1d[2 * e + c] = v1[2 * k + c];
+						e2d[2 * e + c] = v2[2 * k + c];
+						e1f[2 * e + c] = v1f[2 * k + c];
+						e2f[2 * e + c] = v2f[2 * k + c];
+					}
+				}
+			}
+		if ((rc = h->ell_col.reserve(E * 4))) return rc;
+		if ((rc = h->ell_v1f.reserve(E * 8))) return rc;
+		if ((rc = h->ell_v2f.reserve(E * 8))) return rc;
+		if ((rc = h->ell_v1d.reserve(E * 16))) return rc;
+		if ((rc = h->ell_v2d.reserve(E * 16))) return rc;
+		TBTKX_CHECK_HIP(hipMemcpy(h->ell_col.ptr, ec.data(), E * 4, hipMemcpyHostToDevice));
+		TBTKX_CHECK_HIP(hipMemcpy(h->ell_v1f.ptr, e1f.data(), E * 8, hipMemcpyHostToDevice));
+		TBTKX_CHECK_HIP(hipMemcpy(h->ell_v2f.ptr, e2f.data(), E * 8, hipMemcpyHostToDevice));
+		TBTKX_CHECK_HIP(hipMemcpy(h->ell_v1d.ptr, e1d.data(), E * 16, hipMemcpyHostToDevice));
+		TBTKX_CHECK_HIP(hipMemcpy(h->ell_v2d.ptr, e2d.data(), E * 16, hipMemcpyHostToDevice));
+	}
 	h->N = N;
 	h->Z = Z;
 	h->scale = scale;
@@ -295,6 +336,9 @@ int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
 		return TBTKX_OK;
 	case TBTKX_OPT_NT_STREAM:
 		h->nt_stream = value != 0;
+		return TBTKX_OK;
+	case TBTKX_OPT_ELL:
+		h->use_ell = value != 0;
 		return TBTKX_OK;
 	default:
 		set_error("tbtkx_set_option: unknown option " + std::to_string(option));
