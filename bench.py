@@ -19,8 +19,15 @@ coefficients, FP32 storage with FP64 accumulation and the doubling identities,
 against the plain FP64 path on the same model; error = max over compared sites
 of max_E |LDOS - LDOS_FP64| / max_E |LDOS_FP64|.
 
+Further objects on the same line: "uneven_batch" (64 models of uneven size
+in one persistent launch per step), "stochastic_dos" (128^3 cubic, 512
+random-phase vectors, M=4096) and "contraction_mfma" (1e5 x 4096 x 8192
+split-BF16 Green's-function GEMM), each with its own roofline fraction and
+error against the FP64 path.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
-                       [--skip-cpu-baseline] [--skip-batched]
+                       [--parts plain,batched,uneven,dos,mfma,cpu]
+                       [--skip-cpu-baseline]
 """
 from __future__ import annotations
 
@@ -43,6 +50,7 @@ from tbtk_amd import lattice as L  # noqa: E402
 
 METRIC = "LDOS sites/s + SpMV GB/s, batched FP32 vs plain FP64 at fixed error"
 HBM_PEAK_GBPS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 
 # Default workload
 SCALE = 12.0
@@ -57,6 +65,8 @@ M_BATCH = 2000
 B_SITES = 256
 BLOCK_WIDTH = 128     # start vectors per SpMM block (two blocks for 256 sites)
 PLAIN_CMP_SITES = 4
+
+PARTS = ("plain", "batched", "uneven", "dos", "mfma", "cpu")
 
 
 def log(msg):
@@ -192,15 +202,176 @@ def batched_comparison(eng):
     }
 
 
+def uneven_batch(eng):
+    """64 independent models of uneven size (honeycomb ribbons and square
+    lattices, 20k-1M states, 1-64 sites and 500-8000 moments each, seed 4) in
+    one persistent launch per recurrence step: FP32 storage, FP64 moments,
+    doubling, autotuned launch shape, replayed as one hipGraph."""
+    t0 = time.perf_counter()
+    batch = L.config_uneven(seed=4)
+    csrs = [m.csr() for m, _, _ in batch]
+    scales = [kpm.gershgorin_scale(c, 1.05) for c in csrs]
+    t_gen = time.perf_counter() - t0
+    eng.multi_set(csrs, scales)
+    sites = [b[1] for b in batch]
+    Ms = [b[2] for b in batch]
+    N = np.array([m.basis_size for m, _, _ in batch], dtype=np.int64)
+    B = np.array([len(x) for x in sites], dtype=np.int64)
+    M = np.array(Ms, dtype=np.int64)
+    S = (M + 1) // 2
+    W = 6 if max(np.diff(c[0]).max() for c in csrs) > 4 else 4
+    step_bytes_total = float(((12 * W * N + 24 * B * N) * S).sum())
+    # 1: autotune + direct launches; 2: hipGraph capture; 3: graph replay (timed)
+    t0 = time.perf_counter()
+    mu = eng.multi_moments(sites, Ms, X.DOUBLING)
+    t_first = time.perf_counter() - t0
+    ms_direct, launches = eng.last_timing()
+    eng.set_option(X.OPT_GRAPH, 1)
+    try:
+        eng.multi_moments(sites, Ms, X.DOUBLING)
+        t0 = time.perf_counter()
+        mu_g = eng.multi_moments(sites, Ms, X.DOUBLING)
+        t_graph = time.perf_counter() - t0
+        ms_graph, _ = eng.last_timing()
+    finally:
+        eng.set_option(X.OPT_GRAPH, 0)
+    graph_diff = max(float(np.abs(a - b).max()) for a, b in zip(mu, mu_g))
+    # error: the two smallest models, first site, against the plain FP64 path
+    errs = []
+    for i in np.argsort(N)[:2]:
+        m, st, Mi = batch[i]
+        eng.set_csr(csrs[i], scales[i])
+        ref = eng.moments(int(st[0]), [int(st[0])], Mi)[0]
+        g = kpm.jackson(Mi)
+        lo, hi = -0.99 * scales[i], 0.99 * scales[i]
+        eng.lut_generate(Mi, E_PTS, lo, hi, scales[i])
+        l_ref = kpm.ldos_from_gf(eng.contract(ref * g, X.NONPRINCIPAL))
+        l_got = kpm.ldos_from_gf(eng.contract(mu[i][0] * g, X.NONPRINCIPAL))
+        errs.append(float(np.abs(l_got - l_ref).max() / np.abs(l_ref).max()))
+    eng.lut_destroy()
+    dev_s = ms_graph * 1e-3
+    return {
+        "workload": ("64 models, seed 4: honeycomb ribbons (t=2.7) and square lattices (t=1), "
+                     "width 20-400, length 50-2000 (<= 1e6 states), disorder U[-0.5,0.5], "
+                     "1-64 LDOS sites and 500-8000 moments per model, Gershgorin*1.05 scales"),
+        "models": len(batch), "states": int(N.sum()), "sites": int(B.sum()),
+        "nnz": int(sum(c[1].size for c in csrs)),
+        "ell_slots": W,
+        "recurrence_steps": int(S.max()),
+        "dtype": "fp32 storage (H and vectors), fp64 moments, doubling",
+        "sites_per_s": float(B.sum()) / (t_graph),
+        "device_s": dev_s,
+        "direct_device_s": ms_direct * 1e-3,
+        "first_call_s_incl_autotune": t_first,
+        "graph_replay_wall_s": t_graph,
+        "step_launches": int(launches),
+        "algorithmic_bytes": step_bytes_total,
+        "GBps": step_bytes_total / dev_s / 1e9,
+        "hbm_frac": step_bytes_total / dev_s / 1e9 / HBM_PEAK_GBPS,
+        "graph_vs_direct_max_abs_diff": graph_diff,
+        "max_ldos_rel_err_vs_fp64": max(errs),
+        "error_sites_compared": len(errs),
+        "model_build_s": t_gen,
+    }
+
+
+def stochastic_dos(eng):
+    """128^3 cubic lattice (t=1, disorder U[-1,1], seed 5), R=512 random-phase
+    start vectors, M=4096, FP32 storage, FP64 moment sums, doubling."""
+    m = L.config_cubic(128)
+    csr = m.csr()
+    N, Z = m.basis_size, csr[1].size
+    s = kpm.gershgorin_scale(csr, 1.05)
+    eng.set_csr(csr, s)
+    R, M, seed = 512, 4096, 17
+    eng.set_option(X.OPT_BLOCK_WIDTH, 256)
+    try:
+        eng.moments_trace(8, seed, 8)  # allocations
+        t0 = time.perf_counter()
+        mu = eng.moments_trace(R, seed, M)
+        wall = time.perf_counter() - t0
+        ms, launches = eng.last_timing()
+    finally:
+        eng.set_option(X.OPT_BLOCK_WIDTH, 0)
+    # error: 8 vectors in FP32 storage vs the same 8 in FP64 (H and vectors)
+    Rc = 8
+    a = eng.moments_trace(Rc, seed, M)
+    b = eng.moments_trace(Rc, seed, M, X.VEC_FP64 | X.H_FP64)
+    g = kpm.jackson(M)
+    eng.lut_generate(M, E_PTS, -0.99 * s, 0.99 * s, s)
+    dos_a = kpm.ldos_from_gf(eng.contract(a * g, X.NONPRINCIPAL))
+    dos_b = kpm.ldos_from_gf(eng.contract(b * g, X.NONPRINCIPAL))
+    eng.lut_destroy()
+    Bw = 256
+    blocks = R // Bw
+    step_bytes = 12 * Z + 4 * N + 24 * Bw * N
+    per_step = ms * 1e-3 / launches
+    return {
+        "workload": (f"cubic 128^3 (2,097,152 states, t=1, disorder U[-1,1] seed 5), R={R} "
+                     f"random-phase vectors (seed {seed}), M={M}, s=Gershgorin*1.05={s:.6f}"),
+        "nnz": int(Z),
+        "dtype": "fp32 storage (H and vectors), fp64 moment sums, doubling",
+        "seconds": ms * 1e-3,
+        "wall_s": wall,
+        "block_width": Bw, "blocks": blocks, "step_launches": int(launches),
+        "spmm_step_ms": per_step * 1e3,
+        "spmm_GBps": step_bytes / per_step / 1e9,
+        "spmm_hbm_frac": step_bytes / per_step / 1e9 / HBM_PEAK_GBPS,
+        "mu0_over_RN": float(mu[0].real / (R * N)),
+        "max_dos_rel_err_fp32_vs_fp64_same_8_vectors": float(
+            np.abs(dos_a - dos_b).max() / np.abs(dos_b).max()),
+    }
+
+
+def contraction_mfma(eng):
+    """G(E) for K=1e5 coefficient vectors x M=4096 moments x E=8192 energies
+    (Retarded) as a split-BF16 complex GEMM on MFMA, FP32 accumulate."""
+    K, M, E = 100_000, 4096, 8192
+    s = 10.0
+    rng = np.random.default_rng(21)
+    # Moments of single levels at random energies (Jackson-damped), 2048
+    # distinct rows repeated to K.
+    x = rng.uniform(-0.9, 0.9, 2048)
+    n = np.arange(M)
+    base = np.cos(np.outer(np.arccos(x), n)) * kpm.jackson(M)[None, :]
+    base = base * np.exp(1j * rng.uniform(0, 2 * np.pi, (2048, 1)))
+    coeffs = np.tile(base, (K // 2048 + 1, 1))[:K]
+    eng.lut_generate(M, E, -0.99 * s, 0.99 * s, s)
+    eng.coeffs_upload(coeffs)
+    eng.contract_mfma(X.RETARDED, keep_on_device=True)  # table planes, warm-up
+    times = []
+    for _ in range(3):
+        eng.contract_mfma(X.RETARDED, keep_on_device=True)
+        times.append(eng.last_timing()[0])
+    ms = min(times)
+    # error on 64 rows against the FP64 contraction
+    rows = coeffs[:64]
+    eng.coeffs_upload(rows)
+    got = eng.contract_mfma(X.RETARDED)
+    ref = eng.contract(rows, X.RETARDED)
+    eng.lut_destroy()
+    executed = 2.0 * K * E * M * 12
+    algorithmic = 8.0 * K * E * M
+    return {
+        "workload": f"K={K} coefficient vectors x M={M} x E={E}, Retarded, s={s}",
+        "dtype": "split-bf16 (3 terms) x complex, fp32 accumulate",
+        "ms": ms,
+        "executed_TFLOPs": executed / (ms * 1e-3) / 1e12,
+        "algorithmic_TFLOPs": algorithmic / (ms * 1e-3) / 1e12,
+        "mfma_frac_of_bf16_dense_peak": executed / (ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
+        "max_rel_err_vs_fp64": float(np.abs(got - ref).max() / np.abs(ref).max()),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--parts", default=",".join(PARTS),
+                    help="comma list of " + ",".join(PARTS) + " (default: all); without "
+                         "'plain' the line carries no headline value (profiling runs)")
     ap.add_argument("--skip-cpu-baseline", action="store_true")
-    ap.add_argument("--skip-batched", action="store_true")
-    ap.add_argument("--skip-plain", action="store_true",
-                    help="run only the batched comparison (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 1 GiB read + 1 GiB write elementwise kernel (torch) "
                          "whose known bytes scale the rocprofv3 memory counters")
@@ -210,6 +381,12 @@ def main():
               "single MI355X (independent sites/vectors shard across GPUs with no "
               "communication; see DESIGN.md). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    parts = [x for x in args.parts.split(",") if x]
+    for x in parts:
+        if x not in PARTS:
+            ap.error(f"unknown part {x!r}")
+    if args.skip_cpu_baseline and "cpu" in parts:
+        parts.remove("cpu")
 
     if args.calibrate_copy:
         import torch
@@ -218,86 +395,92 @@ def main():
         torch.add(a, 1.0, out=b)
         torch.cuda.synchronize()
         del a, b
-    if args.skip_plain:
-        # Profiling aid: the batched comparison alone (not a bench line).
-        with X.Engine(0) as eng:
-            print(json.dumps({"metric": METRIC, "batched": batched_comparison(eng)}))
-        return
-    log("building 1024x1024 spin lattice")
-    model = L.config_plain_large(seed=2)
-    csr = model.csr()
-    N, Z = model.basis_size, csr[1].size
+
+    out = {"metric": METRIC}
     eng = X.Engine(0)
-    eng.set_csr(csr, SCALE)
-    kpm.check_window(*WINDOW, SCALE)
-    eng.lut_generate(M_PLAIN, E_PTS, WINDOW[0], WINDOW[1], SCALE)
-
-    for _ in range(args.warmup):
-        plain_ldos(eng, model, CENTRE, M_PLAIN)
-    log(f"timing {args.steps} steps")
-    kernel_ms = 0.0
-    launches = 0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ldos, kms, n = plain_ldos(eng, model, CENTRE, M_PLAIN)
-        kernel_ms += kms
-        launches += n
-    elapsed = time.perf_counter() - t0
-    sites_per_step = 2
-    value = args.steps * sites_per_step / elapsed
-    step_bytes = 20 * Z + 52 * N
-    avg_kernel_s = kernel_ms * 1e-3 / launches
-    achieved = step_bytes / avg_kernel_s / 1e9
-    dE = (WINDOW[1] - WINDOW[0]) / (E_PTS - 1)
-    checks = {"ldos_integral": float(ldos.sum() * dE),
-              "ldos_integral_expected": 2.0,
-              "ldos_min": float(ldos.min())}
-
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "LDOS sites/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": ("square lattice 1024x1024 with spin, t=1, J=0.5, disorder U[-1,1] "
-                         "seed 2; plain FP64 single start vector; LDOS at (512,512,sum over "
-                         "spin) = 2 sites per step; s=12, M=4000, E=2000 on [-11.88,11.88], "
-                         "Lorentz 1e-6"),
-            "states": int(N), "nnz": int(Z), "moments": M_PLAIN, "energies": E_PTS,
-            "sites_per_step": sites_per_step,
-        },
-        "physical_sites_per_s": value / 2,
-        "spmv_GBps": achieved,
-        "roofline": {
-            "kernel": "plain_step_lds (FP64 CSR SpMV recurrence step)",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": load_traffic("plain_step_lds"),
-            "algorithmic_bytes_per_launch": step_bytes,
-            "avg_launch_us": avg_kernel_s * 1e6,
-        },
-        "checks": checks,
-    }
-    if not args.skip_batched:
+    model = None
+    if "plain" in parts or "cpu" in parts:
+        log("building 1024x1024 spin lattice")
+        model = L.config_plain_large(seed=2)
+    if "plain" in parts:
+        csr = model.csr()
+        N, Z = model.basis_size, csr[1].size
+        eng.set_csr(csr, SCALE)
+        kpm.check_window(*WINDOW, SCALE)
+        eng.lut_generate(M_PLAIN, E_PTS, WINDOW[0], WINDOW[1], SCALE)
+        for _ in range(args.warmup):
+            plain_ldos(eng, model, CENTRE, M_PLAIN)
+        log(f"timing {args.steps} steps")
+        kernel_ms = 0.0
+        launches = 0
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ldos, kms, n = plain_ldos(eng, model, CENTRE, M_PLAIN)
+            kernel_ms += kms
+            launches += n
+        elapsed = time.perf_counter() - t0
+        eng.lut_destroy()
+        sites_per_step = 2
+        value = args.steps * sites_per_step / elapsed
+        step_bytes = 20 * Z + 52 * N
+        avg_kernel_s = kernel_ms * 1e-3 / launches
+        achieved = step_bytes / avg_kernel_s / 1e9
+        dE = (WINDOW[1] - WINDOW[0]) / (E_PTS - 1)
+        out.update({
+            "value": value,
+            "unit": "LDOS sites/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": elapsed / args.steps * 1e3,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f64",
+            "data": "synthetic",
+            "config": {
+                "workload": ("square lattice 1024x1024 with spin, t=1, J=0.5, disorder "
+                             "U[-1,1] seed 2; plain FP64 single start vector; LDOS at "
+                             "(512,512,sum over spin) = 2 sites per step; s=12, M=4000, "
+                             "E=2000 on [-11.88,11.88], Lorentz 1e-6"),
+                "states": int(N), "nnz": int(Z), "moments": M_PLAIN, "energies": E_PTS,
+                "sites_per_step": sites_per_step,
+            },
+            "physical_sites_per_s": value / 2,
+            "spmv_GBps": achieved,
+            "roofline": {
+                "kernel": "plain_step_lds (FP64 CSR SpMV recurrence step)",
+                "bound": "hbm",
+                "achieved": achieved,
+                "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBPS,
+                "traffic": load_traffic("plain_step_lds"),
+                "algorithmic_bytes_per_launch": step_bytes,
+                "avg_launch_us": avg_kernel_s * 1e6,
+            },
+            "checks": {"ldos_integral": float(ldos.sum() * dE),
+                       "ldos_integral_expected": 2.0,
+                       "ldos_min": float(ldos.min())},
+        })
+    if "batched" in parts:
         log("batched comparison (512x512 Nambu, 256 sites)")
         out["batched"] = batched_comparison(eng)
+    if "uneven" in parts:
+        log("uneven batch (64 models, one persistent launch per step)")
+        out["uneven_batch"] = uneven_batch(eng)
+    if "dos" in parts:
+        log("stochastic DOS (128^3 cubic, R=512, M=4096)")
+        out["stochastic_dos"] = stochastic_dos(eng)
+    if "mfma" in parts:
+        log("MFMA contraction (1e5 x 4096 x 8192)")
+        out["contraction_mfma"] = contraction_mfma(eng)
     eng.close()
-    if not args.skip_cpu_baseline:
+    if "cpu" in parts:
         log("CPU baseline (reference CPU path, bounded sample)")
         out["cpu_baseline"] = cpu_baseline(model, "512,512,+", SCALE, WINDOW, E_PTS,
-                                           M_PLAIN, sites_per_step)
-    else:
+                                           M_PLAIN, 2)
+    elif "plain" in parts:
         out["cpu_baseline"] = None
     print(json.dumps(out))
 

@@ -155,6 +155,30 @@ int tbtkx_moments_block(tbtkx_handle handle, int num_sites,
 int tbtkx_moments_trace(tbtkx_handle handle, int num_vectors, uint64_t seed,
                         int num_moments, int flags, double *out);
 
+/* Uneven batch: upload K independent models at once. Model m has
+ * num_rows[m] states and nnz[m] non-zeros; its CSR (same conventions as
+ * tbtkx_set_csr, indices local to the model) follows the previous model's in
+ * rowptr (num_rows[m] + 1 entries per model), col and val, and scales[m] is
+ * its scale factor. Rows may hold at most 8 non-zeros. No reference
+ * counterpart: the reference handles one model per solver and one start
+ * vector per recurrence (Lib/src/Solver/ChebyshevExpander.cpp:103-199). */
+int tbtkx_multi_set(tbtkx_handle handle, int num_models, const int64_t *num_rows,
+                    const int64_t *nnz, const uint32_t *rowptr, const uint32_t *col,
+                    const double *val, const double *scales);
+
+/* Diagonal moments of every uploaded model in one persistent launch per
+ * recurrence step: model m gets num_sites[m] (1..64) unit start vectors at
+ * its local basis indices (consecutive in sites) and num_moments[m] moments.
+ * out holds, model after model, num_sites[m] x num_moments[m] complex values
+ * (site-major), as tbtkx_moments_block would give per model. FP32 storage,
+ * FP64 moments; flags may be TBTKX_DOUBLING. The launch shape (tile size,
+ * grid share) is autotuned on first use unless TBTKX_OPT_AUTOTUNE is 0 or
+ * TBTKX_OPT_TILE_ROWS fixes it; TBTKX_OPT_GRAPH replays the whole run as a
+ * hipGraph. */
+int tbtkx_multi_moments(tbtkx_handle handle, const int *num_sites,
+                        const int64_t *sites, const int *num_moments, int flags,
+                        double *out);
+
 /* Device time of the last recurrence call: total milliseconds between the
  * first and last step launch (HIP events on the handle's stream) and the
  * number of step launches in it. */
@@ -164,8 +188,9 @@ int tbtkx_last_timing(tbtkx_handle handle, double *milliseconds,
 /* Engine options (0 = default). TBTKX_OPT_PLAIN_KERNEL: 0 = LDS-staged
  * row blocks, 1 = one thread per row. TBTKX_OPT_GRAPH: 1 = replay the step
  * loop as a hipGraph. TBTKX_OPT_BLOCK_WIDTH: start vectors per SpMM block
- * (64, 128, 256). TBTKX_OPT_XCD_REMAP: 1 = give each XCD a contiguous row
- * slice. TBTKX_OPT_TILE_ROWS: rows per wavefront tile of the SpMM (1-32).
+ * (64, 128, 256; 0 = chosen by the width autotuner, which times each width on
+ * the resident model once per model, precision and size class).
+ * TBTKX_OPT_XCD_REMAP: 1 = give each XCD a contiguous row slice. TBTKX_OPT_TILE_ROWS: rows per wavefront tile of the SpMM (1-32).
  * TBTKX_OPT_NT_STREAM: 1 = non-temporal loads/stores for the SpMM rows that
  * are used once per step (default 1). */
 enum {
@@ -175,10 +200,17 @@ enum {
 	TBTKX_OPT_XCD_REMAP = 4,
 	TBTKX_OPT_TILE_ROWS = 5,
 	TBTKX_OPT_NT_STREAM = 6,
-	TBTKX_OPT_ELL = 7       /* 1 = batched step on the tile-major ELL copy of H
+	TBTKX_OPT_ELL = 7,      /* 1 = batched step on the tile-major ELL copy of H
 	                           when no row has more than 8 entries (default) */
+	TBTKX_OPT_AUTOTUNE = 8  /* 1 = time candidate launch shapes on first use
+	                           (default); 0 = fixed defaults */
 };
 int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
+
+/* Effective value of an option: TBTKX_OPT_BLOCK_WIDTH gives the width the
+ * last batched call used (the autotuner's choice when it is 0); TBTKX_OPT_ELL
+ * gives 1 only when the ELL copy exists and is enabled. */
+int tbtkx_get_option(tbtkx_handle handle, int option, int64_t *value);
 
 #ifdef __cplusplus
 }

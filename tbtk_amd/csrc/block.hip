@@ -765,63 +765,157 @@ int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
 	if (rc) return rc;
 	const bool dots = (flags & TBTKX_DOUBLING) != 0;
 	const int S = dots ? (M + 1) / 2 : M - 1;
+	const int nblocks = (total + B - 1) / B;
+	// Every block writes its own moment region, so the blocks run back to
+	// back with one copy to the host at the end.
+	const size_t per_block = dots ? (size_t)(S > 0 ? S : 1) * 2 * B : (size_t)B * M;
 	if ((rc = h->blk_x.reserve((size_t)h->N * B * sizeof(VT)))) return rc;
 	if ((rc = h->blk_y.reserve((size_t)h->N * B * sizeof(VT)))) return rc;
 	if (dots &&
 	    (rc = h->blk_part.reserve((size_t)kDotChunk * r.nwg * 2 * B * 16)))
 		return rc;
-	const size_t mu_elems = dots ? (size_t)(S > 0 ? S : 1) * 2 * B : (size_t)B * M;
-	if ((rc = h->blk_mu.reserve(mu_elems * 16))) return rc;
-	if ((rc = h->blk_sites.reserve((size_t)B * 8))) return rc;
+	if ((rc = h->blk_mu.reserve(per_block * nblocks * 16))) return rc;
+	if ((rc = h->blk_sites.reserve((size_t)(mode == 0 ? total : 1) * 8))) return rc;
+	if (mode == 0)
+		TBTKX_CHECK_HIP(hipMemcpyAsync(h->blk_sites.ptr, sites, (size_t)total * 8,
+		                               hipMemcpyHostToDevice, h->stream));
 
-	const int nblocks = (total + B - 1) / B;
+	auto enqueue = [&]() -> int {
+		for (int blk = 0; blk < nblocks; blk++) {
+			const int live = total - blk * B < B ? total - blk * B : B;
+			double2 *mu = h->blk_mu.as<double2>() + per_block * blk;
+			int erc = run_block<VT, HT, B>(r, live, h->blk_sites.as<int64_t>() + (int64_t)blk * B,
+			                               mode, seed, (int64_t)blk * B, M, mu, mu);
+			if (erc) return erc;
+		}
+		return TBTKX_OK;
+	};
+	TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
+	if (h->use_graph) {
+		GraphCache &gc = h->blk_graph;
+		int64_t key[GraphCache::kKey];
+		GraphCache::make_key(
+		    key,
+		    {total, M, flags, mode, (int64_t)seed, B, (int64_t)r.nwg * 64 + r.tile,
+		     h->use_ell * 4 + h->nt_stream * 2 + h->xcd_remap, h->N, h->ell_W},
+		    {h->blk_x.ptr, h->blk_y.ptr, h->blk_part.ptr, h->blk_mu.ptr, h->blk_sites.ptr,
+		     h->rowptr.ptr, h->col.ptr, h->v1.ptr, h->v2.ptr, h->v1f.ptr, h->v2f.ptr,
+		     h->ell_col.ptr, h->ell_v1f.ptr, h->ell_v2f.ptr});
+		if (!gc.matches(key)) {
+			gc.reset();
+			TBTKX_CHECK_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+			int erc = enqueue();
+			hipError_t ec = hipStreamEndCapture(h->stream, &gc.graph);
+			if (erc) return erc;
+			TBTKX_CHECK_HIP(ec);
+			TBTKX_CHECK_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
+			memcpy(gc.key, key, sizeof key);
+		}
+		TBTKX_CHECK_HIP(hipGraphLaunch(gc.exec, h->stream));
+	} else {
+		r.launches = 0;
+		if ((rc = enqueue())) return rc;
+	}
+	TBTKX_CHECK_HIP(hipEventRecord(h->ev_stop, h->stream));
+	std::vector<double2> buf(per_block * nblocks);
+	TBTKX_CHECK_HIP(hipMemcpyAsync(buf.data(), h->blk_mu.ptr, buf.size() * 16,
+	                               hipMemcpyDeviceToHost, h->stream));
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	float ms = 0;
+	TBTKX_CHECK_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
+	h->last_ms = ms;
+	h->last_launches = (int64_t)nblocks * S;
+	h->last_block_width = B;
+
 	host_mu.assign((size_t)total * M, make_double2(0, 0));
-	std::vector<double2> buf(mu_elems);
-	bool timing = false;
 	for (int blk = 0; blk < nblocks; blk++) {
 		const int live = total - blk * B < B ? total - blk * B : B;
-		if (mode == 0)
-			TBTKX_CHECK_HIP(hipMemcpyAsync(h->blk_sites.ptr, sites + (int64_t)blk * B,
-			                               (size_t)live * 8, hipMemcpyHostToDevice,
-			                               h->stream));
-		if (!timing) {
-			TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
-			timing = true;
-		}
-		rc = run_block<VT, HT, B>(r, live, h->blk_sites.as<int64_t>(), mode, seed,
-		                          (int64_t)blk * B, M, h->blk_mu.as<double2>(),
-		                          h->blk_mu.as<double2>());
-		if (rc) return rc;
-		TBTKX_CHECK_HIP(hipMemcpyAsync(buf.data(), h->blk_mu.ptr, mu_elems * 16,
-		                               hipMemcpyDeviceToHost, h->stream));
-		TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+		const double2 *bb = buf.data() + per_block * blk;
 		for (int b = 0; b < live; b++) {
 			double2 *o = host_mu.data() + (size_t)(blk * B + b) * M;
 			if (!dots) {
-				memcpy(o, buf.data() + (size_t)b * M, (size_t)M * 16);
+				memcpy(o, bb + (size_t)b * M, (size_t)M * 16);
 				continue;
 			}
 			// D[n][0][b] = <Phi_n|Phi_n>, D[n][1][b] = <Phi_n+1|Phi_n>
-			const double2 mu0 = buf[(size_t)0 * 2 * B + b];
-			const double2 mu1 = M > 1 ? buf[(size_t)0 * 2 * B + B + b]
-			                          : make_double2(0, 0);
+			const double2 mu0 = bb[b];
+			const double2 mu1 = M > 1 ? bb[B + b] : make_double2(0, 0);
 			o[0] = mu0;
 			if (M > 1) o[1] = mu1;
 			for (int n = 1; n < S; n++) {
-				const double2 a = buf[(size_t)n * 2 * B + b];
-				const double2 c = buf[(size_t)n * 2 * B + B + b];
+				const double2 a = bb[(size_t)n * 2 * B + b];
+				const double2 c = bb[(size_t)n * 2 * B + B + b];
 				if (2 * n < M) o[2 * n] = make_double2(2 * a.x - mu0.x, 2 * a.y - mu0.y);
 				if (2 * n + 1 < M)
 					o[2 * n + 1] = make_double2(2 * c.x - mu1.x, 2 * c.y - mu1.y);
 			}
 		}
 	}
+	return TBTKX_OK;
+}
+
+// Time a few steps of one block of width B on the resident model (random
+// start vectors, the same kernel variant a real run uses) for the width
+// autotuner; returns milliseconds per step.
+template <typename VT, typename HT, int B>
+int trial_width(tbtkx_context *h, int flags, float *ms_per_step) {
+	constexpr int kRpi = B >= 256 ? 1 : 256 / B;
+	int tile = h->tile_rows > 0 ? h->tile_rows : 4;
+	tile = (tile + kRpi - 1) / kRpi * kRpi;
+	if (tile > 32) tile = 32;
+	BlockRun r{h, B, flags, 0, tile, 0};
+	const bool dots = (flags & TBTKX_DOUBLING) != 0;
+	int rc = persistent_grid<VT, HT, B>(h, dots, r.tile, &r.nwg);
+	if (rc) return rc;
+	if ((rc = h->blk_x.reserve((size_t)h->N * B * sizeof(VT)))) return rc;
+	if ((rc = h->blk_y.reserve((size_t)h->N * B * sizeof(VT)))) return rc;
+	if (dots && (rc = h->blk_part.reserve((size_t)kDotChunk * r.nwg * 2 * B * 16)))
+		return rc;
+	VT *X = h->blk_x.as<VT>(), *Y = h->blk_y.as<VT>();
+	double2 *part = h->blk_part.as<double2>();
+	constexpr int kTrial = 3;
+	block_init<VT><<<grid_for(h->N * B), kThreads, 0, h->stream>>>(X, h->N, B, B, 1,
+	                                                              nullptr, 1, 0);
+	// live = 0: no moment gathers (there are no sites), full vector traffic.
+	if ((rc = launch_step<VT, HT, B>(r, true, dots, X, Y, part, nullptr, 0, nullptr, 1, 0)))
+		return rc;
+	TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
+	for (int i = 0; i < kTrial; i++) {
+		if ((rc = launch_step<VT, HT, B>(r, false, dots, Y, X, part + (int64_t)(i + 1) * r.nwg * 2 * B,
+		                                 nullptr, 0, nullptr, 1, 0)))
+			return rc;
+		VT *t = X;
+		X = Y;
+		Y = t;
+	}
 	TBTKX_CHECK_HIP(hipEventRecord(h->ev_stop, h->stream));
 	TBTKX_CHECK_HIP(hipEventSynchronize(h->ev_stop));
+	TBTKX_CHECK_HIP(hipGetLastError());
 	float ms = 0;
 	TBTKX_CHECK_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
-	h->last_ms = ms;
-	h->last_launches = r.launches;
+	*ms_per_step = ms / kTrial;
+	return TBTKX_OK;
+}
+
+template <typename VT, typename HT>
+int tune_width(tbtkx_context *h, int total, int flags, int *best_B) {
+	static const int kWidths[] = {64, 128, 256};
+	double best = 0;
+	*best_B = 64;
+	const int cap = (total + 63) / 64 * 64;
+	for (int B : kWidths) {
+		if (B > cap && B != 64) continue;
+		float ms = 0;
+		int rc = B == 64    ? trial_width<VT, HT, 64>(h, flags, &ms)
+		         : B == 128 ? trial_width<VT, HT, 128>(h, flags, &ms)
+		                    : trial_width<VT, HT, 256>(h, flags, &ms);
+		if (rc) return rc;
+		const double cost = (double)ms * ((total + B - 1) / B);
+		if (best == 0 || cost < best) {
+			best = cost;
+			*best_B = B;
+		}
+	}
 	return TBTKX_OK;
 }
 
@@ -838,20 +932,38 @@ int dispatch_width(tbtkx_context *h, int B, int total, const int64_t *sites,
 
 int block_entry(tbtkx_context *h, int total, const int64_t *sites, int mode,
                 uint64_t seed, int M, int flags, std::vector<double2> &mu) {
-	int B = h->block_width;
-	if (B == 0 || B == 32) {
-		// Measured on the 512x512 Nambu model: 128 columns per block gives
-		// the most site-steps per second (7.5 us per site-step vs 8.3-8.9
-		// for 64 and 256).
-		B = total <= 64 ? 64 : 128;
-	}
 	int rc = set_device(h);
 	if (rc) return rc;
-	if (flags & TBTKX_VEC_FP64) {
-		if (!(flags & TBTKX_H_FP64)) flags |= TBTKX_H_FP64;
+	if (flags & TBTKX_VEC_FP64) flags |= TBTKX_H_FP64;
+	int B = h->block_width;
+	if (B == 0 || B == 32) {
+		if (!h->autotune) {
+			B = total <= 64 ? 64 : 128;
+		} else {
+			// Width autotuner: time each candidate width on the resident
+			// model once per (model, precision, mode, size class) and keep
+			// the one with the least time for all blocks of this call.
+			const int64_t key[5] = {h->csr_gen, flags, total <= 64 ? 64 : total <= 128 ? 128 : 256,
+			                        h->use_ell * 4 + h->nt_stream * 2 + h->xcd_remap,
+			                        h->tile_rows};
+			if (memcmp(key, h->blk_tune_key, sizeof key) != 0) {
+				int tb = 64;
+				if (flags & TBTKX_VEC_FP64)
+					rc = tune_width<double2, double2>(h, total, flags, &tb);
+				else if (flags & TBTKX_H_FP64)
+					rc = tune_width<float2, double2>(h, total, flags, &tb);
+				else
+					rc = tune_width<float2, float2>(h, total, flags, &tb);
+				if (rc) return rc;
+				memcpy(h->blk_tune_key, key, sizeof key);
+				h->blk_tune_B = tb;
+			}
+			B = h->blk_tune_B;
+		}
+	}
+	if (flags & TBTKX_VEC_FP64)
 		return dispatch_width<double2, double2>(h, B, total, sites, mode, seed, M,
 		                                        flags, mu);
-	}
 	if (flags & TBTKX_H_FP64)
 		return dispatch_width<float2, double2>(h, B, total, sites, mode, seed, M,
 		                                       flags, mu);

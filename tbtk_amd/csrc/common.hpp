@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
+#include <initializer_list>
 #include <map>
 #include <string>
 #include <vector>
@@ -55,11 +57,31 @@ constexpr int kCsrPad = 64;         // zero entries after the last non-zero
 constexpr int kPlainCap = 1536;     // products staged in LDS per row block
 constexpr int kPlainRows = 256;     // rows per row block (one per thread)
 
+// One instantiated graph and the key it was captured for. Keys hold the
+// launch parameters and the device addresses of every buffer the graph
+// touches, so a buffer that was reallocated since the capture never gets
+// replayed.
 struct GraphCache {
+	static constexpr int kKey = 24;
 	hipGraphExec_t exec = nullptr;
 	hipGraph_t graph = nullptr;
-	int64_t key[4] = {-1, -1, -1, -1};
+	int64_t key[kKey];
+	GraphCache() { reset_key(); }
+	void reset_key() {
+		for (auto &k : key) k = -1;
+	}
 	void reset();
+	// Fill `out` with the values, then the addresses, zero after.
+	static void make_key(int64_t (&out)[kKey], std::initializer_list<int64_t> vals,
+	                     std::initializer_list<const void *> ptrs) {
+		int i = 0;
+		for (int64_t v : vals) out[i++] = v;
+		for (const void *q : ptrs) out[i++] = (int64_t)(intptr_t)q;
+		while (i < kKey) out[i++] = 0;
+	}
+	bool matches(const int64_t (&k)[kKey]) const {
+		return exec && memcmp(k, key, sizeof key) == 0;
+	}
 };
 
 }  // namespace tbtkx
@@ -98,6 +120,24 @@ struct tbtkx_context {
 
 	// Batched path buffers.
 	tbtkx::DevBuf blk_x, blk_y, blk_mu, blk_sites, blk_part;
+	int64_t csr_gen = 0;                          // bumped by every tbtkx_set_csr
+	int64_t blk_tune_key[5] = {-1, -1, -1, -1, -1};
+	int blk_tune_B = 0;                           // width chosen by the autotuner
+	int last_block_width = 0;                     // width of the last batched call
+	tbtkx::GraphCache blk_graph;
+
+	// Uneven multi-model batch (tbtkx_multi_*): every model's H as row-major
+	// ELL with mm_W slots per row (FP32, scaled by 1/s_m and 2/s_m), rows of
+	// all models concatenated in upload order.
+	int mm_K = 0, mm_W = 0;
+	bool mm_real = false;
+	int64_t mm_gen = 0;                       // bumped by every tbtkx_multi_set
+	std::vector<int64_t> mm_N, mm_erow0;      // rows, first ELL row per model
+	tbtkx::DevBuf mm_ecol, mm_ev1, mm_ev2;
+	tbtkx::DevBuf mm_desc, mm_tab, mm_x, mm_y, mm_part, mm_seg, mm_D, mm_cols;
+	int64_t mm_tune_key[3] = {-1, -1, -1};    // (generation, flags, widths hash)
+	int mm_tune_g = 0, mm_tune_occ = 0;       // tuned tile groups, grid share
+	tbtkx::GraphCache mm_graph;
 
 	// Options.
 	int plain_kernel = 0;
@@ -107,6 +147,7 @@ struct tbtkx_context {
 	int tile_rows = 0;     // 0 = default
 	int nt_stream = 1;     // non-temporal hints on once-used SpMM rows
 	int use_ell = 1;       // batched step on the ELL copy when it exists
+	int autotune = 1;      // time candidate launch shapes on first use
 
 	tbtkx::GraphCache plain_graph;
 };
@@ -125,5 +166,9 @@ __device__ __forceinline__ int xcd_remap(int hw, int n) {
 }
 
 int set_device(tbtkx_context *h);
+
+// out = in * (d, 0) as complex products (contraction off): the scaling of the
+// CSR values by 1/s and by 2.
+void scale_complex(const double *in, double *out, int64_t Z, double d);
 
 }  // namespace tbtkx

@@ -37,7 +37,7 @@ void GraphCache::reset() {
 	if (graph) (void)hipGraphDestroy(graph);
 	exec = nullptr;
 	graph = nullptr;
-	for (auto &k : key) k = -1;
+	reset_key();
 }
 
 int set_device(tbtkx_context *h) {
@@ -65,6 +65,19 @@ static int pool_init() {
 	g_pool_busy.assign(n, 0);
 	g_pool_ready = true;
 	return TBTKX_OK;
+}
+
+// The reference scales the CSR values in place, first by the complex scalar
+// (1/s, 0) and later by (2, 0) (Lib/src/Solver/ChebyshevExpander.cpp:147,168,
+// SparseMatrix::operator*= at Lib/include/Utilities/TBTK/SparseMatrix.h:
+// 1106-1125). The same two complex products are formed here on the host.
+void scale_complex(const double *in, double *out, int64_t Z, double d) {
+#pragma clang fp contract(off)
+	for (int64_t k = 0; k < Z; k++) {
+		const double a = in[2 * k], b = in[2 * k + 1];
+		out[2 * k] = a * d - b * 0.0;
+		out[2 * k + 1] = a * 0.0 + b * d;
+	}
 }
 
 }  // namespace tbtkx
@@ -142,12 +155,17 @@ int tbtkx_destroy(tbtkx_handle h) {
 	(void)hipSetDevice(h->device);
 	if (h->stream) (void)hipStreamSynchronize(h->stream);
 	h->plain_graph.reset();
+	h->mm_graph.reset();
+	h->blk_graph.reset();
 	for (DevBuf *b : {&h->rowptr, &h->col, &h->v1, &h->v2, &h->v1f, &h->v2f,
 	                  &h->rblk, &h->vec_a, &h->vec_b, &h->coef, &h->to,
 	                  &h->lut, &h->gf_in, &h->gf_out, &h->blk_x, &h->blk_y,
 	                  &h->blk_mu, &h->blk_sites, &h->blk_part, &h->cplanes,
 	                  &h->lplanes, &h->gf_mfma_out, &h->ell_col, &h->ell_v1f,
-	                  &h->ell_v2f, &h->ell_v1d, &h->ell_v2d})
+	                  &h->ell_v2f, &h->ell_v1d, &h->ell_v2d,
+	                  &h->mm_ecol, &h->mm_ev1, &h->mm_ev2, &h->mm_desc, &h->mm_tab,
+	                  &h->mm_x, &h->mm_y, &h->mm_part, &h->mm_seg, &h->mm_D,
+	                  &h->mm_cols})
 		b->release();
 	if (h->ev_start) (void)hipEventDestroy(h->ev_start);
 	if (h->ev_stop) (void)hipEventDestroy(h->ev_stop);
@@ -174,19 +192,6 @@ static std::vector<uint32_t> make_row_blocks(int64_t N, const uint32_t *rowptr) 
 		b.push_back((uint32_t)r);
 	}
 	return b;
-}
-
-// The reference scales the CSR values in place, first by the complex scalar
-// (1/s, 0) and later by (2, 0) (Lib/src/Solver/ChebyshevExpander.cpp:147,168,
-// SparseMatrix::operator*= at Lib/include/Utilities/TBTK/SparseMatrix.h:
-// 1106-1125). The same two complex products are formed here on the host.
-static void scale_complex(const double *in, double *out, int64_t Z, double d) {
-#pragma clang fp contract(off)
-	for (int64_t k = 0; k < Z; k++) {
-		const double a = in[2 * k], b = in[2 * k + 1];
-		out[2 * k] = a * d - b * 0.0;
-		out[2 * k + 1] = a * 0.0 + b * d;
-	}
 }
 
 int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
@@ -290,6 +295,8 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 	h->N = N;
 	h->Z = Z;
 	h->scale = scale;
+	h->csr_gen++;
+	h->blk_graph.reset();
 	h->h_real = true;
 	for (int64_t k = 0; k < Z && h->h_real; k++)
 		if (val[2 * k + 1] != 0.0) h->h_real = false;
@@ -340,8 +347,31 @@ int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
 	case TBTKX_OPT_ELL:
 		h->use_ell = value != 0;
 		return TBTKX_OK;
+	case TBTKX_OPT_AUTOTUNE:
+		h->autotune = value != 0;
+		for (auto &k : h->mm_tune_key) k = -1;
+		return TBTKX_OK;
 	default:
 		set_error("tbtkx_set_option: unknown option " + std::to_string(option));
+		return TBTKX_ERR_ARG;
+	}
+}
+
+int tbtkx_get_option(tbtkx_handle h, int option, int64_t *value) {
+	TBTKX_REQUIRE(h && value, TBTKX_ERR_ARG, "tbtkx_get_option: null pointer");
+	switch (option) {
+	case TBTKX_OPT_PLAIN_KERNEL: *value = h->plain_kernel; return TBTKX_OK;
+	case TBTKX_OPT_GRAPH: *value = h->use_graph; return TBTKX_OK;
+	case TBTKX_OPT_BLOCK_WIDTH:
+		*value = h->block_width ? h->block_width : h->last_block_width;
+		return TBTKX_OK;
+	case TBTKX_OPT_XCD_REMAP: *value = h->xcd_remap; return TBTKX_OK;
+	case TBTKX_OPT_TILE_ROWS: *value = h->tile_rows; return TBTKX_OK;
+	case TBTKX_OPT_NT_STREAM: *value = h->nt_stream; return TBTKX_OK;
+	case TBTKX_OPT_ELL: *value = h->use_ell && h->ell_W > 0; return TBTKX_OK;
+	case TBTKX_OPT_AUTOTUNE: *value = h->autotune; return TBTKX_OK;
+	default:
+		set_error("tbtkx_get_option: unknown option " + std::to_string(option));
 		return TBTKX_ERR_ARG;
 	}
 }

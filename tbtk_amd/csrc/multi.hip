@@ -1,0 +1,822 @@
+// Uneven batch: many independent models in one persistent launch per step.
+//
+// K models, each with its own Hamiltonian H_m, scale factor s_m, B_m <= 64
+// unit start vectors (LDOS sites) and M_m moments, advance through the
+// Chebyshev recurrence together: step n is ONE kernel launch over the row
+// tiles of every model that still needs step n. Models are ordered by their
+// step count, longest first, so the models active at step n are a prefix of
+// that order and the launch covers a prefix of the tile list; the launch
+// shape changes only when a model finishes (at most K "phases").
+//
+// Storage: FP32 complex H (row-major ELL, W slots per row, scaled by 1/s_m
+// and 2/s_m exactly as the single-model path scales its FP32 copies) and FP32
+// complex vectors, row-major per model (N_m rows x B_m columns), rows summed
+// in FP32 with the same fused multiply-add order as the single-model step, so
+// the recurrence vectors are the ones tbtkx_moments_block produces for the
+// same model and sites. Moments are FP64: gather mode reads
+// mu_n[b] = Phi_n[site_b][b] each step; doubling mode fuses
+// D0_n = <Phi_n|Phi_n> and D1_n = <Phi_n+1|Phi_n> into the step (FP64 per
+// lane, then ordered reductions) and uses mu_2n = 2 D0_n - mu_0,
+// mu_2n+1 = 2 D1_n - mu_1 (Weisse et al., Rev. Mod. Phys. 78, 275, eq. 31).
+//
+// Lane mapping for a model of width w = B_m: a wavefront covers
+// Q = floor(64 / w) consecutive rows at once, lane l -> row slot q = l / w,
+// column b = l % w; lanes l >= Q w are idle (they mirror a valid row's loads
+// and store nothing), so no HBM traffic is spent on padding columns. A tile is
+// G row groups (G Q rows). Each wavefront of the persistent grid walks a
+// contiguous range of the phase's tile list, cut so that every wavefront gets
+// the same number of algorithmic bytes.
+//
+// Dot partials: a wavefront sums its lanes' FP64 partials per column when it
+// leaves a model (lanes of one column, ascending row slot) and writes them to
+// its own slot; slots of one model are contiguous. Two ordered passes reduce
+// them (groups of <= 64 slots, then the groups of a model), so the result is
+// deterministic for a given launch shape.
+//
+// Algorithmic HBM bytes per step: sum over active models of
+// 12 W N_m (ELL column + value) + 24 B_m N_m (read Phi_n, read Phi_{n-1},
+// write Phi_{n+1}).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "common.hpp"
+
+namespace tbtkx {
+namespace {
+
+constexpr int kWaves = kThreads / 64;
+constexpr int kRpi = 2;          // row groups in flight per lane
+constexpr int kSegSlots = 64;    // dot slots summed per group in pass 1
+constexpr int kMaxWidth = 64;    // start vectors per model
+
+struct MDesc {
+	int64_t N;       // rows
+	int64_t erow0;   // first row of the model in the ELL arrays
+	int64_t vec0;    // first element of the model's vectors in X / Y
+	int64_t dof;     // first element of the model's D block (doubling)
+	int w;           // columns (start vectors)
+	int Q;           // rows per row group, 64 / w
+};
+
+struct GCol {
+	int64_t xoff;    // X element of (site_b, b)
+	int64_t out;     // first output moment of the column
+	int M;           // moments of the column's model
+	int pad;
+};
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float2 load_nt(const float2 *p) {
+	const f32x2 q = __builtin_nontemporal_load(reinterpret_cast<const f32x2 *>(p));
+	return make_float2(q.x, q.y);
+}
+__device__ __forceinline__ void store_nt(float2 *p, float x, float y) {
+	f32x2 q = {x, y};
+	__builtin_nontemporal_store(q, reinterpret_cast<f32x2 *>(p));
+}
+
+inline unsigned grid_for(int64_t n) {
+	return (unsigned)((n + kThreads - 1) / kThreads);
+}
+
+__global__ __launch_bounds__(kThreads) void multi_fill(float2 *__restrict__ X,
+                                                         int64_t n) {
+	for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+	     i += (int64_t)gridDim.x * kThreads)
+		X[i] = make_float2(0.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(kThreads) void multi_units(float2 *__restrict__ X,
+                                                          const GCol *__restrict__ cols,
+                                                          int ncols) {
+	const int g = blockIdx.x * kThreads + threadIdx.x;
+	if (g < ncols) X[cols[g].xoff] = make_float2(1.0f, 0.0f);
+}
+
+// Gather mode: mu_n of every column whose model has more than n moments.
+__global__ __launch_bounds__(kThreads) void multi_gather(
+    const GCol *__restrict__ cols, int ncols, const float2 *__restrict__ X,
+    double2 *__restrict__ mu, int n) {
+	const int g = blockIdx.x * kThreads + threadIdx.x;
+	if (g >= ncols) return;
+	const GCol c = cols[g];
+	if (n < c.M) {
+		const float2 x = X[c.xoff];
+		mu[c.out + n] = make_double2((double)x.x, (double)x.y);
+	}
+}
+
+template <bool REALH>
+__device__ __forceinline__ void mac(float2 a, float2 x, float &ax, float &ay) {
+	if (REALH) {
+		ax = fmaf(a.x, x.x, ax);
+		ay = fmaf(a.x, x.y, ay);
+	} else {
+		ax = fmaf(a.x, x.x, fmaf(-a.y, x.y, ax));
+		ay = fmaf(a.x, x.y, fmaf(a.y, x.x, ay));
+	}
+}
+
+// One recurrence step of every active model. Tables of the phase: toff[k]
+// (first tile of sorted model k, k <= kact), wstart[gw] (first tile of
+// wavefront gw, gw <= number of wavefronts), wslot[gw] (first dot slot).
+template <int W, bool FIRST, bool DOTS, bool REALH>
+__global__ __launch_bounds__(kThreads) void multi_step(
+    const MDesc *__restrict__ md, const int *__restrict__ toff, int kact,
+    const int *__restrict__ wstart, const int *__restrict__ wslot, int G,
+    const uint32_t *__restrict__ ecol, const float2 *__restrict__ ev,
+    const float2 *__restrict__ X, float2 *__restrict__ Y,
+    double2 *__restrict__ part) {
+	const int lane = threadIdx.x & 63;
+	const int gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
+	int t = wstart[gw];
+	const int te = wstart[gw + 1];
+	if (t >= te) return;
+	// Model of tile t: the last k with toff[k] <= t (every model has a tile).
+	int lo = 0, hi = kact;
+	while (hi - lo > 1) {
+		const int mid = (lo + hi) >> 1;
+		if (toff[mid] <= t) lo = mid;
+		else hi = mid;
+	}
+	int m = lo;
+	int slot = DOTS ? wslot[gw] : 0;
+	for (;;) {
+		const MDesc d = md[m];
+		const int w = d.w, Q = d.Q;
+		const int q = lane / w;
+		const int b = lane - q * w;
+		const bool act = q < Q;
+		const int qc = act ? q : Q - 1;
+		const int64_t ngroups = (d.N + Q - 1) / Q;
+		const int tend = te < toff[m + 1] ? te : toff[m + 1];
+		double d0 = 0.0, d1x = 0.0, d1y = 0.0;
+		for (; t < tend; t++) {
+			const int64_t g0 = (int64_t)(t - toff[m]) * G;
+			for (int gg = 0; gg < G; gg += kRpi) {
+				int64_t row[kRpi];
+				bool live[kRpi];
+				uint32_t c[kRpi][W];
+				float2 a[kRpi][W], xv[kRpi][W], yo[kRpi], xs[kRpi];
+#pragma unroll
+				for (int r = 0; r < kRpi; r++) {
+					const int64_t grp = g0 + gg + r;
+					const int64_t rr = grp * Q + qc;
+					live[r] = act && grp < ngroups && rr < d.N;
+					row[r] = rr < d.N ? rr : d.N - 1;
+					const int64_t e = (d.erow0 + row[r]) * W;
+#pragma unroll
+					for (int u = 0; u < W; u++) {
+						c[r][u] = ecol[e + u];
+						a[r][u] = ev[e + u];
+					}
+					const int64_t o = d.vec0 + row[r] * w + b;
+					if (!FIRST) yo[r] = load_nt(&Y[o]);
+					if (DOTS) xs[r] = X[o];
+				}
+#pragma unroll
+				for (int r = 0; r < kRpi; r++)
+#pragma unroll
+					for (int u = 0; u < W; u++)
+						xv[r][u] = X[d.vec0 + (int64_t)c[r][u] * w + b];
+#pragma unroll
+				for (int r = 0; r < kRpi; r++) {
+					float ax = FIRST ? 0.0f : -yo[r].x;
+					float ay = FIRST ? 0.0f : -yo[r].y;
+#pragma unroll
+					for (int u = 0; u < W; u++) mac<REALH>(a[r][u], xv[r][u], ax, ay);
+					if (!live[r]) continue;
+					store_nt(&Y[d.vec0 + row[r] * w + b], ax, ay);
+					if (DOTS) {
+						const double yx = ax, yy = ay;
+						const double xx = xs[r].x, xy = xs[r].y;
+						d0 = fma(xx, xx, fma(xy, xy, d0));
+						d1x = fma(yx, xx, fma(yy, xy, d1x));
+						d1y = fma(yx, xy, fma(-yy, xx, d1y));
+					}
+				}
+			}
+		}
+		if (DOTS) {
+			// Column b's sum over the row slots q = 0 .. Q-1, ascending.
+			double s0 = 0.0, s1x = 0.0, s1y = 0.0;
+			for (int k = 0; k < Q; k++) {
+				const int src = b + k * w;
+				s0 += __shfl(d0, src);
+				s1x += __shfl(d1x, src);
+				s1y += __shfl(d1y, src);
+			}
+			if (lane < w) {
+				part[((int64_t)slot * 2 + 0) * kMaxWidth + lane] = make_double2(s0, 0.0);
+				part[((int64_t)slot * 2 + 1) * kMaxWidth + lane] = make_double2(s1x, s1y);
+			}
+			slot++;
+		}
+		if (t >= te) break;
+		m++;
+	}
+}
+
+// Pass 1: segment s = (model, slot_lo, slot_hi) of <= 64 slots; one
+// wavefront per (segment, which), lane = column.
+__global__ __launch_bounds__(kThreads) void multi_reduce_slots(
+    const int *__restrict__ seg, int nseg, const double2 *__restrict__ part,
+    double2 *__restrict__ segsum) {
+	const int lane = threadIdx.x & 63;
+	const int gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
+	if (gw >= 2 * nseg) return;
+	const int s = gw >> 1, which = gw & 1;
+	const int lo = seg[3 * s + 1], hi = seg[3 * s + 2];
+	double2 acc = make_double2(0.0, 0.0);
+	for (int k = lo; k < hi; k++) {
+		const double2 v = part[((int64_t)k * 2 + which) * kMaxWidth + lane];
+		acc.x += v.x;
+		acc.y += v.y;
+	}
+	segsum[((int64_t)s * 2 + which) * kMaxWidth + lane] = acc;
+}
+
+// Pass 2: one wavefront per (active model, which): the model's segments in
+// order, into D[dof + (n * 2 + which) * w + b].
+__global__ __launch_bounds__(kThreads) void multi_reduce_models(
+    const MDesc *__restrict__ md, const int *__restrict__ mseg, int kact,
+    const double2 *__restrict__ segsum, double2 *__restrict__ D, int n) {
+	const int lane = threadIdx.x & 63;
+	const int gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
+	if (gw >= 2 * kact) return;
+	const int k = gw >> 1, which = gw & 1;
+	const MDesc d = md[k];
+	if (lane >= d.w) return;
+	double2 acc = make_double2(0.0, 0.0);
+	for (int s = mseg[k]; s < mseg[k + 1]; s++) {
+		const double2 v = segsum[((int64_t)s * 2 + which) * kMaxWidth + lane];
+		acc.x += v.x;
+		acc.y += v.y;
+	}
+	D[d.dof + ((int64_t)n * 2 + which) * d.w + lane] = acc;
+}
+
+// ------------------------------------------------------------------ host --
+
+struct Phase {
+	int kact;          // active models (a prefix of the sorted order)
+	int n_lo, n_hi;    // steps of the phase
+	int G;             // row groups per tile
+	int64_t toff, wstart, wslot, mseg, seg;   // offsets into the table buffer
+	int nseg, nslots;
+};
+
+struct Plan {
+	std::vector<int> order;      // sorted position -> model
+	std::vector<MDesc> desc;     // in sorted order
+	std::vector<int> S;          // steps, sorted order
+	std::vector<Phase> phases;
+	std::vector<int> tab;        // all phase tables
+	int nwg = 0, nw = 0, Smax = 0, max_slots = 0, max_seg = 0;
+	int64_t vec_elems = 0, d_elems = 0;
+};
+
+// Tables of one phase: tile offsets, a byte-balanced cut of the tile list
+// into nw contiguous wavefront ranges, the dot slots of each wavefront and
+// the slot groups of each model.
+void plan_phase(const std::vector<MDesc> &desc, int W, int kact, int Gmax, int nw,
+                Phase &ph, std::vector<int> &tab) {
+	// Tile size: the largest G <= Gmax (a multiple of kRpi) that still gives
+	// every wavefront about four tiles.
+	std::vector<int64_t> groups(kact);
+	int64_t rows_all = 0;
+	for (int k = 0; k < kact; k++) {
+		groups[k] = (desc[k].N + desc[k].Q - 1) / desc[k].Q;
+		rows_all += groups[k];
+	}
+	int G = Gmax;
+	while (G > kRpi && rows_all / G < 4 * (int64_t)nw)
+		G = std::max(kRpi, G / 2 / kRpi * kRpi);
+	ph.G = G;
+	std::vector<int> toff(kact + 1, 0);
+	for (int k = 0; k < kact; k++)
+		toff[k + 1] = toff[k] + (int)((groups[k] + G - 1) / G);
+	const int T = toff[kact];
+	// Cost of each tile in algorithmic bytes.
+	std::vector<double> cum(T + 1, 0.0);
+	for (int k = 0; k < kact; k++) {
+		const double per_row = 12.0 * W + 24.0 * desc[k].w;
+		for (int t = toff[k]; t < toff[k + 1]; t++) {
+			const int64_t r0 = (int64_t)(t - toff[k]) * G * desc[k].Q;
+			int64_t rows = (int64_t)G * desc[k].Q;
+			if (r0 + rows > desc[k].N) rows = desc[k].N - r0;
+			cum[t + 1] = cum[t] + per_row * (double)rows;
+		}
+	}
+	std::vector<int> wstart(nw + 1, T);
+	wstart[0] = 0;
+	int t = 0;
+	for (int i = 1; i < nw; i++) {
+		const double target = cum[T] * i / nw;
+		while (t < T && cum[t] < target) t++;
+		wstart[i] = t;
+	}
+	// Slots: one per (wavefront, model it touches), in wavefront order.
+	std::vector<int> wslot(nw + 1, 0), mslot(kact + 1, -1);
+	int slot = 0, k = 0;
+	for (int i = 0; i < nw; i++) {
+		wslot[i] = slot;
+		const int a = wstart[i], e = wstart[i + 1];
+		if (a >= e) continue;
+		while (toff[k + 1] <= a) k++;
+		int kk = k;
+		for (;;) {
+			if (mslot[kk] < 0) mslot[kk] = slot;
+			slot++;
+			if (toff[kk + 1] >= e) break;
+			kk++;
+		}
+	}
+	wslot[nw] = slot;
+	mslot[kact] = slot;
+	std::vector<int> mseg(kact + 1, 0), seg;
+	for (int j = 0; j < kact; j++) {
+		mseg[j] = (int)(seg.size() / 3);
+		for (int s = mslot[j]; s < mslot[j + 1]; s += kSegSlots) {
+			seg.push_back(j);
+			seg.push_back(s);
+			seg.push_back(std::min(s + kSegSlots, mslot[j + 1]));
+		}
+	}
+	mseg[kact] = (int)(seg.size() / 3);
+	ph.kact = kact;
+	ph.nslots = slot;
+	ph.nseg = (int)(seg.size() / 3);
+	auto put = [&tab](const std::vector<int> &v) {
+		const int64_t off = (int64_t)tab.size();
+		tab.insert(tab.end(), v.begin(), v.end());
+		return off;
+	};
+	ph.toff = put(toff);
+	ph.wstart = put(wstart);
+	ph.wslot = put(wslot);
+	ph.mseg = put(mseg);
+	ph.seg = put(seg.empty() ? std::vector<int>(3, 0) : seg);
+}
+
+template <int W, bool DOTS, bool REALH>
+const void *step_fn() {
+	return (const void *)multi_step<W, false, DOTS, REALH>;
+}
+
+int grid_size(tbtkx_context *h, bool dots, int share, int *nwg) {
+	const void *fn;
+	const bool R = h->mm_real;
+#define TBTKX_FN(WW) \
+	fn = dots ? (R ? step_fn<WW, true, true>() : step_fn<WW, true, false>()) \
+	          : (R ? step_fn<WW, false, true>() : step_fn<WW, false, false>())
+	if (h->mm_W == 4) TBTKX_FN(4);
+	else if (h->mm_W == 6) TBTKX_FN(6);
+	else TBTKX_FN(8);
+#undef TBTKX_FN
+	int per_cu = 0, cus = 0;
+	TBTKX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0));
+	TBTKX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+	                                      h->device));
+	int64_t n = (int64_t)(per_cu > 0 ? per_cu : 1) * cus / (share > 0 ? share : 1);
+	n = (n + 7) / 8 * 8;
+	if (n < 8) n = 8;
+	*nwg = (int)n;
+	return TBTKX_OK;
+}
+
+// Launch shape and tables for the given widths and step counts.
+int make_plan(tbtkx_context *h, const std::vector<int> &width, const std::vector<int> &S,
+              bool dots, int Gmax, int share, Plan &p) {
+	const int K = h->mm_K;
+	p.order.resize(K);
+	std::iota(p.order.begin(), p.order.end(), 0);
+	std::stable_sort(p.order.begin(), p.order.end(),
+	                 [&S](int x, int y) { return S[x] > S[y]; });
+	p.desc.resize(K);
+	p.S.resize(K);
+	int64_t vec = 0, dof = 0;
+	for (int k = 0; k < K; k++) {
+		const int m = p.order[k];
+		MDesc &d = p.desc[k];
+		d.N = h->mm_N[m];
+		d.erow0 = h->mm_erow0[m];
+		d.w = width[m];
+		d.Q = kMaxWidth / d.w;
+		d.vec0 = vec;
+		d.dof = dof;
+		vec += d.N * d.w;
+		dof += (int64_t)S[m] * 2 * d.w;
+		p.S[k] = S[m];
+	}
+	p.vec_elems = vec;
+	p.d_elems = dof;
+	p.Smax = p.S[0];
+	int rc = grid_size(h, dots, share, &p.nwg);
+	if (rc) return rc;
+	p.nw = p.nwg * kWaves;
+	p.phases.clear();
+	p.tab.clear();
+	for (int n = 0; n < p.Smax;) {
+		int kact = 0;
+		while (kact < K && p.S[kact] > n) kact++;
+		const int n_hi = p.S[kact - 1];
+		Phase ph;
+		ph.n_lo = n;
+		ph.n_hi = n_hi;
+		plan_phase(p.desc, h->mm_W, kact, Gmax, p.nw, ph, p.tab);
+		p.max_slots = std::max(p.max_slots, ph.nslots);
+		p.max_seg = std::max(p.max_seg, ph.nseg);
+		p.phases.push_back(ph);
+		n = n_hi;
+	}
+	return TBTKX_OK;
+}
+
+template <int W, bool REALH>
+void launch_step_w(tbtkx_context *h, const Plan &p, const Phase &ph, bool first,
+                   bool dots, const float2 *X, float2 *Y) {
+	const int *tab = h->mm_tab.as<int>();
+	const MDesc *md = h->mm_desc.as<MDesc>();
+	const float2 *ev = first ? h->mm_ev1.as<float2>() : h->mm_ev2.as<float2>();
+	double2 *part = h->mm_part.as<double2>();
+#define TBTKX_MS(F, D)                                                          \
+	multi_step<W, F, D, REALH><<<p.nwg, kThreads, 0, h->stream>>>(            \
+	    md, tab + ph.toff, ph.kact, tab + ph.wstart, tab + ph.wslot, ph.G,      \
+	    h->mm_ecol.as<uint32_t>(), ev, X, Y, part)
+	if (first && dots) TBTKX_MS(true, true);
+	else if (first) TBTKX_MS(true, false);
+	else if (dots) TBTKX_MS(false, true);
+	else TBTKX_MS(false, false);
+#undef TBTKX_MS
+}
+
+void launch_step(tbtkx_context *h, const Plan &p, const Phase &ph, bool first,
+                 bool dots, const float2 *X, float2 *Y) {
+	const bool R = h->mm_real;
+	if (h->mm_W == 4) {
+		if (R) launch_step_w<4, true>(h, p, ph, first, dots, X, Y);
+		else launch_step_w<4, false>(h, p, ph, first, dots, X, Y);
+	} else if (h->mm_W == 6) {
+		if (R) launch_step_w<6, true>(h, p, ph, first, dots, X, Y);
+		else launch_step_w<6, false>(h, p, ph, first, dots, X, Y);
+	} else {
+		if (R) launch_step_w<8, true>(h, p, ph, first, dots, X, Y);
+		else launch_step_w<8, false>(h, p, ph, first, dots, X, Y);
+	}
+}
+
+int reserve_plan(tbtkx_context *h, const Plan &p, int ncols, int64_t mu_elems) {
+	int rc;
+	if ((rc = h->mm_desc.reserve(p.desc.size() * sizeof(MDesc)))) return rc;
+	if ((rc = h->mm_tab.reserve(p.tab.size() * 4))) return rc;
+	if ((rc = h->mm_x.reserve((size_t)p.vec_elems * 8))) return rc;
+	if ((rc = h->mm_y.reserve((size_t)p.vec_elems * 8))) return rc;
+	if ((rc = h->mm_part.reserve((size_t)(p.max_slots > 0 ? p.max_slots : 1) * 2 *
+	                             kMaxWidth * 16)))
+		return rc;
+	if ((rc = h->mm_seg.reserve((size_t)(p.max_seg > 0 ? p.max_seg : 1) * 2 *
+	                            kMaxWidth * 16)))
+		return rc;
+	if ((rc = h->mm_D.reserve((size_t)(mu_elems > 0 ? mu_elems : 1) * 16))) return rc;
+	if ((rc = h->mm_cols.reserve((size_t)ncols * sizeof(GCol)))) return rc;
+	TBTKX_CHECK_HIP(hipMemcpyAsync(h->mm_desc.ptr, p.desc.data(),
+	                               p.desc.size() * sizeof(MDesc),
+	                               hipMemcpyHostToDevice, h->stream));
+	if (!p.tab.empty())
+		TBTKX_CHECK_HIP(hipMemcpyAsync(h->mm_tab.ptr, p.tab.data(), p.tab.size() * 4,
+		                               hipMemcpyHostToDevice, h->stream));
+	return TBTKX_OK;
+}
+
+// Everything from the initial vectors to the last step, on h->stream.
+int enqueue_run(tbtkx_context *h, const Plan &p, bool dots, int ncols) {
+	float2 *X = h->mm_x.as<float2>(), *Y = h->mm_y.as<float2>();
+	const GCol *cols = h->mm_cols.as<GCol>();
+	double2 *D = h->mm_D.as<double2>();
+	const int *tab = h->mm_tab.as<int>();
+	multi_fill<<<2048, kThreads, 0, h->stream>>>(X, p.vec_elems);
+	multi_units<<<grid_for(ncols), kThreads, 0, h->stream>>>(X, cols, ncols);
+	int64_t launches = 0;
+	for (const Phase &ph : p.phases) {
+		for (int n = ph.n_lo; n < ph.n_hi; n++) {
+			if (!dots)
+				multi_gather<<<grid_for(ncols), kThreads, 0, h->stream>>>(
+				    cols, ncols, X, D, n);
+			launch_step(h, p, ph, n == 0, dots, X, Y);
+			launches++;
+			if (dots) {
+				multi_reduce_slots<<<(2 * ph.nseg + kWaves - 1) / kWaves, kThreads, 0,
+				                     h->stream>>>(tab + ph.seg, ph.nseg,
+				                                  h->mm_part.as<double2>(),
+				                                  h->mm_seg.as<double2>());
+				multi_reduce_models<<<(2 * ph.kact + kWaves - 1) / kWaves, kThreads, 0,
+				                      h->stream>>>(h->mm_desc.as<MDesc>(), tab + ph.mseg,
+				                                   ph.kact, h->mm_seg.as<double2>(), D, n);
+			}
+			float2 *tmp = X;
+			X = Y;
+			Y = tmp;
+		}
+	}
+	if (!dots)
+		multi_gather<<<grid_for(ncols), kThreads, 0, h->stream>>>(cols, ncols, X, D,
+		                                                          p.Smax);
+	TBTKX_CHECK_HIP(hipGetLastError());
+	h->last_launches = launches;
+	return TBTKX_OK;
+}
+
+// Autotuner: time a few steps of the first phase for each (tile groups, grid
+// share) candidate on the actual batch and keep the fastest.
+int autotune(tbtkx_context *h, const std::vector<int> &width, const std::vector<int> &S,
+             bool dots, int ncols, int64_t mu_elems) {
+	static const int kG[] = {2, 4, 8, 16};
+	static const int kShare[] = {1, 2};
+	constexpr int kTrial = 4;
+	float best = 0;
+	int bestG = 8, bestShare = 1;
+	for (int share : kShare) {
+		for (int G : kG) {
+			Plan p;
+			int rc = make_plan(h, width, S, dots, G, share, p);
+			if (rc) return rc;
+			if ((rc = reserve_plan(h, p, ncols, mu_elems))) return rc;
+			const Phase &ph = p.phases[0];
+			float2 *X = h->mm_x.as<float2>(), *Y = h->mm_y.as<float2>();
+			multi_fill<<<2048, kThreads, 0, h->stream>>>(X, p.vec_elems);
+			launch_step(h, p, ph, true, dots, X, Y);
+			TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
+			for (int i = 0; i < kTrial; i++) {
+				launch_step(h, p, ph, false, dots, Y, X);
+				float2 *tmp = X;
+				X = Y;
+				Y = tmp;
+			}
+			TBTKX_CHECK_HIP(hipEventRecord(h->ev_stop, h->stream));
+			TBTKX_CHECK_HIP(hipEventSynchronize(h->ev_stop));
+			TBTKX_CHECK_HIP(hipGetLastError());
+			float ms = 0;
+			TBTKX_CHECK_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
+			if (best == 0 || ms < best) {
+				best = ms;
+				bestG = G;
+				bestShare = share;
+			}
+		}
+	}
+	h->mm_tune_g = bestG;
+	h->mm_tune_occ = bestShare;
+	return TBTKX_OK;
+}
+
+}  // namespace
+}  // namespace tbtkx
+
+using namespace tbtkx;
+
+extern "C" {
+
+int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
+                    const int64_t *nnz, const uint32_t *rowptr, const uint32_t *col,
+                    const double *val, const double *scales) {
+	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_multi_set: null handle");
+	TBTKX_REQUIRE(K > 0 && K <= 65536 && num_rows && nnz && rowptr && scales,
+	              TBTKX_ERR_ARG, "tbtkx_multi_set: need 1..65536 models and arrays");
+	int64_t rp0 = 0, z0 = 0, rows = 0;
+	uint32_t maxlen = 0;
+	bool real = true;
+	for (int m = 0; m < K; m++) {
+		const int64_t N = num_rows[m], Z = nnz[m];
+		const std::string tag = "tbtkx_multi_set: model " + std::to_string(m) + ": ";
+		TBTKX_REQUIRE(N > 0 && N < ((int64_t)1 << 31) && Z >= 0 && Z < ((int64_t)1 << 32),
+		              TBTKX_ERR_ARG, tag + "bad size");
+		TBTKX_REQUIRE(scales[m] > 0, TBTKX_ERR_ARG,
+		              tag + "the scale factor has to be larger than 0");
+		TBTKX_REQUIRE(Z == 0 || (col && val), TBTKX_ERR_ARG, tag + "null array");
+		const uint32_t *rp = rowptr + rp0;
+		TBTKX_REQUIRE(rp[0] == 0 && (int64_t)rp[N] == Z, TBTKX_ERR_ARG,
+		              tag + "rowptr[0] must be 0 and rowptr[N] == nnz");
+		for (int64_t r = 0; r < N; r++) {
+			TBTKX_REQUIRE(rp[r] <= rp[r + 1], TBTKX_ERR_ARG, tag + "rowptr not monotone");
+			maxlen = std::max(maxlen, rp[r + 1] - rp[r]);
+		}
+		for (int64_t k = 0; k < Z; k++) {
+			TBTKX_REQUIRE(col[z0 + k] < (uint64_t)N, TBTKX_ERR_ARG,
+			              tag + "column index out of range");
+			if (val[2 * (z0 + k) + 1] != 0.0) real = false;
+		}
+		rp0 += N + 1;
+		z0 += Z;
+		rows += N;
+	}
+	TBTKX_REQUIRE(maxlen <= 8, TBTKX_ERR_ARG,
+	              "tbtkx_multi_set: rows with more than 8 non-zeros are not supported");
+	TBTKX_REQUIRE(rows * 8 < ((int64_t)1 << 40), TBTKX_ERR_ARG,
+	              "tbtkx_multi_set: batch too large");
+	int rc = set_device(h);
+	if (rc) return rc;
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	const int W = maxlen <= 4 ? 4 : maxlen <= 6 ? 6 : 8;
+	std::vector<uint32_t> ec((size_t)rows * W);
+	std::vector<float> e1((size_t)rows * W * 2, 0.0f), e2((size_t)rows * W * 2, 0.0f);
+	h->mm_N.assign(K, 0);
+	h->mm_erow0.assign(K, 0);
+	rp0 = 0;
+	z0 = 0;
+	int64_t erow = 0;
+	std::vector<double> v1, v2;
+	for (int m = 0; m < K; m++) {
+		const int64_t N = num_rows[m], Z = nnz[m];
+		const uint32_t *rp = rowptr + rp0;
+		v1.assign(2 * Z, 0.0);
+		v2.assign(2 * Z, 0.0);
+		scale_complex(val + 2 * z0, v1.data(), Z, 1 / scales[m]);
+		scale_complex(v1.data(), v2.data(), Z, 2);
+		for (int64_t r = 0; r < N; r++) {
+			const int64_t base = (erow + r) * W;
+			for (int u = 0; u < W; u++) {
+				const int64_t k = (int64_t)rp[r] + u;
+				if (k < (int64_t)rp[r + 1]) {
+					ec[base + u] = col[z0 + k];
+					e1[2 * (base + u)] = (float)v1[2 * k];
+					e1[2 * (base + u) + 1] = (float)v1[2 * k + 1];
+					e2[2 * (base + u)] = (float)v2[2 * k];
+					e2[2 * (base + u) + 1] = (float)v2[2 * k + 1];
+				} else {
+					ec[base + u] = (uint32_t)r;
+				}
+			}
+		}
+		h->mm_N[m] = N;
+		h->mm_erow0[m] = erow;
+		erow += N;
+		rp0 += N + 1;
+		z0 += Z;
+	}
+	h->mm_graph.reset();
+	h->mm_K = 0;
+	if ((rc = h->mm_ecol.reserve(ec.size() * 4))) return rc;
+	if ((rc = h->mm_ev1.reserve(e1.size() * 4))) return rc;
+	if ((rc = h->mm_ev2.reserve(e2.size() * 4))) return rc;
+	TBTKX_CHECK_HIP(hipMemcpy(h->mm_ecol.ptr, ec.data(), ec.size() * 4, hipMemcpyHostToDevice));
+	TBTKX_CHECK_HIP(hipMemcpy(h->mm_ev1.ptr, e1.data(), e1.size() * 4, hipMemcpyHostToDevice));
+	TBTKX_CHECK_HIP(hipMemcpy(h->mm_ev2.ptr, e2.data(), e2.size() * 4, hipMemcpyHostToDevice));
+	h->mm_K = K;
+	h->mm_W = W;
+	h->mm_real = real;
+	h->mm_gen++;
+	for (auto &k : h->mm_tune_key) k = -1;
+	return TBTKX_OK;
+}
+
+int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sites,
+                        const int *num_moments, int flags, double *out) {
+	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_multi_moments: null handle");
+	TBTKX_REQUIRE(h->mm_K > 0, TBTKX_ERR_STATE, "tbtkx_multi_moments: no models set");
+	TBTKX_REQUIRE(num_sites && sites && num_moments && out, TBTKX_ERR_ARG,
+	              "tbtkx_multi_moments: null array");
+	TBTKX_REQUIRE((flags & ~TBTKX_DOUBLING) == 0, TBTKX_ERR_ARG,
+	              "tbtkx_multi_moments: only TBTKX_DOUBLING is supported (FP32 storage)");
+	const int K = h->mm_K;
+	const bool dots = (flags & TBTKX_DOUBLING) != 0;
+	std::vector<int> width(K), S(K);
+	std::vector<int64_t> out_off(K + 1, 0);
+	int ncols = 0;
+	for (int m = 0; m < K; m++) {
+		const std::string tag = "tbtkx_multi_moments: model " + std::to_string(m) + ": ";
+		TBTKX_REQUIRE(num_sites[m] >= 1 && num_sites[m] <= kMaxWidth, TBTKX_ERR_ARG,
+		              tag + "needs 1..64 sites");
+		TBTKX_REQUIRE(num_moments[m] >= 1, TBTKX_ERR_ARG,
+		              tag + "numCoefficients has to be larger than 0");
+		for (int b = 0; b < num_sites[m]; b++)
+			TBTKX_REQUIRE(sites[ncols + b] >= 0 && sites[ncols + b] < h->mm_N[m],
+			              TBTKX_ERR_ARG, tag + "site outside the basis");
+		width[m] = num_sites[m];
+		S[m] = dots ? (num_moments[m] + 1) / 2 : num_moments[m] - 1;
+		out_off[m + 1] = out_off[m] + (int64_t)num_sites[m] * num_moments[m];
+		ncols += num_sites[m];
+	}
+	int rc = set_device(h);
+	if (rc) return rc;
+	const int Smax = *std::max_element(S.begin(), S.end());
+
+	// Launch shape: explicit (TBTKX_OPT_TILE_ROWS = row groups per tile) or
+	// autotuned once per (batch, mode, widths).
+	int64_t hw = 1469598103934665603ll;
+	for (int m = 0; m < K; m++) hw = (hw ^ (width[m] * 131 + S[m])) * 1099511628211ll;
+	const int64_t key[3] = {h->mm_gen, flags, hw};
+	int G = 8, share = 1;
+	if (h->tile_rows > 0) {
+		G = std::max(kRpi, (h->tile_rows + kRpi - 1) / kRpi * kRpi);
+	} else if (h->autotune && Smax > 0) {
+		if (memcmp(key, h->mm_tune_key, sizeof key) != 0) {
+			// Output sizes are needed to reserve buffers for the trial runs.
+			int64_t mu_elems = 0;
+			for (int m = 0; m < K; m++)
+				mu_elems += dots ? (int64_t)S[m] * 2 * width[m] : out_off[m + 1] - out_off[m];
+			if ((rc = autotune(h, width, S, dots, ncols, mu_elems))) return rc;
+			memcpy(h->mm_tune_key, key, sizeof key);
+		}
+		G = h->mm_tune_g;
+		share = h->mm_tune_occ;
+	}
+	Plan p;
+	if (Smax > 0) {
+		if ((rc = make_plan(h, width, S, dots, G, share, p))) return rc;
+	} else {
+		// One moment per model: only the initial gather.
+		p.order.resize(K);
+		std::iota(p.order.begin(), p.order.end(), 0);
+		p.desc.resize(K);
+		int64_t vec = 0;
+		for (int m = 0; m < K; m++) {
+			p.desc[m] = MDesc{h->mm_N[m], h->mm_erow0[m], vec, 0, width[m],
+			                  kMaxWidth / width[m]};
+			vec += h->mm_N[m] * width[m];
+		}
+		p.vec_elems = vec;
+	}
+	// Columns: X offset of each unit start vector and its output row.
+	std::vector<int> pos(K);
+	for (int k = 0; k < K; k++) pos[p.order[k]] = k;
+	std::vector<GCol> cols(ncols);
+	for (int m = 0, c = 0; m < K; m++) {
+		const MDesc &d = p.desc[pos[m]];
+		for (int b = 0; b < width[m]; b++, c++) {
+			cols[c].xoff = d.vec0 + sites[c] * d.w + b;
+			cols[c].out = out_off[m] + (int64_t)b * num_moments[m];
+			cols[c].M = num_moments[m];
+			cols[c].pad = 0;
+		}
+	}
+	const int64_t mu_elems = dots ? p.d_elems : out_off[K];
+	if ((rc = reserve_plan(h, p, ncols, mu_elems))) return rc;
+	TBTKX_CHECK_HIP(hipMemcpyAsync(h->mm_cols.ptr, cols.data(), cols.size() * sizeof(GCol),
+	                               hipMemcpyHostToDevice, h->stream));
+
+	TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
+	if (h->use_graph) {
+		GraphCache &gc = h->mm_graph;
+		int64_t mh = hw;
+		for (int m = 0; m < K; m++) mh = (mh ^ num_moments[m]) * 1099511628211ll;
+		int64_t gkey[GraphCache::kKey];
+		GraphCache::make_key(gkey, {h->mm_gen, flags, mh, G * 16 + share, p.nwg},
+		                     {h->mm_desc.ptr, h->mm_tab.ptr, h->mm_x.ptr, h->mm_y.ptr,
+		                      h->mm_part.ptr, h->mm_seg.ptr, h->mm_D.ptr, h->mm_cols.ptr,
+		                      h->mm_ecol.ptr, h->mm_ev1.ptr, h->mm_ev2.ptr});
+		if (!gc.matches(gkey)) {
+			gc.reset();
+			TBTKX_CHECK_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+			int erc = enqueue_run(h, p, dots, ncols);
+			hipError_t ec = hipStreamEndCapture(h->stream, &gc.graph);
+			if (erc) return erc;
+			TBTKX_CHECK_HIP(ec);
+			TBTKX_CHECK_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
+			memcpy(gc.key, gkey, sizeof gkey);
+		}
+		TBTKX_CHECK_HIP(hipGraphLaunch(gc.exec, h->stream));
+		h->last_launches = Smax;
+	} else {
+		if ((rc = enqueue_run(h, p, dots, ncols))) return rc;
+	}
+	TBTKX_CHECK_HIP(hipEventRecord(h->ev_stop, h->stream));
+	std::vector<double2> buf(mu_elems > 0 ? mu_elems : 1);
+	TBTKX_CHECK_HIP(hipMemcpyAsync(buf.data(), h->mm_D.ptr, (size_t)mu_elems * 16,
+	                               hipMemcpyDeviceToHost, h->stream));
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	float ms = 0;
+	TBTKX_CHECK_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
+	h->last_ms = ms;
+
+	double2 *o2 = reinterpret_cast<double2 *>(out);
+	if (!dots) {
+		memcpy(o2, buf.data(), (size_t)out_off[K] * 16);
+		return TBTKX_OK;
+	}
+	for (int m = 0; m < K; m++) {
+		const MDesc &d = p.desc[pos[m]];
+		const int M = num_moments[m], w = width[m];
+		for (int b = 0; b < w; b++) {
+			double2 *o = o2 + out_off[m] + (int64_t)b * M;
+			auto Dv = [&](int n, int which) {
+				return buf[d.dof + ((int64_t)n * 2 + which) * w + b];
+			};
+			const double2 mu0 = Dv(0, 0), mu1 = Dv(0, 1);
+			o[0] = mu0;
+			if (M > 1) o[1] = mu1;
+			for (int n = 1; n < S[m]; n++) {
+				const double2 a = Dv(n, 0), c = Dv(n, 1);
+				if (2 * n < M) o[2 * n] = make_double2(2 * a.x - mu0.x, 2 * a.y - mu0.y);
+				if (2 * n + 1 < M)
+					o[2 * n + 1] = make_double2(2 * c.x - mu1.x, 2 * c.y - mu1.y);
+			}
+		}
+	}
+	return TBTKX_OK;
+}
+
+}  // extern "C"

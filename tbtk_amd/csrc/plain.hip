@@ -238,8 +238,12 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 	TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
 	if (h->use_graph) {
 		GraphCache &gc = h->plain_graph;
-		const int64_t key[4] = {U, M, h->plain_kernel, h->xcd_remap};
-		if (!gc.exec || memcmp(key, gc.key, sizeof key) != 0) {
+		int64_t key[GraphCache::kKey];
+		GraphCache::make_key(key, {U, M, h->plain_kernel, h->xcd_remap, h->N},
+		                     {h->vec_a.ptr, h->vec_b.ptr, h->coef.ptr, h->to.ptr,
+		                      h->rowptr.ptr, h->col.ptr, h->v1.ptr, h->v2.ptr,
+		                      h->rblk.ptr});
+		if (!gc.matches(key)) {
 			gc.reset();
 			TBTKX_CHECK_HIP(hipStreamBeginCapture(h->stream,
 			                                      hipStreamCaptureModeThreadLocal));

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libtbtkx.so")
 ADVANCED, RETARDED, PRINCIPAL, NONPRINCIPAL = 0, 1, 2, 3
 H_FP64, DOUBLING, VEC_FP64 = 1, 2, 4
 (OPT_PLAIN_KERNEL, OPT_GRAPH, OPT_BLOCK_WIDTH, OPT_XCD_REMAP, OPT_TILE_ROWS, OPT_NT_STREAM,
- OPT_ELL) = 1, 2, 3, 4, 5, 6, 7
+ OPT_ELL, OPT_AUTOTUNE) = 1, 2, 3, 4, 5, 6, 7, 8
 
 # (name, argtypes) of every exported function, in header order.
 _P = ctypes.c_void_p
@@ -43,8 +43,11 @@ SIGNATURES = {
     "tbtkx_contract_mfma": [_P, _I, _P],
     "tbtkx_moments_block": [_P, _I, _P, _I, _I, _P],
     "tbtkx_moments_trace": [_P, _I, ctypes.c_uint64, _I, _I, _P],
+    "tbtkx_multi_set": [_P, _I, _P, _P, _P, _P, _P, _P],
+    "tbtkx_multi_moments": [_P, _P, _P, _P, _I, _P],
     "tbtkx_last_timing": [_P, _P, _P],
     "tbtkx_set_option": [_P, _I, _I64],
+    "tbtkx_get_option": [_P, _I, _P],
 }
 
 _lib = None
@@ -126,6 +129,11 @@ class Engine:
     def set_option(self, option: int, value: int):
         _check(self._lib.tbtkx_set_option(self._h, option, int(value)))
 
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int64(0)
+        _check(self._lib.tbtkx_get_option(self._h, option, ctypes.byref(v)))
+        return v.value
+
     # ------------------------------------------------------------ plain ---
     def moments(self, frm: int, to, M: int) -> np.ndarray:
         to = np.ascontiguousarray(np.atleast_1d(to), dtype=np.int64)
@@ -189,6 +197,37 @@ class Engine:
         out = np.zeros(M, dtype=np.complex128)
         _check(self._lib.tbtkx_moments_trace(self._h, R, seed, M, flags, _p(out)))
         return out
+
+    # ----------------------------------------------------- uneven batch ---
+    def multi_set(self, csrs, scales):
+        """Upload several models at once: csrs = [(rowptr, col, val), ...]."""
+        rps = [np.ascontiguousarray(c[0], dtype=np.uint32) for c in csrs]
+        cols = [np.ascontiguousarray(c[1], dtype=np.uint32) for c in csrs]
+        vals = [np.ascontiguousarray(c[2], dtype=np.complex128) for c in csrs]
+        N = np.array([r.size - 1 for r in rps], dtype=np.int64)
+        Z = np.array([c.size for c in cols], dtype=np.int64)
+        rp = np.concatenate(rps)
+        col = np.concatenate(cols) if Z.sum() else np.zeros(1, np.uint32)
+        val = np.concatenate(vals) if Z.sum() else np.zeros(1, np.complex128)
+        sc = np.ascontiguousarray(scales, dtype=np.float64)
+        _check(self._lib.tbtkx_multi_set(self._h, len(csrs), _p(N), _p(Z), _p(rp), _p(col),
+                                         _p(val), _p(sc)))
+        self.multi_N = N
+
+    def multi_moments(self, sites, moments, flags: int = 0):
+        """sites: list of per-model site arrays; moments: per-model M.
+        Returns a list of (B_m, M_m) complex arrays."""
+        ns = np.array([len(s) for s in sites], dtype=np.int32)
+        st = np.ascontiguousarray(np.concatenate([np.asarray(s) for s in sites]), dtype=np.int64)
+        Ms = np.ascontiguousarray(moments, dtype=np.int32)
+        total = int((ns.astype(np.int64) * Ms).sum())
+        out = np.zeros(total, dtype=np.complex128)
+        _check(self._lib.tbtkx_multi_moments(self._h, _p(ns), _p(st), _p(Ms), flags, _p(out)))
+        res, o = [], 0
+        for b, m in zip(ns, Ms):
+            res.append(out[o:o + b * m].reshape(b, m))
+            o += b * m
+        return res
 
     def last_timing(self):
         ms = ctypes.c_double(0)

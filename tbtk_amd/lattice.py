@@ -231,14 +231,18 @@ def honeycomb_ribbon(lx, ly, t=2.7, disorder=0.0, seed=0):
         np.concatenate(vals), dict(lx=lx, ly=ly, t=t, disorder=disorder, seed=seed))
 
 
-def square(lx, ly, t=1.0, disorder=0.0, seed=0):
-    """{x, y}: -t nearest-neighbour hopping, disorder U[-disorder, disorder]."""
+def square(lx, ly, t=1.0, disorder=0.0, seed=0, flux=0.0):
+    """{x, y}: -t nearest-neighbour hopping, disorder U[-disorder, disorder].
+    flux: magnetic flux per plaquette in flux quanta (Landau gauge: the hop
+    (x, y) -> (x, y + 1) carries exp(2 pi i flux x)), which makes H complex."""
     shape = (lx, ly)
     rows, cols, vals = [], [], []
     for axis in (0, 1):
         to, frm = _bonds(shape, axis)
         a, b = _flat(to, shape), _flat(frm, shape)
         amp = np.full(a.size, -t, dtype=np.complex128)
+        if axis == 1 and flux:
+            amp = amp * np.exp(2j * np.pi * flux * frm[0])
         rows += [a, b]; cols += [b, a]; vals += [amp, np.conj(amp)]
     n = lx * ly
     w = np.zeros(n)
@@ -248,7 +252,8 @@ def square(lx, ly, t=1.0, disorder=0.0, seed=0):
     rows.append(d); cols.append(d); vals.append(w.astype(np.complex128))
     return LatticeModel(
         "square", shape, np.concatenate(rows), np.concatenate(cols),
-        np.concatenate(vals), dict(lx=lx, ly=ly, t=t, disorder=disorder, seed=seed))
+        np.concatenate(vals),
+        dict(lx=lx, ly=ly, t=t, disorder=disorder, seed=seed, flux=flux))
 
 
 def cubic(lx, ly, lz, t=1.0, disorder=0.0, seed=0):
@@ -348,3 +353,29 @@ def config_nambu(l=512, seed=3):
 def config_cubic(l=128, seed=5):
     """l^3 cubic lattice, t=1, disorder U[-1,1]."""
     return cubic(l, l, l, t=1.0, disorder=1.0, seed=seed)
+
+
+def config_uneven(seed=4, count=64, max_states=1_000_000):
+    """Uneven batch of `count` independent models drawn from `seed`: each a
+    honeycomb ribbon (t=2.7, 4-site cell) or a square lattice (t=1), width
+    20-400 and length 50-2000 (the length cut so that a model has at most
+    `max_states` states), on-site disorder U[-1/2, 1/2], 500-8000 moments and
+    1-64 LDOS sites drawn uniformly. Returns [(model, sites, M), ...]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        honey = bool(rng.integers(2))
+        width = int(rng.integers(20, 401))
+        length = int(rng.integers(50, 2001))
+        per = 4 if honey else 1
+        length = max(1, min(length, max_states // (per * width)))
+        M = int(rng.integers(500, 8001))
+        B = int(rng.integers(1, 65))
+        mseed = int(rng.integers(1 << 31))
+        if honey:
+            model = honeycomb_ribbon(width, length, t=2.7, disorder=0.5, seed=mseed)
+        else:
+            model = square(width, length, t=1.0, disorder=0.5, seed=mseed)
+        sites = np.sort(rng.choice(model.basis_size, B, replace=False))
+        out.append((model, sites, M))
+    return out

@@ -185,3 +185,40 @@ def test_trace_vs_oracle(engine, oracle_mod, flags, tol):
     ref = oracle_mod.moments_trace(m.csr(), s, vecs, M)
     got = engine.moments_trace(R, seed, M, flags)
     assert np.abs(got - ref).max() < tol * R * N, np.abs(got - ref).max()
+
+
+def test_block_width_autotuner_and_graph(engine):
+    """The width autotuner picks one of the kernel widths and gives the same
+    moments as a fixed width (FP32 vectors are column-independent; only the
+    FP64 dot-sum order differs); the block recurrence replayed as a hipGraph
+    matches eager launches on every replay, with eager work in between."""
+    m = L.nambu_swave(12, 11, disorder=0.5, seed=4)
+    s = m.gershgorin_bound() * 1.05
+    engine.set_csr(m.csr(), s)
+    sites = np.random.default_rng(3).choice(m.basis_size, 150, replace=False)
+    engine.set_option(X.OPT_BLOCK_WIDTH, 0)
+    tuned = engine.moments_block(sites, 81, X.DOUBLING)
+    width = engine.get_option(X.OPT_BLOCK_WIDTH)
+    assert width in (64, 128, 256)
+    engine.set_option(X.OPT_BLOCK_WIDTH, 64)
+    fixed = engine.moments_block(sites, 81, X.DOUBLING)
+    assert np.abs(tuned - fixed).max() < 1e-12
+    gather = engine.moments_block(sites, 81, 0)
+    engine.set_option(X.OPT_BLOCK_WIDTH, 0)
+    engine.set_option(X.OPT_GRAPH, 1)
+    try:
+        for _ in range(3):
+            g = engine.moments_block(sites, 81, X.DOUBLING)
+            assert np.abs(g - tuned).max() < 1e-12
+            g0 = engine.moments_block(sites, 81, 0)
+            assert np.array_equal(g0, gather)
+            engine.moments(int(sites[0]), [int(sites[0])], 7)   # eager work between replays
+        # a larger call reallocates the block buffers; the cached graph of the
+        # first call must not be replayed on the freed ones
+        engine.set_option(X.OPT_GRAPH, 0)
+        engine.moments_block(np.arange(400), 33, X.DOUBLING)
+        engine.set_option(X.OPT_GRAPH, 1)
+        g = engine.moments_block(sites, 81, X.DOUBLING)
+        assert np.abs(g - tuned).max() < 1e-12
+    finally:
+        engine.set_option(X.OPT_GRAPH, 0)

@@ -1,0 +1,157 @@
+"""Uneven multi-model batch (tbtkx_multi_set / tbtkx_multi_moments) on the GPU.
+
+Checks, per model of a small uneven batch:
+- gather mode is bitwise equal to the single-model batched path
+  (tbtkx_moments_block, same FP32 recurrence arithmetic; mu_n is read from the
+  vectors directly);
+- doubling mode agrees with the single-model doubling path to FP64 rounding of
+  the dot-product sums (1e-12 relative);
+- both agree with the FP64 oracle (oracle/kpm_oracle.c, kpm_moments) within
+  the FP32-storage tolerance of the batched path (2e-5 absolute on moments of
+  unit start vectors, |mu_n| <= 1);
+- autotuned, fixed-tile and hipGraph-replayed launches agree.
+"""
+import numpy as np
+import pytest
+
+from tbtk_amd import engine as X
+from tbtk_amd import lattice as L
+
+pytestmark = pytest.mark.gpu
+
+
+def small_batch():
+    """Models of uneven size, width and moment count, including a complex H
+    (square lattice with flux), a model smaller than one row group, widths 1
+    and 64, M = 1 and M = 2."""
+    rng = np.random.default_rng(11)
+    specs = [
+        (L.square(9, 7, disorder=0.5, seed=1), 3, 41),
+        (L.honeycomb_ribbon(4, 6, disorder=0.5, seed=2), 1, 120),
+        (L.square(6, 5, disorder=0.5, seed=3, flux=0.13), 7, 77),
+        (L.honeycomb_ribbon(12, 9, disorder=0.5, seed=4), 64, 64),
+        (L.square(3, 2, disorder=0.5, seed=5), 2, 2),
+        (L.square(30, 40, disorder=0.5, seed=6), 33, 150),
+        (L.square(5, 5, disorder=0.5, seed=7), 4, 1),
+        (L.honeycomb_ribbon(20, 30, disorder=0.5, seed=8), 17, 93),
+    ]
+    out = []
+    for model, B, M in specs:
+        sites = np.sort(rng.choice(model.basis_size, B, replace=False))
+        out.append((model, sites, M))
+    return out
+
+
+def real_batch():
+    return [b for b in small_batch() if not b[0].params.get("flux")]
+
+
+def upload(engine, batch):
+    csrs = [m.csr() for m, _, _ in batch]
+    scales = [m.gershgorin_bound() * 1.05 for m, _, _ in batch]
+    engine.multi_set(csrs, scales)
+    return csrs, scales
+
+
+def single_model(engine, csr, s, sites, M, flags):
+    engine.set_csr(csr, s)
+    return engine.moments_block(sites, M, flags)
+
+
+@pytest.mark.parametrize("which", ["complex", "real"])
+def test_multi_gather_bitwise_vs_block(engine, which):
+    batch = small_batch() if which == "complex" else real_batch()
+    csrs, scales = upload(engine, batch)
+    got = engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], 0)
+    for (m, sites, M), csr, s, g in zip(batch, csrs, scales, got):
+        ref = single_model(engine, csr, s, sites, M, 0)
+        assert g.shape == ref.shape
+        assert np.array_equal(g, ref), (m.name, m.basis_size, np.abs(g - ref).max())
+
+
+@pytest.mark.parametrize("which", ["complex", "real"])
+def test_multi_doubling_vs_block_and_oracle(engine, oracle_mod, which):
+    batch = small_batch() if which == "complex" else real_batch()
+    csrs, scales = upload(engine, batch)
+    got = engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], X.DOUBLING)
+    for (m, sites, M), csr, s, g in zip(batch, csrs, scales, got):
+        ref = single_model(engine, csr, s, sites, M, X.DOUBLING)
+        assert np.abs(g - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), m.name
+        exact = np.array([oracle_mod.moments(csr, s, b, [b], M)[0] for b in sites])
+        assert np.abs(g - exact).max() < 2e-5, (m.name, np.abs(g - exact).max())
+
+
+def test_multi_gather_vs_oracle(engine, oracle_mod):
+    batch = small_batch()
+    csrs, scales = upload(engine, batch)
+    got = engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], 0)
+    for (m, sites, M), csr, s, g in zip(batch, csrs, scales, got):
+        exact = np.array([oracle_mod.moments(csr, s, b, [b], M)[0] for b in sites])
+        assert np.abs(g - exact).max() < 2e-5, (m.name, np.abs(g - exact).max())
+        # mu_0 of a unit start vector is exactly 1
+        assert np.all(g[:, 0] == 1.0)
+
+
+@pytest.mark.parametrize("flags", [0, X.DOUBLING])
+def test_multi_launch_shapes_and_graph_agree(engine, flags):
+    batch = small_batch()
+    upload(engine, batch)
+    sites = [b[1] for b in batch]
+    Ms = [b[2] for b in batch]
+    base = engine.multi_moments(sites, Ms, flags)
+    results = []
+    try:
+        for tile in (2, 6, 16):
+            engine.set_option(X.OPT_TILE_ROWS, tile)
+            results.append(engine.multi_moments(sites, Ms, flags))
+        engine.set_option(X.OPT_TILE_ROWS, 0)
+        engine.set_option(X.OPT_GRAPH, 1)
+        for _ in range(3):
+            results.append(engine.multi_moments(sites, Ms, flags))
+    finally:
+        engine.set_option(X.OPT_TILE_ROWS, 0)
+        engine.set_option(X.OPT_GRAPH, 0)
+    for r in results:
+        for a, b in zip(r, base):
+            if flags == 0:
+                assert np.array_equal(a, b)
+            else:
+                assert np.abs(a - b).max() <= 1e-12
+
+
+def test_multi_graph_replays_with_eager_work_between(engine):
+    batch = small_batch()
+    csrs, scales = upload(engine, batch)
+    sites = [b[1] for b in batch]
+    Ms = [b[2] for b in batch]
+    eager = engine.multi_moments(sites, Ms, X.DOUBLING)
+    engine.set_option(X.OPT_GRAPH, 1)
+    try:
+        for _ in range(3):
+            g = engine.multi_moments(sites, Ms, X.DOUBLING)
+            for a, b in zip(g, eager):
+                assert np.abs(a - b).max() <= 1e-12
+            # unrelated eager work on the same handle between replays
+            engine.set_csr(csrs[0], scales[0])
+            engine.moments_block(sites[0], 9, 0)
+    finally:
+        engine.set_option(X.OPT_GRAPH, 0)
+
+
+def test_multi_rejects_bad_arguments(engine):
+    batch = small_batch()[:2]
+    upload(engine, batch)
+    with pytest.raises(X.EngineError, match="1..64 sites"):
+        engine.multi_moments([np.arange(65), [0]], [10, 10], 0)
+    with pytest.raises(X.EngineError, match="outside the basis"):
+        engine.multi_moments([[0], [batch[1][0].basis_size]], [10, 10], 0)
+    with pytest.raises(X.EngineError, match="larger than 0"):
+        engine.multi_moments([[0], [0]], [10, 0], 0)
+    with pytest.raises(X.EngineError, match="only TBTKX_DOUBLING"):
+        engine.multi_moments([[0], [0]], [10, 10], X.H_FP64)
+    dense = L.coo_to_csr(np.repeat(np.arange(10), 10), np.tile(np.arange(10), 10),
+                         np.ones(100, np.complex128), 10)
+    with pytest.raises(X.EngineError, match="more than 8"):
+        engine.multi_set([dense], [20.0])
+    with pytest.raises(X.EngineError, match="scale factor"):
+        engine.multi_set([batch[0][0].csr()], [0.0])
