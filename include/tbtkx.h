@@ -202,8 +202,11 @@ enum {
 	TBTKX_OPT_NT_STREAM = 6,
 	TBTKX_OPT_ELL = 7,      /* 1 = batched step on the tile-major ELL copy of H
 	                           when no row has more than 8 entries (default) */
-	TBTKX_OPT_AUTOTUNE = 8  /* 1 = time candidate launch shapes on first use
+	TBTKX_OPT_AUTOTUNE = 8, /* 1 = time candidate launch shapes on first use
 	                           (default); 0 = fixed defaults */
+	TBTKX_OPT_MFMA_PIPE = 9 /* 1 = MFMA contraction with double-buffered
+	                           direct-to-LDS operand loads (default); 0 = one
+	                           LDS stage loaded through registers */
 };
 int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
 

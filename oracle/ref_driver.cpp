@@ -5,13 +5,15 @@
 // baseline of bench.py. It is never part of the product path.
 //
 // Usage: ref_driver mode=<mode> key=value ...
-//   modes: csr | coef | gf | gfcoef | ldos | sampledos | timeldos
+//   modes: csr | coef | gf | gfcoef | ldos | sampledos | timeldos |
+//          density | mag | splodos | expval
 //   model=<file>        binary HoppingAmplitude list (format in
 //                       tbtk_amd/lattice.py: write_amplitudes)
 //   scale=, M=, broadening=, window=lo,hi,res, lookup=0|1, type=<GF type>
 //   from=i,j,k  to=i,j,k;i,j,k  patterns=i,j,+;...   ('*'=IDX_ALL,
-//   '+'=IDX_SUM_ALL, 's'=IDX_SPIN)  coefin=<file of complex doubles>
+//   '+'=IDX_SUM_ALL, 's'=IDX_SPIN, 'x','y','z'=IDX_X,IDX_Y,IDX_Z)  coefin=<file of complex doubles>
 //   out=<file or prefix>  Ms=100,200 (timeldos)  n=, seed= (sampledos)
+//   T=, mu= (occupation)  pattern=, ranges= (Ranges format)
 #include "TBTK/Model.h"
 #include "TBTK/PropertyExtractor/ChebyshevExpander.h"
 #include "TBTK/Solver/ChebyshevExpander.h"
@@ -63,6 +65,9 @@ static Index parseIndex(const string &s){
 		if(t == "*") sub.push_back(IDX_ALL);
 		else if(t == "+") sub.push_back(IDX_SUM_ALL);
 		else if(t == "s") sub.push_back(IDX_SPIN);
+		else if(t == "x") sub.push_back(IDX_X);
+		else if(t == "y") sub.push_back(IDX_Y);
+		else if(t == "z") sub.push_back(IDX_Z);
 		else sub.push_back(atoi(t.c_str()));
 	}
 	return Index(sub);
@@ -240,6 +245,48 @@ int main(int argc, char **argv){
 			for(int e = 0; e < res; e++)
 				flat.push_back(gf({t, from}, e));
 		writeBin(arg("out"), flat.data(), flat.size()*sizeof(complex<double>));
+		return 0;
+	}
+	// Thermodynamic-equilibrium properties: T= (K) and mu= (eV) set the
+	// Model's Fermi-Dirac occupation; pattern= with ranges= selects the
+	// Ranges format, patterns= the Custom format. Spin matrices are written
+	// as (0,0), (0,1), (1,0), (1,1) per entry.
+	model.setTemperature(atof(arg("T", "0").c_str()));
+	model.setChemicalPotential(atof(arg("mu", "0").c_str()));
+	auto writeSpin = [](const string &path, const vector<SpinMatrix> &d){
+		vector<complex<double>> flat;
+		for(const SpinMatrix &m : d)
+			for(int n = 0; n < 4; n++)
+				flat.push_back(m.at(n/2, n%2));
+		writeBin(path, flat.data(), flat.size()*sizeof(complex<double>));
+	};
+	if(mode == "density"){
+		Property::Density density = args.count("ranges")
+			? pe.calculateDensity(parseIndex(arg("pattern")), parseIndex(arg("ranges")))
+			: pe.calculateDensity(parseIndices(arg("patterns")));
+		const vector<double> &d = density.getData();
+		writeBin(arg("out"), d.data(), d.size()*sizeof(double));
+		return 0;
+	}
+	if(mode == "mag"){
+		Property::Magnetization mag = args.count("ranges")
+			? pe.calculateMagnetization(parseIndex(arg("pattern")), parseIndex(arg("ranges")))
+			: pe.calculateMagnetization(parseIndices(arg("patterns")));
+		writeSpin(arg("out"), mag.getData());
+		return 0;
+	}
+	if(mode == "splodos"){
+		Property::SpinPolarizedLDOS sp = args.count("ranges")
+			? pe.calculateSpinPolarizedLDOS(parseIndex(arg("pattern")), parseIndex(arg("ranges")))
+			: pe.calculateSpinPolarizedLDOS(parseIndices(arg("patterns")));
+		writeSpin(arg("out"), sp.getData());
+		return 0;
+	}
+	if(mode == "expval"){
+		complex<double> v = pe.calculateExpectationValue(
+			parseIndex(arg("to")), parseIndex(arg("from"))
+		);
+		writeBin(arg("out"), &v, sizeof(v));
 		return 0;
 	}
 	if(mode == "ldos"){

@@ -148,6 +148,7 @@ struct tbtkx_context {
 	int nt_stream = 1;     // non-temporal hints on once-used SpMM rows
 	int use_ell = 1;       // batched step on the ELL copy when it exists
 	int autotune = 1;      // time candidate launch shapes on first use
+	int mfma_pipe = 1;     // double-buffered direct-to-LDS MFMA contraction
 
 	tbtkx::GraphCache plain_graph;
 };

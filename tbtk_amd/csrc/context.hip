@@ -351,6 +351,9 @@ int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
 		h->autotune = value != 0;
 		for (auto &k : h->mm_tune_key) k = -1;
 		return TBTKX_OK;
+	case TBTKX_OPT_MFMA_PIPE:
+		h->mfma_pipe = value != 0;
+		return TBTKX_OK;
 	default:
 		set_error("tbtkx_set_option: unknown option " + std::to_string(option));
 		return TBTKX_ERR_ARG;
@@ -370,6 +373,7 @@ int tbtkx_get_option(tbtkx_handle h, int option, int64_t *value) {
 	case TBTKX_OPT_NT_STREAM: *value = h->nt_stream; return TBTKX_OK;
 	case TBTKX_OPT_ELL: *value = h->use_ell && h->ell_W > 0; return TBTKX_OK;
 	case TBTKX_OPT_AUTOTUNE: *value = h->autotune; return TBTKX_OK;
+	case TBTKX_OPT_MFMA_PIPE: *value = h->mfma_pipe; return TBTKX_OK;
 	default:
 		set_error("tbtkx_get_option: unknown option " + std::to_string(option));
 		return TBTKX_ERR_ARG;

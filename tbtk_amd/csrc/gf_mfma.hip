@@ -185,6 +185,165 @@ __global__ __launch_bounds__(kThreads) void contract_mfma(
 			}
 }
 
+// Pipelined variant. The operand tiles of chunk c+1 stream into the second
+// of two LDS stages by direct global->LDS loads (global_load_lds_dwordx4:
+// lane i of a wavefront writes 16 B at the wave-uniform LDS address + 16 i)
+// while the MFMAs of chunk c run, so no VGPRs hold staged data and the load
+// latency hides behind a whole chunk of MFMAs. LDS rows are 64 B (32 bf16,
+// no padding); the four 16 B quads of row r are stored XOR-swizzled by
+// (r >> 2) & 3, which makes the 16 lanes of each ds_read_b128 phase hit 16
+// distinct bank groups. 2 stages x 2 operands x 4 planes x 128 rows x 64 B =
+// 128 KB of LDS.
+constexpr int kRowB = 32;
+
+__device__ __forceinline__ int swz(int row, int quad) { return quad ^ ((row >> 2) & 3); }
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int TYPE>
+__global__ __launch_bounds__(kThreads) void contract_mfma_pipe(
+    const uint16_t *__restrict__ Cp, const uint16_t *__restrict__ Lp,
+    float2 *__restrict__ G, int K, int E, int Mp, int ntile_e) {
+	__shared__ __attribute__((aligned(16))) uint16_t sm[2][2][4][kTileK * kRowB];
+	const int tid = threadIdx.x;
+	const int wave = tid >> 6, lane = tid & 63;
+	const int tk = blockIdx.x / ntile_e, te = blockIdx.x - tk * ntile_e;
+	const int k0 = tk * kTileK, e0 = te * kTileE;
+	const int wr = wave >> 1, wc = wave & 1;   // 2 x 2 wavefronts
+	const int64_t planeC = (int64_t)K * Mp, planeL = (int64_t)E * Mp;
+	const int nc = Mp / kChunk;
+
+	// A stage is 64 loads of 1 KB (operand, plane, 16 rows); wavefront w
+	// issues loads w, w + 4, ..., w + 60. Lane l fetches row 16 (j & 7) + l / 4,
+	// the quad that belongs in LDS slot l % 4. Rows past K or E re-read the
+	// last valid row (their results are never stored).
+	const int lrow = lane >> 2;
+	auto issue = [&](int c, int buf) {
+		const int m0 = c * kChunk;
+#pragma unroll
+		for (int i = 0; i < 16; i++) {
+			const int j = wave + 4 * i;
+			const int op = i >= 8;
+			const int p = (j >> 3) & 3, rb = 16 * (j & 7);
+			const int r = rb + lrow;
+			const int q = swz(r, lane & 3);
+			const uint16_t *src;
+			if (!op) {
+				const int row = k0 + r < K ? k0 + r : K - 1;
+				src = Cp + p * planeC + (int64_t)row * Mp + m0 + q * 8;
+			} else {
+				const int row = e0 + r < E ? e0 + r : E - 1;
+				src = Lp + p * planeL + (int64_t)row * Mp + m0 + q * 8;
+			}
+			__builtin_amdgcn_global_load_lds((const void *)src,
+			                                 (lds_void *)&sm[buf][op][p][rb * kRowB], 16, 0, 0);
+		}
+	};
+
+	f32x16 accR[2][2], accI[2][2];
+#pragma unroll
+	for (int a = 0; a < 2; a++)
+#pragma unroll
+		for (int b = 0; b < 2; b++) {
+			accR[a][b] = (f32x16){};
+			accI[a][b] = (f32x16){};
+		}
+
+	issue(0, 0);
+	for (int c = 0; c < nc; c++) {
+		// This wavefront's loads of chunk c are done; the barrier makes every
+		// wavefront's visible and tells that all finished reading the stage
+		// the next loads overwrite (chunk c - 1's).
+		__builtin_amdgcn_s_waitcnt(0);
+		__syncthreads();
+		if (c + 1 < nc) issue(c + 1, (c + 1) & 1);
+		const int buf = c & 1;
+#pragma unroll
+		for (int ks = 0; ks < kChunk; ks += 16) {
+			const int kq = ks / 8 + (lane >> 5);
+			bf16x8 A[4][2], Bm[4][2];
+#pragma unroll
+			for (int p = 0; p < 4; p++)
+#pragma unroll
+				for (int t = 0; t < 2; t++) {
+					const int ra = wr * 64 + t * 32 + (lane & 31);
+					const int rb = wc * 64 + t * 32 + (lane & 31);
+					A[p][t] = *reinterpret_cast<const bf16x8 *>(
+					    &sm[buf][0][p][ra * kRowB + swz(ra, kq) * 8]);
+					Bm[p][t] = *reinterpret_cast<const bf16x8 *>(
+					    &sm[buf][1][p][rb * kRowB + swz(rb, kq) * 8]);
+				}
+			if (TYPE == TBTKX_RETARDED || TYPE == TBTKX_ADVANCED) {
+				// L' = (br, +-bi): re += ar br - ai bi', im += ar bi' + ai br
+				bf16x8 sih[2], sil[2], naih[2], nail[2];
+#pragma unroll
+				for (int t = 0; t < 2; t++) {
+					sih[t] = TYPE == TBTKX_RETARDED ? Bm[2][t] : neg(Bm[2][t]);
+					sil[t] = TYPE == TBTKX_RETARDED ? Bm[3][t] : neg(Bm[3][t]);
+					naih[t] = neg(A[2][t]);
+					nail[t] = neg(A[3][t]);
+				}
+				// Term by term over the eight independent accumulators.
+#define TBTKX_MM(ACC, X, Y)                                                     \
+	_Pragma("unroll") for (int a = 0; a < 2; a++)                          \
+	    _Pragma("unroll") for (int b = 0; b < 2; b++) ACC[a][b] = mma(X, Y, ACC[a][b]);
+				TBTKX_MM(accR, A[0][a], Bm[0][b])
+				TBTKX_MM(accI, A[0][a], sih[b])
+				TBTKX_MM(accR, A[0][a], Bm[1][b])
+				TBTKX_MM(accI, A[0][a], sil[b])
+				TBTKX_MM(accR, A[1][a], Bm[0][b])
+				TBTKX_MM(accI, A[1][a], sih[b])
+				TBTKX_MM(accR, naih[a], sih[b])
+				TBTKX_MM(accI, A[2][a], Bm[0][b])
+				TBTKX_MM(accR, naih[a], sil[b])
+				TBTKX_MM(accI, A[2][a], Bm[1][b])
+				TBTKX_MM(accR, nail[a], sih[b])
+				TBTKX_MM(accI, A[3][a], Bm[0][b])
+			} else if (TYPE == TBTKX_PRINCIPAL) {
+				// L' = -Re L: re += ar (-br), im += ai (-br)
+				bf16x8 nbh[2], nbl[2];
+#pragma unroll
+				for (int t = 0; t < 2; t++) {
+					nbh[t] = neg(Bm[0][t]);
+					nbl[t] = neg(Bm[1][t]);
+				}
+				TBTKX_MM(accR, A[0][a], nbh[b])
+				TBTKX_MM(accI, A[2][a], nbh[b])
+				TBTKX_MM(accR, A[0][a], nbl[b])
+				TBTKX_MM(accI, A[2][a], nbl[b])
+				TBTKX_MM(accR, A[1][a], nbh[b])
+				TBTKX_MM(accI, A[3][a], nbh[b])
+			} else {
+				// L' = -i Im L = (0, -bi): re += ai bi, im += -ar bi
+				bf16x8 nbh[2], nbl[2];
+#pragma unroll
+				for (int t = 0; t < 2; t++) {
+					nbh[t] = neg(Bm[2][t]);
+					nbl[t] = neg(Bm[3][t]);
+				}
+				TBTKX_MM(accR, A[2][a], Bm[2][b])
+				TBTKX_MM(accI, A[0][a], nbh[b])
+				TBTKX_MM(accR, A[2][a], Bm[3][b])
+				TBTKX_MM(accI, A[0][a], nbl[b])
+				TBTKX_MM(accR, A[3][a], Bm[2][b])
+				TBTKX_MM(accI, A[1][a], nbh[b])
+#undef TBTKX_MM
+			}
+		}
+	}
+#pragma unroll
+	for (int a = 0; a < 2; a++)
+#pragma unroll
+		for (int b = 0; b < 2; b++)
+#pragma unroll
+			for (int r = 0; r < 16; r++) {
+				const int row = k0 + wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+				const int col = e0 + wc * 64 + b * 32 + (lane & 31);
+				if (row < K && col < E)
+					G[(int64_t)row * E + col] = make_float2(accR[a][b][r], accI[a][b][r]);
+			}
+}
+
 }  // namespace
 }  // namespace tbtkx
 
@@ -239,20 +398,20 @@ int tbtkx_contract_mfma(tbtkx_handle h, int type, double *out) {
 	const uint16_t *Cp = h->cplanes.as<uint16_t>(), *Lp = h->lplanes.as<uint16_t>();
 	float2 *G = h->gf_mfma_out.as<float2>();
 	TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
+#define TBTKX_CM(T)                                                              \
+	do {                                                                     \
+		if (h->mfma_pipe)                                                \
+			contract_mfma_pipe<T><<<grid, kThreads, 0, h->stream>>>(Cp, Lp, G, K, E, Mp, nte); \
+		else                                                             \
+			contract_mfma<T><<<grid, kThreads, 0, h->stream>>>(Cp, Lp, G, K, E, Mp, nte); \
+	} while (0)
 	switch (type) {
-	case TBTKX_RETARDED:
-		contract_mfma<TBTKX_RETARDED><<<grid, kThreads, 0, h->stream>>>(Cp, Lp, G, K, E, Mp, nte);
-		break;
-	case TBTKX_ADVANCED:
-		contract_mfma<TBTKX_ADVANCED><<<grid, kThreads, 0, h->stream>>>(Cp, Lp, G, K, E, Mp, nte);
-		break;
-	case TBTKX_PRINCIPAL:
-		contract_mfma<TBTKX_PRINCIPAL><<<grid, kThreads, 0, h->stream>>>(Cp, Lp, G, K, E, Mp, nte);
-		break;
-	default:
-		contract_mfma<TBTKX_NONPRINCIPAL><<<grid, kThreads, 0, h->stream>>>(Cp, Lp, G, K, E, Mp, nte);
-		break;
+	case TBTKX_RETARDED: TBTKX_CM(TBTKX_RETARDED); break;
+	case TBTKX_ADVANCED: TBTKX_CM(TBTKX_ADVANCED); break;
+	case TBTKX_PRINCIPAL: TBTKX_CM(TBTKX_PRINCIPAL); break;
+	default: TBTKX_CM(TBTKX_NONPRINCIPAL); break;
 	}
+#undef TBTKX_CM
 	TBTKX_CHECK_HIP(hipGetLastError());
 	TBTKX_CHECK_HIP(hipEventRecord(h->ev_stop, h->stream));
 	TBTKX_CHECK_HIP(hipEventSynchronize(h->ev_stop));

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libtbtkx.so")
 ADVANCED, RETARDED, PRINCIPAL, NONPRINCIPAL = 0, 1, 2, 3
 H_FP64, DOUBLING, VEC_FP64 = 1, 2, 4
 (OPT_PLAIN_KERNEL, OPT_GRAPH, OPT_BLOCK_WIDTH, OPT_XCD_REMAP, OPT_TILE_ROWS, OPT_NT_STREAM,
- OPT_ELL, OPT_AUTOTUNE) = 1, 2, 3, 4, 5, 6, 7, 8
+ OPT_ELL, OPT_AUTOTUNE, OPT_MFMA_PIPE) = 1, 2, 3, 4, 5, 6, 7, 8, 9
 
 # (name, argtypes) of every exported function, in header order.
 _P = ctypes.c_void_p

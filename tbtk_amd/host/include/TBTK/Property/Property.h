@@ -1,5 +1,5 @@
-/* Energy-resolved property containers the Chebyshev path returns:
- * GreensFunction, LDOS and DOS. Memory layout = (linear index of the
+/* Property containers the Chebyshev path returns: GreensFunction, LDOS,
+ * DOS, Density, Magnetization and SpinPolarizedLDOS. Memory layout = (linear index of the
  * IndexTree entry, or the Ranges-format offset) x energy resolution, as
  * Lib/include/TBTK/Property/AbstractProperty.h:107-147 and
  * EnergyResolvedProperty.h:914-932 (getDeltaE). */
@@ -8,6 +8,7 @@
 
 #include "TBTK/IndexTree.h"
 #include "TBTK/Range.h"
+#include "TBTK/SpinMatrix.h"
 #include "TBTK/TBTKMacros.h"
 
 #include <complex>
@@ -120,6 +121,32 @@ public:
 class DOS : public EnergyResolvedProperty<double> {
 public:
 	explicit DOS(const Range &window) : EnergyResolvedProperty<double>(window) {}
+};
+
+/* Occupation per index (one double per block), as
+ * Lib/include/TBTK/Property/Density.h. */
+class Density : public AbstractProperty<double> {
+public:
+	explicit Density(const std::vector<int> &ranges) : AbstractProperty<double>(ranges, 1) {}
+	explicit Density(const IndexTree &tree) : AbstractProperty<double>(tree, 1) {}
+};
+
+/* Spin density matrix per index, as Lib/include/TBTK/Property/Magnetization.h. */
+class Magnetization : public AbstractProperty<SpinMatrix> {
+public:
+	explicit Magnetization(const std::vector<int> &ranges)
+	    : AbstractProperty<SpinMatrix>(ranges, 1) {}
+	explicit Magnetization(const IndexTree &tree) : AbstractProperty<SpinMatrix>(tree, 1) {}
+};
+
+/* Energy-resolved spin density matrix per index, as
+ * Lib/include/TBTK/Property/SpinPolarizedLDOS.h. */
+class SpinPolarizedLDOS : public EnergyResolvedProperty<SpinMatrix> {
+public:
+	SpinPolarizedLDOS(const std::vector<int> &ranges, const Range &window)
+	    : EnergyResolvedProperty<SpinMatrix>(ranges, window) {}
+	SpinPolarizedLDOS(const IndexTree &tree, const Range &window)
+	    : EnergyResolvedProperty<SpinMatrix>(tree, window) {}
 };
 
 }  // namespace Property

@@ -91,6 +91,21 @@ public:
 		stochasticSeed = seed;
 	}
 
+	/* Thermodynamic-equilibrium properties (Lib/src/PropertyExtractor/
+	 * ChebyshevExpander.cpp:297-453, 507-717): the NonPrincipal Green's
+	 * functions weighted by the Model's occupation function
+	 * (getStatistics(), getChemicalPotential(), getTemperature()) and
+	 * summed over the energy window with dE = window[1] - window[0], in the
+	 * reference's order of operations. IDX_SPIN marks the spin subindex of
+	 * Magnetization and SpinPolarizedLDOS patterns. */
+	std::complex<double> calculateExpectationValue(Index to, Index from);
+	Property::Density calculateDensity(Index pattern, Index ranges);
+	Property::Density calculateDensity(std::vector<Index> patterns);
+	Property::Magnetization calculateMagnetization(Index pattern, Index ranges);
+	Property::Magnetization calculateMagnetization(std::vector<Index> patterns);
+	Property::SpinPolarizedLDOS calculateSpinPolarizedLDOS(Index pattern, Index ranges);
+	Property::SpinPolarizedLDOS calculateSpinPolarizedLDOS(std::vector<Index> patterns);
+
 	Solver::ChebyshevExpander &getSolver() { return *solver; }
 
 private:
@@ -104,6 +119,22 @@ private:
 	 * of `ldos` given by the offsets. */
 	void accumulateLDOS(const std::vector<Index> &indices,
 	                    const std::vector<int> &offsets, Property::LDOS &ldos);
+	static void rangesIndices(Index pattern, Index ranges, int blockSize,
+	                          std::vector<Index> &indices, std::vector<int> &offsets);
+	static std::vector<int> loopRanges(Index pattern, Index ranges);
+	double occupation(double energy) const;
+	std::vector<std::vector<std::complex<double>>> diagonalGreensFunctions(
+	    const std::vector<Index> &indices);
+	void spinGreensFunctions(const Index &index, int spinIndex,
+	                         std::vector<std::complex<double>> (&g)[4]);
+	void accumulateDensity(const std::vector<Index> &indices,
+	                       const std::vector<int> &offsets, Property::Density &density);
+	void accumulateMagnetization(const std::vector<Index> &indices,
+	                             const std::vector<int> &offsets, int spinIndex,
+	                             Property::Magnetization &magnetization);
+	void accumulateSpinPolarizedLDOS(const std::vector<Index> &indices,
+	                                 const std::vector<int> &offsets, int spinIndex,
+	                                 Property::SpinPolarizedLDOS &splodos);
 };
 
 }  // namespace PropertyExtractor

@@ -1,6 +1,7 @@
 // Drop-in PropertyExtractor::ChebyshevExpander and the PropertyExtractor
 // base behaviour it uses.
 #include "TBTK/PropertyExtractor/ChebyshevExpander.h"
+#include "TBTK/Functions.h"
 #include "TBTK/TBTKMacros.h"
 
 #include <cmath>
@@ -271,19 +272,16 @@ Property::LDOS ChebyshevExpander::calculateLDOS(std::vector<Index> patterns) {
 	return ldos;
 }
 
-// Ranges format: PropertyExtractor::calculate, Lib/include/TBTK/
-// PropertyExtractor/PropertyExtractor.h:780-847, with ensureCompliantRanges
-// and getLoopRanges (PropertyExtractor.cpp:292-321).
-Property::LDOS ChebyshevExpander::calculateLDOS(Index pattern, Index ranges) {
+// Ranges format: the index walk of PropertyExtractor::calculate,
+// Lib/include/TBTK/PropertyExtractor/PropertyExtractor.h:780-847, after
+// ensureCompliantRanges (PropertyExtractor.cpp:292-303): every concrete index
+// with its block offset, in the reference's order (last subindex first,
+// IDX_SUM_ALL summed into one block).
+void ChebyshevExpander::rangesIndices(Index pattern, Index ranges, int blockSize,
+                                      std::vector<Index> &indices,
+                                      std::vector<int> &offsets) {
 	for (unsigned int n = 0; n < pattern.getSize(); n++)
 		if (pattern.at(n) >= 0) ranges.at(n) = 1;
-	std::vector<int> loopRanges;
-	for (unsigned int n = 0; n < ranges.getSize(); n++)
-		if (pattern.at(n).isRangeIndex()) loopRanges.push_back(ranges.at(n));
-	Property::LDOS ldos(loopRanges, getEnergyWindow());
-
-	std::vector<Index> indices;
-	std::vector<int> offsets;
 	struct Walker {
 		const Index &ranges;
 		std::vector<Index> &indices;
@@ -310,7 +308,24 @@ Property::LDOS ChebyshevExpander::calculateLDOS(Index pattern, Index ranges) {
 			}
 		}
 	} walker{ranges, indices, offsets};
-	walker.walk(pattern, 0, (int)getEnergyWindow().getResolution());
+	walker.walk(pattern, 0, blockSize);
+}
+
+// getLoopRanges, Lib/src/PropertyExtractor/PropertyExtractor.cpp:305-321.
+std::vector<int> ChebyshevExpander::loopRanges(Index pattern, Index ranges) {
+	for (unsigned int n = 0; n < pattern.getSize(); n++)
+		if (pattern.at(n) >= 0) ranges.at(n) = 1;
+	std::vector<int> r;
+	for (unsigned int n = 0; n < ranges.getSize(); n++)
+		if (pattern.at(n).isRangeIndex()) r.push_back(ranges.at(n));
+	return r;
+}
+
+Property::LDOS ChebyshevExpander::calculateLDOS(Index pattern, Index ranges) {
+	Property::LDOS ldos(loopRanges(pattern, ranges), getEnergyWindow());
+	std::vector<Index> indices;
+	std::vector<int> offsets;
+	rangesIndices(pattern, ranges, (int)getEnergyWindow().getResolution(), indices, offsets);
 	accumulateLDOS(indices, offsets, ldos);
 	return ldos;
 }
@@ -350,6 +365,239 @@ Property::DOS ChebyshevExpander::calculateDOS() {
 			for (unsigned int n = 0; n < E; n++) data[n] += std::imag(gi[n]) / M_PI;
 	}
 	return dos;
+}
+
+// ----------------------------------------- thermodynamic-equilibrium weights
+
+// PropertyExtractor::getThermodynamicEquilibriumOccupation,
+// Lib/include/TBTK/PropertyExtractor/PropertyExtractor.h:910-934.
+double ChebyshevExpander::occupation(double energy) const {
+	const Model &model = solver->getModel();
+	switch (model.getStatistics()) {
+	case Statistics::FermiDirac:
+		return Functions::fermiDiracDistribution(energy, model.getChemicalPotential(),
+		                                         model.getTemperature());
+	case Statistics::BoseEinstein:
+		return Functions::boseEinsteinDistribution(energy, model.getChemicalPotential(),
+		                                           model.getTemperature());
+	}
+	TBTKExit("PropertyExtractor::getThermodynamicEquilibriumOccupation()",
+	         "Unknown statistics.", "This should never happen, contact the developer.");
+}
+
+// Diagonal NonPrincipal Green's functions of the given indices (block
+// recurrence when enabled, as accumulateLDOS).
+std::vector<std::vector<std::complex<double>>> ChebyshevExpander::diagonalGreensFunctions(
+    const std::vector<Index> &indices) {
+	std::vector<std::vector<std::complex<double>>> coefficients;
+	if (solver->getUseBlockRecurrence()) {
+		coefficients = solver->calculateCoefficientsBlock(indices);
+	} else {
+		coefficients.reserve(indices.size());
+		for (const Index &idx : indices)
+			coefficients.push_back(solver->calculateCoefficients(idx, idx));
+	}
+	return solver->generateGreensFunctions(coefficients,
+	                                       Solver::ChebyshevExpander::Type::NonPrincipal);
+}
+
+// The four NonPrincipal Green's functions G(to spin n/2, from spin n%2) of an
+// index, n = 0..3 (up-up, up-down, down-up, down-down). One recurrence per
+// from-spin gathers both to-spins; the moments of each pair are the ones
+// separate calls give.
+void ChebyshevExpander::spinGreensFunctions(const Index &index, int spinIndex,
+                                            std::vector<std::complex<double>> (&g)[4]) {
+	for (int c = 0; c < 2; c++) {
+		Index from(index);
+		from.at(spinIndex) = c;
+		std::vector<Index> to = {Index(index), Index(index)};
+		to[0].at(spinIndex) = 0;
+		to[1].at(spinIndex) = 1;
+		std::vector<std::vector<std::complex<double>>> gf = solver->generateGreensFunctions(
+		    solver->calculateCoefficients(to, from),
+		    Solver::ChebyshevExpander::Type::NonPrincipal);
+		g[0 * 2 + c] = gf[0];
+		g[1 * 2 + c] = gf[1];
+	}
+}
+
+// Lib/src/PropertyExtractor/ChebyshevExpander.cpp:297-327.
+std::complex<double> ChebyshevExpander::calculateExpectationValue(Index to, Index from) {
+	const std::complex<double> i(0, 1);
+	std::complex<double> expectationValue = 0.;
+	Property::GreensFunction greensFunction =
+	    calculateGreensFunction(to, from, Property::GreensFunction::Type::NonPrincipal);
+	const std::vector<std::complex<double>> &greensFunctionData = greensFunction.getData();
+	const Range &energyWindow = getEnergyWindow();
+	const double dE = (energyWindow[1] - energyWindow[0]);
+	for (unsigned int e = 0; e < energyWindow.getResolution(); e++) {
+		const double weight = occupation(energyWindow[e]);
+		expectationValue -= weight * std::conj(i * greensFunctionData[e]) * dE / M_PI;
+	}
+	return expectationValue;
+}
+
+// The density callback, Lib/src/PropertyExtractor/ChebyshevExpander.cpp:
+// 575-607: data[offset] += f(E) Im G_NonPrincipal(E)/pi dE, e ascending, in
+// the order of the indices.
+void ChebyshevExpander::accumulateDensity(const std::vector<Index> &indices,
+                                          const std::vector<int> &offsets,
+                                          Property::Density &density) {
+	std::vector<std::vector<std::complex<double>>> g = diagonalGreensFunctions(indices);
+	std::vector<double> &data = density.getDataRW();
+	const Range &energyWindow = getEnergyWindow();
+	const double dE = energyWindow[1] - energyWindow[0];
+	for (size_t k = 0; k < indices.size(); k++)
+		for (unsigned int e = 0; e < energyWindow.getResolution(); e++) {
+			const double weight = occupation(energyWindow[e]);
+			data[offsets[k]] += weight * std::imag(g[k][e]) / M_PI * dE;
+		}
+}
+
+Property::Density ChebyshevExpander::calculateDensity(Index pattern, Index ranges) {
+	Property::Density density(loopRanges(pattern, ranges));
+	std::vector<Index> indices;
+	std::vector<int> offsets;
+	rangesIndices(pattern, ranges, 1, indices, offsets);
+	accumulateDensity(indices, offsets, density);
+	return density;
+}
+
+// Lib/src/PropertyExtractor/ChebyshevExpander.cpp:352-379.
+Property::Density ChebyshevExpander::calculateDensity(std::vector<Index> patterns) {
+	const HoppingAmplitudeSet &has = solver->getModel().getHoppingAmplitudeSet();
+	IndexTree allIndices = generateIndexTree(patterns, has, false, false);
+	IndexTree layout = generateIndexTree(patterns, has, true, true);
+	Property::Density density(layout);
+	std::vector<Index> indices;
+	std::vector<int> offsets;
+	for (const Index &idx : allIndices) {
+		indices.push_back(idx);
+		offsets.push_back(density.getOffset(idx));
+	}
+	accumulateDensity(indices, offsets, density);
+	return density;
+}
+
+// The magnetization callback, Lib/src/PropertyExtractor/ChebyshevExpander.
+// cpp:609-647: data[offset].at(n/2, n%2) += f(E) (-i) G_n(E)/pi dE.
+void ChebyshevExpander::accumulateMagnetization(const std::vector<Index> &indices,
+                                                const std::vector<int> &offsets,
+                                                int spinIndex,
+                                                Property::Magnetization &magnetization) {
+	const std::complex<double> i(0, 1);
+	std::vector<SpinMatrix> &data = magnetization.getDataRW();
+	const Range &energyWindow = getEnergyWindow();
+	const double dE = energyWindow[1] - energyWindow[0];
+	for (size_t k = 0; k < indices.size(); k++) {
+		std::vector<std::complex<double>> g[4];
+		spinGreensFunctions(indices[k], spinIndex, g);
+		for (int n = 0; n < 4; n++)
+			for (unsigned int e = 0; e < energyWindow.getResolution(); e++) {
+				const double weight = occupation(energyWindow[e]);
+				data[offsets[k]].at(n / 2, n % 2) += weight * (-i) * g[n][e] / M_PI * dE;
+			}
+	}
+}
+
+// The spin-polarized LDOS callback, Lib/src/PropertyExtractor/
+// ChebyshevExpander.cpp:686-717: data[offset + e].at(n/2, n%2) += -i G_n(E)/pi.
+void ChebyshevExpander::accumulateSpinPolarizedLDOS(const std::vector<Index> &indices,
+                                                    const std::vector<int> &offsets,
+                                                    int spinIndex,
+                                                    Property::SpinPolarizedLDOS &splodos) {
+	const std::complex<double> i(0, 1);
+	std::vector<SpinMatrix> &data = splodos.getDataRW();
+	const unsigned int E = getEnergyWindow().getResolution();
+	for (size_t k = 0; k < indices.size(); k++) {
+		std::vector<std::complex<double>> g[4];
+		spinGreensFunctions(indices[k], spinIndex, g);
+		for (int n = 0; n < 4; n++)
+			for (unsigned int e = 0; e < E; e++)
+				data[offsets[k] + e].at(n / 2, n % 2) += -i * g[n][e] / M_PI;
+	}
+}
+
+// Ranges format with IDX_SPIN: the spin subindex is fixed to 0 in the walk
+// and set by the callback (Lib/src/PropertyExtractor/ChebyshevExpander.cpp:
+// 381-412, 507-546).
+static int takeSpinIndex(Index &pattern, Index &ranges, const char *function) {
+	for (unsigned int n = 0; n < pattern.getSize(); n++)
+		if (pattern.at(n).isSpinIndex()) {
+			pattern.at(n) = 0;
+			ranges.at(n) = 1;
+			return (int)n;
+		}
+	TBTKExit(function, "No spin index indicated.",
+	         "Use IDX_SPIN to indicate the position of the spin index.");
+}
+
+// Custom format: the position of IDX_SPIN in an index of allIndices (kept as
+// a wildcard), as PropertyExtractor::calculate finds it in the memory layout
+// (PropertyExtractor.h:849-894).
+static int spinPosition(const Index &index, const char *function) {
+	int pos = -1;
+	for (unsigned int n = 0; n < index.getSize(); n++)
+		if (index.at(n).isSpinIndex()) {
+			TBTKAssert(pos == -1, "PropertyExtractor::calculate()", "Several spin indeces found.",
+			           "Use IDX_SPIN at most once per pattern to indicate spin index.");
+			pos = (int)n;
+		}
+	if (pos == -1)
+		TBTKExit(function, "No spin index indicated.",
+		         "Use IDX_SPIN to indicate the position of the spin index.");
+	return pos;
+}
+
+Property::Magnetization ChebyshevExpander::calculateMagnetization(Index pattern, Index ranges) {
+	const int spinIndex = takeSpinIndex(
+	    pattern, ranges, "PropertyExtractor::ChebyshevExpander::calculateMagnetization()");
+	Property::Magnetization magnetization(loopRanges(pattern, ranges));
+	std::vector<Index> indices;
+	std::vector<int> offsets;
+	rangesIndices(pattern, ranges, 1, indices, offsets);
+	accumulateMagnetization(indices, offsets, spinIndex, magnetization);
+	return magnetization;
+}
+
+Property::Magnetization ChebyshevExpander::calculateMagnetization(std::vector<Index> patterns) {
+	const HoppingAmplitudeSet &has = solver->getModel().getHoppingAmplitudeSet();
+	IndexTree allIndices = generateIndexTree(patterns, has, false, true);
+	IndexTree layout = generateIndexTree(patterns, has, true, true);
+	Property::Magnetization magnetization(layout);
+	for (const Index &idx : allIndices) {
+		const int spinIndex =
+		    spinPosition(idx, "PropertyExtractor::ChebyshevExpander::calculateMagnetization()");
+		accumulateMagnetization({idx}, {magnetization.getOffset(idx)}, spinIndex,
+		                        magnetization);
+	}
+	return magnetization;
+}
+
+Property::SpinPolarizedLDOS ChebyshevExpander::calculateSpinPolarizedLDOS(Index pattern,
+                                                                           Index ranges) {
+	const int spinIndex = takeSpinIndex(
+	    pattern, ranges, "PropertyExtractor::ChebsyhevExpander::calculateSpinPolarizedLDOS()");
+	Property::SpinPolarizedLDOS splodos(loopRanges(pattern, ranges), getEnergyWindow());
+	std::vector<Index> indices;
+	std::vector<int> offsets;
+	rangesIndices(pattern, ranges, (int)getEnergyWindow().getResolution(), indices, offsets);
+	accumulateSpinPolarizedLDOS(indices, offsets, spinIndex, splodos);
+	return splodos;
+}
+
+Property::SpinPolarizedLDOS ChebyshevExpander::calculateSpinPolarizedLDOS(
+    std::vector<Index> patterns) {
+	const HoppingAmplitudeSet &has = solver->getModel().getHoppingAmplitudeSet();
+	IndexTree allIndices = generateIndexTree(patterns, has, false, true);
+	IndexTree layout = generateIndexTree(patterns, has, true, true);
+	Property::SpinPolarizedLDOS splodos(layout, getEnergyWindow());
+	for (const Index &idx : allIndices) {
+		const int spinIndex = spinPosition(
+		    idx, "PropertyExtractor::ChebsyhevExpander::calculateSpinPolarizedLDOS()");
+		accumulateSpinPolarizedLDOS({idx}, {splodos.getOffset(idx)}, spinIndex, splodos);
+	}
+	return splodos;
 }
 
 }  // namespace PropertyExtractor

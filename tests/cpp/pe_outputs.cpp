@@ -115,6 +115,40 @@ int main(int argc, char **argv) {
 		for (const Index &ti : std::vector<Index>{{15, 15, 0}, {16, 15, 0}})
 			for (int e = 0; e < 300; e++) out.push_back(gp({ti, {15, 15, 0}}, e));
 		write("square30_spin_gfpattern_NonPrincipal", out);
+
+		// Thermodynamic-equilibrium properties at mu = 0.3, T = 300 K and 0.
+		auto spins = [](const std::vector<SpinMatrix> &d) {
+			std::vector<complex<double>> flat;
+			for (const SpinMatrix &m : d)
+				for (int n = 0; n < 4; n++) flat.push_back(m.at(n / 2, n % 2));
+			return flat;
+		};
+		model.setChemicalPotential(0.3);
+		for (double T : {300.0, 0.0}) {
+			model.setTemperature(T);
+			const std::string tag = T == 0 ? "T0" : "T300";
+			write("thermo_density_custom_" + tag,
+			      pe.calculateDensity({{15, 15, IDX_SUM_ALL}, {3, 4, IDX_ALL}, {0, 0, 1}})
+			          .getData());
+			write("thermo_density_ranges_" + tag,
+			      pe.calculateDensity({IDX_X, 15, IDX_SUM_ALL}, {4, 1, 2}).getData());
+			write("thermo_mag_custom_" + tag,
+			      spins(pe.calculateMagnetization({{15, 15, IDX_SPIN}, {2, 3, IDX_SPIN}})
+			                .getData()));
+			write("thermo_mag_ranges_" + tag,
+			      spins(pe.calculateMagnetization({IDX_X, 15, IDX_SPIN}, {3, 1, 2}).getData()));
+			write("thermo_expval_offdiag_" + tag,
+			      std::vector<complex<double>>{
+			          pe.calculateExpectationValue({15, 15, 0}, {16, 15, 0})});
+			write("thermo_expval_diag_" + tag,
+			      std::vector<complex<double>>{
+			          pe.calculateExpectationValue({15, 15, 1}, {15, 15, 1})});
+		}
+		model.setTemperature(300);
+		write("thermo_splodos_custom",
+		      spins(pe.calculateSpinPolarizedLDOS({{15, 15, IDX_SPIN}}).getData()));
+		write("thermo_splodos_ranges",
+		      spins(pe.calculateSpinPolarizedLDOS({IDX_X, 14, IDX_SPIN}, {2, 1, 2}).getData()));
 	}
 
 	{	// periodic5: known-answer model, M = 100, broadening 0

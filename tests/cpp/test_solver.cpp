@@ -7,6 +7,7 @@
 //           Test/include/TBTK/Test/PropertyExtractor/ChebyshevExpander.h:19-45)
 #include "minitest.h"
 
+#include "TBTK/Functions.h"
 #include "TBTK/Model.h"
 #include "TBTK/PropertyExtractor/ChebyshevExpander.h"
 #include "TBTK/Solver/ChebyshevExpander.h"
@@ -173,6 +174,44 @@ TEST(Api, propertyExtractorWindow) {
 	EXPECT_EXIT_CODE({
 		Streams::setStdMuteErr();
 		pe.setEnergyWindow(-10, 10, 100);
+	}, 1);
+}
+
+TEST(Api, spinPropertiesNeedSpinIndex) {
+	Model model = periodicSquare(5, -2, 1);
+	Solver::ChebyshevExpander solver;
+	solver.setModel(model);
+	solver.setScaleFactor(10);
+	PropertyExtractor::ChebyshevExpander pe(solver);
+	pe.setEnergyWindow(-9, 9, 100);
+	// No IDX_SPIN: the reference exits before any computation
+	// (Lib/src/PropertyExtractor/ChebyshevExpander.cpp:393-399, 520-526).
+	EXPECT_EXIT_CODE({
+		Streams::setStdMuteErr();
+		pe.calculateMagnetization({IDX_X, 0}, {5, 1});
+	}, 1);
+	EXPECT_EXIT_CODE({
+		Streams::setStdMuteErr();
+		pe.calculateSpinPolarizedLDOS({IDX_X, 0}, {5, 1});
+	}, 1);
+}
+
+TEST(Api, occupationFunctions) {
+	// k_B in eV/K as the reference derives it (Constants.cpp:38).
+	EXPECT_TRUE(Functions::boltzmannConstant() == 1.380649e-23 / 1.602176634e-19);
+	EXPECT_TRUE(Functions::fermiDiracDistribution(0.2, 0.3, 0) == 1.);
+	EXPECT_TRUE(Functions::fermiDiracDistribution(0.3, 0.3, 0) == 0.5);
+	EXPECT_TRUE(Functions::fermiDiracDistribution(0.4, 0.3, 0) == 0.);
+	EXPECT_TRUE(Functions::fermiDiracDistribution(0.3, 0.3, 300) == 0.5);
+	// f(mu + x) + f(mu - x) = 1 and the Bose-Einstein function's pole
+	const double x = 0.0123, T = 250;
+	const double s = Functions::fermiDiracDistribution(0.3 + x, 0.3, T) +
+	                 Functions::fermiDiracDistribution(0.3 - x, 0.3, T);
+	EXPECT_TRUE(std::abs(s - 1) < 1e-15);
+	EXPECT_TRUE(Functions::boseEinsteinDistribution(0.5, 0.3, 300) > 0);
+	EXPECT_EXIT_CODE({
+		Streams::setStdMuteErr();
+		Functions::boseEinsteinDistribution(0.5, 0.3, 0);
 	}, 1);
 }
 

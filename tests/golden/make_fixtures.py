@@ -72,6 +72,45 @@ def moments_fixture(name, model, scale, M, frm, tos, broadenings=(0.0,),
     return path
 
 
+def thermo_fixture(p1):
+    """Thermodynamic-equilibrium properties (Fermi-Dirac weights) on the
+    30x30 spin model (amplitude file p1): Density, Magnetization and
+    SpinPolarizedLDOS in the Custom and the Ranges formats, expectation
+    values; T = 300 K and T = 0, mu = 0.3."""
+    th = {}
+    base = dict(model=p1, scale=10.0, M=400, broadening=1e-6, window="-9.5,9.5,300",
+                lookup=1, mu=0.3)
+    for T in (300.0, 0.0):
+        tag = f"T{T:g}"
+        out = os.path.join(TMP, "th.bin")
+        O.run_ref(mode="density", patterns="15,15,+;3,4,*;0,0,1", T=T, out=out, **base)
+        th[f"density_custom_{tag}"] = np.fromfile(out, np.float64)
+        O.run_ref(mode="density", pattern="x,15,+", ranges="4,1,2", T=T, out=out, **base)
+        th[f"density_ranges_{tag}"] = np.fromfile(out, np.float64)
+        O.run_ref(mode="mag", patterns="15,15,s;2,3,s", T=T, out=out, **base)
+        th[f"mag_custom_{tag}"] = np.fromfile(out, np.complex128)
+        O.run_ref(mode="mag", pattern="x,15,s", ranges="3,1,2", T=T, out=out, **base)
+        th[f"mag_ranges_{tag}"] = np.fromfile(out, np.complex128)
+        O.run_ref(mode="expval", to="15,15,0", **{"from": "16,15,0"}, T=T, out=out, **base)
+        th[f"expval_offdiag_{tag}"] = np.fromfile(out, np.complex128)
+        O.run_ref(mode="expval", to="15,15,1", **{"from": "15,15,1"}, T=T, out=out, **base)
+        th[f"expval_diag_{tag}"] = np.fromfile(out, np.complex128)
+    out = os.path.join(TMP, "th.bin")
+    O.run_ref(mode="splodos", patterns="15,15,s", T=300.0, out=out, **base)
+    th["splodos_custom"] = np.fromfile(out, np.complex128)
+    O.run_ref(mode="splodos", pattern="x,14,s", ranges="2,1,2", T=300.0, out=out, **base)
+    th["splodos_ranges"] = np.fromfile(out, np.complex128)
+    save("square30_spin_thermo",
+         dict(scale=10.0, M=400, broadening=1e-6, window=[-9.5, 9.5, 300], lookup=1,
+              mu=0.3, T=[300.0, 0.0],
+              density_custom="15,15,+;3,4,*;0,0,1", density_ranges=["x,15,+", "4,1,2"],
+              mag_custom="15,15,s;2,3,s", mag_ranges=["x,15,s", "3,1,2"],
+              expval=[["15,15,0", "16,15,0"], ["15,15,1", "15,15,1"]],
+              splodos_custom="15,15,s", splodos_ranges=["x,14,s", "2,1,2"]),
+         **th)
+
+
+
 def main():
     if not O.ref_available():
         sys.exit("build oracle/_ref/ref_driver first (make -f oracle/Makefile.ref)")
@@ -105,6 +144,8 @@ def main():
               broadening=1e-6, gf=dict(M=400, window=[-9.5, 9.5, 300],
                                       frm=[15, 15, 0], to=[[15, 15, 0], [16, 15, 0]])),
          **ld)
+
+    thermo_fixture(p1)
 
     # Green's functions from given coefficients (no model): c = {1,2,3} on
     # Range(-5,5,10) as in Test/.../Solver/ChebyshevExpander.h:371-565, and a

@@ -88,3 +88,19 @@ def test_ring_ldos_and_sampledos(outputs):
         assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), f
     fx = load_fixture("periodic5_sampledos")
     assert np.array_equal(_read(outputs, "periodic5_sampledos", np.float64), fx["dos"])
+
+
+def test_thermodynamic_properties_bitexact(outputs):
+    """Density, Magnetization, SpinPolarizedLDOS (Custom and Ranges formats)
+    and expectation values at T = 300 K and T = 0 equal the reference's bit
+    for bit (tests/golden/square30_spin_thermo.npz)."""
+    fx = load_fixture("square30_spin_thermo")
+    for T in ("T300", "T0"):
+        for name, dt in (("density_custom", np.float64), ("density_ranges", np.float64),
+                         ("mag_custom", np.complex128), ("mag_ranges", np.complex128),
+                         ("expval_offdiag", np.complex128), ("expval_diag", np.complex128)):
+            got = _read(outputs, f"thermo_{name}_{T}", dt)
+            assert np.array_equal(got, fx[f"{name}_{T}"]), (name, T)
+    for name in ("splodos_custom", "splodos_ranges"):
+        got = _read(outputs, f"thermo_{name}", np.complex128)
+        assert np.array_equal(got, fx[name]), name

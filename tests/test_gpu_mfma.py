@@ -15,8 +15,15 @@ from tbtk_amd import lattice as L
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(params=[1, 0], ids=["pipelined", "single-stage"])
+def mfma_kernel(engine, request):
+    engine.set_option(X.OPT_MFMA_PIPE, request.param)
+    yield request.param
+    engine.set_option(X.OPT_MFMA_PIPE, 1)
+
+
 @pytest.mark.parametrize("gtype", [X.RETARDED, X.ADVANCED, X.PRINCIPAL, X.NONPRINCIPAL])
-def test_mfma_random_coefficients(engine, gtype):
+def test_mfma_random_coefficients(engine, mfma_kernel, gtype):
     K, M, E, s = 300, 500, 700, 8.0
     rng = np.random.default_rng(gtype)
     c = (rng.normal(size=(K, M)) + 1j * rng.normal(size=(K, M))) / np.sqrt(M)
@@ -28,7 +35,23 @@ def test_mfma_random_coefficients(engine, gtype):
     assert err < 1e-4, err
 
 
-def test_mfma_ldos_from_recurrence(engine):
+def test_mfma_kernels_agree_bitwise(engine):
+    """Both kernels add the same BF16 products into FP32 in the same order per
+    accumulator, so their outputs are identical."""
+    K, M, E, s = 260, 96, 390, 6.0
+    rng = np.random.default_rng(9)
+    c = (rng.normal(size=(K, M)) + 1j * rng.normal(size=(K, M))) / np.sqrt(M)
+    engine.lut_generate(M, E, -0.95 * s, 0.95 * s, s)
+    engine.coeffs_upload(c)
+    out = {}
+    for pipe in (1, 0):
+        engine.set_option(X.OPT_MFMA_PIPE, pipe)
+        out[pipe] = engine.contract_mfma(X.RETARDED)
+    engine.set_option(X.OPT_MFMA_PIPE, 1)
+    assert np.array_equal(out[0], out[1])
+
+
+def test_mfma_ldos_from_recurrence(engine, mfma_kernel):
     m = L.nambu_swave(10, 9, disorder=0.5, seed=3)
     s = kpm.gershgorin_scale(m.csr(), 1.05)
     engine.set_csr(m.csr(), s)
