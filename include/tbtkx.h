@@ -204,15 +204,24 @@ enum {
 	                           when no row has more than 8 entries (default) */
 	TBTKX_OPT_AUTOTUNE = 8, /* 1 = time candidate launch shapes on first use
 	                           (default); 0 = fixed defaults */
-	TBTKX_OPT_MFMA_PIPE = 9 /* 1 = MFMA contraction with double-buffered
+	TBTKX_OPT_MFMA_PIPE = 9, /* 1 = MFMA contraction with double-buffered
 	                           direct-to-LDS operand loads (default); 0 = one
 	                           LDS stage loaded through registers */
+	TBTKX_OPT_STRIP_ROWS = 10, /* batched step traversal: S > 0 = rows in L2
+	                           strips of S rows per layer of the band (the
+	                           largest |col - row|), each XCD owning one part;
+	                           -1 = natural row order; 0 = autotuned (default) */
+	TBTKX_OPT_GRID_SHARE = 11, /* batched step grid = occupancy x CUs / share
+	                           (0 = autotuned, default) */
+	TBTKX_OPT_STRIP_CHUNK = 12 /* strip traversal: consecutive tiles dealt to
+	                           one workgroup at a time (multiple of 4, 0 = 4) */
 };
 int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
 
-/* Effective value of an option: TBTKX_OPT_BLOCK_WIDTH gives the width the
- * last batched call used (the autotuner's choice when it is 0); TBTKX_OPT_ELL
- * gives 1 only when the ELL copy exists and is enabled. */
+/* Effective value of an option: TBTKX_OPT_BLOCK_WIDTH, TBTKX_OPT_STRIP_ROWS
+ * and TBTKX_OPT_GRID_SHARE give what the last batched call used (the
+ * autotuner's choice when they are 0); TBTKX_OPT_ELL gives 1 only when the
+ * ELL copy exists and is enabled. */
 int tbtkx_get_option(tbtkx_handle handle, int option, int64_t *value);
 
 #ifdef __cplusplus

@@ -495,12 +495,16 @@ __global__ __launch_bounds__(kThreads) void spmm_step_ell(
 					if (nt) RowNT<VT, CPL>::load(Y + base[r], yo[r]);
 					else Row<VT, CPL>::load(Y + base[r], yo[r]);
 				}
-				if (DOTS) Row<VT, CPL>::load(X + base[r], xs[r]);
 #pragma unroll
 				for (int u = 0; u < W; u++) {
 					const uint32_t c = bcast(wcol, i * W + u);
 					aa[r][u] = CA<AT>{(AT)bcast(wval.x, i * W + u), (AT)bcast(wval.y, i * W + u)};
 					Row<VT, CPL>::load(X + (int64_t)c * B + lane * CPL, xv[r][u]);
+				}
+				// Slot 0 of an ELL row is the row itself (diagonal first).
+				if (DOTS) {
+#pragma unroll
+					for (int j = 0; j < CPL; j++) xs[r][j] = xv[r][0][j];
 				}
 			}
 #pragma unroll
@@ -537,6 +541,224 @@ __global__ __launch_bounds__(kThreads) void spmm_step_ell(
 		}
 		wcol = ncol;
 		wval = nval;
+	}
+
+	if (DOTS) {
+		__shared__ double2 red[kWaves][2][B];
+#pragma unroll
+		for (int j = 0; j < CPL; j++) {
+			red[wave][0][lane * CPL + j] = make_double2(d0[j], 0.0);
+			red[wave][1][lane * CPL + j] = d1[j];
+		}
+		__syncthreads();
+		for (int i = threadIdx.x; i < 2 * B; i += kThreads) {
+			const int w = i / B, b = i - w * B;
+			double2 s = red[0][w][b];
+#pragma unroll
+			for (int q = 1; q < kWaves; q++) {
+				s.x += red[q][w][b].x;
+				s.y += red[q][w][b].y;
+			}
+			part[(int64_t)blockIdx.x * 2 * B + i] = s;
+		}
+	}
+}
+
+// Traversal of the rows in L2-sized strips. Rows are viewed as layers of D
+// (the band: max |col - row|, the slowest lattice stride) and every layer
+// is cut into strips of S rows; the traversal visits strip k of layers 0, 1,
+// 2, ... before strip k+1, so a row's neighbours one stride away (the
+// previous and next layer) are visited S rows before and after it instead of
+// D rows. Segments (strip k, layer a) are cut into tiles of at most T rows.
+// XCD x (hardware workgroup id mod 8) owns the x-th eighth of that tile
+// list, and its wavefronts take consecutive tiles round-robin, so the rows in
+// flight on one XCD (and the Phi_n rows they gather) form a narrow front
+// whose footprint (front + 2 S rows) fits its L2.
+struct StripMap {
+	int64_t N, D;     // rows, layer length (band)
+	int S, A, tps;    // strip rows, layers, tiles per segment
+	int64_t ntiles;   // strips * A * tps
+	int ch;           // consecutive tiles a workgroup's 4 wavefronts share
+};
+
+// Tile t of the strip traversal: first row and row count (0 for the empty
+// tail of a partial strip or layer). The tile list has fewer than 2^31
+// entries (checked on the host), so the divisions are 32-bit.
+__device__ __forceinline__ int strip_tile(const StripMap &m, int T, int64_t t, int64_t *row0) {
+	const uint32_t tt = (uint32_t)t;
+	const uint32_t seg = tt / (uint32_t)m.tps;
+	const int j = (int)(tt - seg * (uint32_t)m.tps);
+	const uint32_t k = seg / (uint32_t)m.A;
+	const int64_t a = seg - k * (uint32_t)m.A;
+	const int64_t c0 = (int64_t)k * m.S + (int64_t)j * T;
+	int64_t n = m.S - (int64_t)j * T < T ? m.S - (int64_t)j * T : T;
+	if (c0 + n > m.D) n = m.D - c0;
+	*row0 = a * m.D + c0;
+	if (*row0 + n > m.N) n = m.N - *row0;
+	return n > 0 ? (int)n : 0;
+}
+
+template <typename VT, typename HT, typename AT, int B, bool FIRST, bool DOTS,
+          bool REALH, int W>
+__global__ __launch_bounds__(kThreads) void spmm_step_strip(
+    const uint32_t *__restrict__ ecol, const HT *__restrict__ eval,
+    const VT *__restrict__ X, VT *Y, StripMap sm, int nwg, int nt,
+    double2 *__restrict__ part, const int64_t *__restrict__ sites, int live,
+    double2 *__restrict__ mu, int M, int nprev) {
+	constexpr int CPL = B / 64;
+	constexpr int RPI = CPL >= 4 ? 1 : 4 / CPL;
+	// A tile is the rows a wavefront keeps in flight, so the XCD's front is
+	// (wavefronts x RPI) rows wide.
+	constexpr int T = RPI;
+	const int wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+
+	if (!DOTS) {
+		const int g = blockIdx.x * kThreads + threadIdx.x;
+		if (g < live)
+			mu[(int64_t)g * M + nprev] = C2<VT>::load(X[sites[g] * B + g]);
+	}
+
+	double d0[CPL];
+	double2 d1[CPL];
+#pragma unroll
+	for (int j = 0; j < CPL; j++) {
+		d0[j] = 0.0;
+		d1[j] = make_double2(0.0, 0.0);
+	}
+
+	// The XCD's part of the tile list is cut into chunks of sm.ch consecutive
+	// tiles dealt round-robin to its workgroups; the 4 wavefronts of a
+	// workgroup interleave over the chunk's tiles, so the rows one CU works
+	// on at a time are adjacent and their nearby neighbours hit its L1.
+	const int xcd = blockIdx.x & 7;
+	const int local = blockIdx.x >> 3;
+	const int per = nwg >> 3;
+	const int64_t s_lo = sm.ntiles * xcd / 8, s_hi = sm.ntiles * (xcd + 1) / 8;
+	const int spc = sm.ch / kWaves;          // tiles per wavefront per chunk
+	// k-th tile of this wavefront (s_hi when past the end).
+	auto tile_of = [&](int64_t k) -> int64_t {
+		const int64_t chunk = local + (k / spc) * per;
+		const int64_t tt = s_lo + chunk * sm.ch + wave + (k % spc) * kWaves;
+		return tt < s_hi ? tt : s_hi;
+	};
+	int64_t kk = 0;
+	int64_t t = tile_of(0);
+
+	// Per tile, in issue order: broadcast the tile's window (loaded one tile
+	// ahead), issue its gathers and Phi_{n-1} / Phi_n row loads, issue the
+	// stores of the previous tile's rows (kept in registers until now),
+	// prefetch the next window, then wait for this tile's loads and compute.
+	// The window registers are consumed before any of the tile's loads are
+	// issued, so the wait on them never covers this tile's own round trip,
+	// and the stores overlap the next tile's loads.
+	uint32_t wcol = 0;
+	HT wval = {};
+	int64_t r0 = 0;
+	int nrows = 0;
+	if (t < s_hi) {
+		nrows = strip_tile(sm, T, t, &r0);
+		if (lane < nrows * W) {
+			wcol = ecol[r0 * W + lane];
+			wval = eval[r0 * W + lane];
+		}
+	}
+	VT pst[RPI][CPL];
+	int64_t pbase[RPI];
+	int pn = 0;
+	while (t < s_hi) {
+		uint32_t cc[RPI][W];
+		CA<AT> aa[RPI][W];
+#pragma unroll
+		for (int r = 0; r < RPI; r++) {
+			const int i = r < nrows ? r : (nrows > 0 ? nrows - 1 : 0);
+#pragma unroll
+			for (int u = 0; u < W; u++) {
+				cc[r][u] = bcast(wcol, i * W + u);
+				aa[r][u] = CA<AT>{(AT)bcast(wval.x, i * W + u), (AT)bcast(wval.y, i * W + u)};
+			}
+		}
+		int64_t base[RPI];
+		VT xv[RPI][W][CPL];
+		VT yo[RPI][CPL], xs[RPI][CPL];
+#pragma unroll
+		for (int r = 0; r < RPI; r++) {
+			const int i = r < nrows ? r : (nrows > 0 ? nrows - 1 : 0);
+			base[r] = (r0 + i) * B + lane * CPL;
+#pragma unroll
+			for (int u = 0; u < W; u++)
+				Row<VT, CPL>::load(X + (int64_t)cc[r][u] * B + lane * CPL, xv[r][u]);
+			if (!FIRST) {
+				if (nt) RowNT<VT, CPL>::load(Y + base[r], yo[r]);
+				else Row<VT, CPL>::load(Y + base[r], yo[r]);
+			}
+			// Slot 0 of an ELL row is the row itself (diagonal first).
+			if (DOTS) {
+#pragma unroll
+				for (int j = 0; j < CPL; j++) xs[r][j] = xv[r][0][j];
+			}
+		}
+#pragma unroll
+		for (int r = 0; r < RPI; r++) {
+			if (r >= pn) break;
+			if (nt) RowNT<VT, CPL>::store(Y + pbase[r], pst[r]);
+			else Row<VT, CPL>::store(Y + pbase[r], pst[r]);
+		}
+		// Tiles of a chunk beyond the part's end are skipped; a later chunk
+		// of this wavefront can only start further on, so the first tile
+		// index at or past s_hi ends the walk.
+		const int64_t tn = tile_of(++kk);
+		uint32_t ncol = 0;
+		HT nval = {};
+		int64_t rn = 0;
+		int nn = 0;
+		if (tn < s_hi) {
+			nn = strip_tile(sm, T, tn, &rn);
+			if (lane < nn * W) {
+				ncol = ecol[rn * W + lane];
+				nval = eval[rn * W + lane];
+			}
+		}
+#pragma unroll
+		for (int r = 0; r < RPI; r++) {
+			CA<AT> acc[CPL];
+#pragma unroll
+			for (int j = 0; j < CPL; j++) {
+				if (FIRST) {
+					acc[j] = CA<AT>{(AT)0, (AT)0};
+				} else {
+					const CA<AT> p = to_acc<AT>(yo[r][j]);
+					acc[j] = CA<AT>{-p.x, -p.y};
+				}
+			}
+			accumulate_chunk<VT, AT, CPL, W, REALH>(aa[r], xv[r], acc);
+#pragma unroll
+			for (int j = 0; j < CPL; j++)
+				pst[r][j] = C2<VT>::store(make_double2((double)acc[j].x, (double)acc[j].y));
+			pbase[r] = base[r];
+			if (DOTS && r < nrows) {
+#pragma unroll
+				for (int j = 0; j < CPL; j++) {
+					const double2 y = C2<VT>::load(pst[r][j]);
+					const double2 x = C2<VT>::load(xs[r][j]);
+					d0[j] = fma(x.x, x.x, fma(x.y, x.y, d0[j]));
+					d1[j].x = fma(y.x, x.x, fma(y.y, x.y, d1[j].x));
+					d1[j].y = fma(y.x, x.y, fma(-y.y, x.x, d1[j].y));
+				}
+			}
+		}
+		pn = nrows;
+		wcol = ncol;
+		wval = nval;
+		r0 = rn;
+		nrows = nn;
+		t = tn;
+	}
+#pragma unroll
+	for (int r = 0; r < RPI; r++) {
+		if (r >= pn) break;
+		if (nt) RowNT<VT, CPL>::store(Y + pbase[r], pst[r]);
+		else Row<VT, CPL>::store(Y + pbase[r], pst[r]);
 	}
 
 	if (DOTS) {
@@ -597,7 +819,25 @@ struct BlockRun {
 	int nwg;
 	int tile;          // rows per wavefront tile
 	int64_t launches = 0;
+	int S = -1;        // strip rows of the strip traversal (-1: natural order)
+	int share = 1;     // grid = occupancy x CUs / share
+	StripMap sm{};
 };
+
+// Strip map for S rows per strip on the resident model (ELL layout), tiles
+// of T rows (the step kernel's rows in flight).
+StripMap make_strip(const tbtkx_context *h, int S, int T) {
+	StripMap m{};
+	m.N = h->N;
+	m.D = h->band < h->N ? h->band : h->N;
+	if (S > m.D) S = (int)m.D;
+	m.S = S;
+	m.A = (int)((m.N + m.D - 1) / m.D);
+	m.tps = (S + T - 1) / T;
+	m.ntiles = (int64_t)((m.D + S - 1) / S) * m.A * m.tps;
+	m.ch = h->strip_chunk > 0 ? h->strip_chunk : 4;
+	return m;
+}
 
 template <typename VT, typename HT, int B>
 int launch_step(BlockRun &r, bool first, bool dots, const VT *X, VT *Y,
@@ -621,9 +861,18 @@ int launch_step(BlockRun &r, bool first, bool dots, const VT *X, VT *Y,
 		else
 			ev = first ? (const HT *)h->ell_v1d.ptr : (const HT *)h->ell_v2d.ptr;
 #define TBTKX_ELL(F, D, R, WW)                                                 \
-	spmm_step_ell<VT, HT, AT, B, F, D, R, WW><<<r.nwg, kThreads, 0, h->stream>>>( \
-	    ec, ev, X, Y, h->N, r.nwg, h->xcd_remap, h->nt_stream, part, sites,    \
-	    live, mu, M, nprev)
+	do {                                                                   \
+		if (r.S > 0)                                                   \
+			spmm_step_strip<VT, HT, AT, B, F, D, R, WW>            \
+			    <<<r.nwg, kThreads, 0, h->stream>>>(               \
+			        ec, ev, X, Y, r.sm, r.nwg, h->nt_stream, part, \
+			        sites, live, mu, M, nprev);                    \
+		else                                                           \
+			spmm_step_ell<VT, HT, AT, B, F, D, R, WW>              \
+			    <<<r.nwg, kThreads, 0, h->stream>>>(               \
+			        ec, ev, X, Y, h->N, r.nwg, h->xcd_remap,       \
+			        h->nt_stream, part, sites, live, mu, M, nprev); \
+	} while (0)
 #define TBTKX_ELL_W(F, D, R)                                                   \
 	do {                                                                   \
 		if (h->ell_W == 4) TBTKX_ELL(F, D, R, 4);                      \
@@ -705,31 +954,49 @@ int run_block(BlockRun &r, int live, const int64_t *d_sites, int mode,
 // Workgroups of the persistent step grid: as many as fit on the device at
 // once for the steady-state kernel variant, a multiple of 8 (one share per
 // XCD), and no more than the row tiles need.
-template <typename VT, typename HT, int B>
-int persistent_grid(tbtkx_context *h, bool dots, int tile, int *nwg);
-
 template <typename VT, typename HT, typename AT, int B, bool D, bool R>
-const void *ell_fn(int W) {
+const void *ell_fn(int W, bool strip) {
+	if (strip) {
+		if (W == 4) return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 4>;
+		if (W == 6) return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 6>;
+		return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 8>;
+	}
 	if (W == 4) return (const void *)spmm_step_ell<VT, HT, AT, B, false, D, R, 4>;
 	if (W == 6) return (const void *)spmm_step_ell<VT, HT, AT, B, false, D, R, 6>;
 	return (const void *)spmm_step_ell<VT, HT, AT, B, false, D, R, 8>;
 }
 
+// Workgroups of the persistent step grid: the occupancy of the steady-state
+// kernel variant times the CUs, divided by `share`, a multiple of 8 (one
+// share per XCD), and no more than the row tiles need. With strips (r.S > 0)
+// r.sm is set up as well.
 template <typename VT, typename HT, int B>
-int persistent_grid(tbtkx_context *h, bool dots, int tile, int *nwg) {
+int persistent_grid(tbtkx_context *h, bool dots, BlockRun &r) {
+	int tile = r.tile;
+	int64_t tiles_total = 0;
 	using AT = typename std::conditional<std::is_same<HT, float2>::value &&
 	                                         std::is_same<VT, float2>::value,
 	                                     float, double>::type;
 	const void *fn;
-	if (h->ell_W > 0 && h->use_ell) {
+	const bool ell = h->ell_W > 0 && h->use_ell;
+	if (!ell) r.S = -1;
+	if (ell) {
 		// The occupancy of the steady-state ELL variant.
 		tile = 64 / h->ell_W;
+		const bool strip = r.S > 0;
+		if (strip) {
+			constexpr int kStripT = B >= 256 ? 1 : 256 / B;   // RPI of the step
+			r.sm = make_strip(h, r.S, kStripT);
+			TBTKX_REQUIRE(r.sm.ntiles < ((int64_t)1 << 31), TBTKX_ERR_ARG,
+			              "batched step: too many strip tiles; use a larger strip");
+			tiles_total = r.sm.ntiles;
+		}
 		if (dots)
-			fn = h->h_real ? ell_fn<VT, HT, AT, B, true, true>(h->ell_W)
-			               : ell_fn<VT, HT, AT, B, true, false>(h->ell_W);
+			fn = h->h_real ? ell_fn<VT, HT, AT, B, true, true>(h->ell_W, strip)
+			               : ell_fn<VT, HT, AT, B, true, false>(h->ell_W, strip);
 		else
-			fn = h->h_real ? ell_fn<VT, HT, AT, B, false, true>(h->ell_W)
-			               : ell_fn<VT, HT, AT, B, false, false>(h->ell_W);
+			fn = h->h_real ? ell_fn<VT, HT, AT, B, false, true>(h->ell_W, strip)
+			               : ell_fn<VT, HT, AT, B, false, false>(h->ell_W, strip);
 	} else if (dots) {
 		fn = h->h_real ? (const void *)spmm_step<VT, HT, AT, B, false, true, true>
 		               : (const void *)spmm_step<VT, HT, AT, B, false, true, false>;
@@ -741,13 +1008,13 @@ int persistent_grid(tbtkx_context *h, bool dots, int tile, int *nwg) {
 	TBTKX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0));
 	TBTKX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
 	                                      h->device));
-	int64_t n = (int64_t)(per_cu > 0 ? per_cu : 1) * cus;
-	const int64_t tiles = (h->N + tile - 1) / tile;
+	int64_t n = (int64_t)(per_cu > 0 ? per_cu : 1) * cus / (r.share > 0 ? r.share : 1);
+	const int64_t tiles = tiles_total > 0 ? tiles_total : (h->N + tile - 1) / tile;
 	const int64_t need = (tiles + kWaves - 1) / kWaves;
 	if (n > need) n = need;
 	n = (n + 7) / 8 * 8;
 	if (n < 8) n = 8;
-	*nwg = (int)n;
+	r.nwg = (int)n;
 	return TBTKX_OK;
 }
 
@@ -761,7 +1028,9 @@ int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
 	tile = (tile + kRpi - 1) / kRpi * kRpi;
 	if (tile > 32) tile = 32;
 	BlockRun r{h, B, flags, 0, tile, 0};
-	int rc = persistent_grid<VT, HT, B>(h, (flags & TBTKX_DOUBLING) != 0, r.tile, &r.nwg);
+	r.S = h->run_strip;
+	r.share = h->run_share;
+	int rc = persistent_grid<VT, HT, B>(h, (flags & TBTKX_DOUBLING) != 0, r);
 	if (rc) return rc;
 	const bool dots = (flags & TBTKX_DOUBLING) != 0;
 	const int S = dots ? (M + 1) / 2 : M - 1;
@@ -797,7 +1066,8 @@ int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
 		GraphCache::make_key(
 		    key,
 		    {total, M, flags, mode, (int64_t)seed, B, (int64_t)r.nwg * 64 + r.tile,
-		     h->use_ell * 4 + h->nt_stream * 2 + h->xcd_remap, h->N, h->ell_W},
+		     h->use_ell * 4 + h->nt_stream * 2 + h->xcd_remap, h->N, h->ell_W, r.S,
+		     h->strip_chunk},
 		    {h->blk_x.ptr, h->blk_y.ptr, h->blk_part.ptr, h->blk_mu.ptr, h->blk_sites.ptr,
 		     h->rowptr.ptr, h->col.ptr, h->v1.ptr, h->v2.ptr, h->v1f.ptr, h->v2f.ptr,
 		     h->ell_col.ptr, h->ell_v1f.ptr, h->ell_v2f.ptr});
@@ -826,6 +1096,8 @@ int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
 	h->last_ms = ms;
 	h->last_launches = (int64_t)nblocks * S;
 	h->last_block_width = B;
+	h->last_strip = r.S;
+	h->last_share = r.share;
 
 	host_mu.assign((size_t)total * M, make_double2(0, 0));
 	for (int blk = 0; blk < nblocks; blk++) {
@@ -858,14 +1130,16 @@ int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
 // start vectors, the same kernel variant a real run uses) for the width
 // autotuner; returns milliseconds per step.
 template <typename VT, typename HT, int B>
-int trial_width(tbtkx_context *h, int flags, float *ms_per_step) {
+int trial_width(tbtkx_context *h, int flags, int S, int share, float *ms_per_step) {
 	constexpr int kRpi = B >= 256 ? 1 : 256 / B;
 	int tile = h->tile_rows > 0 ? h->tile_rows : 4;
 	tile = (tile + kRpi - 1) / kRpi * kRpi;
 	if (tile > 32) tile = 32;
 	BlockRun r{h, B, flags, 0, tile, 0};
+	r.S = S;
+	r.share = share;
 	const bool dots = (flags & TBTKX_DOUBLING) != 0;
-	int rc = persistent_grid<VT, HT, B>(h, dots, r.tile, &r.nwg);
+	int rc = persistent_grid<VT, HT, B>(h, dots, r);
 	if (rc) return rc;
 	if ((rc = h->blk_x.reserve((size_t)h->N * B * sizeof(VT)))) return rc;
 	if ((rc = h->blk_y.reserve((size_t)h->N * B * sizeof(VT)))) return rc;
@@ -897,24 +1171,58 @@ int trial_width(tbtkx_context *h, int flags, float *ms_per_step) {
 	return TBTKX_OK;
 }
 
+// Launch-shape autotuner: every free parameter among the block width B, the
+// strip rows S (natural order or two strip widths sized from the L2: rows
+// of about 1 MB and 256 KB of Phi) and the grid share is tried for three
+// steps on the resident model; the shape with the least time for all blocks
+// of the call wins.
 template <typename VT, typename HT>
-int tune_width(tbtkx_context *h, int total, int flags, int *best_B) {
-	static const int kWidths[] = {64, 128, 256};
+int tune_shape(tbtkx_context *h, int total, int flags, int *best_B, int *best_S,
+               int *best_share) {
+	std::vector<int> widths;
+	if (h->block_width == 64 || h->block_width == 128 || h->block_width == 256)
+		widths.push_back(h->block_width);
+	else
+		widths = {64, 128, 256};
+	const bool ell = h->ell_W > 0 && h->use_ell;
 	double best = 0;
-	*best_B = 64;
+	*best_B = widths[0];
+	*best_S = -1;
+	*best_share = 1;
 	const int cap = (total + 63) / 64 * 64;
-	for (int B : kWidths) {
-		if (B > cap && B != 64) continue;
-		float ms = 0;
-		int rc = B == 64    ? trial_width<VT, HT, 64>(h, flags, &ms)
-		         : B == 128 ? trial_width<VT, HT, 128>(h, flags, &ms)
-		                    : trial_width<VT, HT, 256>(h, flags, &ms);
-		if (rc) return rc;
-		const double cost = (double)ms * ((total + B - 1) / B);
-		if (best == 0 || cost < best) {
-			best = cost;
-			*best_B = B;
+	for (int B : widths) {
+		if (B > cap && B != widths[0]) continue;
+		std::vector<int> strips;
+		if (h->strip_rows != 0 || !ell) {
+			strips.push_back(ell ? h->strip_rows : -1);
+		} else {
+			strips.push_back(-1);
+			const int64_t rowbytes = (int64_t)B * sizeof(VT);
+			for (int64_t target : {(int64_t)1 << 20, (int64_t)1 << 18}) {
+				int S = 32;
+				while ((int64_t)S * 2 * rowbytes <= target) S *= 2;
+				if (h->band >= 2 * S) strips.push_back(S);
+			}
 		}
+		std::vector<int> shares;
+		if (h->grid_share > 0) shares.push_back(h->grid_share);
+		else shares = {1, 2, 4};
+		for (int S : strips)
+			for (int share : shares) {
+				if (S < 0 && share > 2) continue;
+				float ms = 0;
+				int rc = B == 64    ? trial_width<VT, HT, 64>(h, flags, S, share, &ms)
+				         : B == 128 ? trial_width<VT, HT, 128>(h, flags, S, share, &ms)
+				                    : trial_width<VT, HT, 256>(h, flags, S, share, &ms);
+				if (rc) return rc;
+				const double cost = (double)ms * ((total + B - 1) / B);
+				if (best == 0 || cost < best) {
+					best = cost;
+					*best_B = B;
+					*best_S = S;
+					*best_share = share;
+				}
+			}
 	}
 	return TBTKX_OK;
 }
@@ -936,30 +1244,38 @@ int block_entry(tbtkx_context *h, int total, const int64_t *sites, int mode,
 	if (rc) return rc;
 	if (flags & TBTKX_VEC_FP64) flags |= TBTKX_H_FP64;
 	int B = h->block_width;
-	if (B == 0 || B == 32) {
-		if (!h->autotune) {
-			B = total <= 64 ? 64 : 128;
-		} else {
-			// Width autotuner: time each candidate width on the resident
-			// model once per (model, precision, mode, size class) and keep
-			// the one with the least time for all blocks of this call.
-			const int64_t key[5] = {h->csr_gen, flags, total <= 64 ? 64 : total <= 128 ? 128 : 256,
-			                        h->use_ell * 4 + h->nt_stream * 2 + h->xcd_remap,
-			                        h->tile_rows};
-			if (memcmp(key, h->blk_tune_key, sizeof key) != 0) {
-				int tb = 64;
-				if (flags & TBTKX_VEC_FP64)
-					rc = tune_width<double2, double2>(h, total, flags, &tb);
-				else if (flags & TBTKX_H_FP64)
-					rc = tune_width<float2, double2>(h, total, flags, &tb);
-				else
-					rc = tune_width<float2, float2>(h, total, flags, &tb);
-				if (rc) return rc;
-				memcpy(h->blk_tune_key, key, sizeof key);
-				h->blk_tune_B = tb;
-			}
-			B = h->blk_tune_B;
+	if (B == 32) B = 0;
+	const bool ell = h->ell_W > 0 && h->use_ell;
+	h->run_strip = ell ? (h->strip_rows != 0 ? h->strip_rows : -1) : -1;
+	h->run_share = h->grid_share > 0 ? h->grid_share : 1;
+	const bool free_shape = B == 0 || (ell && h->strip_rows == 0) || h->grid_share == 0;
+	if (free_shape && !h->autotune) {
+		if (B == 0) B = total <= 64 ? 64 : 128;
+	} else if (free_shape) {
+		// Launch-shape autotuner: time the candidates on the resident
+		// model once per (model, precision, mode, size class, options) and
+		// keep the fastest for the whole call.
+		const int64_t key[8] = {h->csr_gen, flags, total <= 64 ? 64 : total <= 128 ? 128 : 256,
+		                        h->use_ell * 4 + h->nt_stream * 2 + h->xcd_remap,
+		                        h->tile_rows * 65536 + h->strip_chunk, h->block_width,
+		                        h->strip_rows, h->grid_share};
+		if (memcmp(key, h->blk_tune_key, sizeof key) != 0) {
+			int tb = 64, ts = -1, tsh = 1;
+			if (flags & TBTKX_VEC_FP64)
+				rc = tune_shape<double2, double2>(h, total, flags, &tb, &ts, &tsh);
+			else if (flags & TBTKX_H_FP64)
+				rc = tune_shape<float2, double2>(h, total, flags, &tb, &ts, &tsh);
+			else
+				rc = tune_shape<float2, float2>(h, total, flags, &tb, &ts, &tsh);
+			if (rc) return rc;
+			memcpy(h->blk_tune_key, key, sizeof key);
+			h->blk_tune_B = tb;
+			h->blk_tune_S = ts;
+			h->blk_tune_share = tsh;
 		}
+		B = h->blk_tune_B;
+		h->run_strip = h->blk_tune_S;
+		h->run_share = h->blk_tune_share;
 	}
 	if (flags & TBTKX_VEC_FP64)
 		return dispatch_width<double2, double2>(h, B, total, sites, mode, seed, M,

@@ -102,6 +102,7 @@ struct tbtkx_context {
 	// 0 when a row is longer than 8): tiles of 64/ell_W rows.
 	tbtkx::DevBuf ell_col, ell_v1f, ell_v2f, ell_v1d, ell_v2d;
 	int ell_W = 0;
+	int64_t band = 0;      // max |col - row| over the non-zeros (N if diagonal)
 	// Row blocks for the LDS-staged plain kernel.
 	tbtkx::DevBuf rblk;
 	int nblk = 0;
@@ -121,9 +122,11 @@ struct tbtkx_context {
 	// Batched path buffers.
 	tbtkx::DevBuf blk_x, blk_y, blk_mu, blk_sites, blk_part;
 	int64_t csr_gen = 0;                          // bumped by every tbtkx_set_csr
-	int64_t blk_tune_key[5] = {-1, -1, -1, -1, -1};
-	int blk_tune_B = 0;                           // width chosen by the autotuner
-	int last_block_width = 0;                     // width of the last batched call
+	int64_t blk_tune_key[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+	int blk_tune_B = 0, blk_tune_S = -1, blk_tune_share = 1;   // tuned shape
+	int run_strip = -1, run_share = 1;            // shape of the current call
+	int last_block_width = 0;                     // shape of the last batched call
+	int last_strip = -1, last_share = 1;
 	tbtkx::GraphCache blk_graph;
 
 	// Uneven multi-model batch (tbtkx_multi_*): every model's H as row-major
@@ -149,6 +152,11 @@ struct tbtkx_context {
 	int use_ell = 1;       // batched step on the ELL copy when it exists
 	int autotune = 1;      // time candidate launch shapes on first use
 	int mfma_pipe = 1;     // double-buffered direct-to-LDS MFMA contraction
+	int strip_rows = 0;    // L2 strip width S of the batched step (0 = tuned,
+	                       // -1 = natural row order, the tile-major ELL step)
+	int grid_share = 0;    // batched grid = occupancy x CUs / share (0 = tuned)
+	int strip_chunk = 0;   // consecutive strip tiles per workgroup (multiple
+	                       // of 4; 0 = 4)
 
 	tbtkx::GraphCache plain_graph;
 };
@@ -167,6 +175,23 @@ __device__ __forceinline__ int xcd_remap(int hw, int n) {
 }
 
 int set_device(tbtkx_context *h);
+
+// Slot order of a row in the ELL copies of H: the diagonal entry first (k =
+// -1 when the row has none: a zero value in column r), then the other
+// entries in ascending column order. With the row itself in slot 0, the
+// kernels that need <Phi_n| for the moment dot products reuse that gather.
+// Returns the number of slots; `slots` must hold the row length + 1.
+inline int ell_row_slots(const uint32_t *rowptr, const uint32_t *col, int64_t r,
+                         int64_t *slots) {
+	int64_t diag = -1;
+	for (int64_t k = rowptr[r]; k < (int64_t)rowptr[r + 1]; k++)
+		if ((int64_t)col[k] == r && diag < 0) diag = k;
+	int n = 0;
+	slots[n++] = diag;
+	for (int64_t k = rowptr[r]; k < (int64_t)rowptr[r + 1]; k++)
+		if (k != diag) slots[n++] = k;
+	return n;
+}
 
 // out = in * (d, 0) as complex products (contraction off): the scaling of the
 // CSR values by 1/s and by 2.

@@ -252,11 +252,25 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 	TBTKX_CHECK_HIP(hipMemcpy(h->v2f.ptr, v2f.data(), Zp * 8, hipMemcpyHostToDevice));
 	TBTKX_CHECK_HIP(hipMemcpy(h->rblk.ptr, blocks.data(), blocks.size() * 4,
 	                          hipMemcpyHostToDevice));
-	// Tile-major ELL copy (see spmm_step_ell): W slots per row, tiles of
-	// 64/W rows; padding slots point at the row itself with value zero.
+	// Tile-major ELL copy (see spmm_step_ell): W slots per row in the order
+	// of ell_row_slots (diagonal first), tiles of 64/W rows; padding slots
+	// point at the row itself with value zero.
 	uint32_t maxlen = 0;
-	for (int64_t r = 0; r < N; r++)
-		if (rowptr[r + 1] - rowptr[r] > maxlen) maxlen = rowptr[r + 1] - rowptr[r];
+	int64_t band = 0;   // max |col - row|: the slowest lattice stride
+	std::vector<int64_t> slots;
+	for (int64_t r = 0; r < N; r++) {
+		uint32_t len = rowptr[r + 1] - rowptr[r];
+		bool diag = false;
+		for (uint32_t k = rowptr[r]; k < rowptr[r + 1]; k++) {
+			const int64_t d = (int64_t)col[k] - r;
+			if (d > band) band = d;
+			if (-d > band) band = -d;
+			if (d == 0) diag = true;
+		}
+		if (!diag) len++;
+		if (len > maxlen) maxlen = len;
+	}
+	h->band = band > 0 ? band : N;
 	h->ell_W = maxlen <= 4 ? 4 : maxlen <= 6 ? 6 : maxlen <= 8 ? 8 : 0;
 	if (h->ell_W) {
 		const int W = h->ell_W, T = 64 / W;
@@ -264,23 +278,23 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 		std::vector<uint32_t> ec(E);
 		std::vector<float> e1f(2 * E, 0.0f), e2f(2 * E, 0.0f);
 		std::vector<double> e1d(2 * E, 0.0), e2d(2 * E, 0.0);
-		for (int64_t t = 0; t < ntiles; t++)
-			for (int i = 0; i < T; i++) {
-				const int64_t r = t * T + i;
-				for (int u = 0; u < W; u++) {
-					const int64_t e = (t * T + i) * W + u;
-					ec[e] = (uint32_t)(r < N ? r : 0);
-					if (r >= N || (int64_t)rowptr[r] + u >= (int64_t)rowptr[r + 1]) continue;
-					const int64_t k = rowptr[r] + u;
-					ec[e] = col[k];
-					for (int c = 0; c < 2; c++) {
-						e1d[2 * e + c] = v1[2 * k + c];
-						e2d[2 * e + c] = v2[2 * k + c];
-						e1f[2 * e + c] = v1f[2 * k + c];
-						e2f[2 * e + c] = v2f[2 * k + c];
-					}
+		slots.resize(W + 1);
+		for (int64_t r = 0; r < ntiles * T; r++) {
+			const int n = r < N ? ell_row_slots(rowptr, col, r, slots.data()) : 0;
+			for (int u = 0; u < W; u++) {
+				const int64_t e = r * W + u;
+				ec[e] = (uint32_t)(r < N ? r : 0);
+				if (u >= n || slots[u] < 0) continue;
+				const int64_t k = slots[u];
+				ec[e] = col[k];
+				for (int c = 0; c < 2; c++) {
+					e1d[2 * e + c] = v1[2 * k + c];
+					e2d[2 * e + c] = v2[2 * k + c];
+					e1f[2 * e + c] = v1f[2 * k + c];
+					e2f[2 * e + c] = v2f[2 * k + c];
 				}
 			}
+		}
 		if ((rc = h->ell_col.reserve(E * 4))) return rc;
 		if ((rc = h->ell_v1f.reserve(E * 8))) return rc;
 		if ((rc = h->ell_v2f.reserve(E * 8))) return rc;
@@ -354,6 +368,21 @@ int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
 	case TBTKX_OPT_MFMA_PIPE:
 		h->mfma_pipe = value != 0;
 		return TBTKX_OK;
+	case TBTKX_OPT_STRIP_ROWS:
+		TBTKX_REQUIRE(value >= -1 && value <= (1 << 24), TBTKX_ERR_ARG,
+		              "tbtkx_set_option: strip rows must be -1, 0 or in [1, 2^24]");
+		h->strip_rows = (int)value;
+		return TBTKX_OK;
+	case TBTKX_OPT_GRID_SHARE:
+		TBTKX_REQUIRE(value >= 0 && value <= 64, TBTKX_ERR_ARG,
+		              "tbtkx_set_option: grid share must be in [0, 64]");
+		h->grid_share = (int)value;
+		return TBTKX_OK;
+	case TBTKX_OPT_STRIP_CHUNK:
+		TBTKX_REQUIRE(value >= 0 && value <= 4096 && value % 4 == 0, TBTKX_ERR_ARG,
+		              "tbtkx_set_option: strip chunk must be a multiple of 4 in [0, 4096]");
+		h->strip_chunk = (int)value;
+		return TBTKX_OK;
 	default:
 		set_error("tbtkx_set_option: unknown option " + std::to_string(option));
 		return TBTKX_ERR_ARG;
@@ -374,6 +403,15 @@ int tbtkx_get_option(tbtkx_handle h, int option, int64_t *value) {
 	case TBTKX_OPT_ELL: *value = h->use_ell && h->ell_W > 0; return TBTKX_OK;
 	case TBTKX_OPT_AUTOTUNE: *value = h->autotune; return TBTKX_OK;
 	case TBTKX_OPT_MFMA_PIPE: *value = h->mfma_pipe; return TBTKX_OK;
+	case TBTKX_OPT_STRIP_ROWS:
+		*value = h->strip_rows ? h->strip_rows : h->last_strip;
+		return TBTKX_OK;
+	case TBTKX_OPT_GRID_SHARE:
+		*value = h->grid_share ? h->grid_share : h->last_share;
+		return TBTKX_OK;
+	case TBTKX_OPT_STRIP_CHUNK:
+		*value = h->strip_chunk ? h->strip_chunk : 4;
+		return TBTKX_OK;
 	default:
 		set_error("tbtkx_get_option: unknown option " + std::to_string(option));
 		return TBTKX_ERR_ARG;

@@ -175,13 +175,15 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 					}
 					const int64_t o = d.vec0 + row[r] * w + b;
 					if (!FIRST) yo[r] = load_nt(&Y[o]);
-					if (DOTS) xs[r] = X[o];
 				}
 #pragma unroll
 				for (int r = 0; r < kRpi; r++)
 #pragma unroll
 					for (int u = 0; u < W; u++)
 						xv[r][u] = X[d.vec0 + (int64_t)c[r][u] * w + b];
+				// Slot 0 is the row itself (diagonal first).
+#pragma unroll
+				for (int r = 0; r < kRpi; r++) xs[r] = xv[r][0];
 #pragma unroll
 				for (int r = 0; r < kRpi; r++) {
 					float ax = FIRST ? 0.0f : -yo[r].x;
@@ -600,21 +602,27 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 		const uint32_t *rp = rowptr + rp0;
 		TBTKX_REQUIRE(rp[0] == 0 && (int64_t)rp[N] == Z, TBTKX_ERR_ARG,
 		              tag + "rowptr[0] must be 0 and rowptr[N] == nnz");
-		for (int64_t r = 0; r < N; r++) {
+		for (int64_t r = 0; r < N; r++)
 			TBTKX_REQUIRE(rp[r] <= rp[r + 1], TBTKX_ERR_ARG, tag + "rowptr not monotone");
-			maxlen = std::max(maxlen, rp[r + 1] - rp[r]);
-		}
 		for (int64_t k = 0; k < Z; k++) {
 			TBTKX_REQUIRE(col[z0 + k] < (uint64_t)N, TBTKX_ERR_ARG,
 			              tag + "column index out of range");
 			if (val[2 * (z0 + k) + 1] != 0.0) real = false;
+		}
+		for (int64_t r = 0; r < N; r++) {
+			uint32_t len = rp[r + 1] - rp[r];
+			bool diag = false;
+			for (uint32_t k = rp[r]; k < rp[r + 1]; k++)
+				if ((int64_t)col[z0 + k] == r) diag = true;
+			maxlen = std::max(maxlen, len + (diag ? 0u : 1u));
 		}
 		rp0 += N + 1;
 		z0 += Z;
 		rows += N;
 	}
 	TBTKX_REQUIRE(maxlen <= 8, TBTKX_ERR_ARG,
-	              "tbtkx_multi_set: rows with more than 8 non-zeros are not supported");
+	              "tbtkx_multi_set: rows with more than 8 non-zeros (7 without a "
+	              "diagonal entry) are not supported");
 	TBTKX_REQUIRE(rows * 8 < ((int64_t)1 << 40), TBTKX_ERR_ARG,
 	              "tbtkx_multi_set: batch too large");
 	int rc = set_device(h);
@@ -629,6 +637,7 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 	z0 = 0;
 	int64_t erow = 0;
 	std::vector<double> v1, v2;
+	std::vector<int64_t> slots(W + 1);
 	for (int m = 0; m < K; m++) {
 		const int64_t N = num_rows[m], Z = nnz[m];
 		const uint32_t *rp = rowptr + rp0;
@@ -636,11 +645,14 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 		v2.assign(2 * Z, 0.0);
 		scale_complex(val + 2 * z0, v1.data(), Z, 1 / scales[m]);
 		scale_complex(v1.data(), v2.data(), Z, 2);
+		// Row slots as the single-model ELL copy (ell_row_slots: diagonal
+		// first), on the model's local column indices.
 		for (int64_t r = 0; r < N; r++) {
 			const int64_t base = (erow + r) * W;
+			const int n = ell_row_slots(rp, col + z0, r, slots.data());
 			for (int u = 0; u < W; u++) {
-				const int64_t k = (int64_t)rp[r] + u;
-				if (k < (int64_t)rp[r + 1]) {
+				const int64_t k = u < n ? slots[u] : -1;
+				if (k >= 0) {
 					ec[base + u] = col[z0 + k];
 					e1[2 * (base + u)] = (float)v1[2 * k];
 					e1[2 * (base + u) + 1] = (float)v1[2 * k + 1];

@@ -222,3 +222,31 @@ def test_block_width_autotuner_and_graph(engine):
         assert np.abs(g - tuned).max() < 1e-12
     finally:
         engine.set_option(X.OPT_GRAPH, 0)
+
+
+@pytest.mark.parametrize("strip,share", [(-1, 1), (7, 1), (12, 2), (40, 4), (64, 1), (10 ** 6, 2)])
+def test_block_strip_traversal_bitwise(engine, strip, share):
+    """The L2 strip traversal only changes the order rows are visited in:
+    gather-mode moments are bitwise equal to the natural order for strip
+    widths that do not divide the band, exceed it, or leave partial layers."""
+    m = L.nambu_swave(9, 13, disorder=0.5, seed=6)     # band 52, N = 468
+    s = m.gershgorin_bound() * 1.05
+    engine.set_csr(m.csr(), s)
+    sites = np.random.default_rng(8).choice(m.basis_size, 90, replace=False)
+    try:
+        engine.set_option(X.OPT_BLOCK_WIDTH, 64)
+        engine.set_option(X.OPT_STRIP_ROWS, -1)
+        engine.set_option(X.OPT_GRID_SHARE, 1)
+        ref = engine.moments_block(sites, 47, 0)
+        ref_d = engine.moments_block(sites, 47, X.DOUBLING)
+        engine.set_option(X.OPT_STRIP_ROWS, strip)
+        engine.set_option(X.OPT_GRID_SHARE, share)
+        got = engine.moments_block(sites, 47, 0)
+        got_d = engine.moments_block(sites, 47, X.DOUBLING)
+        assert engine.get_option(X.OPT_STRIP_ROWS) == strip
+    finally:
+        engine.set_option(X.OPT_BLOCK_WIDTH, 0)
+        engine.set_option(X.OPT_STRIP_ROWS, 0)
+        engine.set_option(X.OPT_GRID_SHARE, 0)
+    assert np.array_equal(got, ref)
+    assert np.abs(got_d - ref_d).max() < 1e-12
