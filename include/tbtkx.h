@@ -213,8 +213,11 @@ enum {
 	                           -1 = natural row order; 0 = autotuned (default) */
 	TBTKX_OPT_GRID_SHARE = 11, /* batched step grid = occupancy x CUs / share
 	                           (0 = autotuned, default) */
-	TBTKX_OPT_STRIP_CHUNK = 12 /* strip traversal: consecutive tiles dealt to
+	TBTKX_OPT_STRIP_CHUNK = 12, /* strip traversal: consecutive tiles dealt to
 	                           one workgroup at a time (multiple of 4, 0 = 4) */
+	TBTKX_OPT_ROWS_IN_FLIGHT = 13 /* strip traversal, FP32: 1 or 2 times the
+	                           default rows a wavefront keeps in flight
+	                           (0 = autotuned, default) */
 };
 int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
 

@@ -595,18 +595,27 @@ __device__ __forceinline__ int strip_tile(const StripMap &m, int T, int64_t t, i
 	if (c0 + n > m.D) n = m.D - c0;
 	*row0 = a * m.D + c0;
 	if (*row0 + n > m.N) n = m.N - *row0;
-	return n > 0 ? (int)n : 0;
+	if (n <= 0) {
+		// Empty tail tile: row 0 keeps the (unused) loads of the tile in
+		// bounds; it stores nothing.
+		*row0 = 0;
+		return 0;
+	}
+	return (int)n;
 }
 
 template <typename VT, typename HT, typename AT, int B, bool FIRST, bool DOTS,
-          bool REALH, int W>
+          bool REALH, int W, int RMUL>
 __global__ __launch_bounds__(kThreads) void spmm_step_strip(
     const uint32_t *__restrict__ ecol, const HT *__restrict__ eval,
     const VT *__restrict__ X, VT *Y, StripMap sm, int nwg, int nt,
     double2 *__restrict__ part, const int64_t *__restrict__ sites, int live,
     double2 *__restrict__ mu, int M, int nprev) {
 	constexpr int CPL = B / 64;
-	constexpr int RPI = CPL >= 4 ? 1 : 4 / CPL;
+	// Rows a wavefront keeps in flight: 4 / CPL (at least 1), times RMUL
+	// (2 doubles the bytes in flight per wavefront for the price of fewer
+	// resident wavefronts).
+	constexpr int RPI = (CPL >= 4 ? 1 : 4 / CPL) * RMUL;
 	// A tile is the rows a wavefront keeps in flight, so the XCD's front is
 	// (wavefronts x RPI) rows wide.
 	constexpr int T = RPI;
@@ -821,8 +830,14 @@ struct BlockRun {
 	int64_t launches = 0;
 	int S = -1;        // strip rows of the strip traversal (-1: natural order)
 	int share = 1;     // grid = occupancy x CUs / share
+	int rmul = 1;      // rows in flight multiplier of the strip step (1 or 2)
 	StripMap sm{};
 };
+
+// The strip step with doubled rows in flight is built for FP32 H and
+// vectors only (the batched default).
+template <typename VT, typename HT>
+constexpr bool kFp32Pair = std::is_same<VT, float2>::value && std::is_same<HT, float2>::value;
 
 // Strip map for S rows per strip on the resident model (ELL layout), tiles
 // of T rows (the step kernel's rows in flight).
@@ -862,8 +877,13 @@ int launch_step(BlockRun &r, bool first, bool dots, const VT *X, VT *Y,
 			ev = first ? (const HT *)h->ell_v1d.ptr : (const HT *)h->ell_v2d.ptr;
 #define TBTKX_ELL(F, D, R, WW)                                                 \
 	do {                                                                   \
-		if (r.S > 0)                                                   \
-			spmm_step_strip<VT, HT, AT, B, F, D, R, WW>            \
+		if (r.S > 0 && r.rmul == 2 && kFp32Pair<VT, HT>)               \
+			spmm_step_strip<VT, HT, AT, B, F, D, R, WW, 2>         \
+			    <<<r.nwg, kThreads, 0, h->stream>>>(               \
+			        ec, ev, X, Y, r.sm, r.nwg, h->nt_stream, part, \
+			        sites, live, mu, M, nprev);                    \
+		else if (r.S > 0)                                              \
+			spmm_step_strip<VT, HT, AT, B, F, D, R, WW, 1>         \
 			    <<<r.nwg, kThreads, 0, h->stream>>>(               \
 			        ec, ev, X, Y, r.sm, r.nwg, h->nt_stream, part, \
 			        sites, live, mu, M, nprev);                    \
@@ -955,11 +975,16 @@ int run_block(BlockRun &r, int live, const int64_t *d_sites, int mode,
 // once for the steady-state kernel variant, a multiple of 8 (one share per
 // XCD), and no more than the row tiles need.
 template <typename VT, typename HT, typename AT, int B, bool D, bool R>
-const void *ell_fn(int W, bool strip) {
+const void *ell_fn(int W, bool strip, int rmul) {
+	if (strip && rmul == 2 && kFp32Pair<VT, HT>) {
+		if (W == 4) return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 4, 2>;
+		if (W == 6) return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 6, 2>;
+		return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 8, 2>;
+	}
 	if (strip) {
-		if (W == 4) return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 4>;
-		if (W == 6) return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 6>;
-		return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 8>;
+		if (W == 4) return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 4, 1>;
+		if (W == 6) return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 6, 1>;
+		return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 8, 1>;
 	}
 	if (W == 4) return (const void *)spmm_step_ell<VT, HT, AT, B, false, D, R, 4>;
 	if (W == 6) return (const void *)spmm_step_ell<VT, HT, AT, B, false, D, R, 6>;
@@ -984,19 +1009,21 @@ int persistent_grid(tbtkx_context *h, bool dots, BlockRun &r) {
 		// The occupancy of the steady-state ELL variant.
 		tile = 64 / h->ell_W;
 		const bool strip = r.S > 0;
+		if (!kFp32Pair<VT, HT>) r.rmul = 1;
 		if (strip) {
-			constexpr int kStripT = B >= 256 ? 1 : 256 / B;   // RPI of the step
+			// Tile rows = rows in flight of the step kernel.
+			const int kStripT = (B >= 256 ? 1 : 256 / B) * r.rmul;
 			r.sm = make_strip(h, r.S, kStripT);
 			TBTKX_REQUIRE(r.sm.ntiles < ((int64_t)1 << 31), TBTKX_ERR_ARG,
 			              "batched step: too many strip tiles; use a larger strip");
 			tiles_total = r.sm.ntiles;
 		}
 		if (dots)
-			fn = h->h_real ? ell_fn<VT, HT, AT, B, true, true>(h->ell_W, strip)
-			               : ell_fn<VT, HT, AT, B, true, false>(h->ell_W, strip);
+			fn = h->h_real ? ell_fn<VT, HT, AT, B, true, true>(h->ell_W, strip, r.rmul)
+			               : ell_fn<VT, HT, AT, B, true, false>(h->ell_W, strip, r.rmul);
 		else
-			fn = h->h_real ? ell_fn<VT, HT, AT, B, false, true>(h->ell_W, strip)
-			               : ell_fn<VT, HT, AT, B, false, false>(h->ell_W, strip);
+			fn = h->h_real ? ell_fn<VT, HT, AT, B, false, true>(h->ell_W, strip, r.rmul)
+			               : ell_fn<VT, HT, AT, B, false, false>(h->ell_W, strip, r.rmul);
 	} else if (dots) {
 		fn = h->h_real ? (const void *)spmm_step<VT, HT, AT, B, false, true, true>
 		               : (const void *)spmm_step<VT, HT, AT, B, false, true, false>;
@@ -1030,6 +1057,7 @@ int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
 	BlockRun r{h, B, flags, 0, tile, 0};
 	r.S = h->run_strip;
 	r.share = h->run_share;
+	r.rmul = h->run_rmul;
 	int rc = persistent_grid<VT, HT, B>(h, (flags & TBTKX_DOUBLING) != 0, r);
 	if (rc) return rc;
 	const bool dots = (flags & TBTKX_DOUBLING) != 0;
@@ -1063,14 +1091,15 @@ int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
 	if (h->use_graph) {
 		GraphCache &gc = h->blk_graph;
 		int64_t key[GraphCache::kKey];
-		GraphCache::make_key(
+		const bool key_ok = GraphCache::make_key(
 		    key,
 		    {total, M, flags, mode, (int64_t)seed, B, (int64_t)r.nwg * 64 + r.tile,
 		     h->use_ell * 4 + h->nt_stream * 2 + h->xcd_remap, h->N, h->ell_W, r.S,
-		     h->strip_chunk},
+		     h->strip_chunk, r.rmul},
 		    {h->blk_x.ptr, h->blk_y.ptr, h->blk_part.ptr, h->blk_mu.ptr, h->blk_sites.ptr,
 		     h->rowptr.ptr, h->col.ptr, h->v1.ptr, h->v2.ptr, h->v1f.ptr, h->v2f.ptr,
 		     h->ell_col.ptr, h->ell_v1f.ptr, h->ell_v2f.ptr});
+		TBTKX_REQUIRE(key_ok, TBTKX_ERR_STATE, "batched step: graph key too long");
 		if (!gc.matches(key)) {
 			gc.reset();
 			TBTKX_CHECK_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -1098,6 +1127,7 @@ int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
 	h->last_block_width = B;
 	h->last_strip = r.S;
 	h->last_share = r.share;
+	h->last_rmul = r.rmul;
 
 	host_mu.assign((size_t)total * M, make_double2(0, 0));
 	for (int blk = 0; blk < nblocks; blk++) {
@@ -1130,7 +1160,7 @@ int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
 // start vectors, the same kernel variant a real run uses) for the width
 // autotuner; returns milliseconds per step.
 template <typename VT, typename HT, int B>
-int trial_width(tbtkx_context *h, int flags, int S, int share, float *ms_per_step) {
+int trial_width(tbtkx_context *h, int flags, int S, int share, int rmul, float *ms_per_step) {
 	constexpr int kRpi = B >= 256 ? 1 : 256 / B;
 	int tile = h->tile_rows > 0 ? h->tile_rows : 4;
 	tile = (tile + kRpi - 1) / kRpi * kRpi;
@@ -1138,6 +1168,7 @@ int trial_width(tbtkx_context *h, int flags, int S, int share, float *ms_per_ste
 	BlockRun r{h, B, flags, 0, tile, 0};
 	r.S = S;
 	r.share = share;
+	r.rmul = rmul;
 	const bool dots = (flags & TBTKX_DOUBLING) != 0;
 	int rc = persistent_grid<VT, HT, B>(h, dots, r);
 	if (rc) return rc;
@@ -1178,7 +1209,7 @@ int trial_width(tbtkx_context *h, int flags, int S, int share, float *ms_per_ste
 // of the call wins.
 template <typename VT, typename HT>
 int tune_shape(tbtkx_context *h, int total, int flags, int *best_B, int *best_S,
-               int *best_share) {
+               int *best_share, int *best_rmul) {
 	std::vector<int> widths;
 	if (h->block_width == 64 || h->block_width == 128 || h->block_width == 256)
 		widths.push_back(h->block_width);
@@ -1189,6 +1220,7 @@ int tune_shape(tbtkx_context *h, int total, int flags, int *best_B, int *best_S,
 	*best_B = widths[0];
 	*best_S = -1;
 	*best_share = 1;
+	*best_rmul = 1;
 	const int cap = (total + 63) / 64 * 64;
 	for (int B : widths) {
 		if (B > cap && B != widths[0]) continue;
@@ -1207,22 +1239,29 @@ int tune_shape(tbtkx_context *h, int total, int flags, int *best_B, int *best_S,
 		std::vector<int> shares;
 		if (h->grid_share > 0) shares.push_back(h->grid_share);
 		else shares = {1, 2, 4};
+		std::vector<int> rmuls;
+		if (h->rows_in_flight > 0) rmuls.push_back(h->rows_in_flight);
+		else if (kFp32Pair<VT, HT>) rmuls = {1, 2};
+		else rmuls.push_back(1);
 		for (int S : strips)
-			for (int share : shares) {
-				if (S < 0 && share > 2) continue;
-				float ms = 0;
-				int rc = B == 64    ? trial_width<VT, HT, 64>(h, flags, S, share, &ms)
-				         : B == 128 ? trial_width<VT, HT, 128>(h, flags, S, share, &ms)
-				                    : trial_width<VT, HT, 256>(h, flags, S, share, &ms);
-				if (rc) return rc;
-				const double cost = (double)ms * ((total + B - 1) / B);
-				if (best == 0 || cost < best) {
-					best = cost;
-					*best_B = B;
-					*best_S = S;
-					*best_share = share;
+			for (int share : shares)
+				for (int rmul : rmuls) {
+					if (S < 0 && (share > 2 || rmul > 1)) continue;
+					float ms = 0;
+					int rc = B == 64 ? trial_width<VT, HT, 64>(h, flags, S, share, rmul, &ms)
+					         : B == 128
+					             ? trial_width<VT, HT, 128>(h, flags, S, share, rmul, &ms)
+					             : trial_width<VT, HT, 256>(h, flags, S, share, rmul, &ms);
+					if (rc) return rc;
+					const double cost = (double)ms * ((total + B - 1) / B);
+					if (best == 0 || cost < best) {
+						best = cost;
+						*best_B = B;
+						*best_S = S;
+						*best_share = share;
+						*best_rmul = rmul;
+					}
 				}
-			}
 	}
 	return TBTKX_OK;
 }
@@ -1248,7 +1287,9 @@ int block_entry(tbtkx_context *h, int total, const int64_t *sites, int mode,
 	const bool ell = h->ell_W > 0 && h->use_ell;
 	h->run_strip = ell ? (h->strip_rows != 0 ? h->strip_rows : -1) : -1;
 	h->run_share = h->grid_share > 0 ? h->grid_share : 1;
-	const bool free_shape = B == 0 || (ell && h->strip_rows == 0) || h->grid_share == 0;
+	h->run_rmul = h->rows_in_flight > 0 ? h->rows_in_flight : 1;
+	const bool free_shape = B == 0 || (ell && h->strip_rows == 0) || h->grid_share == 0 ||
+	                        (ell && h->rows_in_flight == 0);
 	if (free_shape && !h->autotune) {
 		if (B == 0) B = total <= 64 ? 64 : 128;
 	} else if (free_shape) {
@@ -1258,24 +1299,26 @@ int block_entry(tbtkx_context *h, int total, const int64_t *sites, int mode,
 		const int64_t key[8] = {h->csr_gen, flags, total <= 64 ? 64 : total <= 128 ? 128 : 256,
 		                        h->use_ell * 4 + h->nt_stream * 2 + h->xcd_remap,
 		                        h->tile_rows * 65536 + h->strip_chunk, h->block_width,
-		                        h->strip_rows, h->grid_share};
+		                        h->strip_rows, h->grid_share * 16 + h->rows_in_flight};
 		if (memcmp(key, h->blk_tune_key, sizeof key) != 0) {
-			int tb = 64, ts = -1, tsh = 1;
+			int tb = 64, ts = -1, tsh = 1, trm = 1;
 			if (flags & TBTKX_VEC_FP64)
-				rc = tune_shape<double2, double2>(h, total, flags, &tb, &ts, &tsh);
+				rc = tune_shape<double2, double2>(h, total, flags, &tb, &ts, &tsh, &trm);
 			else if (flags & TBTKX_H_FP64)
-				rc = tune_shape<float2, double2>(h, total, flags, &tb, &ts, &tsh);
+				rc = tune_shape<float2, double2>(h, total, flags, &tb, &ts, &tsh, &trm);
 			else
-				rc = tune_shape<float2, float2>(h, total, flags, &tb, &ts, &tsh);
+				rc = tune_shape<float2, float2>(h, total, flags, &tb, &ts, &tsh, &trm);
 			if (rc) return rc;
 			memcpy(h->blk_tune_key, key, sizeof key);
 			h->blk_tune_B = tb;
 			h->blk_tune_S = ts;
 			h->blk_tune_share = tsh;
+			h->blk_tune_rmul = trm;
 		}
 		B = h->blk_tune_B;
 		h->run_strip = h->blk_tune_S;
 		h->run_share = h->blk_tune_share;
+		h->run_rmul = h->blk_tune_rmul;
 	}
 	if (flags & TBTKX_VEC_FP64)
 		return dispatch_width<double2, double2>(h, B, total, sites, mode, seed, M,

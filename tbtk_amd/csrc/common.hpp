@@ -62,7 +62,7 @@ constexpr int kPlainRows = 256;     // rows per row block (one per thread)
 // touches, so a buffer that was reallocated since the capture never gets
 // replayed.
 struct GraphCache {
-	static constexpr int kKey = 24;
+	static constexpr int kKey = 40;
 	hipGraphExec_t exec = nullptr;
 	hipGraph_t graph = nullptr;
 	int64_t key[kKey];
@@ -71,13 +71,16 @@ struct GraphCache {
 		for (auto &k : key) k = -1;
 	}
 	void reset();
-	// Fill `out` with the values, then the addresses, zero after.
-	static void make_key(int64_t (&out)[kKey], std::initializer_list<int64_t> vals,
+	// Fill `out` with the values, then the addresses, zero after. Returns
+	// false (and writes nothing past the array) if they do not fit.
+	static bool make_key(int64_t (&out)[kKey], std::initializer_list<int64_t> vals,
 	                     std::initializer_list<const void *> ptrs) {
+		if (vals.size() + ptrs.size() > (size_t)kKey) return false;
 		int i = 0;
 		for (int64_t v : vals) out[i++] = v;
 		for (const void *q : ptrs) out[i++] = (int64_t)(intptr_t)q;
 		while (i < kKey) out[i++] = 0;
+		return true;
 	}
 	bool matches(const int64_t (&k)[kKey]) const {
 		return exec && memcmp(k, key, sizeof key) == 0;
@@ -123,10 +126,10 @@ struct tbtkx_context {
 	tbtkx::DevBuf blk_x, blk_y, blk_mu, blk_sites, blk_part;
 	int64_t csr_gen = 0;                          // bumped by every tbtkx_set_csr
 	int64_t blk_tune_key[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
-	int blk_tune_B = 0, blk_tune_S = -1, blk_tune_share = 1;   // tuned shape
-	int run_strip = -1, run_share = 1;            // shape of the current call
+	int blk_tune_B = 0, blk_tune_S = -1, blk_tune_share = 1, blk_tune_rmul = 1;
+	int run_strip = -1, run_share = 1, run_rmul = 1;   // shape of the current call
 	int last_block_width = 0;                     // shape of the last batched call
-	int last_strip = -1, last_share = 1;
+	int last_strip = -1, last_share = 1, last_rmul = 1;
 	tbtkx::GraphCache blk_graph;
 
 	// Uneven multi-model batch (tbtkx_multi_*): every model's H as row-major
@@ -157,6 +160,7 @@ struct tbtkx_context {
 	int grid_share = 0;    // batched grid = occupancy x CUs / share (0 = tuned)
 	int strip_chunk = 0;   // consecutive strip tiles per workgroup (multiple
 	                       // of 4; 0 = 4)
+	int rows_in_flight = 0;   // strip step rows-in-flight multiplier (0 = tuned)
 
 	tbtkx::GraphCache plain_graph;
 };

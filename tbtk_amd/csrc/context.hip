@@ -383,6 +383,11 @@ int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
 		              "tbtkx_set_option: strip chunk must be a multiple of 4 in [0, 4096]");
 		h->strip_chunk = (int)value;
 		return TBTKX_OK;
+	case TBTKX_OPT_ROWS_IN_FLIGHT:
+		TBTKX_REQUIRE(value >= 0 && value <= 2, TBTKX_ERR_ARG,
+		              "tbtkx_set_option: rows in flight multiplier must be 0, 1 or 2");
+		h->rows_in_flight = (int)value;
+		return TBTKX_OK;
 	default:
 		set_error("tbtkx_set_option: unknown option " + std::to_string(option));
 		return TBTKX_ERR_ARG;
@@ -411,6 +416,9 @@ int tbtkx_get_option(tbtkx_handle h, int option, int64_t *value) {
 		return TBTKX_OK;
 	case TBTKX_OPT_STRIP_CHUNK:
 		*value = h->strip_chunk ? h->strip_chunk : 4;
+		return TBTKX_OK;
+	case TBTKX_OPT_ROWS_IN_FLIGHT:
+		*value = h->rows_in_flight ? h->rows_in_flight : h->last_rmul;
 		return TBTKX_OK;
 	default:
 		set_error("tbtkx_get_option: unknown option " + std::to_string(option));

@@ -776,10 +776,12 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 		int64_t mh = hw;
 		for (int m = 0; m < K; m++) mh = (mh ^ num_moments[m]) * 1099511628211ll;
 		int64_t gkey[GraphCache::kKey];
-		GraphCache::make_key(gkey, {h->mm_gen, flags, mh, G * 16 + share, p.nwg},
-		                     {h->mm_desc.ptr, h->mm_tab.ptr, h->mm_x.ptr, h->mm_y.ptr,
-		                      h->mm_part.ptr, h->mm_seg.ptr, h->mm_D.ptr, h->mm_cols.ptr,
-		                      h->mm_ecol.ptr, h->mm_ev1.ptr, h->mm_ev2.ptr});
+		const bool key_ok = GraphCache::make_key(
+		    gkey, {h->mm_gen, flags, mh, G * 16 + share, p.nwg},
+		    {h->mm_desc.ptr, h->mm_tab.ptr, h->mm_x.ptr, h->mm_y.ptr, h->mm_part.ptr,
+		     h->mm_seg.ptr, h->mm_D.ptr, h->mm_cols.ptr, h->mm_ecol.ptr, h->mm_ev1.ptr,
+		     h->mm_ev2.ptr});
+		TBTKX_REQUIRE(key_ok, TBTKX_ERR_STATE, "tbtkx_multi_moments: graph key too long");
 		if (!gc.matches(gkey)) {
 			gc.reset();
 			TBTKX_CHECK_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));

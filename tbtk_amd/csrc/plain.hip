@@ -239,10 +239,11 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 	if (h->use_graph) {
 		GraphCache &gc = h->plain_graph;
 		int64_t key[GraphCache::kKey];
-		GraphCache::make_key(key, {U, M, h->plain_kernel, h->xcd_remap, h->N},
-		                     {h->vec_a.ptr, h->vec_b.ptr, h->coef.ptr, h->to.ptr,
-		                      h->rowptr.ptr, h->col.ptr, h->v1.ptr, h->v2.ptr,
-		                      h->rblk.ptr});
+		const bool key_ok = GraphCache::make_key(
+		    key, {U, M, h->plain_kernel, h->xcd_remap, h->N},
+		    {h->vec_a.ptr, h->vec_b.ptr, h->coef.ptr, h->to.ptr, h->rowptr.ptr, h->col.ptr,
+		     h->v1.ptr, h->v2.ptr, h->rblk.ptr});
+		TBTKX_REQUIRE(key_ok, TBTKX_ERR_STATE, "tbtkx_moments: graph key too long");
 		if (!gc.matches(key)) {
 			gc.reset();
 			TBTKX_CHECK_HIP(hipStreamBeginCapture(h->stream,

@@ -224,8 +224,9 @@ def test_block_width_autotuner_and_graph(engine):
         engine.set_option(X.OPT_GRAPH, 0)
 
 
-@pytest.mark.parametrize("strip,share", [(-1, 1), (7, 1), (12, 2), (40, 4), (64, 1), (10 ** 6, 2)])
-def test_block_strip_traversal_bitwise(engine, strip, share):
+@pytest.mark.parametrize("strip,share,rmul", [(-1, 1, 1), (7, 1, 1), (12, 2, 2), (40, 4, 1),
+                                              (64, 1, 2), (10 ** 6, 2, 1)])
+def test_block_strip_traversal_bitwise(engine, strip, share, rmul):
     """The L2 strip traversal only changes the order rows are visited in:
     gather-mode moments are bitwise equal to the natural order for strip
     widths that do not divide the band, exceed it, or leave partial layers."""
@@ -237,10 +238,12 @@ def test_block_strip_traversal_bitwise(engine, strip, share):
         engine.set_option(X.OPT_BLOCK_WIDTH, 64)
         engine.set_option(X.OPT_STRIP_ROWS, -1)
         engine.set_option(X.OPT_GRID_SHARE, 1)
+        engine.set_option(X.OPT_ROWS_IN_FLIGHT, 1)
         ref = engine.moments_block(sites, 47, 0)
         ref_d = engine.moments_block(sites, 47, X.DOUBLING)
         engine.set_option(X.OPT_STRIP_ROWS, strip)
         engine.set_option(X.OPT_GRID_SHARE, share)
+        engine.set_option(X.OPT_ROWS_IN_FLIGHT, rmul)
         got = engine.moments_block(sites, 47, 0)
         got_d = engine.moments_block(sites, 47, X.DOUBLING)
         assert engine.get_option(X.OPT_STRIP_ROWS) == strip
@@ -248,5 +251,6 @@ def test_block_strip_traversal_bitwise(engine, strip, share):
         engine.set_option(X.OPT_BLOCK_WIDTH, 0)
         engine.set_option(X.OPT_STRIP_ROWS, 0)
         engine.set_option(X.OPT_GRID_SHARE, 0)
+        engine.set_option(X.OPT_ROWS_IN_FLIGHT, 0)
     assert np.array_equal(got, ref)
     assert np.abs(got_d - ref_d).max() < 1e-12
