@@ -63,7 +63,6 @@ CENTRE = (512, 512)
 # Batched comparison
 M_BATCH = 2000
 B_SITES = 256
-BLOCK_WIDTH = 128     # start vectors per SpMM block (two blocks for 256 sites)
 PLAIN_CMP_SITES = 4
 
 PARTS = ("plain", "batched", "uneven", "dos", "mfma", "cpu")
@@ -147,7 +146,9 @@ def batched_comparison(eng):
     N, Z = m.basis_size, csr[1].size
     s = kpm.gershgorin_scale(csr, 1.05)
     eng.set_csr(csr, s)
-    eng.set_option(X.OPT_BLOCK_WIDTH, BLOCK_WIDTH)
+    # Launch shape (block width, L2 strip, grid share, rows in flight) from
+    # the engine's autotuner, which times the candidates on this model once.
+    eng.set_option(X.OPT_BLOCK_WIDTH, 0)
     rng = np.random.default_rng(3)
     sites = np.sort(rng.choice(N, B_SITES, replace=False))
     lo, hi = -0.99 * s, 0.99 * s
@@ -157,8 +158,8 @@ def batched_comparison(eng):
     def contract(mu):
         return eng.contract(mu * g[None, :], X.NONPRINCIPAL).imag / np.pi
 
-    # warm-up (allocations), then timed
-    eng.moments_block(sites[:64], 16, X.DOUBLING)
+    # warm-up (allocations and the shape autotuner for 256 sites), then timed
+    eng.moments_block(sites, 16, X.DOUBLING)
     t0 = time.perf_counter()
     mu = eng.moments_block(sites, M_BATCH, X.DOUBLING)
     ms_rec, launches = eng.last_timing()
@@ -178,7 +179,10 @@ def batched_comparison(eng):
 
     err = max(np.abs(ld_batched[i] - ld_plain[i]).max() / np.abs(ld_plain[i]).max()
               for i in range(PLAIN_CMP_SITES))
-    B = BLOCK_WIDTH
+    B = eng.get_option(X.OPT_BLOCK_WIDTH)
+    shape = {"block_width": B, "strip_rows": eng.get_option(X.OPT_STRIP_ROWS),
+             "grid_share": eng.get_option(X.OPT_GRID_SHARE),
+             "rows_in_flight_x": eng.get_option(X.OPT_ROWS_IN_FLIGHT)}
     step_bytes = 12 * Z + 4 * N + 24 * B * N
     per_step = ms_rec * 1e-3 / launches
     sps_b = B_SITES / t_batched
@@ -195,6 +199,7 @@ def batched_comparison(eng):
         "error_sites_compared": PLAIN_CMP_SITES,
         "batched_dtype": "fp32 storage (H and vectors), fp64 accumulate, doubling",
         "spmm_block_width": B,
+        "spmm_shape": shape,
         "spmm_step_ms": per_step * 1e3,
         "spmm_GBps": step_bytes / per_step / 1e9,
         "spmm_hbm_frac": step_bytes / per_step / 1e9 / HBM_PEAK_GBPS,
@@ -284,15 +289,16 @@ def stochastic_dos(eng):
     s = kpm.gershgorin_scale(csr, 1.05)
     eng.set_csr(csr, s)
     R, M, seed = 512, 4096, 17
-    eng.set_option(X.OPT_BLOCK_WIDTH, 256)
-    try:
-        eng.moments_trace(8, seed, 8)  # allocations
-        t0 = time.perf_counter()
-        mu = eng.moments_trace(R, seed, M)
-        wall = time.perf_counter() - t0
-        ms, launches = eng.last_timing()
-    finally:
-        eng.set_option(X.OPT_BLOCK_WIDTH, 0)
+    eng.set_option(X.OPT_BLOCK_WIDTH, 0)
+    eng.moments_trace(R, seed, 8)  # allocations and the shape autotuner
+    t0 = time.perf_counter()
+    mu = eng.moments_trace(R, seed, M)
+    wall = time.perf_counter() - t0
+    ms, launches = eng.last_timing()
+    Bw = eng.get_option(X.OPT_BLOCK_WIDTH)
+    shape = {"block_width": Bw, "strip_rows": eng.get_option(X.OPT_STRIP_ROWS),
+             "grid_share": eng.get_option(X.OPT_GRID_SHARE),
+             "rows_in_flight_x": eng.get_option(X.OPT_ROWS_IN_FLIGHT)}
     # error: 8 vectors in FP32 storage vs the same 8 in FP64 (H and vectors)
     Rc = 8
     a = eng.moments_trace(Rc, seed, M)
@@ -302,8 +308,7 @@ def stochastic_dos(eng):
     dos_a = kpm.ldos_from_gf(eng.contract(a * g, X.NONPRINCIPAL))
     dos_b = kpm.ldos_from_gf(eng.contract(b * g, X.NONPRINCIPAL))
     eng.lut_destroy()
-    Bw = 256
-    blocks = R // Bw
+    blocks = (R + Bw - 1) // Bw
     step_bytes = 12 * Z + 4 * N + 24 * Bw * N
     per_step = ms * 1e-3 / launches
     return {
@@ -313,7 +318,8 @@ def stochastic_dos(eng):
         "dtype": "fp32 storage (H and vectors), fp64 moment sums, doubling",
         "seconds": ms * 1e-3,
         "wall_s": wall,
-        "block_width": Bw, "blocks": blocks, "step_launches": int(launches),
+        "block_width": Bw, "spmm_shape": shape, "blocks": blocks,
+        "step_launches": int(launches),
         "spmm_step_ms": per_step * 1e3,
         "spmm_GBps": step_bytes / per_step / 1e9,
         "spmm_hbm_frac": step_bytes / per_step / 1e9 / HBM_PEAK_GBPS,

@@ -139,6 +139,7 @@ struct tbtkx_context {
 	bool mm_real = false;
 	int64_t mm_gen = 0;                       // bumped by every tbtkx_multi_set
 	std::vector<int64_t> mm_N, mm_erow0;      // rows, first ELL row per model
+	std::vector<int64_t> mm_band;             // max |col - row| per model
 	tbtkx::DevBuf mm_ecol, mm_ev1, mm_ev2;
 	tbtkx::DevBuf mm_desc, mm_tab, mm_x, mm_y, mm_part, mm_seg, mm_D, mm_cols;
 	int64_t mm_tune_key[3] = {-1, -1, -1};    // (generation, flags, widths hash)

@@ -48,6 +48,7 @@ namespace {
 
 constexpr int kWaves = kThreads / 64;
 constexpr int kRpi = 2;          // row groups in flight per lane
+constexpr int kRpi1 = 4;         // rows in flight per wavefront for w > 32
 constexpr int kSegSlots = 64;    // dot slots summed per group in pass 1
 constexpr int kMaxWidth = 64;    // start vectors per model
 
@@ -56,8 +57,13 @@ struct MDesc {
 	int64_t erow0;   // first row of the model in the ELL arrays
 	int64_t vec0;    // first element of the model's vectors in X / Y
 	int64_t dof;     // first element of the model's D block (doubling)
+	int64_t D;       // layer length of the strip traversal (the model's band)
 	int w;           // columns (start vectors)
 	int Q;           // rows per row group, 64 / w
+	int S;           // strip rows
+	int A;           // layers, ceil(N / D)
+	int tps;         // tiles per strip segment
+	int pad;
 };
 
 struct GCol {
@@ -120,21 +126,30 @@ __device__ __forceinline__ void mac(float2 a, float2 x, float &ax, float &ay) {
 	}
 }
 
-// One recurrence step of every active model. Tables of the phase: toff[k]
-// (first tile of sorted model k, k <= kact), wstart[gw] (first tile of
-// wavefront gw, gw <= number of wavefronts), wslot[gw] (first dot slot).
+// One recurrence step of every active model. Each model is traversed in
+// L2 strips (layers of D rows, strips of S rows; a tile is G row groups of
+// Q rows inside one strip segment), the models' tile lists are concatenated
+// (toff[k]: first tile of sorted model k) and the phase's list is cut into
+// 8 byte-balanced parts, one per XCD (xpart). Inside a part the XCD's
+// wavefronts take consecutive tiles round-robin, so the rows in flight on an
+// XCD form a narrow front. Dot partials: a wavefront flushes its sums when
+// it leaves a model, into slot pbase[k*8+x] + its rank among the
+// wavefronts that touch the model's piece of part x (pstart: the piece's
+// first tile).
 template <int W, bool FIRST, bool DOTS, bool REALH>
 __global__ __launch_bounds__(kThreads) void multi_step(
     const MDesc *__restrict__ md, const int *__restrict__ toff, int kact,
-    const int *__restrict__ wstart, const int *__restrict__ wslot, int G,
-    const uint32_t *__restrict__ ecol, const float2 *__restrict__ ev,
-    const float2 *__restrict__ X, float2 *__restrict__ Y,
-    double2 *__restrict__ part) {
+    const int *__restrict__ xpart, const int *__restrict__ pbase,
+    const int *__restrict__ pstart, int G, const uint32_t *__restrict__ ecol,
+    const float2 *__restrict__ ev, const float2 *__restrict__ X,
+    float2 *__restrict__ Y, double2 *__restrict__ part) {
 	const int lane = threadIdx.x & 63;
-	const int gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
-	int t = wstart[gw];
-	const int te = wstart[gw + 1];
-	if (t >= te) return;
+	const int x = blockIdx.x & 7;
+	const int nwx = (gridDim.x >> 3) * kWaves;                  // wavefronts per XCD
+	const int wx = (blockIdx.x >> 3) * kWaves + (threadIdx.x >> 6);
+	const int plo = xpart[x], phi = xpart[x + 1];
+	int t = plo + wx;
+	if (t >= phi) return;
 	// Model of tile t: the last k with toff[k] <= t (every model has a tile).
 	int lo = 0, hi = kact;
 	while (hi - lo > 1) {
@@ -143,7 +158,6 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 		else hi = mid;
 	}
 	int m = lo;
-	int slot = DOTS ? wslot[gw] : 0;
 	for (;;) {
 		const MDesc d = md[m];
 		const int w = d.w, Q = d.Q;
@@ -151,27 +165,123 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 		const int b = lane - q * w;
 		const bool act = q < Q;
 		const int qc = act ? q : Q - 1;
-		const int64_t ngroups = (d.N + Q - 1) / Q;
-		const int tend = te < toff[m + 1] ? te : toff[m + 1];
+		const int64_t tr = (int64_t)G * Q;   // rows per tile
+		const bool shared_window = Q * W <= 64;
+		const int tend = phi < toff[m + 1] ? phi : toff[m + 1];
 		double d0 = 0.0, d1x = 0.0, d1y = 0.0;
-		for (; t < tend; t++) {
-			const int64_t g0 = (int64_t)(t - toff[m]) * G;
+		if (Q == 1) {
+			// One row per group (w > 32): every lane needs the same ELL
+			// entries, so a tile's rows are read as one window and broadcast
+			// with readlane (wave-uniform columns, scalar addresses), as in
+			// the single-model step.
+			for (; t < tend; t += nwx) {
+				const uint32_t lt = (uint32_t)(t - toff[m]);
+				const uint32_t seg = lt / (uint32_t)d.tps;
+				const uint32_t j = lt - seg * (uint32_t)d.tps;
+				const uint32_t kk = seg / (uint32_t)d.A;
+				const int64_t a = seg - kk * (uint32_t)d.A;
+				const int64_t c0 = (int64_t)kk * d.S + (int64_t)j * tr;
+				int64_t rows = d.S - (int64_t)j * tr < tr ? d.S - (int64_t)j * tr : tr;
+				if (c0 + rows > d.D) rows = d.D - c0;
+				int64_t row0 = a * d.D + c0;
+				if (row0 + rows > d.N) rows = d.N - row0;
+				if (rows <= 0) continue;
+				for (int gg = 0; gg < rows; gg += kRpi1) {
+					// Window: lane l holds entry l % W of row gg + l / W.
+					int64_t wrow = row0 + gg + lane / W;
+					if (wrow > row0 + rows - 1) wrow = row0 + rows - 1;
+					const int64_t we = (d.erow0 + wrow) * W + lane % W;
+					const uint32_t wc = ecol[we];
+					const float2 wv = ev[we];
+					int64_t row[kRpi1];
+					uint32_t c[kRpi1][W];
+					float2 a2[kRpi1][W], xv[kRpi1][W], yo[kRpi1];
+#pragma unroll
+					for (int r = 0; r < kRpi1; r++) {
+						const int i = gg + r < rows ? r : (int)(rows - 1 - gg);
+						row[r] = row0 + gg + i;
+#pragma unroll
+						for (int u = 0; u < W; u++) {
+							c[r][u] = (uint32_t)__builtin_amdgcn_readlane((int)wc, i * W + u);
+							a2[r][u] = make_float2(
+							    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv.x), i * W + u)),
+							    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv.y), i * W + u)));
+						}
+#pragma unroll
+						for (int u = 0; u < W; u++)
+							xv[r][u] = X[d.vec0 + (int64_t)c[r][u] * w + b];
+						if (!FIRST) yo[r] = load_nt(&Y[d.vec0 + row[r] * w + b]);
+					}
+#pragma unroll
+					for (int r = 0; r < kRpi1; r++) {
+						float ax = FIRST ? 0.0f : -yo[r].x;
+						float ay = FIRST ? 0.0f : -yo[r].y;
+#pragma unroll
+						for (int u = 0; u < W; u++) mac<REALH>(a2[r][u], xv[r][u], ax, ay);
+						if (!act || gg + r >= rows) continue;
+						store_nt(&Y[d.vec0 + row[r] * w + b], ax, ay);
+						if (DOTS) {
+							const double yx = ax, yy = ay;
+							const double xx = xv[r][0].x, xy = xv[r][0].y;
+							d0 = fma(xx, xx, fma(xy, xy, d0));
+							d1x = fma(yx, xx, fma(yy, xy, d1x));
+							d1y = fma(yx, xy, fma(-yy, xx, d1y));
+						}
+					}
+				}
+			}
+		}
+		for (; t < tend; t += nwx) {
+			// Tile lt of the model: strip k, layer a, j-th tile of the segment.
+			const uint32_t lt = (uint32_t)(t - toff[m]);
+			const uint32_t seg = lt / (uint32_t)d.tps;
+			const uint32_t j = lt - seg * (uint32_t)d.tps;
+			const uint32_t kk = seg / (uint32_t)d.A;
+			const int64_t a = seg - kk * (uint32_t)d.A;
+			const int64_t c0 = (int64_t)kk * d.S + (int64_t)j * tr;
+			int64_t rows = d.S - (int64_t)j * tr < tr ? d.S - (int64_t)j * tr : tr;
+			if (c0 + rows > d.D) rows = d.D - c0;
+			int64_t row0 = a * d.D + c0;
+			if (row0 + rows > d.N) rows = d.N - row0;
+			if (rows <= 0) {
+				rows = 0;
+				row0 = 0;
+			}
 			for (int gg = 0; gg < G; gg += kRpi) {
 				int64_t row[kRpi];
 				bool live[kRpi];
 				uint32_t c[kRpi][W];
-				float2 a[kRpi][W], xv[kRpi][W], yo[kRpi], xs[kRpi];
+				float2 a2[kRpi][W], xv[kRpi][W], yo[kRpi], xs[kRpi];
 #pragma unroll
 				for (int r = 0; r < kRpi; r++) {
-					const int64_t grp = g0 + gg + r;
-					const int64_t rr = grp * Q + qc;
-					live[r] = act && grp < ngroups && rr < d.N;
-					row[r] = rr < d.N ? rr : d.N - 1;
-					const int64_t e = (d.erow0 + row[r]) * W;
+					const int64_t off = (int64_t)(gg + r) * Q + qc;
+					live[r] = act && off < rows;
+					row[r] = row0 + (off < rows ? off : (rows > 0 ? rows - 1 : 0));
+					if (shared_window) {
+						// The group's Q rows are consecutive, so their Q W
+						// entries are one contiguous window: lane l loads
+						// entry l, and each lane takes its row's entries
+						// from the lanes that loaded them.
+						const int64_t gfirst = row0 + (int64_t)(gg + r) * Q;
+						const int64_t glast = rows > 0 ? row0 + rows - 1 : row0;
+						int64_t wrow = gfirst + lane / W;
+						if (wrow > glast) wrow = glast;
+						const int64_t we = (d.erow0 + wrow) * W + lane % W;
+						const uint32_t wc = ecol[we];
+						const float2 wv = ev[we];
+						const int base = qc * W;
 #pragma unroll
-					for (int u = 0; u < W; u++) {
-						c[r][u] = ecol[e + u];
-						a[r][u] = ev[e + u];
+						for (int u = 0; u < W; u++) {
+							c[r][u] = (uint32_t)__shfl((int)wc, base + u);
+							a2[r][u] = make_float2(__shfl(wv.x, base + u), __shfl(wv.y, base + u));
+						}
+					} else {
+						const int64_t e = (d.erow0 + row[r]) * W;
+#pragma unroll
+						for (int u = 0; u < W; u++) {
+							c[r][u] = ecol[e + u];
+							a2[r][u] = ev[e + u];
+						}
 					}
 					const int64_t o = d.vec0 + row[r] * w + b;
 					if (!FIRST) yo[r] = load_nt(&Y[o]);
@@ -189,7 +299,7 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 					float ax = FIRST ? 0.0f : -yo[r].x;
 					float ay = FIRST ? 0.0f : -yo[r].y;
 #pragma unroll
-					for (int u = 0; u < W; u++) mac<REALH>(a[r][u], xv[r][u], ax, ay);
+					for (int u = 0; u < W; u++) mac<REALH>(a2[r][u], xv[r][u], ax, ay);
 					if (!live[r]) continue;
 					store_nt(&Y[d.vec0 + row[r] * w + b], ax, ay);
 					if (DOTS) {
@@ -211,14 +321,16 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 				s1x += __shfl(d1x, src);
 				s1y += __shfl(d1y, src);
 			}
+			const int pi = m * 8 + x;
+			const int s0w = (pstart[pi] - plo) % nwx;
+			const int slot = pbase[pi] + (wx - s0w + nwx) % nwx;
 			if (lane < w) {
 				part[((int64_t)slot * 2 + 0) * kMaxWidth + lane] = make_double2(s0, 0.0);
 				part[((int64_t)slot * 2 + 1) * kMaxWidth + lane] = make_double2(s1x, s1y);
 			}
-			slot++;
 		}
-		if (t >= te) break;
-		m++;
+		if (t >= phi) break;
+		while (toff[m + 1] <= t) m++;
 	}
 }
 
@@ -266,8 +378,7 @@ __global__ __launch_bounds__(kThreads) void multi_reduce_models(
 struct Phase {
 	int kact;          // active models (a prefix of the sorted order)
 	int n_lo, n_hi;    // steps of the phase
-	int G;             // row groups per tile
-	int64_t toff, wstart, wslot, mseg, seg;   // offsets into the table buffer
+	int64_t toff, xpart, pbase, pstart, mseg, seg;   // offsets into the table buffer
 	int nseg, nslots;
 };
 
@@ -275,69 +386,52 @@ struct Plan {
 	std::vector<int> order;      // sorted position -> model
 	std::vector<MDesc> desc;     // in sorted order
 	std::vector<int> S;          // steps, sorted order
+	std::vector<int64_t> tiles;  // tiles per model, sorted order
 	std::vector<Phase> phases;
 	std::vector<int> tab;        // all phase tables
-	int nwg = 0, nw = 0, Smax = 0, max_slots = 0, max_seg = 0;
+	int nwg = 0, G = 8, Smax = 0, max_slots = 0, max_seg = 0;
 	int64_t vec_elems = 0, d_elems = 0;
 };
 
-// Tables of one phase: tile offsets, a byte-balanced cut of the tile list
-// into nw contiguous wavefront ranges, the dot slots of each wavefront and
-// the slot groups of each model.
-void plan_phase(const std::vector<MDesc> &desc, int W, int kact, int Gmax, int nw,
-                Phase &ph, std::vector<int> &tab) {
-	// Tile size: the largest G <= Gmax (a multiple of kRpi) that still gives
-	// every wavefront about four tiles.
-	std::vector<int64_t> groups(kact);
-	int64_t rows_all = 0;
-	for (int k = 0; k < kact; k++) {
-		groups[k] = (desc[k].N + desc[k].Q - 1) / desc[k].Q;
-		rows_all += groups[k];
-	}
-	int G = Gmax;
-	while (G > kRpi && rows_all / G < 4 * (int64_t)nw)
-		G = std::max(kRpi, G / 2 / kRpi * kRpi);
-	ph.G = G;
+// Tables of one phase: tile offsets, the byte-balanced cut of the tile list
+// into 8 XCD parts, the pieces (model x part) with the base of their dot
+// slots (model-major, so a model's slots are contiguous) and the slot groups
+// of each model for the ordered reduction.
+void plan_phase(const Plan &p, int W, int kact, int nwx, Phase &ph, std::vector<int> &tab) {
 	std::vector<int> toff(kact + 1, 0);
-	for (int k = 0; k < kact; k++)
-		toff[k + 1] = toff[k] + (int)((groups[k] + G - 1) / G);
-	const int T = toff[kact];
-	// Cost of each tile in algorithmic bytes.
-	std::vector<double> cum(T + 1, 0.0);
+	std::vector<double> cost(kact), cum(kact + 1, 0.0);
 	for (int k = 0; k < kact; k++) {
-		const double per_row = 12.0 * W + 24.0 * desc[k].w;
-		for (int t = toff[k]; t < toff[k + 1]; t++) {
-			const int64_t r0 = (int64_t)(t - toff[k]) * G * desc[k].Q;
-			int64_t rows = (int64_t)G * desc[k].Q;
-			if (r0 + rows > desc[k].N) rows = desc[k].N - r0;
-			cum[t + 1] = cum[t] + per_row * (double)rows;
+		toff[k + 1] = toff[k] + (int)p.tiles[k];
+		cost[k] = (double)p.desc[k].N * (12.0 * W + 24.0 * p.desc[k].w);
+		cum[k + 1] = cum[k] + cost[k];
+	}
+	const int T = toff[kact];
+	// XCD parts: cut where the cumulative cost (uniform per tile inside a
+	// model) crosses i/8 of the total.
+	std::vector<int> xpart(9, T);
+	xpart[0] = 0;
+	int k = 0;
+	for (int i = 1; i < 8; i++) {
+		const double target = cum[kact] * i / 8;
+		while (k < kact - 1 && cum[k + 1] <= target) k++;
+		const double per_tile = cost[k] / (double)(toff[k + 1] - toff[k]);
+		int tcut = toff[k] + (int)((target - cum[k]) / per_tile);
+		tcut = std::min(std::max(tcut, toff[k]), toff[k + 1]);
+		xpart[i] = std::max(tcut, xpart[i - 1]);
+	}
+	// Pieces and slots.
+	std::vector<int> pbase(kact * 8, -1), pstart(kact * 8, 0), mslot(kact + 1, 0);
+	int slot = 0;
+	for (int j = 0; j < kact; j++) {
+		mslot[j] = slot;
+		for (int x = 0; x < 8; x++) {
+			const int a = std::max(toff[j], xpart[x]), e = std::min(toff[j + 1], xpart[x + 1]);
+			if (a >= e) continue;
+			pbase[j * 8 + x] = slot;
+			pstart[j * 8 + x] = a;
+			slot += std::min(e - a, nwx);
 		}
 	}
-	std::vector<int> wstart(nw + 1, T);
-	wstart[0] = 0;
-	int t = 0;
-	for (int i = 1; i < nw; i++) {
-		const double target = cum[T] * i / nw;
-		while (t < T && cum[t] < target) t++;
-		wstart[i] = t;
-	}
-	// Slots: one per (wavefront, model it touches), in wavefront order.
-	std::vector<int> wslot(nw + 1, 0), mslot(kact + 1, -1);
-	int slot = 0, k = 0;
-	for (int i = 0; i < nw; i++) {
-		wslot[i] = slot;
-		const int a = wstart[i], e = wstart[i + 1];
-		if (a >= e) continue;
-		while (toff[k + 1] <= a) k++;
-		int kk = k;
-		for (;;) {
-			if (mslot[kk] < 0) mslot[kk] = slot;
-			slot++;
-			if (toff[kk + 1] >= e) break;
-			kk++;
-		}
-	}
-	wslot[nw] = slot;
 	mslot[kact] = slot;
 	std::vector<int> mseg(kact + 1, 0), seg;
 	for (int j = 0; j < kact; j++) {
@@ -358,8 +452,9 @@ void plan_phase(const std::vector<MDesc> &desc, int W, int kact, int Gmax, int n
 		return off;
 	};
 	ph.toff = put(toff);
-	ph.wstart = put(wstart);
-	ph.wslot = put(wslot);
+	ph.xpart = put(xpart);
+	ph.pbase = put(pbase);
+	ph.pstart = put(pstart);
 	ph.mseg = put(mseg);
 	ph.seg = put(seg.empty() ? std::vector<int>(3, 0) : seg);
 }
@@ -391,16 +486,19 @@ int grid_size(tbtkx_context *h, bool dots, int share, int *nwg) {
 }
 
 // Launch shape and tables for the given widths and step counts.
-int make_plan(tbtkx_context *h, const std::vector<int> &width, const std::vector<int> &S,
+int make_plan(tbtkx_context *h, const std::vector<int> &width, const std::vector<int> &steps,
               bool dots, int Gmax, int share, Plan &p) {
 	const int K = h->mm_K;
 	p.order.resize(K);
 	std::iota(p.order.begin(), p.order.end(), 0);
 	std::stable_sort(p.order.begin(), p.order.end(),
-	                 [&S](int x, int y) { return S[x] > S[y]; });
+	                 [&steps](int x, int y) { return steps[x] > steps[y]; });
 	p.desc.resize(K);
 	p.S.resize(K);
 	int64_t vec = 0, dof = 0;
+	p.G = Gmax;
+	p.tiles.resize(K);
+	int64_t tiles_all = 0;
 	for (int k = 0; k < K; k++) {
 		const int m = p.order[k];
 		MDesc &d = p.desc[k];
@@ -410,16 +508,31 @@ int make_plan(tbtkx_context *h, const std::vector<int> &width, const std::vector
 		d.Q = kMaxWidth / d.w;
 		d.vec0 = vec;
 		d.dof = dof;
+		// Strips: the rows whose Phi rows are reused one layer later span
+		// 2 S rows; about 512 KB of them per strip, at least one tile.
+		const int64_t tr = (int64_t)p.G * d.Q;
+		d.D = h->mm_band[m];
+		int64_t S = (int64_t)32768 / d.w;
+		if (S < tr) S = tr;
+		if (2 * S > d.D) S = d.D;
+		d.S = (int)S;
+		d.A = (int)((d.N + d.D - 1) / d.D);
+		d.tps = (int)((S + tr - 1) / tr);
+		d.pad = 0;
+		p.tiles[k] = ((d.D + S - 1) / S) * d.A * d.tps;
+		tiles_all += p.tiles[k];
 		vec += d.N * d.w;
-		dof += (int64_t)S[m] * 2 * d.w;
-		p.S[k] = S[m];
+		dof += (int64_t)steps[m] * 2 * d.w;
+		p.S[k] = steps[m];
 	}
+	TBTKX_REQUIRE(tiles_all < ((int64_t)1 << 31), TBTKX_ERR_ARG,
+	              "tbtkx_multi_moments: batch too large for one launch");
 	p.vec_elems = vec;
 	p.d_elems = dof;
 	p.Smax = p.S[0];
 	int rc = grid_size(h, dots, share, &p.nwg);
 	if (rc) return rc;
-	p.nw = p.nwg * kWaves;
+	const int nwx = p.nwg / 8 * kWaves;
 	p.phases.clear();
 	p.tab.clear();
 	for (int n = 0; n < p.Smax;) {
@@ -429,7 +542,7 @@ int make_plan(tbtkx_context *h, const std::vector<int> &width, const std::vector
 		Phase ph;
 		ph.n_lo = n;
 		ph.n_hi = n_hi;
-		plan_phase(p.desc, h->mm_W, kact, Gmax, p.nw, ph, p.tab);
+		plan_phase(p, h->mm_W, kact, nwx, ph, p.tab);
 		p.max_slots = std::max(p.max_slots, ph.nslots);
 		p.max_seg = std::max(p.max_seg, ph.nseg);
 		p.phases.push_back(ph);
@@ -447,8 +560,8 @@ void launch_step_w(tbtkx_context *h, const Plan &p, const Phase &ph, bool first,
 	double2 *part = h->mm_part.as<double2>();
 #define TBTKX_MS(F, D)                                                          \
 	multi_step<W, F, D, REALH><<<p.nwg, kThreads, 0, h->stream>>>(            \
-	    md, tab + ph.toff, ph.kact, tab + ph.wstart, tab + ph.wslot, ph.G,      \
-	    h->mm_ecol.as<uint32_t>(), ev, X, Y, part)
+	    md, tab + ph.toff, ph.kact, tab + ph.xpart, tab + ph.pbase,            \
+	    tab + ph.pstart, p.G, h->mm_ecol.as<uint32_t>(), ev, X, Y, part)
 	if (first && dots) TBTKX_MS(true, true);
 	else if (first) TBTKX_MS(true, false);
 	else if (dots) TBTKX_MS(false, true);
@@ -633,6 +746,7 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 	std::vector<float> e1((size_t)rows * W * 2, 0.0f), e2((size_t)rows * W * 2, 0.0f);
 	h->mm_N.assign(K, 0);
 	h->mm_erow0.assign(K, 0);
+	h->mm_band.assign(K, 0);
 	rp0 = 0;
 	z0 = 0;
 	int64_t erow = 0;
@@ -663,6 +777,13 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 				}
 			}
 		}
+		int64_t band = 0;   // max |col - row|: the model's slowest stride
+		for (int64_t r = 0; r < N; r++)
+			for (int64_t k = rp[r]; k < (int64_t)rp[r + 1]; k++) {
+				const int64_t dd = (int64_t)col[z0 + k] - r;
+				band = std::max(band, dd < 0 ? -dd : dd);
+			}
+		h->mm_band[m] = band > 0 ? band : N;
 		h->mm_N[m] = N;
 		h->mm_erow0[m] = erow;
 		erow += N;
@@ -746,8 +867,13 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 		p.desc.resize(K);
 		int64_t vec = 0;
 		for (int m = 0; m < K; m++) {
-			p.desc[m] = MDesc{h->mm_N[m], h->mm_erow0[m], vec, 0, width[m],
-			                  kMaxWidth / width[m]};
+			MDesc &d = p.desc[m];
+			d = MDesc{};
+			d.N = h->mm_N[m];
+			d.erow0 = h->mm_erow0[m];
+			d.vec0 = vec;
+			d.w = width[m];
+			d.Q = kMaxWidth / width[m];
 			vec += h->mm_N[m] * width[m];
 		}
 		p.vec_elems = vec;
