@@ -16,11 +16,15 @@
 // the reduction index is contiguous for both operands; Mp = M rounded up to
 // 32, zero padded). Output G[K][E] complex FP32.
 //
-// Work mapping: a workgroup of 4 wavefronts computes a 128 x 128 complex
-// output tile; each wavefront a 64 x 64 tile as 2 x 2 MFMA tiles of
-// v_mfma_f32_32x32x16_bf16 for the real and imaginary parts. The reduction
-// runs over chunks of 32: both operand tiles (4 planes each) are staged in
-// LDS, then each wavefront reads its fragments and issues the MFMAs.
+// Kernels. contract_mfma_pipe8 (the default): double-buffered direct
+// global->LDS staging, 8 wavefronts per workgroup (2 per SIMD); 128 x 128
+// tiles for Retarded/Advanced, 128 x 256 tiles over the two used table
+// planes for Principal/NonPrincipal. contract_mfma (TBTKX_OPT_MFMA_PIPE = 0):
+// single-stage, 4 wavefronts, 128 x 128 tiles, operands staged through
+// VGPRs; it adds the same products in the same order per accumulator, so
+// both give identical outputs. Both use v_mfma_f32_32x32x16_bf16 with FP32
+// accumulators for the real and imaginary parts, the reduction in chunks of
+// 32 staged in LDS.
 // Executed MFMA flops per launch: 2 * K * E * M * 12 (Retarded/Advanced)
 // or * 6 (Principal/NonPrincipal).
 #include <cstring>
@@ -185,14 +189,14 @@ __global__ __launch_bounds__(kThreads) void contract_mfma(
 			}
 }
 
-// Pipelined variant. The operand tiles of chunk c+1 stream into the second
-// of two LDS stages by direct global->LDS loads (global_load_lds_dwordx4:
-// lane i of a wavefront writes 16 B at the wave-uniform LDS address + 16 i)
-// while the MFMAs of chunk c run, so no VGPRs hold staged data and the load
-// latency hides behind a whole chunk of MFMAs. LDS rows are 64 B (32 bf16,
-// no padding); the four 16 B quads of row r are stored XOR-swizzled by
-// (r >> 2) & 3, which makes the 16 lanes of each ds_read_b128 phase hit 16
-// distinct bank groups. 2 stages x 2 operands x 4 planes x 128 rows x 64 B =
+// Staging of the pipelined kernel. The operand tiles of chunk c+1 stream
+// into the second of two LDS stages by direct global->LDS loads
+// (global_load_lds_dwordx4: lane i of a wavefront writes 16 B at the
+// wave-uniform LDS address + 16 i) while the MFMAs of chunk c run, so no
+// VGPRs hold staged data and the load latency hides behind a whole chunk of
+// MFMAs. LDS rows are 64 B (32 bf16, no padding); the four 16 B quads of
+// row r are stored XOR-swizzled by (r >> 2) & 3, which makes the 16 lanes of
+// each ds_read_b128 phase hit 16 distinct bank groups. 2 stages x 64 KB =
 // 128 KB of LDS.
 constexpr int kRowB = 32;
 
@@ -200,51 +204,87 @@ __device__ __forceinline__ int swz(int row, int quad) { return quad ^ ((row >> 2
 
 typedef __attribute__((address_space(3))) void lds_void;
 
+// Pipelined kernel with two wavefronts per SIMD. A workgroup of 8
+// wavefronts (2 x 4) shares each double-buffered 64 KB stage, so one
+// wavefront's fragment reads overlap the other's MFMAs on the same SIMD.
+//   Retarded / Advanced: 128 x 128 tile, all four planes of both operands
+//   staged; each wavefront computes 64 x 32 (2 x 1 MFMA tiles for the real
+//   and imaginary parts).
+//   Principal / NonPrincipal: the table operand is one real plane pair
+//   (Re L or Im L), so a stage holds the 4 coefficient planes of 128 rows and
+//   the 2 used table planes of 256 rows; 128 x 256 tile, each wavefront
+//   64 x 64 (2 x 2 MFMA tiles).
+// Either way a k-step reads 12 fragments per 24 MFMAs per wavefront. Each
+// accumulator receives the same products in the same order as in the
+// single-stage kernel.
+constexpr int kThreads8 = 512;
+
 template <int TYPE>
-__global__ __launch_bounds__(kThreads) void contract_mfma_pipe(
+struct PipeShape {
+	static constexpr bool kHalf = TYPE == TBTKX_PRINCIPAL || TYPE == TBTKX_NONPRINCIPAL;
+	static constexpr int kTE = kHalf ? 256 : 128;   // energies per workgroup
+	static constexpr int kNBP = kHalf ? 2 : 4;      // table planes staged
+	static constexpr int kNB = kHalf ? 2 : 1;       // MFMA tiles along E per wavefront
+};
+
+template <int TYPE>
+__global__ __launch_bounds__(kThreads8) void contract_mfma_pipe8(
     const uint16_t *__restrict__ Cp, const uint16_t *__restrict__ Lp,
     float2 *__restrict__ G, int K, int E, int Mp, int ntile_e) {
-	__shared__ __attribute__((aligned(16))) uint16_t sm[2][2][4][kTileK * kRowB];
+	typedef PipeShape<TYPE> S;
+	constexpr int kNB = S::kNB, kNBP = S::kNBP, kTE = S::kTE;
+	constexpr int kPlaneA = kTileK * kRowB, kPlaneB = kTE * kRowB;
+	constexpr int kGroupsB = kTE / 16;   // 16-row load groups per table plane
+	static_assert(4 * kPlaneA == kNBP * kPlaneB, "stage halves differ");
+	__shared__ __attribute__((aligned(16))) uint16_t sm[2][2][4 * kPlaneA];
 	const int tid = threadIdx.x;
 	const int wave = tid >> 6, lane = tid & 63;
 	const int tk = blockIdx.x / ntile_e, te = blockIdx.x - tk * ntile_e;
-	const int k0 = tk * kTileK, e0 = te * kTileE;
-	const int wr = wave >> 1, wc = wave & 1;   // 2 x 2 wavefronts
+	const int k0 = tk * kTileK, e0 = te * kTE;
+	const int wr = wave >> 2, wc = wave & 3;   // 2 x 4 wavefronts
 	const int64_t planeC = (int64_t)K * Mp, planeL = (int64_t)E * Mp;
 	const int nc = Mp / kChunk;
+	// first table plane staged: Re L (Principal) or Im L (NonPrincipal)
+	const int lp0 = TYPE == TBTKX_NONPRINCIPAL ? 2 : 0;
 
-	// A stage is 64 loads of 1 KB (operand, plane, 16 rows); wavefront w
-	// issues loads w, w + 4, ..., w + 60. Lane l fetches row 16 (j & 7) + l / 4,
-	// the quad that belongs in LDS slot l % 4. Rows past K or E re-read the
-	// last valid row (their results are never stored).
+	// 64 loads of 1 KB per stage: j < 32 are (coefficient plane j / 8, rows
+	// 16 (j % 8) ..), j >= 32 are (staged table plane (j - 32) / kGroupsB,
+	// rows 16 ((j - 32) % kGroupsB) ..). Wavefront w issues j = w, w + 8,
+	// ..., w + 56; lane l fetches row (group start) + l / 4, the quad that
+	// belongs in LDS slot l % 4. Rows past K or E re-read the last valid row
+	// (their results are never stored).
 	const int lrow = lane >> 2;
 	auto issue = [&](int c, int buf) {
 		const int m0 = c * kChunk;
 #pragma unroll
-		for (int i = 0; i < 16; i++) {
-			const int j = wave + 4 * i;
-			const int op = i >= 8;
-			const int p = (j >> 3) & 3, rb = 16 * (j & 7);
-			const int r = rb + lrow;
-			const int q = swz(r, lane & 3);
+		for (int i = 0; i < 8; i++) {
+			const int j = wave + 8 * i;
 			const uint16_t *src;
-			if (!op) {
+			int off;
+			if (i < 4) {
+				const int p = j >> 3, rb = 16 * (j & 7);
+				const int r = rb + lrow;
 				const int row = k0 + r < K ? k0 + r : K - 1;
-				src = Cp + p * planeC + (int64_t)row * Mp + m0 + q * 8;
+				src = Cp + p * planeC + (int64_t)row * Mp + m0 + swz(r, lane & 3) * 8;
+				off = p * kPlaneA + rb * kRowB;
 			} else {
+				const int jj = j - 32;
+				const int p = jj / kGroupsB, rb = 16 * (jj % kGroupsB);
+				const int r = rb + lrow;
 				const int row = e0 + r < E ? e0 + r : E - 1;
-				src = Lp + p * planeL + (int64_t)row * Mp + m0 + q * 8;
+				src = Lp + (lp0 + p) * planeL + (int64_t)row * Mp + m0 + swz(r, lane & 3) * 8;
+				off = p * kPlaneB + rb * kRowB;
 			}
 			__builtin_amdgcn_global_load_lds((const void *)src,
-			                                 (lds_void *)&sm[buf][op][p][rb * kRowB], 16, 0, 0);
+			                                 (lds_void *)&sm[buf][i >= 4][off], 16, 0, 0);
 		}
 	};
 
-	f32x16 accR[2][2], accI[2][2];
+	f32x16 accR[2][kNB], accI[2][kNB];
 #pragma unroll
 	for (int a = 0; a < 2; a++)
 #pragma unroll
-		for (int b = 0; b < 2; b++) {
+		for (int b = 0; b < kNB; b++) {
 			accR[a][b] = (f32x16){};
 			accI[a][b] = (f32x16){};
 		}
@@ -261,32 +301,39 @@ __global__ __launch_bounds__(kThreads) void contract_mfma_pipe(
 #pragma unroll
 		for (int ks = 0; ks < kChunk; ks += 16) {
 			const int kq = ks / 8 + (lane >> 5);
-			bf16x8 A[4][2], Bm[4][2];
+			bf16x8 A[4][2], Bm[kNBP][kNB];
 #pragma unroll
 			for (int p = 0; p < 4; p++)
 #pragma unroll
 				for (int t = 0; t < 2; t++) {
 					const int ra = wr * 64 + t * 32 + (lane & 31);
-					const int rb = wc * 64 + t * 32 + (lane & 31);
 					A[p][t] = *reinterpret_cast<const bf16x8 *>(
-					    &sm[buf][0][p][ra * kRowB + swz(ra, kq) * 8]);
-					Bm[p][t] = *reinterpret_cast<const bf16x8 *>(
-					    &sm[buf][1][p][rb * kRowB + swz(rb, kq) * 8]);
+					    &sm[buf][0][p * kPlaneA + ra * kRowB + swz(ra, kq) * 8]);
 				}
+#pragma unroll
+			for (int p = 0; p < kNBP; p++)
+#pragma unroll
+				for (int t = 0; t < kNB; t++) {
+					const int rb = wc * (32 * kNB) + t * 32 + (lane & 31);
+					Bm[p][t] = *reinterpret_cast<const bf16x8 *>(
+					    &sm[buf][1][p * kPlaneB + rb * kRowB + swz(rb, kq) * 8]);
+				}
+#define TBTKX_MM(ACC, X, Y)                                                     \
+	_Pragma("unroll") for (int a = 0; a < 2; a++)                          \
+	    _Pragma("unroll") for (int b = 0; b < kNB; b++) ACC[a][b] = mma(X, Y, ACC[a][b]);
 			if (TYPE == TBTKX_RETARDED || TYPE == TBTKX_ADVANCED) {
 				// L' = (br, +-bi): re += ar br - ai bi', im += ar bi' + ai br
-				bf16x8 sih[2], sil[2], naih[2], nail[2];
+				bf16x8 sih[kNB], sil[kNB], naih[2], nail[2];
+#pragma unroll
+				for (int t = 0; t < kNB; t++) {
+					sih[t] = TYPE == TBTKX_RETARDED ? Bm[2 % kNBP][t] : neg(Bm[2 % kNBP][t]);
+					sil[t] = TYPE == TBTKX_RETARDED ? Bm[3 % kNBP][t] : neg(Bm[3 % kNBP][t]);
+				}
 #pragma unroll
 				for (int t = 0; t < 2; t++) {
-					sih[t] = TYPE == TBTKX_RETARDED ? Bm[2][t] : neg(Bm[2][t]);
-					sil[t] = TYPE == TBTKX_RETARDED ? Bm[3][t] : neg(Bm[3][t]);
 					naih[t] = neg(A[2][t]);
 					nail[t] = neg(A[3][t]);
 				}
-				// Term by term over the eight independent accumulators.
-#define TBTKX_MM(ACC, X, Y)                                                     \
-	_Pragma("unroll") for (int a = 0; a < 2; a++)                          \
-	    _Pragma("unroll") for (int b = 0; b < 2; b++) ACC[a][b] = mma(X, Y, ACC[a][b]);
 				TBTKX_MM(accR, A[0][a], Bm[0][b])
 				TBTKX_MM(accI, A[0][a], sih[b])
 				TBTKX_MM(accR, A[0][a], Bm[1][b])
@@ -299,46 +346,41 @@ __global__ __launch_bounds__(kThreads) void contract_mfma_pipe(
 				TBTKX_MM(accI, A[2][a], Bm[1][b])
 				TBTKX_MM(accR, nail[a], sih[b])
 				TBTKX_MM(accI, A[3][a], Bm[0][b])
-			} else if (TYPE == TBTKX_PRINCIPAL) {
-				// L' = -Re L: re += ar (-br), im += ai (-br)
-				bf16x8 nbh[2], nbl[2];
-#pragma unroll
-				for (int t = 0; t < 2; t++) {
-					nbh[t] = neg(Bm[0][t]);
-					nbl[t] = neg(Bm[1][t]);
-				}
-				TBTKX_MM(accR, A[0][a], nbh[b])
-				TBTKX_MM(accI, A[2][a], nbh[b])
-				TBTKX_MM(accR, A[0][a], nbl[b])
-				TBTKX_MM(accI, A[2][a], nbl[b])
-				TBTKX_MM(accR, A[1][a], nbh[b])
-				TBTKX_MM(accI, A[3][a], nbh[b])
 			} else {
-				// L' = -i Im L = (0, -bi): re += ai bi, im += -ar bi
-				bf16x8 nbh[2], nbl[2];
+				bf16x8 nB[2][kNB];
 #pragma unroll
-				for (int t = 0; t < 2; t++) {
-					nbh[t] = neg(Bm[2][t]);
-					nbl[t] = neg(Bm[3][t]);
+				for (int p = 0; p < 2; p++)
+#pragma unroll
+					for (int t = 0; t < kNB; t++) nB[p][t] = neg(Bm[p][t]);
+				if (TYPE == TBTKX_PRINCIPAL) {
+					// L' = -Re L: re += ar (-br), im += ai (-br)
+					TBTKX_MM(accR, A[0][a], nB[0][b])
+					TBTKX_MM(accI, A[2][a], nB[0][b])
+					TBTKX_MM(accR, A[0][a], nB[1][b])
+					TBTKX_MM(accI, A[2][a], nB[1][b])
+					TBTKX_MM(accR, A[1][a], nB[0][b])
+					TBTKX_MM(accI, A[3][a], nB[0][b])
+				} else {
+					// L' = -i Im L = (0, -bi): re += ai bi, im += -ar bi
+					TBTKX_MM(accR, A[2][a], Bm[0][b])
+					TBTKX_MM(accI, A[0][a], nB[0][b])
+					TBTKX_MM(accR, A[2][a], Bm[1][b])
+					TBTKX_MM(accI, A[0][a], nB[1][b])
+					TBTKX_MM(accR, A[3][a], Bm[0][b])
+					TBTKX_MM(accI, A[1][a], nB[0][b])
 				}
-				TBTKX_MM(accR, A[2][a], Bm[2][b])
-				TBTKX_MM(accI, A[0][a], nbh[b])
-				TBTKX_MM(accR, A[2][a], Bm[3][b])
-				TBTKX_MM(accI, A[0][a], nbl[b])
-				TBTKX_MM(accR, A[3][a], Bm[2][b])
-				TBTKX_MM(accI, A[1][a], nbh[b])
-#undef TBTKX_MM
 			}
+#undef TBTKX_MM
 		}
 	}
 #pragma unroll
 	for (int a = 0; a < 2; a++)
 #pragma unroll
-		for (int b = 0; b < 2; b++)
+		for (int b = 0; b < kNB; b++)
 #pragma unroll
 			for (int r = 0; r < 16; r++) {
 				const int row = k0 + wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-				const int col = e0 + wc * 64 + b * 32 + (lane & 31);
+				const int col = e0 + wc * (32 * kNB) + b * 32 + (lane & 31);
 				if (row < K && col < E)
 					G[(int64_t)row * E + col] = make_float2(accR[a][b][r], accI[a][b][r]);
 			}
@@ -400,10 +442,13 @@ int tbtkx_contract_mfma(tbtkx_handle h, int type, double *out) {
 	TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
 #define TBTKX_CM(T)                                                              \
 	do {                                                                     \
-		if (h->mfma_pipe)                                                \
-			contract_mfma_pipe<T><<<grid, kThreads, 0, h->stream>>>(Cp, Lp, G, K, E, Mp, nte); \
-		else                                                             \
+		if (h->mfma_pipe) {                                              \
+			const int n8 = (E + PipeShape<T>::kTE - 1) / PipeShape<T>::kTE; \
+			contract_mfma_pipe8<T><<<(unsigned)ntk * (unsigned)n8, kThreads8, 0, h->stream>>>( \
+			    Cp, Lp, G, K, E, Mp, n8);                            \
+		} else {                                                         \
 			contract_mfma<T><<<grid, kThreads, 0, h->stream>>>(Cp, Lp, G, K, E, Mp, nte); \
+		}                                                                \
 	} while (0)
 	switch (type) {
 	case TBTKX_RETARDED: TBTKX_CM(TBTKX_RETARDED); break;

@@ -35,9 +35,13 @@ def test_mfma_random_coefficients(engine, mfma_kernel, gtype):
     assert err < 1e-4, err
 
 
-def test_mfma_kernels_agree_bitwise(engine):
-    """Both kernels add the same BF16 products into FP32 in the same order per
-    accumulator, so their outputs are identical."""
+@pytest.mark.parametrize("gtype", [X.RETARDED, X.ADVANCED, X.PRINCIPAL, X.NONPRINCIPAL])
+def test_mfma_kernels_agree_bitwise(engine, gtype):
+    """The single-stage and the pipelined kernels (128 x 128 tiles for the
+    complex types, 128 x 256 tiles over two table planes for Principal and
+    NonPrincipal) add the same BF16 products into FP32 in the same order per
+    accumulator, so their outputs are identical; K and E are not multiples
+    of either tile."""
     K, M, E, s = 260, 96, 390, 6.0
     rng = np.random.default_rng(9)
     c = (rng.normal(size=(K, M)) + 1j * rng.normal(size=(K, M))) / np.sqrt(M)
@@ -46,7 +50,7 @@ def test_mfma_kernels_agree_bitwise(engine):
     out = {}
     for pipe in (1, 0):
         engine.set_option(X.OPT_MFMA_PIPE, pipe)
-        out[pipe] = engine.contract_mfma(X.RETARDED)
+        out[pipe] = engine.contract_mfma(gtype)
     engine.set_option(X.OPT_MFMA_PIPE, 1)
     assert np.array_equal(out[0], out[1])
 
