@@ -166,7 +166,9 @@ def batched_comparison(eng):
     ld_batched = contract(mu)
     t_batched = time.perf_counter() - t0
 
+    packed = eng.get_option(X.OPT_REAL) == 1
     eng.moments(int(sites[0]), [int(sites[0])], 16)
+    plain_real = eng.get_option(X.OPT_REAL) == 1
     t0 = time.perf_counter()
     ref = []
     plain_ms = 0.0
@@ -197,13 +199,17 @@ def batched_comparison(eng):
         "speedup_vs_plain_fp64": sps_b / sps_p,
         "max_ldos_rel_err_vs_fp64": float(err),
         "error_sites_compared": PLAIN_CMP_SITES,
-        "batched_dtype": "fp32 storage (H and vectors), fp64 accumulate, doubling",
+        "batched_dtype": ("fp32 storage (H and vectors), fp64 accumulate, doubling"
+                          + (", real H: two real vectors per complex column" if packed else "")),
         "spmm_block_width": B,
+        "vectors_per_block": 2 * B if packed else B,
         "spmm_shape": shape,
         "spmm_step_ms": per_step * 1e3,
         "spmm_GBps": step_bytes / per_step / 1e9,
         "spmm_hbm_frac": step_bytes / per_step / 1e9 / HBM_PEAK_GBPS,
-        "plain_spmv_GBps": (20 * Z + 52 * N) / (plain_ms * 1e-3 / (PLAIN_CMP_SITES * (M_BATCH - 1))) / 1e9,
+        "plain_arithmetic": "real fp64" if plain_real else "complex fp64",
+        "plain_spmv_GBps": ((12 * Z + 28 * N) if plain_real else (20 * Z + 52 * N))
+        / (plain_ms * 1e-3 / (PLAIN_CMP_SITES * (M_BATCH - 1))) / 1e9,
     }
 
 
@@ -225,12 +231,15 @@ def uneven_batch(eng):
     M = np.array(Ms, dtype=np.int64)
     S = (M + 1) // 2
     W = 6 if max(np.diff(c[0]).max() for c in csrs) > 4 else 4
-    step_bytes_total = float(((12 * W * N + 24 * B * N) * S).sum())
     # 1: autotune + direct launches; 2: hipGraph capture; 3: graph replay (timed)
     t0 = time.perf_counter()
     mu = eng.multi_moments(sites, Ms, X.DOUBLING)
     t_first = time.perf_counter() - t0
     ms_direct, launches = eng.last_timing()
+    # complex columns per model: two real vectors per column for a real batch
+    packed = eng.get_option(X.OPT_REAL) == 1
+    cols = (B + 1) // 2 if packed else B
+    step_bytes_total = float(((12 * W * N + 24 * cols * N) * S).sum())
     eng.set_option(X.OPT_GRAPH, 1)
     try:
         eng.multi_moments(sites, Ms, X.DOUBLING)
@@ -263,7 +272,8 @@ def uneven_batch(eng):
         "nnz": int(sum(c[1].size for c in csrs)),
         "ell_slots": W,
         "recurrence_steps": int(S.max()),
-        "dtype": "fp32 storage (H and vectors), fp64 moments, doubling",
+        "dtype": ("fp32 storage (H and vectors), fp64 moments, doubling"
+                  + (", real H: two real vectors per complex column" if packed else "")),
         "sites_per_s": float(B.sum()) / (t_graph),
         "device_s": dev_s,
         "direct_device_s": ms_direct * 1e-3,
@@ -472,7 +482,9 @@ def main():
         eng.lut_destroy()
         sites_per_step = 2
         value = args.steps * sites_per_step / elapsed
-        step_bytes = 20 * Z + 52 * N
+        # A real H runs the real FP64 recurrence (same moments, fewer bytes).
+        real = eng.get_option(X.OPT_REAL) == 1
+        step_bytes = 12 * Z + 28 * N if real else 20 * Z + 52 * N
         avg_kernel_s = kernel_ms * 1e-3 / launches
         achieved = step_bytes / avg_kernel_s / 1e9
         dE = (WINDOW[1] - WINDOW[0]) / (E_PTS - 1)
@@ -499,13 +511,15 @@ def main():
             "physical_sites_per_s": value / 2,
             "spmv_GBps": achieved,
             "roofline": {
-                "kernel": "plain_step_lds (FP64 CSR SpMV recurrence step)",
+                "kernel": ("plain_step_lds<double> (real FP64 CSR SpMV recurrence step; "
+                           "H is real)" if real else
+                           "plain_step_lds<double2> (complex FP64 CSR SpMV recurrence step)"),
                 "bound": "hbm",
                 "achieved": achieved,
                 "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBPS,
-                "traffic": load_traffic("plain_step_lds"),
+                "traffic": load_traffic("plain_step_lds_real" if real else "plain_step_lds"),
                 "algorithmic_bytes_per_launch": step_bytes,
                 "avg_launch_us": avg_kernel_s * 1e6,
             },

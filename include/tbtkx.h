@@ -215,16 +215,26 @@ enum {
 	                           (0 = autotuned, default) */
 	TBTKX_OPT_STRIP_CHUNK = 12, /* strip traversal: consecutive tiles dealt to
 	                           one workgroup at a time (multiple of 4, 0 = 4) */
-	TBTKX_OPT_ROWS_IN_FLIGHT = 13 /* strip traversal, FP32: 1 or 2 times the
+	TBTKX_OPT_ROWS_IN_FLIGHT = 13, /* strip traversal, FP32: 1 or 2 times the
 	                           default rows a wavefront keeps in flight
 	                           (0 = autotuned, default) */
+	TBTKX_OPT_REAL = 14     /* 1 = when every value of H is real and the start
+	                           vectors are unit vectors, run the recurrence in
+	                           real arithmetic (default): the plain path on
+	                           real FP64 vectors, the batched paths with two
+	                           start vectors per complex column (one in the
+	                           real, one in the imaginary part), so half the
+	                           vector bytes. The moments equal those of the
+	                           complex recurrence (whose imaginary parts stay
+	                           exactly zero). 0 = always complex */
 };
 int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
 
 /* Effective value of an option: TBTKX_OPT_BLOCK_WIDTH, TBTKX_OPT_STRIP_ROWS
  * and TBTKX_OPT_GRID_SHARE give what the last batched call used (the
  * autotuner's choice when they are 0); TBTKX_OPT_ELL gives 1 only when the
- * ELL copy exists and is enabled. */
+ * ELL copy exists and is enabled; TBTKX_OPT_REAL gives 1 when the last
+ * recurrence call (plain, batched or uneven batch) ran in real arithmetic. */
 int tbtkx_get_option(tbtkx_handle handle, int option, int64_t *value);
 
 #ifdef __cplusplus

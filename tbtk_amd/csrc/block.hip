@@ -48,6 +48,39 @@ template <> struct C2<double2> {
 	__device__ static double2 store(double2 a) { return a; }
 };
 
+// Packed real vectors: with a real H and real start vectors the real and
+// imaginary parts of a complex column evolve independently, so column c
+// carries two real vectors, vector 2c in the real part and 2c + 1 in the
+// imaginary part. The step kernels are the same; only the start block, the
+// moment gather and the doubling dots see the packing.
+
+// Moment of live vector g at its site.
+template <typename VT>
+__device__ __forceinline__ double2 site_moment(const VT *__restrict__ X,
+                                               const int64_t *__restrict__ sites, int g,
+                                               int B, int packed) {
+	if (!packed) return C2<VT>::load(X[sites[g] * B + g]);
+	const double2 v = C2<VT>::load(X[sites[g] * B + (g >> 1)]);
+	return make_double2((g & 1) ? v.y : v.x, 0.0);
+}
+
+// Doubling dots of one element, FP64: d0 += |x|^2 and d1 += conj(y) x for a
+// complex column; per component for a packed column, d0 += (x.re^2, x.im^2)
+// and d1 += (y.re x.re, y.im x.im).
+__device__ __forceinline__ void dot_update(double2 x, double2 y, int packed, double2 &d0,
+                                           double2 &d1) {
+	if (packed) {
+		d0.x = fma(x.x, x.x, d0.x);
+		d0.y = fma(x.y, x.y, d0.y);
+		d1.x = fma(y.x, x.x, d1.x);
+		d1.y = fma(y.y, x.y, d1.y);
+	} else {
+		d0.x = fma(x.x, x.x, fma(x.y, x.y, d0.x));
+		d1.x = fma(y.x, x.x, fma(y.y, x.y, d1.x));
+		d1.y = fma(y.x, x.y, fma(-y.y, x.x, d1.y));
+	}
+}
+
 // A lane's CPL consecutive complex values of one block row, moved with the
 // widest aligned accesses (16 B): FP32 pairs as float4, FP64 as double2.
 template <typename VT, int CPL> struct Row {
@@ -138,19 +171,24 @@ template <int CPL> struct RowNT<double2, CPL> {
 	}
 };
 
-// Initial block: unit vectors at sites[b] (mode 0) or random phases
-// exp(2 pi i u(seed, (col0 + b) N + row)) (mode 1). Columns past the number
-// of live vectors are zero.
+// Initial block: unit vectors at sites[b] (mode 0; packed: two per column)
+// or random phases exp(2 pi i u(seed, (col0 + b) N + row)) (mode 1). Columns
+// past the live vectors are zero.
 template <typename VT>
 __global__ __launch_bounds__(kThreads) void block_init(
     VT *__restrict__ X, int64_t N, int B, int live, int mode,
-    const int64_t *__restrict__ sites, uint64_t seed, int64_t col0) {
+    const int64_t *__restrict__ sites, uint64_t seed, int64_t col0, int packed) {
 	const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
 	if (i >= N * B) return;
 	const int64_t row = i / B;
 	const int b = (int)(i - row * B);
 	double2 v = make_double2(0.0, 0.0);
-	if (b < live) {
+	if (mode == 0 && packed) {
+		// live counts real vectors: 2b in the real part, 2b + 1 in the
+		// imaginary part of column b
+		if (2 * b < live && sites[2 * b] == row) v.x = 1.0;
+		if (2 * b + 1 < live && sites[2 * b + 1] == row) v.y = 1.0;
+	} else if (b < live) {
 		if (mode == 0) {
 			if (sites[b] == row) v.x = 1.0;
 		} else {
@@ -261,7 +299,7 @@ __global__ __launch_bounds__(kThreads) void spmm_step(
     const HT *__restrict__ v, const VT *__restrict__ X, VT *Y, int64_t N,
     int tile, int nwg, int remap, int nt, double2 *__restrict__ part,
     const int64_t *__restrict__ sites, int live, double2 *__restrict__ mu,
-    int M, int nprev) {
+    int M, int nprev, int packed) {
 	constexpr int CPL = B / 64;   // complex values per lane
 	// Rows in flight per wavefront: CPL * RPI = 4 complex columns per lane.
 	constexpr int RPI = CPL >= 4 ? 1 : 4 / CPL;
@@ -274,15 +312,13 @@ __global__ __launch_bounds__(kThreads) void spmm_step(
 	if (!DOTS) {
 		const int g = blockIdx.x * kThreads + threadIdx.x;
 		if (g < live)
-			mu[(int64_t)g * M + nprev] =
-			    C2<VT>::load(X[sites[g] * B + g]);
+			mu[(int64_t)g * M + nprev] = site_moment<VT>(X, sites, g, B, packed);
 	}
 
-	double d0[CPL];
-	double2 d1[CPL];
+	double2 d0[CPL], d1[CPL];
 #pragma unroll
 	for (int j = 0; j < CPL; j++) {
-		d0[j] = 0.0;
+		d0[j] = make_double2(0.0, 0.0);
 		d1[j] = make_double2(0.0, 0.0);
 	}
 
@@ -379,10 +415,7 @@ __global__ __launch_bounds__(kThreads) void spmm_step(
 					for (int j = 0; j < CPL; j++) {
 						const double2 y = C2<VT>::load(st[j]);
 						const double2 x = C2<VT>::load(xs[r][j]);
-						d0[j] = fma(x.x, x.x, fma(x.y, x.y, d0[j]));
-						// conj(y) * x
-						d1[j].x = fma(y.x, x.x, fma(y.y, x.y, d1[j].x));
-						d1[j].y = fma(y.x, x.y, fma(-y.y, x.x, d1[j].y));
+						dot_update(x, y, packed, d0[j], d1[j]);
 					}
 				}
 			}
@@ -394,7 +427,7 @@ __global__ __launch_bounds__(kThreads) void spmm_step(
 		__shared__ double2 red[kWaves][2][B];
 #pragma unroll
 		for (int j = 0; j < CPL; j++) {
-			red[wave][0][lane * CPL + j] = make_double2(d0[j], 0.0);
+			red[wave][0][lane * CPL + j] = d0[j];
 			red[wave][1][lane * CPL + j] = d1[j];
 		}
 		__syncthreads();
@@ -427,7 +460,7 @@ __global__ __launch_bounds__(kThreads) void spmm_step_ell(
     const uint32_t *__restrict__ ecol, const HT *__restrict__ eval,
     const VT *__restrict__ X, VT *Y, int64_t N, int nwg, int remap, int nt,
     double2 *__restrict__ part, const int64_t *__restrict__ sites, int live,
-    double2 *__restrict__ mu, int M, int nprev) {
+    double2 *__restrict__ mu, int M, int nprev, int packed) {
 	constexpr int CPL = B / 64;
 	constexpr int T = 64 / W;          // rows per tile
 	constexpr int TW = T * W;          // entries per tile (<= 64)
@@ -438,14 +471,13 @@ __global__ __launch_bounds__(kThreads) void spmm_step_ell(
 	if (!DOTS) {
 		const int g = blockIdx.x * kThreads + threadIdx.x;
 		if (g < live)
-			mu[(int64_t)g * M + nprev] = C2<VT>::load(X[sites[g] * B + g]);
+			mu[(int64_t)g * M + nprev] = site_moment<VT>(X, sites, g, B, packed);
 	}
 
-	double d0[CPL];
-	double2 d1[CPL];
+	double2 d0[CPL], d1[CPL];
 #pragma unroll
 	for (int j = 0; j < CPL; j++) {
-		d0[j] = 0.0;
+		d0[j] = make_double2(0.0, 0.0);
 		d1[j] = make_double2(0.0, 0.0);
 	}
 
@@ -532,9 +564,7 @@ __global__ __launch_bounds__(kThreads) void spmm_step_ell(
 					for (int j = 0; j < CPL; j++) {
 						const double2 y = C2<VT>::load(st[j]);
 						const double2 x = C2<VT>::load(xs[r][j]);
-						d0[j] = fma(x.x, x.x, fma(x.y, x.y, d0[j]));
-						d1[j].x = fma(y.x, x.x, fma(y.y, x.y, d1[j].x));
-						d1[j].y = fma(y.x, x.y, fma(-y.y, x.x, d1[j].y));
+						dot_update(x, y, packed, d0[j], d1[j]);
 					}
 				}
 			}
@@ -547,7 +577,7 @@ __global__ __launch_bounds__(kThreads) void spmm_step_ell(
 		__shared__ double2 red[kWaves][2][B];
 #pragma unroll
 		for (int j = 0; j < CPL; j++) {
-			red[wave][0][lane * CPL + j] = make_double2(d0[j], 0.0);
+			red[wave][0][lane * CPL + j] = d0[j];
 			red[wave][1][lane * CPL + j] = d1[j];
 		}
 		__syncthreads();
@@ -610,7 +640,7 @@ __global__ __launch_bounds__(kThreads) void spmm_step_strip(
     const uint32_t *__restrict__ ecol, const HT *__restrict__ eval,
     const VT *__restrict__ X, VT *Y, StripMap sm, int nwg, int nt,
     double2 *__restrict__ part, const int64_t *__restrict__ sites, int live,
-    double2 *__restrict__ mu, int M, int nprev) {
+    double2 *__restrict__ mu, int M, int nprev, int packed) {
 	constexpr int CPL = B / 64;
 	// Rows a wavefront keeps in flight: 4 / CPL (at least 1), times RMUL
 	// (2 doubles the bytes in flight per wavefront for the price of fewer
@@ -625,14 +655,13 @@ __global__ __launch_bounds__(kThreads) void spmm_step_strip(
 	if (!DOTS) {
 		const int g = blockIdx.x * kThreads + threadIdx.x;
 		if (g < live)
-			mu[(int64_t)g * M + nprev] = C2<VT>::load(X[sites[g] * B + g]);
+			mu[(int64_t)g * M + nprev] = site_moment<VT>(X, sites, g, B, packed);
 	}
 
-	double d0[CPL];
-	double2 d1[CPL];
+	double2 d0[CPL], d1[CPL];
 #pragma unroll
 	for (int j = 0; j < CPL; j++) {
-		d0[j] = 0.0;
+		d0[j] = make_double2(0.0, 0.0);
 		d1[j] = make_double2(0.0, 0.0);
 	}
 
@@ -750,9 +779,7 @@ __global__ __launch_bounds__(kThreads) void spmm_step_strip(
 				for (int j = 0; j < CPL; j++) {
 					const double2 y = C2<VT>::load(pst[r][j]);
 					const double2 x = C2<VT>::load(xs[r][j]);
-					d0[j] = fma(x.x, x.x, fma(x.y, x.y, d0[j]));
-					d1[j].x = fma(y.x, x.x, fma(y.y, x.y, d1[j].x));
-					d1[j].y = fma(y.x, x.y, fma(-y.y, x.x, d1[j].y));
+					dot_update(x, y, packed, d0[j], d1[j]);
 				}
 			}
 		}
@@ -774,7 +801,7 @@ __global__ __launch_bounds__(kThreads) void spmm_step_strip(
 		__shared__ double2 red[kWaves][2][B];
 #pragma unroll
 		for (int j = 0; j < CPL; j++) {
-			red[wave][0][lane * CPL + j] = make_double2(d0[j], 0.0);
+			red[wave][0][lane * CPL + j] = d0[j];
 			red[wave][1][lane * CPL + j] = d1[j];
 		}
 		__syncthreads();
@@ -812,9 +839,9 @@ __global__ __launch_bounds__(kThreads) void reduce_parts(
 template <typename VT>
 __global__ __launch_bounds__(kThreads) void gather_block(
     const VT *__restrict__ X, const int64_t *__restrict__ sites, int live,
-    int B, double2 *__restrict__ mu, int M, int n) {
+    int B, double2 *__restrict__ mu, int M, int n, int packed) {
 	const int g = blockIdx.x * kThreads + threadIdx.x;
-	if (g < live) mu[(int64_t)g * M + n] = C2<VT>::load(X[sites[g] * B + g]);
+	if (g < live) mu[(int64_t)g * M + n] = site_moment<VT>(X, sites, g, B, packed);
 }
 
 inline unsigned grid_for(int64_t n) {
@@ -857,7 +884,7 @@ StripMap make_strip(const tbtkx_context *h, int S, int T) {
 template <typename VT, typename HT, int B>
 int launch_step(BlockRun &r, bool first, bool dots, const VT *X, VT *Y,
                 double2 *part, const int64_t *sites, int live, double2 *mu,
-                int M, int nprev) {
+                int M, int nprev, int packed) {
 	tbtkx_context *h = r.h;
 	const HT *v = first ? h->v1.as<HT>() : h->v2.as<HT>();
 	if (std::is_same<HT, float2>::value)
@@ -881,17 +908,17 @@ int launch_step(BlockRun &r, bool first, bool dots, const VT *X, VT *Y,
 			spmm_step_strip<VT, HT, AT, B, F, D, R, WW, 2>         \
 			    <<<r.nwg, kThreads, 0, h->stream>>>(               \
 			        ec, ev, X, Y, r.sm, r.nwg, h->nt_stream, part, \
-			        sites, live, mu, M, nprev);                    \
+			        sites, live, mu, M, nprev, packed);            \
 		else if (r.S > 0)                                              \
 			spmm_step_strip<VT, HT, AT, B, F, D, R, WW, 1>         \
 			    <<<r.nwg, kThreads, 0, h->stream>>>(               \
 			        ec, ev, X, Y, r.sm, r.nwg, h->nt_stream, part, \
-			        sites, live, mu, M, nprev);                    \
+			        sites, live, mu, M, nprev, packed);            \
 		else                                                           \
 			spmm_step_ell<VT, HT, AT, B, F, D, R, WW>              \
 			    <<<r.nwg, kThreads, 0, h->stream>>>(               \
 			        ec, ev, X, Y, h->N, r.nwg, h->xcd_remap,       \
-			        h->nt_stream, part, sites, live, mu, M, nprev); \
+			        h->nt_stream, part, sites, live, mu, M, nprev, packed); \
 	} while (0)
 #define TBTKX_ELL_W(F, D, R)                                                   \
 	do {                                                                   \
@@ -917,7 +944,7 @@ int launch_step(BlockRun &r, bool first, bool dots, const VT *X, VT *Y,
 #define TBTKX_STEP(F, D, R)                                                    \
 	spmm_step<VT, HT, AT, B, F, D, R><<<r.nwg, kThreads, 0, h->stream>>>(  \
 	    rp, cl, v, X, Y, h->N, r.tile, r.nwg, h->xcd_remap, h->nt_stream,   \
-	    part, sites, live, mu, M, nprev)
+	    part, sites, live, mu, M, nprev, packed)
 #define TBTKX_STEP_R(F, D)                                                     \
 	do {                                                                   \
 		if (h->h_real) TBTKX_STEP(F, D, true);                         \
@@ -938,20 +965,20 @@ int launch_step(BlockRun &r, bool first, bool dots, const VT *X, VT *Y,
 template <typename VT, typename HT, int B>
 int run_block(BlockRun &r, int live, const int64_t *d_sites, int mode,
               uint64_t seed, int64_t col0, int M, double2 *mu_dev,
-              double2 *D_dev) {
+              double2 *D_dev, int packed) {
 	tbtkx_context *h = r.h;
 	const bool dots = (r.flags & TBTKX_DOUBLING) != 0;
 	VT *X = h->blk_x.as<VT>(), *Y = h->blk_y.as<VT>();
 	double2 *part = h->blk_part.as<double2>();
 	block_init<VT><<<grid_for(h->N * B), kThreads, 0, h->stream>>>(
-	    X, h->N, B, live, mode, d_sites, seed, col0);
+	    X, h->N, B, live, mode, d_sites, seed, col0, packed);
 	const int S = dots ? (M + 1) / 2 : M - 1;
 	int chunk0 = 0;
 	for (int n = 0; n < S; n++) {
 		// Step n computes Phi_{n+1} from Phi_n (X) and Phi_{n-1} (Y).
 		double2 *p = part + (int64_t)(n - chunk0) * r.nwg * 2 * B;
 		int rc = launch_step<VT, HT, B>(r, n == 0, dots, X, Y, p, d_sites,
-		                                live, mu_dev, M, n);
+		                                live, mu_dev, M, n, packed);
 		if (rc) return rc;
 		VT *t = X;
 		X = Y;
@@ -966,7 +993,7 @@ int run_block(BlockRun &r, int live, const int64_t *d_sites, int mode,
 	}
 	if (!dots)
 		gather_block<VT><<<grid_for(live), kThreads, 0, h->stream>>>(
-		    X, d_sites, live, B, mu_dev, M, M - 1);
+		    X, d_sites, live, B, mu_dev, M, M - 1, packed);
 	TBTKX_CHECK_HIP(hipGetLastError());
 	return TBTKX_OK;
 }
@@ -1047,7 +1074,7 @@ int persistent_grid(tbtkx_context *h, bool dots, BlockRun &r) {
 
 template <typename VT, typename HT, int B>
 int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
-            uint64_t seed, int M, int flags, std::vector<double2> &host_mu) {
+            uint64_t seed, int M, int flags, int packed, std::vector<double2> &host_mu) {
 	// Tile: rows a wavefront takes per round-robin turn, a multiple of the
 	// rows it keeps in flight (RPI of spmm_step), at most 32.
 	constexpr int kRpi = B >= 256 ? 1 : 256 / B;
@@ -1062,10 +1089,12 @@ int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
 	if (rc) return rc;
 	const bool dots = (flags & TBTKX_DOUBLING) != 0;
 	const int S = dots ? (M + 1) / 2 : M - 1;
-	const int nblocks = (total + B - 1) / B;
+	// Vectors per block: B columns, two real vectors per column when packed.
+	const int vpb = packed ? 2 * B : B;
+	const int nblocks = (total + vpb - 1) / vpb;
 	// Every block writes its own moment region, so the blocks run back to
 	// back with one copy to the host at the end.
-	const size_t per_block = dots ? (size_t)(S > 0 ? S : 1) * 2 * B : (size_t)B * M;
+	const size_t per_block = dots ? (size_t)(S > 0 ? S : 1) * 2 * B : (size_t)vpb * M;
 	if ((rc = h->blk_x.reserve((size_t)h->N * B * sizeof(VT)))) return rc;
 	if ((rc = h->blk_y.reserve((size_t)h->N * B * sizeof(VT)))) return rc;
 	if (dots &&
@@ -1079,10 +1108,10 @@ int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
 
 	auto enqueue = [&]() -> int {
 		for (int blk = 0; blk < nblocks; blk++) {
-			const int live = total - blk * B < B ? total - blk * B : B;
+			const int live = total - blk * vpb < vpb ? total - blk * vpb : vpb;
 			double2 *mu = h->blk_mu.as<double2>() + per_block * blk;
-			int erc = run_block<VT, HT, B>(r, live, h->blk_sites.as<int64_t>() + (int64_t)blk * B,
-			                               mode, seed, (int64_t)blk * B, M, mu, mu);
+			int erc = run_block<VT, HT, B>(r, live, h->blk_sites.as<int64_t>() + (int64_t)blk * vpb,
+			                               mode, seed, (int64_t)blk * B, M, mu, mu, packed);
 			if (erc) return erc;
 		}
 		return TBTKX_OK;
@@ -1095,7 +1124,7 @@ int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
 		    key,
 		    {total, M, flags, mode, (int64_t)seed, B, (int64_t)r.nwg * 64 + r.tile,
 		     h->use_ell * 4 + h->nt_stream * 2 + h->xcd_remap, h->N, h->ell_W, r.S,
-		     h->strip_chunk, r.rmul},
+		     h->strip_chunk, r.rmul, packed},
 		    {h->blk_x.ptr, h->blk_y.ptr, h->blk_part.ptr, h->blk_mu.ptr, h->blk_sites.ptr,
 		     h->rowptr.ptr, h->col.ptr, h->v1.ptr, h->v2.ptr, h->v1f.ptr, h->v2f.ptr,
 		     h->ell_col.ptr, h->ell_v1f.ptr, h->ell_v2f.ptr});
@@ -1128,28 +1157,36 @@ int run_all(tbtkx_context *h, int total, const int64_t *sites, int mode,
 	h->last_strip = r.S;
 	h->last_share = r.share;
 	h->last_rmul = r.rmul;
+	h->last_real = packed;
 
 	host_mu.assign((size_t)total * M, make_double2(0, 0));
 	for (int blk = 0; blk < nblocks; blk++) {
-		const int live = total - blk * B < B ? total - blk * B : B;
+		const int live = total - blk * vpb < vpb ? total - blk * vpb : vpb;
 		const double2 *bb = buf.data() + per_block * blk;
 		for (int b = 0; b < live; b++) {
-			double2 *o = host_mu.data() + (size_t)(blk * B + b) * M;
+			double2 *o = host_mu.data() + (size_t)(blk * vpb + b) * M;
 			if (!dots) {
 				memcpy(o, bb + (size_t)b * M, (size_t)M * 16);
 				continue;
 			}
-			// D[n][0][b] = <Phi_n|Phi_n>, D[n][1][b] = <Phi_n+1|Phi_n>
-			const double2 mu0 = bb[b];
-			const double2 mu1 = M > 1 ? bb[B + b] : make_double2(0, 0);
+			// D[n][0][c] = <Phi_n|Phi_n>, D[n][1][c] = <Phi_n+1|Phi_n> of
+			// column c; packed: per component, vector b in column b / 2.
+			const int c = packed ? b >> 1 : b;
+			auto dval = [&](size_t i) {
+				const double2 d = bb[i + c];
+				if (!packed) return d;
+				return make_double2((b & 1) ? d.y : d.x, 0.0);
+			};
+			const double2 mu0 = dval(0);
+			const double2 mu1 = M > 1 ? dval(B) : make_double2(0, 0);
 			o[0] = mu0;
 			if (M > 1) o[1] = mu1;
 			for (int n = 1; n < S; n++) {
-				const double2 a = bb[(size_t)n * 2 * B + b];
-				const double2 c = bb[(size_t)n * 2 * B + B + b];
+				const double2 a = dval((size_t)n * 2 * B);
+				const double2 e = dval((size_t)n * 2 * B + B);
 				if (2 * n < M) o[2 * n] = make_double2(2 * a.x - mu0.x, 2 * a.y - mu0.y);
 				if (2 * n + 1 < M)
-					o[2 * n + 1] = make_double2(2 * c.x - mu1.x, 2 * c.y - mu1.y);
+					o[2 * n + 1] = make_double2(2 * e.x - mu1.x, 2 * e.y - mu1.y);
 			}
 		}
 	}
@@ -1180,14 +1217,14 @@ int trial_width(tbtkx_context *h, int flags, int S, int share, int rmul, float *
 	double2 *part = h->blk_part.as<double2>();
 	constexpr int kTrial = 3;
 	block_init<VT><<<grid_for(h->N * B), kThreads, 0, h->stream>>>(X, h->N, B, B, 1,
-	                                                              nullptr, 1, 0);
+	                                                              nullptr, 1, 0, 0);
 	// live = 0: no moment gathers (there are no sites), full vector traffic.
-	if ((rc = launch_step<VT, HT, B>(r, true, dots, X, Y, part, nullptr, 0, nullptr, 1, 0)))
+	if ((rc = launch_step<VT, HT, B>(r, true, dots, X, Y, part, nullptr, 0, nullptr, 1, 0, 0)))
 		return rc;
 	TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
 	for (int i = 0; i < kTrial; i++) {
 		if ((rc = launch_step<VT, HT, B>(r, false, dots, Y, X, part + (int64_t)(i + 1) * r.nwg * 2 * B,
-		                                 nullptr, 0, nullptr, 1, 0)))
+		                                 nullptr, 0, nullptr, 1, 0, 0)))
 			return rc;
 		VT *t = X;
 		X = Y;
@@ -1207,6 +1244,7 @@ int trial_width(tbtkx_context *h, int flags, int S, int share, int rmul, float *
 // of about 1 MB and 256 KB of Phi) and the grid share is tried for three
 // steps on the resident model; the shape with the least time for all blocks
 // of the call wins.
+// `total` counts complex columns (packed real vectors: two per column).
 template <typename VT, typename HT>
 int tune_shape(tbtkx_context *h, int total, int flags, int *best_B, int *best_S,
                int *best_share, int *best_rmul) {
@@ -1268,12 +1306,12 @@ int tune_shape(tbtkx_context *h, int total, int flags, int *best_B, int *best_S,
 
 template <typename VT, typename HT>
 int dispatch_width(tbtkx_context *h, int B, int total, const int64_t *sites,
-                   int mode, uint64_t seed, int M, int flags,
+                   int mode, uint64_t seed, int M, int flags, int packed,
                    std::vector<double2> &mu) {
 	switch (B) {
-	case 64: return run_all<VT, HT, 64>(h, total, sites, mode, seed, M, flags, mu);
-	case 128: return run_all<VT, HT, 128>(h, total, sites, mode, seed, M, flags, mu);
-	default: return run_all<VT, HT, 256>(h, total, sites, mode, seed, M, flags, mu);
+	case 64: return run_all<VT, HT, 64>(h, total, sites, mode, seed, M, flags, packed, mu);
+	case 128: return run_all<VT, HT, 128>(h, total, sites, mode, seed, M, flags, packed, mu);
+	default: return run_all<VT, HT, 256>(h, total, sites, mode, seed, M, flags, packed, mu);
 	}
 }
 
@@ -1284,6 +1322,9 @@ int block_entry(tbtkx_context *h, int total, const int64_t *sites, int mode,
 	if (flags & TBTKX_VEC_FP64) flags |= TBTKX_H_FP64;
 	int B = h->block_width;
 	if (B == 32) B = 0;
+	// Unit start vectors on a real H: two real vectors per complex column.
+	const int packed = mode == 0 && h->real_path && h->h_real;
+	const int cols = packed ? (total + 1) / 2 : total;
 	const bool ell = h->ell_W > 0 && h->use_ell;
 	h->run_strip = ell ? (h->strip_rows != 0 ? h->strip_rows : -1) : -1;
 	h->run_share = h->grid_share > 0 ? h->grid_share : 1;
@@ -1291,23 +1332,24 @@ int block_entry(tbtkx_context *h, int total, const int64_t *sites, int mode,
 	const bool free_shape = B == 0 || (ell && h->strip_rows == 0) || h->grid_share == 0 ||
 	                        (ell && h->rows_in_flight == 0);
 	if (free_shape && !h->autotune) {
-		if (B == 0) B = total <= 64 ? 64 : 128;
+		if (B == 0) B = cols <= 64 ? 64 : 128;
 	} else if (free_shape) {
 		// Launch-shape autotuner: time the candidates on the resident
 		// model once per (model, precision, mode, size class, options) and
 		// keep the fastest for the whole call.
-		const int64_t key[8] = {h->csr_gen, flags, total <= 64 ? 64 : total <= 128 ? 128 : 256,
+		const int64_t key[8] = {h->csr_gen, flags + (packed << 16),
+		                        cols <= 64 ? 64 : cols <= 128 ? 128 : 256,
 		                        h->use_ell * 4 + h->nt_stream * 2 + h->xcd_remap,
 		                        h->tile_rows * 65536 + h->strip_chunk, h->block_width,
 		                        h->strip_rows, h->grid_share * 16 + h->rows_in_flight};
 		if (memcmp(key, h->blk_tune_key, sizeof key) != 0) {
 			int tb = 64, ts = -1, tsh = 1, trm = 1;
 			if (flags & TBTKX_VEC_FP64)
-				rc = tune_shape<double2, double2>(h, total, flags, &tb, &ts, &tsh, &trm);
+				rc = tune_shape<double2, double2>(h, cols, flags, &tb, &ts, &tsh, &trm);
 			else if (flags & TBTKX_H_FP64)
-				rc = tune_shape<float2, double2>(h, total, flags, &tb, &ts, &tsh, &trm);
+				rc = tune_shape<float2, double2>(h, cols, flags, &tb, &ts, &tsh, &trm);
 			else
-				rc = tune_shape<float2, float2>(h, total, flags, &tb, &ts, &tsh, &trm);
+				rc = tune_shape<float2, float2>(h, cols, flags, &tb, &ts, &tsh, &trm);
 			if (rc) return rc;
 			memcpy(h->blk_tune_key, key, sizeof key);
 			h->blk_tune_B = tb;
@@ -1322,12 +1364,12 @@ int block_entry(tbtkx_context *h, int total, const int64_t *sites, int mode,
 	}
 	if (flags & TBTKX_VEC_FP64)
 		return dispatch_width<double2, double2>(h, B, total, sites, mode, seed, M,
-		                                        flags, mu);
+		                                        flags, packed, mu);
 	if (flags & TBTKX_H_FP64)
 		return dispatch_width<float2, double2>(h, B, total, sites, mode, seed, M,
-		                                       flags, mu);
+		                                       flags, packed, mu);
 	return dispatch_width<float2, float2>(h, B, total, sites, mode, seed, M, flags,
-	                                      mu);
+	                                      packed, mu);
 }
 
 }  // namespace

@@ -101,6 +101,8 @@ struct tbtkx_context {
 	double scale = 0;
 	bool h_real = false;   // every value of H has a zero imaginary part
 	tbtkx::DevBuf rowptr, col, v1, v2, v1f, v2f;
+	// Real parts of v1 and v2 (FP64), present when h_real.
+	tbtkx::DevBuf v1r, v2r;
 	// Tile-major ELL copy of H for the batched step (ell_W = slots per row,
 	// 0 when a row is longer than 8): tiles of 64/ell_W rows.
 	tbtkx::DevBuf ell_col, ell_v1f, ell_v2f, ell_v1d, ell_v2d;
@@ -137,6 +139,7 @@ struct tbtkx_context {
 	// all models concatenated in upload order.
 	int mm_K = 0, mm_W = 0;
 	bool mm_real = false;
+	int mm_packed = 0;                        // current call packs real vectors
 	int64_t mm_gen = 0;                       // bumped by every tbtkx_multi_set
 	std::vector<int64_t> mm_N, mm_erow0;      // rows, first ELL row per model
 	std::vector<int64_t> mm_band;             // max |col - row| per model
@@ -162,6 +165,8 @@ struct tbtkx_context {
 	int strip_chunk = 0;   // consecutive strip tiles per workgroup (multiple
 	                       // of 4; 0 = 4)
 	int rows_in_flight = 0;   // strip step rows-in-flight multiplier (0 = tuned)
+	int real_path = 1;     // real arithmetic for real H and real start vectors
+	int last_real = 0;     // the last recurrence call ran in real arithmetic
 
 	tbtkx::GraphCache plain_graph;
 };

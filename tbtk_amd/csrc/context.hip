@@ -158,7 +158,7 @@ int tbtkx_destroy(tbtkx_handle h) {
 	h->mm_graph.reset();
 	h->blk_graph.reset();
 	for (DevBuf *b : {&h->rowptr, &h->col, &h->v1, &h->v2, &h->v1f, &h->v2f,
-	                  &h->rblk, &h->vec_a, &h->vec_b, &h->coef, &h->to,
+	                  &h->v1r, &h->v2r, &h->rblk, &h->vec_a, &h->vec_b, &h->coef, &h->to,
 	                  &h->lut, &h->gf_in, &h->gf_out, &h->blk_x, &h->blk_y,
 	                  &h->blk_mu, &h->blk_sites, &h->blk_part, &h->cplanes,
 	                  &h->lplanes, &h->gf_mfma_out, &h->ell_col, &h->ell_v1f,
@@ -234,6 +234,9 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 		v2f[k] = (float)v2[k];
 	}
 	std::vector<uint32_t> blocks = make_row_blocks(N, rowptr);
+	bool real = true;
+	for (int64_t k = 0; k < Z && real; k++)
+		if (val[2 * k + 1] != 0.0) real = false;
 
 	if ((rc = h->rowptr.reserve((N + 1) * 4))) return rc;
 	if ((rc = h->col.reserve(Zp * 4))) return rc;
@@ -252,6 +255,18 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 	TBTKX_CHECK_HIP(hipMemcpy(h->v2f.ptr, v2f.data(), Zp * 8, hipMemcpyHostToDevice));
 	TBTKX_CHECK_HIP(hipMemcpy(h->rblk.ptr, blocks.data(), blocks.size() * 4,
 	                          hipMemcpyHostToDevice));
+	if (real) {
+		// Real parts of H/s and 2H/s for the real plain recurrence.
+		std::vector<double> r1(Zp, 0.0), r2(Zp, 0.0);
+		for (int64_t k = 0; k < Z; k++) {
+			r1[k] = v1[2 * k];
+			r2[k] = v2[2 * k];
+		}
+		if ((rc = h->v1r.reserve(Zp * 8))) return rc;
+		if ((rc = h->v2r.reserve(Zp * 8))) return rc;
+		TBTKX_CHECK_HIP(hipMemcpy(h->v1r.ptr, r1.data(), Zp * 8, hipMemcpyHostToDevice));
+		TBTKX_CHECK_HIP(hipMemcpy(h->v2r.ptr, r2.data(), Zp * 8, hipMemcpyHostToDevice));
+	}
 	// Tile-major ELL copy (see spmm_step_ell): W slots per row in the order
 	// of ell_row_slots (diagonal first), tiles of 64/W rows; padding slots
 	// point at the row itself with value zero.
@@ -311,9 +326,7 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 	h->scale = scale;
 	h->csr_gen++;
 	h->blk_graph.reset();
-	h->h_real = true;
-	for (int64_t k = 0; k < Z && h->h_real; k++)
-		if (val[2 * k + 1] != 0.0) h->h_real = false;
+	h->h_real = real;
 	h->nblk = (int)blocks.size() - 1;
 	return TBTKX_OK;
 }
@@ -388,6 +401,12 @@ int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
 		              "tbtkx_set_option: rows in flight multiplier must be 0, 1 or 2");
 		h->rows_in_flight = (int)value;
 		return TBTKX_OK;
+	case TBTKX_OPT_REAL:
+		TBTKX_REQUIRE(value == 0 || value == 1, TBTKX_ERR_ARG,
+		              "tbtkx_set_option: real arithmetic option must be 0 or 1");
+		h->real_path = (int)value;
+		h->plain_graph.reset();
+		return TBTKX_OK;
 	default:
 		set_error("tbtkx_set_option: unknown option " + std::to_string(option));
 		return TBTKX_ERR_ARG;
@@ -419,6 +438,9 @@ int tbtkx_get_option(tbtkx_handle h, int option, int64_t *value) {
 		return TBTKX_OK;
 	case TBTKX_OPT_ROWS_IN_FLIGHT:
 		*value = h->rows_in_flight ? h->rows_in_flight : h->last_rmul;
+		return TBTKX_OK;
+	case TBTKX_OPT_REAL:
+		*value = h->last_real;
 		return TBTKX_OK;
 	default:
 		set_error("tbtkx_get_option: unknown option " + std::to_string(option));

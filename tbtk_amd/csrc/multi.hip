@@ -19,7 +19,13 @@
 // lane, then ordered reductions) and uses mu_2n = 2 D0_n - mu_0,
 // mu_2n+1 = 2 D1_n - mu_1 (Weisse et al., Rev. Mod. Phys. 78, 275, eq. 31).
 //
-// Lane mapping for a model of width w = B_m: a wavefront covers
+// Packed real vectors: when every model's H is real, the real and imaginary
+// parts of a complex column evolve independently, so a model's B_m real
+// unit vectors take w = ceil(B_m / 2) complex columns (vector 2c in the real
+// part of column c, 2c + 1 in the imaginary part); the step is unchanged,
+// the gathers and dots read per component. Otherwise w = B_m.
+//
+// Lane mapping for a model of width w: a wavefront covers
 // Q = floor(64 / w) consecutive rows at once, lane l -> row slot q = l / w,
 // column b = l % w; lanes l >= Q w are idle (they mirror a valid row's loads
 // and store nothing), so no HBM traffic is spent on padding columns. A tile is
@@ -34,7 +40,7 @@
 // deterministic for a given launch shape.
 //
 // Algorithmic HBM bytes per step: sum over active models of
-// 12 W N_m (ELL column + value) + 24 B_m N_m (read Phi_n, read Phi_{n-1},
+// 12 W N_m (ELL column + value) + 24 w N_m (read Phi_n, read Phi_{n-1},
 // write Phi_{n+1}).
 #include <algorithm>
 #include <cmath>
@@ -67,10 +73,11 @@ struct MDesc {
 };
 
 struct GCol {
-	int64_t xoff;    // X element of (site_b, b)
-	int64_t out;     // first output moment of the column
-	int M;           // moments of the column's model
-	int pad;
+	int64_t xoff;    // X element of the vector's site in its column
+	int64_t out;     // first output moment of the vector
+	int M;           // moments of the vector's model
+	int comp;        // 0: complex column; packed real vectors: 1 = the
+	                 // column's real part, 2 = its imaginary part
 };
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -99,7 +106,12 @@ __global__ __launch_bounds__(kThreads) void multi_units(float2 *__restrict__ X,
                                                           const GCol *__restrict__ cols,
                                                           int ncols) {
 	const int g = blockIdx.x * kThreads + threadIdx.x;
-	if (g < ncols) X[cols[g].xoff] = make_float2(1.0f, 0.0f);
+	if (g >= ncols) return;
+	const GCol c = cols[g];
+	// Two packed vectors of one column may start at the same site: each
+	// writes only its own component.
+	float *e = reinterpret_cast<float *>(X + c.xoff);
+	e[c.comp == 2 ? 1 : 0] = 1.0f;
 }
 
 // Gather mode: mu_n of every column whose model has more than n moments.
@@ -111,7 +123,25 @@ __global__ __launch_bounds__(kThreads) void multi_gather(
 	const GCol c = cols[g];
 	if (n < c.M) {
 		const float2 x = X[c.xoff];
-		mu[c.out + n] = make_double2((double)x.x, (double)x.y);
+		if (c.comp == 0) mu[c.out + n] = make_double2((double)x.x, (double)x.y);
+		else mu[c.out + n] = make_double2((double)(c.comp == 2 ? x.y : x.x), 0.0);
+	}
+}
+
+// Doubling dots of one element, FP64: d0 += |x|^2 and d1 += conj(y) x for a
+// complex column; per component for a column of two packed real vectors,
+// d0 += (x.re^2, x.im^2) and d1 += (y.re x.re, y.im x.im).
+__device__ __forceinline__ void dot_acc(double xx, double xy, double yx, double yy, int packed,
+                                        double &d0x, double &d0y, double &d1x, double &d1y) {
+	if (packed) {
+		d0x = fma(xx, xx, d0x);
+		d0y = fma(xy, xy, d0y);
+		d1x = fma(yx, xx, d1x);
+		d1y = fma(yy, xy, d1y);
+	} else {
+		d0x = fma(xx, xx, fma(xy, xy, d0x));
+		d1x = fma(yx, xx, fma(yy, xy, d1x));
+		d1y = fma(yx, xy, fma(-yy, xx, d1y));
 	}
 }
 
@@ -142,7 +172,7 @@ __global__ __launch_bounds__(kThreads) void multi_step(
     const int *__restrict__ xpart, const int *__restrict__ pbase,
     const int *__restrict__ pstart, int G, const uint32_t *__restrict__ ecol,
     const float2 *__restrict__ ev, const float2 *__restrict__ X,
-    float2 *__restrict__ Y, double2 *__restrict__ part) {
+    float2 *__restrict__ Y, double2 *__restrict__ part, int packed) {
 	const int lane = threadIdx.x & 63;
 	const int x = blockIdx.x & 7;
 	const int nwx = (gridDim.x >> 3) * kWaves;                  // wavefronts per XCD
@@ -168,7 +198,7 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 		const int64_t tr = (int64_t)G * Q;   // rows per tile
 		const bool shared_window = Q * W <= 64;
 		const int tend = phi < toff[m + 1] ? phi : toff[m + 1];
-		double d0 = 0.0, d1x = 0.0, d1y = 0.0;
+		double d0x = 0.0, d0y = 0.0, d1x = 0.0, d1y = 0.0;
 		if (Q == 1) {
 			// One row per group (w > 32): every lane needs the same ELL
 			// entries, so a tile's rows are read as one window and broadcast
@@ -221,11 +251,8 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 						if (!act || gg + r >= rows) continue;
 						store_nt(&Y[d.vec0 + row[r] * w + b], ax, ay);
 						if (DOTS) {
-							const double yx = ax, yy = ay;
-							const double xx = xv[r][0].x, xy = xv[r][0].y;
-							d0 = fma(xx, xx, fma(xy, xy, d0));
-							d1x = fma(yx, xx, fma(yy, xy, d1x));
-							d1y = fma(yx, xy, fma(-yy, xx, d1y));
+							dot_acc(xv[r][0].x, xv[r][0].y, ax, ay, packed, d0x, d0y, d1x,
+							        d1y);
 						}
 					}
 				}
@@ -303,21 +330,18 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 					if (!live[r]) continue;
 					store_nt(&Y[d.vec0 + row[r] * w + b], ax, ay);
 					if (DOTS) {
-						const double yx = ax, yy = ay;
-						const double xx = xs[r].x, xy = xs[r].y;
-						d0 = fma(xx, xx, fma(xy, xy, d0));
-						d1x = fma(yx, xx, fma(yy, xy, d1x));
-						d1y = fma(yx, xy, fma(-yy, xx, d1y));
+						dot_acc(xs[r].x, xs[r].y, ax, ay, packed, d0x, d0y, d1x, d1y);
 					}
 				}
 			}
 		}
 		if (DOTS) {
 			// Column b's sum over the row slots q = 0 .. Q-1, ascending.
-			double s0 = 0.0, s1x = 0.0, s1y = 0.0;
+			double s0x = 0.0, s0y = 0.0, s1x = 0.0, s1y = 0.0;
 			for (int k = 0; k < Q; k++) {
 				const int src = b + k * w;
-				s0 += __shfl(d0, src);
+				s0x += __shfl(d0x, src);
+				s0y += __shfl(d0y, src);
 				s1x += __shfl(d1x, src);
 				s1y += __shfl(d1y, src);
 			}
@@ -325,7 +349,7 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 			const int s0w = (pstart[pi] - plo) % nwx;
 			const int slot = pbase[pi] + (wx - s0w + nwx) % nwx;
 			if (lane < w) {
-				part[((int64_t)slot * 2 + 0) * kMaxWidth + lane] = make_double2(s0, 0.0);
+				part[((int64_t)slot * 2 + 0) * kMaxWidth + lane] = make_double2(s0x, s0y);
 				part[((int64_t)slot * 2 + 1) * kMaxWidth + lane] = make_double2(s1x, s1y);
 			}
 		}
@@ -561,7 +585,7 @@ void launch_step_w(tbtkx_context *h, const Plan &p, const Phase &ph, bool first,
 #define TBTKX_MS(F, D)                                                          \
 	multi_step<W, F, D, REALH><<<p.nwg, kThreads, 0, h->stream>>>(            \
 	    md, tab + ph.toff, ph.kact, tab + ph.xpart, tab + ph.pbase,            \
-	    tab + ph.pstart, p.G, h->mm_ecol.as<uint32_t>(), ev, X, Y, part)
+	    tab + ph.pstart, p.G, h->mm_ecol.as<uint32_t>(), ev, X, Y, part, h->mm_packed)
 	if (first && dots) TBTKX_MS(true, true);
 	else if (first) TBTKX_MS(true, false);
 	else if (dots) TBTKX_MS(false, true);
@@ -816,6 +840,10 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 	              "tbtkx_multi_moments: only TBTKX_DOUBLING is supported (FP32 storage)");
 	const int K = h->mm_K;
 	const bool dots = (flags & TBTKX_DOUBLING) != 0;
+	// Real H in every model (the start vectors are unit vectors): two real
+	// vectors per complex column.
+	const int packed = h->real_path && h->mm_real;
+	h->mm_packed = packed;
 	std::vector<int> width(K), S(K);
 	std::vector<int64_t> out_off(K + 1, 0);
 	int ncols = 0;
@@ -828,7 +856,7 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 		for (int b = 0; b < num_sites[m]; b++)
 			TBTKX_REQUIRE(sites[ncols + b] >= 0 && sites[ncols + b] < h->mm_N[m],
 			              TBTKX_ERR_ARG, tag + "site outside the basis");
-		width[m] = num_sites[m];
+		width[m] = packed ? (num_sites[m] + 1) / 2 : num_sites[m];
 		S[m] = dots ? (num_moments[m] + 1) / 2 : num_moments[m] - 1;
 		out_off[m + 1] = out_off[m] + (int64_t)num_sites[m] * num_moments[m];
 		ncols += num_sites[m];
@@ -841,7 +869,7 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 	// autotuned once per (batch, mode, widths).
 	int64_t hw = 1469598103934665603ll;
 	for (int m = 0; m < K; m++) hw = (hw ^ (width[m] * 131 + S[m])) * 1099511628211ll;
-	const int64_t key[3] = {h->mm_gen, flags, hw};
+	const int64_t key[3] = {h->mm_gen, flags + (packed << 16), hw};
 	int G = 8, share = 1;
 	if (h->tile_rows > 0) {
 		G = std::max(kRpi, (h->tile_rows + kRpi - 1) / kRpi * kRpi);
@@ -884,11 +912,11 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 	std::vector<GCol> cols(ncols);
 	for (int m = 0, c = 0; m < K; m++) {
 		const MDesc &d = p.desc[pos[m]];
-		for (int b = 0; b < width[m]; b++, c++) {
-			cols[c].xoff = d.vec0 + sites[c] * d.w + b;
+		for (int b = 0; b < num_sites[m]; b++, c++) {
+			cols[c].xoff = d.vec0 + sites[c] * d.w + (packed ? b >> 1 : b);
 			cols[c].out = out_off[m] + (int64_t)b * num_moments[m];
 			cols[c].M = num_moments[m];
-			cols[c].pad = 0;
+			cols[c].comp = packed ? 1 + (b & 1) : 0;
 		}
 	}
 	const int64_t mu_elems = dots ? p.d_elems : out_off[K];
@@ -903,7 +931,7 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 		for (int m = 0; m < K; m++) mh = (mh ^ num_moments[m]) * 1099511628211ll;
 		int64_t gkey[GraphCache::kKey];
 		const bool key_ok = GraphCache::make_key(
-		    gkey, {h->mm_gen, flags, mh, G * 16 + share, p.nwg},
+		    gkey, {h->mm_gen, flags, mh, G * 16 + share, p.nwg, packed},
 		    {h->mm_desc.ptr, h->mm_tab.ptr, h->mm_x.ptr, h->mm_y.ptr, h->mm_part.ptr,
 		     h->mm_seg.ptr, h->mm_D.ptr, h->mm_cols.ptr, h->mm_ecol.ptr, h->mm_ev1.ptr,
 		     h->mm_ev2.ptr});
@@ -931,6 +959,7 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 	float ms = 0;
 	TBTKX_CHECK_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
 	h->last_ms = ms;
+	h->last_real = packed;
 
 	double2 *o2 = reinterpret_cast<double2 *>(out);
 	if (!dots) {
@@ -940,10 +969,14 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 	for (int m = 0; m < K; m++) {
 		const MDesc &d = p.desc[pos[m]];
 		const int M = num_moments[m], w = width[m];
-		for (int b = 0; b < w; b++) {
+		for (int b = 0; b < num_sites[m]; b++) {
 			double2 *o = o2 + out_off[m] + (int64_t)b * M;
+			// Column of vector b; packed: its component.
+			const int c = packed ? b >> 1 : b;
 			auto Dv = [&](int n, int which) {
-				return buf[d.dof + ((int64_t)n * 2 + which) * w + b];
+				const double2 v = buf[d.dof + ((int64_t)n * 2 + which) * w + c];
+				if (!packed) return v;
+				return make_double2((b & 1) ? v.y : v.x, 0.0);
 			};
 			const double2 mu0 = Dv(0, 0), mu1 = Dv(0, 1);
 			o[0] = mu0;

@@ -10,60 +10,88 @@
 //
 // HBM traffic per step (algorithmic): 20 B per non-zero (16 B value + 4 B
 // column), 4 B row pointer, 16 B read of x, 16 B read and 16 B write of the
-// in-place j_{n-2} -> j_n row: 20 Z + 52 N bytes.
+// in-place j_{n-2} -> j_n row: 20 Z + 52 N bytes. For a real H (and the real
+// unit start vector) the recurrence runs on real FP64 values and vectors:
+// 12 Z + 28 N bytes.
 #include <cstring>
+#include <type_traits>
 
 #include "common.hpp"
 
 namespace tbtkx {
 namespace {
 
+// FP64 arithmetic of one recurrence element: complex (double2) or, for a
+// real H and a real start vector, real (double). A complex product is formed
+// as (vr*xr - vi*xi, vr*xi + vi*xr) with no fused multiply-add. With real
+// operands the complex recurrence keeps every imaginary part exactly zero,
+// and its real parts come out of the same operations in the same order as
+// the real recurrence's values, so both give the same moments (a zero may
+// differ in sign).
+template <typename S> struct F64;
+template <> struct F64<double2> {
+	__device__ static double2 mul(double2 a, double2 c) {
+#pragma clang fp contract(off)
+		return make_double2(a.x * c.x - a.y * c.y, a.x * c.y + a.y * c.x);
+	}
+	__device__ static double2 add(double2 a, double2 b) {
+#pragma clang fp contract(off)
+		return make_double2(a.x + b.x, a.y + b.y);
+	}
+	__device__ static double2 neg(double2 p) { return make_double2(-p.x, -p.y); }
+	__device__ static double2 zero() { return make_double2(0.0, 0.0); }
+	__device__ static double2 unit(bool one) { return make_double2(one ? 1.0 : 0.0, 0.0); }
+	__device__ static double2 moment(double2 x) { return x; }
+};
+template <> struct F64<double> {
+	__device__ static double mul(double a, double c) {
+#pragma clang fp contract(off)
+		return a * c;
+	}
+	__device__ static double add(double a, double b) {
+#pragma clang fp contract(off)
+		return a + b;
+	}
+	__device__ static double neg(double p) { return -p; }
+	__device__ static double zero() { return 0.0; }
+	__device__ static double unit(bool one) { return one ? 1.0 : 0.0; }
+	__device__ static double2 moment(double x) { return make_double2(x, 0.0); }
+};
+
 // Row-block kernel: the workgroup streams its block's non-zeros with
 // consecutive threads on consecutive non-zeros (coalesced value/column loads
 // and gathers of x), forms the products in registers, stages them in LDS and
 // then lets one thread per row add them up in column order.
-template <bool FIRST>
+template <typename S, bool FIRST>
 __global__ __launch_bounds__(kThreads) void plain_step_lds(
     const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ col,
-    const double2 *__restrict__ v, const double2 *__restrict__ x,
-    double2 *y, const uint32_t *__restrict__ rblk, int nblk, int remap,
+    const S *__restrict__ v, const S *__restrict__ x,
+    S *y, const uint32_t *__restrict__ rblk, int nblk, int remap,
     const int64_t *__restrict__ to, int K, double2 *__restrict__ coef,
     int M, int nprev) {
 #pragma clang fp contract(off)
-	__shared__ double2 prod[kPlainCap];
+	typedef F64<S> A;
+	__shared__ S prod[kPlainCap];
 	const int tid = threadIdx.x;
 
 	// Moments of the previous vector (x = j_{nprev}); x is not written in
 	// this launch, so any workgroup may read it.
 	const int64_t g = (int64_t)blockIdx.x * kThreads + tid;
-	if (g < K) coef[g * M + nprev] = x[to[g]];
+	if (g < K) coef[g * M + nprev] = A::moment(x[to[g]]);
 
 	const int b = remap ? xcd_remap(blockIdx.x, nblk) : (int)blockIdx.x;
 	const uint32_t r0 = rblk[b], r1 = rblk[b + 1];
 	const uint32_t k0 = rowptr[r0], k1 = rowptr[r1];
 
 	if (k1 - k0 <= (uint32_t)kPlainCap) {
-		for (uint32_t k = k0 + tid; k < k1; k += kThreads) {
-			const double2 a = v[k];
-			const double2 c = x[col[k]];
-			prod[k - k0] = make_double2(a.x * c.x - a.y * c.y,
-			                            a.x * c.y + a.y * c.x);
-		}
+		for (uint32_t k = k0 + tid; k < k1; k += kThreads)
+			prod[k - k0] = A::mul(v[k], x[col[k]]);
 		__syncthreads();
 		const uint32_t row = r0 + tid;
 		if (row < r1) {
-			double2 acc;
-			if (FIRST) {
-				acc = make_double2(0.0, 0.0);
-			} else {
-				const double2 p = y[row];
-				acc = make_double2(-p.x, -p.y);
-			}
+			S acc = FIRST ? A::zero() : A::neg(y[row]);
 			const uint32_t e = rowptr[row + 1] - k0;
-			for (uint32_t k = rowptr[row] - k0; k < e; k++) {
-				acc.x = acc.x + prod[k].x;
-				acc.y = acc.y + prod[k].y;
-			}
+			for (uint32_t k = rowptr[row] - k0; k < e; k++) acc = A::add(acc, prod[k]);
 			y[row] = acc;
 		}
 		return;
@@ -71,86 +99,67 @@ __global__ __launch_bounds__(kThreads) void plain_step_lds(
 
 	// A single row with more than kPlainCap non-zeros: stage it chunk by
 	// chunk and let thread 0 keep the sequential sum.
-	double2 acc = make_double2(0.0, 0.0);
-	if (tid == 0 && !FIRST) {
-		const double2 p = y[r0];
-		acc = make_double2(-p.x, -p.y);
-	}
+	S acc = A::zero();
+	if (tid == 0 && !FIRST) acc = A::neg(y[r0]);
 	for (uint32_t base = k0; base < k1; base += kPlainCap) {
 		const uint32_t end = base + kPlainCap < k1 ? base + kPlainCap : k1;
-		for (uint32_t k = base + tid; k < end; k += kThreads) {
-			const double2 a = v[k];
-			const double2 c = x[col[k]];
-			prod[k - base] = make_double2(a.x * c.x - a.y * c.y,
-			                              a.x * c.y + a.y * c.x);
-		}
+		for (uint32_t k = base + tid; k < end; k += kThreads)
+			prod[k - base] = A::mul(v[k], x[col[k]]);
 		__syncthreads();
 		if (tid == 0)
-			for (uint32_t k = 0; k < end - base; k++) {
-				acc.x = acc.x + prod[k].x;
-				acc.y = acc.y + prod[k].y;
-			}
+			for (uint32_t k = 0; k < end - base; k++) acc = A::add(acc, prod[k]);
 		__syncthreads();
 	}
 	if (tid == 0) y[r0] = acc;
 }
 
 // One thread per row, loads straight from memory (no LDS).
-template <bool FIRST>
+template <typename S, bool FIRST>
 __global__ __launch_bounds__(kThreads) void plain_step_row(
     const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ col,
-    const double2 *__restrict__ v, const double2 *__restrict__ x,
-    double2 *y, int64_t N, int nwg, int remap, const int64_t *__restrict__ to,
+    const S *__restrict__ v, const S *__restrict__ x,
+    S *y, int64_t N, int nwg, int remap, const int64_t *__restrict__ to,
     int K, double2 *__restrict__ coef, int M, int nprev) {
 #pragma clang fp contract(off)
+	typedef F64<S> A;
 	const int64_t g = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-	if (g < K) coef[g * M + nprev] = x[to[g]];
+	if (g < K) coef[g * M + nprev] = A::moment(x[to[g]]);
 	const int b = remap ? xcd_remap(blockIdx.x, nwg) : (int)blockIdx.x;
 	const int64_t row = (int64_t)b * kThreads + threadIdx.x;
 	if (row >= N) return;
-	double2 acc;
-	if (FIRST) {
-		acc = make_double2(0.0, 0.0);
-	} else {
-		const double2 p = y[row];
-		acc = make_double2(-p.x, -p.y);
-	}
+	S acc = FIRST ? A::zero() : A::neg(y[row]);
 	const uint32_t e = rowptr[row + 1];
-	for (uint32_t k = rowptr[row]; k < e; k++) {
-		const double2 a = v[k];
-		const double2 c = x[col[k]];
-		acc.x = acc.x + (a.x * c.x - a.y * c.y);
-		acc.y = acc.y + (a.x * c.y + a.y * c.x);
-	}
+	for (uint32_t k = rowptr[row]; k < e; k++) acc = A::add(acc, A::mul(v[k], x[col[k]]));
 	y[row] = acc;
 }
 
-__global__ __launch_bounds__(kThreads) void init_unit(double2 *a, int64_t N,
-                                                      int64_t from) {
+template <typename S>
+__global__ __launch_bounds__(kThreads) void init_unit(S *a, int64_t N, int64_t from) {
 	const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-	if (i < N) a[i] = make_double2(i == from ? 1.0 : 0.0, 0.0);
+	if (i < N) a[i] = F64<S>::unit(i == from);
 }
 
+template <typename S>
 __global__ __launch_bounds__(kThreads) void gather_moments(
-    const double2 *__restrict__ x, const int64_t *__restrict__ to, int K,
+    const S *__restrict__ x, const int64_t *__restrict__ to, int K,
     double2 *__restrict__ coef, int M, int n) {
 	const int64_t g = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-	if (g < K) coef[g * M + n] = x[to[g]];
-}
-
-__global__ __launch_bounds__(kThreads) void zero_c128(double2 *p, int64_t n) {
-	const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-	if (i < n) p[i] = make_double2(0.0, 0.0);
+	if (g < K) coef[g * M + n] = F64<S>::moment(x[to[g]]);
 }
 
 inline unsigned grid_for(int64_t n) {
 	return (unsigned)((n + kThreads - 1) / kThreads);
 }
 
-// Enqueue the step launches n = 1 .. M-1 and the final gather.
+// Enqueue the step launches n = 1 .. M-1 and the final gather; S = double2
+// (complex) or double (real H and start vector: v1r/v2r, vectors of N
+// doubles in the same buffers).
+template <typename S>
 int enqueue_steps(tbtkx_context *h, int K, int M) {
-	const double2 *v1 = h->v1.as<double2>(), *v2 = h->v2.as<double2>();
-	double2 *a = h->vec_a.as<double2>(), *b = h->vec_b.as<double2>();
+	const bool real = std::is_same<S, double>::value;
+	const S *v1 = real ? h->v1r.as<S>() : h->v1.as<S>();
+	const S *v2 = real ? h->v2r.as<S>() : h->v2.as<S>();
+	S *a = h->vec_a.as<S>(), *b = h->vec_b.as<S>();
 	double2 *coef = h->coef.as<double2>();
 	const int64_t *to = h->to.as<int64_t>();
 	const uint32_t *rp = h->rowptr.as<uint32_t>(), *cl = h->col.as<uint32_t>();
@@ -161,32 +170,31 @@ int enqueue_steps(tbtkx_context *h, int K, int M) {
 	// The moment gather rides on the step launches; their grid covers
 	// K <= N threads (to-indices are unique basis indices).
 	for (int n = 1; n < M; n++) {
-		const double2 *x = (n & 1) ? a : b;
-		double2 *y = (n & 1) ? b : a;
-		const double2 *v = n == 1 ? v1 : v2;
+		const S *x = (n & 1) ? a : b;
+		S *y = (n & 1) ? b : a;
+		const S *v = n == 1 ? v1 : v2;
 		if (lds) {
 			if (n == 1)
-				plain_step_lds<true><<<grid, kThreads, 0, h->stream>>>(
+				plain_step_lds<S, true><<<grid, kThreads, 0, h->stream>>>(
 				    rp, cl, v, x, y, h->rblk.as<uint32_t>(), h->nblk,
 				    remap, to, K, coef, M, n - 1);
 			else
-				plain_step_lds<false><<<grid, kThreads, 0, h->stream>>>(
+				plain_step_lds<S, false><<<grid, kThreads, 0, h->stream>>>(
 				    rp, cl, v, x, y, h->rblk.as<uint32_t>(), h->nblk,
 				    remap, to, K, coef, M, n - 1);
 		} else {
 			if (n == 1)
-				plain_step_row<true><<<grid, kThreads, 0, h->stream>>>(
+				plain_step_row<S, true><<<grid, kThreads, 0, h->stream>>>(
 				    rp, cl, v, x, y, h->N, nwg_row, remap, to, K, coef,
 				    M, n - 1);
 			else
-				plain_step_row<false><<<grid, kThreads, 0, h->stream>>>(
+				plain_step_row<S, false><<<grid, kThreads, 0, h->stream>>>(
 				    rp, cl, v, x, y, h->N, nwg_row, remap, to, K, coef,
 				    M, n - 1);
 		}
 	}
-	const double2 *last = ((M - 1) & 1) ? b : a;
-	gather_moments<<<grid_for(K), kThreads, 0, h->stream>>>(last, to, K, coef,
-	                                                         M, M - 1);
+	const S *last = ((M - 1) & 1) ? b : a;
+	gather_moments<S><<<grid_for(K), kThreads, 0, h->stream>>>(last, to, K, coef, M, M - 1);
 	TBTKX_CHECK_HIP(hipGetLastError());
 	return TBTKX_OK;
 }
@@ -231,24 +239,32 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 	if ((rc = h->coef.reserve((size_t)U * M * 16))) return rc;
 	TBTKX_CHECK_HIP(hipMemcpyAsync(h->to.ptr, uniq.data(), (size_t)U * 8,
 	                               hipMemcpyHostToDevice, h->stream));
-	init_unit<<<grid_for(h->N), kThreads, 0, h->stream>>>(h->vec_a.as<double2>(),
-	                                                      h->N, from);
+	// Real arithmetic when every value of H is real (the start vector is a
+	// unit vector).
+	const bool real = h->real_path && h->h_real;
+	if (real)
+		init_unit<double><<<grid_for(h->N), kThreads, 0, h->stream>>>(
+		    h->vec_a.as<double>(), h->N, from);
+	else
+		init_unit<double2><<<grid_for(h->N), kThreads, 0, h->stream>>>(
+		    h->vec_a.as<double2>(), h->N, from);
 	TBTKX_CHECK_HIP(hipGetLastError());
+	auto enqueue = [&]() { return real ? enqueue_steps<double>(h, U, M) : enqueue_steps<double2>(h, U, M); };
 
 	TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
 	if (h->use_graph) {
 		GraphCache &gc = h->plain_graph;
 		int64_t key[GraphCache::kKey];
 		const bool key_ok = GraphCache::make_key(
-		    key, {U, M, h->plain_kernel, h->xcd_remap, h->N},
+		    key, {U, M, h->plain_kernel, h->xcd_remap, h->N, real},
 		    {h->vec_a.ptr, h->vec_b.ptr, h->coef.ptr, h->to.ptr, h->rowptr.ptr, h->col.ptr,
-		     h->v1.ptr, h->v2.ptr, h->rblk.ptr});
+		     h->v1.ptr, h->v2.ptr, h->v1r.ptr, h->v2r.ptr, h->rblk.ptr});
 		TBTKX_REQUIRE(key_ok, TBTKX_ERR_STATE, "tbtkx_moments: graph key too long");
 		if (!gc.matches(key)) {
 			gc.reset();
 			TBTKX_CHECK_HIP(hipStreamBeginCapture(h->stream,
 			                                      hipStreamCaptureModeThreadLocal));
-			int erc = enqueue_steps(h, U, M);
+			int erc = enqueue();
 			hipError_t ec = hipStreamEndCapture(h->stream, &gc.graph);
 			if (erc) return erc;
 			TBTKX_CHECK_HIP(ec);
@@ -258,7 +274,7 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 		}
 		TBTKX_CHECK_HIP(hipGraphLaunch(gc.exec, h->stream));
 	} else {
-		if ((rc = enqueue_steps(h, U, M))) return rc;
+		if ((rc = enqueue())) return rc;
 	}
 	TBTKX_CHECK_HIP(hipEventRecord(h->ev_stop, h->stream));
 
@@ -270,6 +286,7 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 	TBTKX_CHECK_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
 	h->last_ms = ms;
 	h->last_launches = M > 1 ? M - 1 : 0;
+	h->last_real = real;
 
 	for (int k = 0; k < K; k++) {
 		double *o = out + (size_t)k * M * 2;
