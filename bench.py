@@ -27,7 +27,7 @@ error against the FP64 path.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
                        [--parts plain,batched,uneven,dos,mfma,cpu]
-                       [--skip-cpu-baseline]
+                       [--skip-cpu-baseline] [--calibrate-copy] [--complex]
 """
 from __future__ import annotations
 
@@ -435,6 +435,9 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 1 GiB read + 1 GiB write elementwise kernel (torch) "
                          "whose known bytes scale the rocprofv3 memory counters")
+    ap.add_argument("--complex", action="store_true",
+                    help="run every recurrence in complex arithmetic, also for a real H "
+                         "(TBTKX_OPT_REAL = 0), to measure the complex kernels")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the Chebyshev path on a "
@@ -458,6 +461,8 @@ def main():
 
     out = {"metric": METRIC}
     eng = X.Engine(0)
+    if args.complex:
+        eng.set_option(X.OPT_REAL, 0)
     model = None
     if "plain" in parts or "cpu" in parts:
         log("building 1024x1024 spin lattice")
