@@ -155,6 +155,50 @@ int tbtkx_moments_block(tbtkx_handle handle, int num_sites,
 int tbtkx_moments_trace(tbtkx_handle handle, int num_vectors, uint64_t seed,
                         int num_moments, int flags, double *out);
 
+/* Lattice models built on the device. Replaces, for models given as a
+ * lattice, the host pipeline HoppingAmplitudeSet -> getSparseMatrix ->
+ * SparseMatrix LIL -> CSR (Lib/include/Core/TBTK/HoppingAmplitudeSet.h:397-424,
+ * Lib/include/Utilities/TBTK/SparseMatrix.h:1309-1530) with the same basis
+ * numbering: the model's Index is {x, y, z, orbital} (or a prefix with the
+ * unused extents 1), numbered depth-first over subindex values,
+ * row = ((x * cells[1] + y) * cells[2] + z) * orbitals + orbital.
+ *
+ * Term t adds amplitude (re, im) to H[(c + d, to), (c, from)] for every cell
+ * c whose partner c + d lies inside the lattice, or wraps around along a
+ * periodic axis (so a hermitian model lists both directions, as
+ * "model << HoppingAmplitude(...) + HC" does). onsite (optional, 2 N
+ * doubles) is every state's diagonal amplitude; with onsite, every row has
+ * a diagonal entry. Entries of one (row, column) are summed in the order
+ * onsite, then terms in list order. Rows hold at most 15 distinct columns
+ * besides the diagonal. */
+typedef struct {
+	int32_t d[3];      /* cell offset of the to-cell from the from-cell */
+	int32_t to, from;  /* orbitals */
+	double re, im;     /* amplitude */
+} tbtkx_lattice_term;
+
+typedef struct {
+	int32_t cells[3];     /* cells along x, y, z (>= 1) */
+	int32_t periodic[3];  /* 1 = periodic boundary along the axis */
+	int32_t orbitals;     /* states per cell (>= 1) */
+	int32_t num_terms;
+	const tbtkx_lattice_term *terms;
+} tbtkx_lattice;
+
+/* Build the Hamiltonian of a lattice on the device and make it the handle's
+ * model, as tbtkx_set_csr does for a host CSR (same device layouts, so every
+ * recurrence entry point works on it). */
+int tbtkx_set_lattice(tbtkx_handle handle, const tbtkx_lattice *lattice,
+                      const double *onsite, double scale);
+
+/* Size and copy of the handle's CSR (from tbtkx_set_csr or
+ * tbtkx_set_lattice): rowptr (num_rows + 1), col (nnz) and the values
+ * H * (1/s) as interleaved complex doubles, exactly as the device holds
+ * them. */
+int tbtkx_csr_size(tbtkx_handle handle, int64_t *num_rows, int64_t *nnz);
+int tbtkx_csr_copy(tbtkx_handle handle, uint32_t *rowptr, uint32_t *col,
+                   double *values_over_s);
+
 /* Uneven batch: upload K independent models at once. Model m has
  * num_rows[m] states and nnz[m] non-zeros; its CSR (same conventions as
  * tbtkx_set_csr, indices local to the model) follows the previous model's in
@@ -165,6 +209,14 @@ int tbtkx_moments_trace(tbtkx_handle handle, int num_vectors, uint64_t seed,
 int tbtkx_multi_set(tbtkx_handle handle, int num_models, const int64_t *num_rows,
                     const int64_t *nnz, const uint32_t *rowptr, const uint32_t *col,
                     const double *val, const double *scales);
+
+/* Uneven batch from lattices built on the device: model m is lattices[m]
+ * with on-site amplitudes onsite[m] (2 N_m doubles, or NULL) and scale
+ * factor scales[m]; otherwise as tbtkx_multi_set (rows of at most 8
+ * non-zeros). Replaces the same host pipeline as tbtkx_set_lattice. */
+int tbtkx_multi_set_lattices(tbtkx_handle handle, int num_models,
+                             const tbtkx_lattice *lattices, const double *const *onsite,
+                             const double *scales);
 
 /* Diagonal moments of every uploaded model in one persistent launch per
  * recurrence step: model m gets num_sites[m] (1..64) unit start vectors at

@@ -103,6 +103,9 @@ struct tbtkx_context {
 	tbtkx::DevBuf rowptr, col, v1, v2, v1f, v2f;
 	// Real parts of v1 and v2 (FP64), present when h_real.
 	tbtkx::DevBuf v1r, v2r;
+	// Staging of tbtkx_set_lattice / tbtkx_multi_set_lattices: terms,
+	// on-site amplitudes, row counts, per-workgroup extents, scan storage.
+	tbtkx::DevBuf lat_terms, lat_onsite, lat_count, lat_part, lat_tmp;
 	// Tile-major ELL copy of H for the batched step (ell_W = slots per row,
 	// 0 when a row is longer than 8): tiles of 64/ell_W rows.
 	tbtkx::DevBuf ell_col, ell_v1f, ell_v2f, ell_v1d, ell_v2d;
@@ -206,5 +209,7 @@ inline int ell_row_slots(const uint32_t *rowptr, const uint32_t *col, int64_t r,
 // out = in * (d, 0) as complex products (contraction off): the scaling of the
 // CSR values by 1/s and by 2.
 void scale_complex(const double *in, double *out, int64_t Z, double d);
+// Row blocks of the LDS-staged plain kernel from a host row pointer.
+std::vector<uint32_t> make_row_blocks(int64_t N, const uint32_t *rowptr);
 
 }  // namespace tbtkx

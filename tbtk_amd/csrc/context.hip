@@ -80,6 +80,26 @@ void scale_complex(const double *in, double *out, int64_t Z, double d) {
 	}
 }
 
+// Row blocks for the LDS-staged kernel: at most kPlainRows rows and
+// kPlainCap non-zeros each; a row longer than kPlainCap is a block of its own.
+std::vector<uint32_t> make_row_blocks(int64_t N, const uint32_t *rowptr) {
+	std::vector<uint32_t> b;
+	b.push_back(0);
+	int64_t r = 0;
+	while (r < N) {
+		const int64_t start = r;
+		int64_t nnz = 0;
+		while (r < N && r - start < kPlainRows &&
+		       nnz + (rowptr[r + 1] - rowptr[r]) <= kPlainCap) {
+			nnz += rowptr[r + 1] - rowptr[r];
+			r++;
+		}
+		if (r == start) r++;
+		b.push_back((uint32_t)r);
+	}
+	return b;
+}
+
 }  // namespace tbtkx
 
 using namespace tbtkx;
@@ -158,7 +178,8 @@ int tbtkx_destroy(tbtkx_handle h) {
 	h->mm_graph.reset();
 	h->blk_graph.reset();
 	for (DevBuf *b : {&h->rowptr, &h->col, &h->v1, &h->v2, &h->v1f, &h->v2f,
-	                  &h->v1r, &h->v2r, &h->rblk, &h->vec_a, &h->vec_b, &h->coef, &h->to,
+	                  &h->v1r, &h->v2r, &h->lat_terms, &h->lat_onsite, &h->lat_count,
+	                  &h->lat_part, &h->lat_tmp, &h->rblk, &h->vec_a, &h->vec_b, &h->coef, &h->to,
 	                  &h->lut, &h->gf_in, &h->gf_out, &h->blk_x, &h->blk_y,
 	                  &h->blk_mu, &h->blk_sites, &h->blk_part, &h->cplanes,
 	                  &h->lplanes, &h->gf_mfma_out, &h->ell_col, &h->ell_v1f,
@@ -172,26 +193,6 @@ int tbtkx_destroy(tbtkx_handle h) {
 	if (h->stream) (void)hipStreamDestroy(h->stream);
 	delete h;
 	return TBTKX_OK;
-}
-
-// Row blocks for the LDS-staged kernel: at most kPlainRows rows and
-// kPlainCap non-zeros each; a row longer than kPlainCap is a block of its own.
-static std::vector<uint32_t> make_row_blocks(int64_t N, const uint32_t *rowptr) {
-	std::vector<uint32_t> b;
-	b.push_back(0);
-	int64_t r = 0;
-	while (r < N) {
-		const int64_t start = r;
-		int64_t nnz = 0;
-		while (r < N && r - start < kPlainRows &&
-		       nnz + (rowptr[r + 1] - rowptr[r]) <= kPlainCap) {
-			nnz += rowptr[r + 1] - rowptr[r];
-			r++;
-		}
-		if (r == start) r++;
-		b.push_back((uint32_t)r);
-	}
-	return b;
 }
 
 int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,

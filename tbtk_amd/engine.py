@@ -44,12 +44,40 @@ SIGNATURES = {
     "tbtkx_contract_mfma": [_P, _I, _P],
     "tbtkx_moments_block": [_P, _I, _P, _I, _I, _P],
     "tbtkx_moments_trace": [_P, _I, ctypes.c_uint64, _I, _I, _P],
+    "tbtkx_set_lattice": [_P, _P, _P, _D],
+    "tbtkx_csr_size": [_P, _P, _P],
+    "tbtkx_csr_copy": [_P, _P, _P, _P],
     "tbtkx_multi_set": [_P, _I, _P, _P, _P, _P, _P, _P],
+    "tbtkx_multi_set_lattices": [_P, _I, _P, _P, _P],
     "tbtkx_multi_moments": [_P, _P, _P, _P, _I, _P],
     "tbtkx_last_timing": [_P, _P, _P],
     "tbtkx_set_option": [_P, _I, _I64],
     "tbtkx_get_option": [_P, _I, _P],
 }
+
+# tbtkx_lattice_term and tbtkx_lattice (include/tbtkx.h).
+LATTICE_TERM = np.dtype([("d", "<i4", (3,)), ("to", "<i4"), ("frm", "<i4"),
+                         ("re", "<f8"), ("im", "<f8")], align=True)
+
+
+class _Lattice(ctypes.Structure):
+    _fields_ = [("cells", ctypes.c_int32 * 3), ("periodic", ctypes.c_int32 * 3),
+                ("orbitals", ctypes.c_int32), ("num_terms", ctypes.c_int32),
+                ("terms", ctypes.c_void_p)]
+
+
+def _lattice_struct(spec):
+    """(ctypes struct, arrays to keep alive) for a lattice.LatticeSpec."""
+    terms = np.ascontiguousarray(spec.terms, dtype=LATTICE_TERM)
+    s = _Lattice()
+    for a in range(3):
+        s.cells[a] = int(spec.cells[a])
+        s.periodic[a] = int(bool(spec.periodic[a]))
+    s.orbitals = int(spec.orbitals)
+    s.num_terms = int(terms.size)
+    s.terms = terms.ctypes.data if terms.size else None
+    return s, terms
+
 
 _lib = None
 
@@ -199,7 +227,52 @@ class Engine:
         _check(self._lib.tbtkx_moments_trace(self._h, R, seed, M, flags, _p(out)))
         return out
 
+    def set_lattice(self, spec, scale: float):
+        """Build the Hamiltonian of a lattice.LatticeSpec on the device."""
+        lat, keep = _lattice_struct(spec)
+        onsite = None
+        if spec.onsite is not None:
+            onsite = np.ascontiguousarray(spec.onsite, dtype=np.complex128)
+        _check(self._lib.tbtkx_set_lattice(self._h, ctypes.byref(lat),
+                                           _p(onsite) if onsite is not None else None,
+                                           float(scale)))
+        self.N = spec.basis_size
+        return keep
+
+    def csr_copy(self):
+        """(rowptr, col, H/s) of the resident model, as the device holds them."""
+        n = ctypes.c_int64(0)
+        z = ctypes.c_int64(0)
+        _check(self._lib.tbtkx_csr_size(self._h, ctypes.byref(n), ctypes.byref(z)))
+        rp = np.zeros(n.value + 1, dtype=np.uint32)
+        col = np.zeros(max(z.value, 1), dtype=np.uint32)
+        val = np.zeros(max(z.value, 1), dtype=np.complex128)
+        _check(self._lib.tbtkx_csr_copy(self._h, _p(rp), _p(col), _p(val)))
+        return rp, col[:z.value], val[:z.value]
+
     # ----------------------------------------------------- uneven batch ---
+    def multi_set_lattices(self, specs, scales):
+        """Upload several lattice models, each built on the device."""
+        K = len(specs)
+        lats = (_Lattice * K)()
+        keep = []
+        ptrs = (ctypes.c_void_p * K)()
+        any_onsite = False
+        for m, spec in enumerate(specs):
+            s, terms = _lattice_struct(spec)
+            lats[m] = s
+            keep.append(terms)
+            if spec.onsite is not None:
+                o = np.ascontiguousarray(spec.onsite, dtype=np.complex128)
+                keep.append(o)
+                ptrs[m] = o.ctypes.data
+                any_onsite = True
+        sc = np.ascontiguousarray(scales, dtype=np.float64)
+        _check(self._lib.tbtkx_multi_set_lattices(
+            self._h, K, ctypes.cast(lats, ctypes.c_void_p),
+            ctypes.cast(ptrs, ctypes.c_void_p) if any_onsite else None, _p(sc)))
+        self.multi_N = np.array([sp.basis_size for sp in specs], dtype=np.int64)
+
     def multi_set(self, csrs, scales):
         """Upload several models at once: csrs = [(rowptr, col, val), ...]."""
         rps = [np.ascontiguousarray(c[0], dtype=np.uint32) for c in csrs]

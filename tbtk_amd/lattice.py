@@ -130,6 +130,40 @@ def _flat(multi, shape):
     return np.ravel_multi_index(tuple(multi), shape).astype(np.int64)
 
 
+def _site_disorder(lx, ly, disorder, seed):
+    """U[-disorder, disorder] per lattice site (lexicographic), or zeros."""
+    if disorder:
+        return np.random.default_rng(seed).uniform(-disorder, disorder, size=(lx, ly))
+    return np.zeros((lx, ly))
+
+
+def _uniform_onsite(n, disorder, seed):
+    """U[-disorder, disorder] per state, or zeros, as complex amplitudes."""
+    w = np.zeros(n)
+    if disorder:
+        w = np.random.default_rng(seed).uniform(-disorder, disorder, size=n)
+    return w.astype(np.complex128)
+
+
+def _square_spin_onsite(lx, ly, J, mu, disorder, seed):
+    site_w = _site_disorder(lx, ly, disorder, seed)
+    g = _grid(lx, ly, 2)
+    s = g[2]
+    diag = -mu + (-J * 2 * (0.5 - s)) + site_w[g[0], g[1]]
+    return diag.astype(np.complex128)
+
+
+def _nambu_onsite(lx, ly, mu, J, disorder, seed):
+    site_w = _site_disorder(lx, ly, disorder, seed)
+    g = _grid(lx, ly, 4)
+    s = g[2]
+    w = site_w[g[0], g[1]]
+    se = np.where(s < 2, s, s - 2)
+    zee = -J * 2 * (0.5 - se)
+    diag = np.where(s < 2, -mu + zee + w, -(-mu + zee + w))
+    return diag.astype(np.complex128)
+
+
 def square_spin(lx, ly, t=1.0, J=0.5, mu=0.0, disorder=0.0, seed=0):
     """{x, y, s}: -t nearest-neighbour hopping, Zeeman -J*2*(1/2-s), -mu,
     on-site disorder U[-disorder, disorder] per site (same for both spins)."""
@@ -140,14 +174,8 @@ def square_spin(lx, ly, t=1.0, J=0.5, mu=0.0, disorder=0.0, seed=0):
         a, b = _flat(to, shape), _flat(frm, shape)
         amp = np.full(a.size, -t, dtype=np.complex128)
         rows += [a, b]; cols += [b, a]; vals += [amp, np.conj(amp)]
-    site_w = np.zeros((lx, ly))
-    if disorder:
-        site_w = np.random.default_rng(seed).uniform(-disorder, disorder, size=(lx, ly))
-    g = _grid(*shape)
-    s = g[2]
-    diag = -mu + (-J * 2 * (0.5 - s)) + site_w[g[0], g[1]]
     d = np.arange(np.prod(shape), dtype=np.int64)
-    rows.append(d); cols.append(d); vals.append(diag.astype(np.complex128))
+    rows.append(d); cols.append(d); vals.append(_square_spin_onsite(lx, ly, J, mu, disorder, seed))
     return LatticeModel(
         "square_spin", shape, np.concatenate(rows), np.concatenate(cols),
         np.concatenate(vals),
@@ -161,23 +189,14 @@ def nambu_swave(lx, ly, t=1.0, mu=-1.0, delta=0.3, J=0.5, disorder=0.0, seed=0):
     from={x,y,s}: conj(D) for s=0, -conj(D) for s=1; + HC)."""
     shape = (lx, ly, 4)
     rows, cols, vals = [], [], []
-    site_w = np.zeros((lx, ly))
-    if disorder:
-        site_w = np.random.default_rng(seed).uniform(-disorder, disorder, size=(lx, ly))
     for axis in (0, 1):
         to, frm = _bonds(shape, axis)
         a, b = _flat(to, shape), _flat(frm, shape)
         orb = frm[2]
         amp = np.where(orb < 2, -t, t).astype(np.complex128)
         rows += [a, b]; cols += [b, a]; vals += [amp, np.conj(amp)]
-    g = _grid(*shape)
-    s = g[2]
-    w = site_w[g[0], g[1]]
-    se = np.where(s < 2, s, s - 2)
-    zee = -J * 2 * (0.5 - se)
-    diag = np.where(s < 2, -mu + zee + w, -(-mu + zee + w))
     d = np.arange(np.prod(shape), dtype=np.int64)
-    rows.append(d); cols.append(d); vals.append(diag.astype(np.complex128))
+    rows.append(d); cols.append(d); vals.append(_nambu_onsite(lx, ly, mu, J, disorder, seed))
     # pair potential
     site = _grid(lx, ly)
     for so in (0, 1):
@@ -221,11 +240,8 @@ def honeycomb_ribbon(lx, ly, t=2.7, disorder=0.0, seed=0):
     add(np.vstack([c[0], c[1] + 1, np.full(c.shape[1], 3)]),
         np.vstack([c, np.full(c.shape[1], 2)]))
     n = lx * ly * 4
-    w = np.zeros(n)
-    if disorder:
-        w = np.random.default_rng(seed).uniform(-disorder, disorder, size=n)
     d = np.arange(n, dtype=np.int64)
-    rows.append(d); cols.append(d); vals.append(w.astype(np.complex128))
+    rows.append(d); cols.append(d); vals.append(_uniform_onsite(n, disorder, seed))
     return LatticeModel(
         "honeycomb_ribbon", shape, np.concatenate(rows), np.concatenate(cols),
         np.concatenate(vals), dict(lx=lx, ly=ly, t=t, disorder=disorder, seed=seed))
@@ -245,11 +261,8 @@ def square(lx, ly, t=1.0, disorder=0.0, seed=0, flux=0.0):
             amp = amp * np.exp(2j * np.pi * flux * frm[0])
         rows += [a, b]; cols += [b, a]; vals += [amp, np.conj(amp)]
     n = lx * ly
-    w = np.zeros(n)
-    if disorder:
-        w = np.random.default_rng(seed).uniform(-disorder, disorder, size=n)
     d = np.arange(n, dtype=np.int64)
-    rows.append(d); cols.append(d); vals.append(w.astype(np.complex128))
+    rows.append(d); cols.append(d); vals.append(_uniform_onsite(n, disorder, seed))
     return LatticeModel(
         "square", shape, np.concatenate(rows), np.concatenate(cols),
         np.concatenate(vals),
@@ -266,11 +279,8 @@ def cubic(lx, ly, lz, t=1.0, disorder=0.0, seed=0):
         amp = np.full(a.size, -t, dtype=np.complex128)
         rows += [a, b]; cols += [b, a]; vals += [amp, np.conj(amp)]
     n = lx * ly * lz
-    w = np.zeros(n)
-    if disorder:
-        w = np.random.default_rng(seed).uniform(-disorder, disorder, size=n)
     d = np.arange(n, dtype=np.int64)
-    rows.append(d); cols.append(d); vals.append(w.astype(np.complex128))
+    rows.append(d); cols.append(d); vals.append(_uniform_onsite(n, disorder, seed))
     return LatticeModel(
         "cubic", shape, np.concatenate(rows), np.concatenate(cols),
         np.concatenate(vals), dict(lx=lx, ly=ly, lz=lz, t=t, disorder=disorder, seed=seed))
@@ -309,6 +319,137 @@ def ring(size, t=-1.0):
     return LatticeModel(
         "ring", shape, np.concatenate([to, frm]), np.concatenate([frm, to]),
         np.concatenate([amp, np.conj(amp)]), dict(size=size, t=t))
+
+
+@dataclass
+class LatticeSpec:
+    """A lattice for the device generator (``tbtkx_set_lattice``): the unit
+    cell's terms, the cell counts, boundary conditions and every state's
+    on-site amplitude. Term ``((dx, dy, dz), to, frm, re, im)`` puts
+    ``re + i im`` on H[(c + d, to), (c, frm)] for every cell c whose partner
+    lies inside (or wraps along a periodic axis). The ``*_spec`` builders give
+    exactly the CSR of the same-named generator: same amplitudes (the
+    reverse bond is ``conj(amp)`` as there), same on-site array, and the
+    summation order onsite-then-terms of any duplicate entries."""
+    name: str
+    cells: tuple            # (Lx, Ly, Lz)
+    orbitals: int
+    periodic: tuple         # (px, py, pz)
+    terms: list             # [((dx, dy, dz), to, frm, re, im), ...]
+    onsite: np.ndarray | None
+    shape: tuple            # Index extents of the equivalent LatticeModel
+    params: dict = field(default_factory=dict)
+
+    @property
+    def basis_size(self) -> int:
+        return int(np.prod(self.cells)) * int(self.orbitals)
+
+    def max_row_entries(self) -> int:
+        """Entries of the fullest row (on-site plus the terms of one
+        to-orbital); rows at open edges have fewer."""
+        per = [0 if self.onsite is None else 1] * self.orbitals
+        for _, to, _, _, _ in self.terms:
+            per[to] += 1
+        return max(per)
+
+    def gershgorin_bound(self) -> float:
+        """max over rows of sum |H_ij| (an upper bound of the spectral
+        radius), from the terms without building the CSR; entries that
+        coincide are counted separately, which only loosens the bound."""
+        cell = np.indices(self.cells).reshape(3, -1).astype(np.int64)
+        Ly, Lz, O = self.cells[1], self.cells[2], self.orbitals
+        s = np.zeros(self.basis_size) if self.onsite is None else np.abs(self.onsite)
+        for dvec, to, frm, re, im in self.terms:
+            fc = cell - np.array(dvec, dtype=np.int64)[:, None]
+            ok = np.ones(cell.shape[1], dtype=bool)
+            for a in range(3):
+                if not self.periodic[a]:
+                    ok &= (fc[a] >= 0) & (fc[a] < self.cells[a])
+            tc = cell[:, ok]
+            s[((tc[0] * Ly + tc[1]) * Lz + tc[2]) * O + to] += abs(complex(re, im))
+        return float(s.max())
+
+
+def _bond(terms, d, to, frm, amp):
+    """Append a bond and its hermitian partner, the latter as conj(amp)."""
+    a = np.complex128(amp)
+    b = np.conj(a)
+    terms.append((tuple(d), to, frm, float(a.real), float(a.imag)))
+    terms.append((tuple(-x for x in d), frm, to, float(b.real), float(b.imag)))
+
+
+_AXES = ((1, 0, 0), (0, 1, 0), (0, 0, 1))
+
+
+def square_spin_spec(lx, ly, t=1.0, J=0.5, mu=0.0, disorder=0.0, seed=0):
+    terms = []
+    for e in _AXES[:2]:
+        for s in (0, 1):
+            _bond(terms, e, s, s, -t)
+    return LatticeSpec("square_spin", (lx, ly, 1), 2, (0, 0, 0), terms,
+                       _square_spin_onsite(lx, ly, J, mu, disorder, seed), (lx, ly, 2),
+                       dict(lx=lx, ly=ly, t=t, J=J, mu=mu, disorder=disorder, seed=seed))
+
+
+def nambu_swave_spec(lx, ly, t=1.0, mu=-1.0, delta=0.3, J=0.5, disorder=0.0, seed=0):
+    terms = []
+    for e in _AXES[:2]:
+        for s in range(4):
+            _bond(terms, e, s, s, -t if s < 2 else t)
+    D = complex(delta)
+    for so in (0, 1):
+        _bond(terms, (0, 0, 0), 3 - so, so, np.conj(D) if so == 0 else -np.conj(D))
+    return LatticeSpec("nambu_swave", (lx, ly, 1), 4, (0, 0, 0), terms,
+                       _nambu_onsite(lx, ly, mu, J, disorder, seed), (lx, ly, 4),
+                       dict(lx=lx, ly=ly, t=t, mu=mu, delta=delta, J=J, disorder=disorder,
+                            seed=seed))
+
+
+def honeycomb_ribbon_spec(lx, ly, t=2.7, disorder=0.0, seed=0):
+    terms = []
+    for n0, n1 in ((0, 1), (1, 2), (2, 3)):
+        _bond(terms, (0, 0, 0), n1, n0, -t)
+    _bond(terms, (1, 0, 0), 0, 3, -t)
+    _bond(terms, (0, 1, 0), 0, 1, -t)
+    _bond(terms, (0, 1, 0), 3, 2, -t)
+    return LatticeSpec("honeycomb_ribbon", (lx, ly, 1), 4, (0, 0, 0), terms,
+                       _uniform_onsite(lx * ly * 4, disorder, seed), (lx, ly, 4),
+                       dict(lx=lx, ly=ly, t=t, disorder=disorder, seed=seed))
+
+
+def square_spec(lx, ly, t=1.0, disorder=0.0, seed=0):
+    terms = []
+    for e in _AXES[:2]:
+        _bond(terms, e, 0, 0, -t)
+    return LatticeSpec("square", (lx, ly, 1), 1, (0, 0, 0), terms,
+                       _uniform_onsite(lx * ly, disorder, seed), (lx, ly),
+                       dict(lx=lx, ly=ly, t=t, disorder=disorder, seed=seed, flux=0.0))
+
+
+def cubic_spec(lx, ly, lz, t=1.0, disorder=0.0, seed=0):
+    terms = []
+    for e in _AXES:
+        _bond(terms, e, 0, 0, -t)
+    return LatticeSpec("cubic", (lx, ly, lz), 1, (0, 0, 0), terms,
+                       _uniform_onsite(lx * ly * lz, disorder, seed), (lx, ly, lz),
+                       dict(lx=lx, ly=ly, lz=lz, t=t, disorder=disorder, seed=seed))
+
+
+def periodic_square_spec(size, t=1.0, mu=-2.0):
+    terms = []
+    for e in _AXES[:2]:
+        _bond(terms, e, 0, 0, -t)
+    return LatticeSpec("periodic_square", (size, size, 1), 1, (1, 1, 0), terms,
+                       np.full(size * size, -mu, dtype=np.complex128), (size, size),
+                       dict(size=size, t=t, mu=mu))
+
+
+def ring_spec(size, t=-1.0):
+    # H[n][n+1] = t (to-cell n, from-cell n+1: d = -1), then its partner
+    terms = []
+    _bond(terms, (-1, 0, 0), 0, 0, t)
+    return LatticeSpec("ring", (size, 1, 1), 1, (1, 0, 0), terms, None, (size,),
+                       dict(size=size, t=t))
 
 
 def write_amplitudes(model: LatticeModel, path: str) -> None:
@@ -378,4 +519,40 @@ def config_uneven(seed=4, count=64, max_states=1_000_000):
             model = square(width, length, t=1.0, disorder=0.5, seed=mseed)
         sites = np.sort(rng.choice(model.basis_size, B, replace=False))
         out.append((model, sites, M))
+    return out
+
+
+def config_plain_large_spec(seed=2):
+    """The 1024x1024 spin lattice of config_plain_large as a LatticeSpec."""
+    return square_spin_spec(1024, 1024, t=1.0, J=0.5, mu=0.0, disorder=1.0, seed=seed)
+
+
+def config_nambu_spec(l=512, seed=3):
+    return nambu_swave_spec(l, l, t=1.0, mu=-1.0, delta=0.3, J=0.5, disorder=0.5, seed=seed)
+
+
+def config_cubic_spec(l=128, seed=5):
+    return cubic_spec(l, l, l, t=1.0, disorder=1.0, seed=seed)
+
+
+def config_uneven_specs(seed=4, count=64, max_states=1_000_000):
+    """config_uneven as LatticeSpecs (the same models, sites and moment
+    counts from the same draws), for tbtkx_multi_set_lattices."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        honey = bool(rng.integers(2))
+        width = int(rng.integers(20, 401))
+        length = int(rng.integers(50, 2001))
+        per = 4 if honey else 1
+        length = max(1, min(length, max_states // (per * width)))
+        M = int(rng.integers(500, 8001))
+        B = int(rng.integers(1, 65))
+        mseed = int(rng.integers(1 << 31))
+        if honey:
+            spec = honeycomb_ribbon_spec(width, length, t=2.7, disorder=0.5, seed=mseed)
+        else:
+            spec = square_spec(width, length, t=1.0, disorder=0.5, seed=mseed)
+        sites = np.sort(rng.choice(spec.basis_size, B, replace=False))
+        out.append((spec, sites, M))
     return out
