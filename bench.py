@@ -223,7 +223,7 @@ def uneven_batch(eng):
     t0 = time.perf_counter()
     batch = L.config_uneven_specs(seed=4)
     specs = [b[0] for b in batch]
-    scales = [sp.gershgorin_bound() * 1.05 for sp in specs]
+    scales = [eng.lattice_gershgorin(sp) * 1.05 for sp in specs]
     eng.multi_set_lattices(specs, scales)
     t_gen = time.perf_counter() - t0
     sites = [b[1] for b in batch]

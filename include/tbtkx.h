@@ -191,6 +191,13 @@ typedef struct {
 int tbtkx_set_lattice(tbtkx_handle handle, const tbtkx_lattice *lattice,
                       const double *onsite, double scale);
 
+/* Gershgorin bound of a lattice model, max over rows of sum_c |H_rc|
+ * (the merged entries of the row), computed on the device without building
+ * the model: an upper bound of the spectral radius from which to choose
+ * the scale factor. */
+int tbtkx_lattice_gershgorin(tbtkx_handle handle, const tbtkx_lattice *lattice,
+                             const double *onsite, double *bound);
+
 /* Size and copy of the handle's CSR (from tbtkx_set_csr or
  * tbtkx_set_lattice): rowptr (num_rows + 1), col (nnz) and the values
  * H * (1/s) as interleaved complex doubles, exactly as the device holds

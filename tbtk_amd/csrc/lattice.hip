@@ -92,13 +92,17 @@ __device__ int lattice_row(const LatDev &g, const tbtkx_lattice_term *__restrict
 }
 
 // Per row: the number of entries; per workgroup: the largest ELL row length
-// (entries, plus one when the diagonal is absent) and the band max |col - r|.
+// (entries, plus one when the diagonal is absent), the band max |col - r|
+// and the largest row sum of |H_rc| (the Gershgorin bound), as
+// part[3 b .. 3 b + 2] (the bound's bits in the third).
 __global__ __launch_bounds__(kThreads) void lattice_count(
     LatDev g, const tbtkx_lattice_term *__restrict__ terms, const double2 *__restrict__ onsite,
     int64_t N, uint32_t *__restrict__ count, int64_t *__restrict__ part) {
 	__shared__ int64_t s_len[kThreads], s_band[kThreads];
+	__shared__ double s_sum[kThreads];
 	const int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x;
 	int64_t len = 0, band = 0;
+	double rsum = 0.0;
 	if (r < N) {
 		uint32_t cols[kMaxRow];
 		double2 vals[kMaxRow];
@@ -108,23 +112,27 @@ __global__ __launch_bounds__(kThreads) void lattice_count(
 			const int64_t d = (int64_t)cols[k] - r;
 			band = max(band, d < 0 ? -d : d);
 			if (d == 0) diag = true;
+			rsum += hypot(vals[k].x, vals[k].y);
 		}
 		len = n + (diag ? 0 : 1);
 		if (count) count[r] = (uint32_t)n;
 	}
 	s_len[threadIdx.x] = len;
 	s_band[threadIdx.x] = band;
+	s_sum[threadIdx.x] = rsum;
 	__syncthreads();
 	for (int s = kThreads / 2; s > 0; s >>= 1) {
 		if ((int)threadIdx.x < s) {
 			s_len[threadIdx.x] = max(s_len[threadIdx.x], s_len[threadIdx.x + s]);
 			s_band[threadIdx.x] = max(s_band[threadIdx.x], s_band[threadIdx.x + s]);
+			s_sum[threadIdx.x] = fmax(s_sum[threadIdx.x], s_sum[threadIdx.x + s]);
 		}
 		__syncthreads();
 	}
 	if (threadIdx.x == 0) {
-		part[2 * blockIdx.x] = s_len[0];
-		part[2 * blockIdx.x + 1] = s_band[0];
+		part[3 * blockIdx.x] = s_len[0];
+		part[3 * blockIdx.x + 1] = s_band[0];
+		part[3 * blockIdx.x + 2] = __double_as_longlong(s_sum[0]);
 	}
 }
 
@@ -291,29 +299,34 @@ int stage_lattice(tbtkx_context *h, const tbtkx_lattice *lat, const double *onsi
 	return TBTKX_OK;
 }
 
-// Largest ELL row length and band of a lattice (and per-row counts into
-// h->lat_count when `counts`).
+// Largest ELL row length, band and Gershgorin bound of a lattice (and
+// per-row counts into h->lat_count when `counts`).
 int lattice_extent(tbtkx_context *h, const LatDev &g, int64_t N, bool counts, int64_t *maxlen,
-                   int64_t *band) {
+                   int64_t *band, double *bound = nullptr) {
 	int rc;
 	const unsigned nb = grid_for(N);
-	if ((rc = h->lat_part.reserve((size_t)nb * 16))) return rc;
+	if ((rc = h->lat_part.reserve((size_t)nb * 24))) return rc;
 	if (counts && (rc = h->lat_count.reserve((size_t)N * 4))) return rc;
 	lattice_count<<<nb, kThreads, 0, h->stream>>>(
 	    g, h->lat_terms.as<tbtkx_lattice_term>(),
 	    g.has_onsite ? h->lat_onsite.as<double2>() : nullptr, N,
 	    counts ? h->lat_count.as<uint32_t>() : nullptr, h->lat_part.as<int64_t>());
 	TBTKX_CHECK_HIP(hipGetLastError());
-	std::vector<int64_t> part((size_t)nb * 2);
+	std::vector<int64_t> part((size_t)nb * 3);
 	TBTKX_CHECK_HIP(hipMemcpyAsync(part.data(), h->lat_part.ptr, part.size() * 8,
 	                               hipMemcpyDeviceToHost, h->stream));
 	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
 	*maxlen = 0;
 	*band = 0;
+	double gb = 0.0;
 	for (unsigned b = 0; b < nb; b++) {
-		*maxlen = std::max(*maxlen, part[2 * b]);
-		*band = std::max(*band, part[2 * b + 1]);
+		*maxlen = std::max(*maxlen, part[3 * b]);
+		*band = std::max(*band, part[3 * b + 1]);
+		double x;
+		memcpy(&x, &part[3 * b + 2], 8);
+		gb = std::max(gb, x);
 	}
+	if (bound) *bound = gb;
 	return TBTKX_OK;
 }
 
@@ -419,6 +432,22 @@ int tbtkx_set_lattice(tbtkx_handle h, const tbtkx_lattice *lat, const double *on
 	h->h_real = real;
 	h->nblk = (int)blocks.size() - 1;
 	return TBTKX_OK;
+}
+
+int tbtkx_lattice_gershgorin(tbtkx_handle h, const tbtkx_lattice *lat, const double *onsite,
+                             double *bound) {
+	TBTKX_REQUIRE(h && bound, TBTKX_ERR_ARG, "tbtkx_lattice_gershgorin: null pointer");
+	LatDev g{};
+	int64_t N = 0;
+	int max_row = 0;
+	bool real = true;
+	int rc = check_lattice(lat, onsite, "tbtkx_lattice_gershgorin: ", &g, &N, &max_row, &real);
+	if (rc) return rc;
+	if ((rc = set_device(h))) return rc;
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	if ((rc = stage_lattice(h, lat, onsite, N))) return rc;
+	int64_t maxlen = 0, band = 0;
+	return lattice_extent(h, g, N, false, &maxlen, &band, bound);
 }
 
 int tbtkx_csr_size(tbtkx_handle h, int64_t *num_rows, int64_t *nnz) {

@@ -45,6 +45,7 @@ SIGNATURES = {
     "tbtkx_moments_block": [_P, _I, _P, _I, _I, _P],
     "tbtkx_moments_trace": [_P, _I, ctypes.c_uint64, _I, _I, _P],
     "tbtkx_set_lattice": [_P, _P, _P, _D],
+    "tbtkx_lattice_gershgorin": [_P, _P, _P, _P],
     "tbtkx_csr_size": [_P, _P, _P],
     "tbtkx_csr_copy": [_P, _P, _P, _P],
     "tbtkx_multi_set": [_P, _I, _P, _P, _P, _P, _P, _P],
@@ -238,6 +239,18 @@ class Engine:
                                            float(scale)))
         self.N = spec.basis_size
         return keep
+
+    def lattice_gershgorin(self, spec) -> float:
+        """max_r sum_c |H_rc| of a lattice.LatticeSpec, computed on the device."""
+        lat, keep = _lattice_struct(spec)
+        onsite = None
+        if spec.onsite is not None:
+            onsite = np.ascontiguousarray(spec.onsite, dtype=np.complex128)
+        out = ctypes.c_double(0)
+        _check(self._lib.tbtkx_lattice_gershgorin(
+            self._h, ctypes.byref(lat), _p(onsite) if onsite is not None else None,
+            ctypes.byref(out)))
+        return out.value
 
     def csr_copy(self):
         """(rowptr, col, H/s) of the resident model, as the device holds them."""

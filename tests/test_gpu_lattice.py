@@ -133,3 +133,13 @@ def test_lattice_rejects_bad_input(engine):
     wide.terms = wide.terms + [((1, 1, 0), 0, 0, 0.5, 0.0), ((-1, -1, 0), 0, 0, 0.5, 0.0)]
     with pytest.raises(X.EngineError, match="more than 8 non-zeros"):
         engine.multi_set_lattices([wide], [10.0])
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_device_gershgorin(engine, name):
+    """Device max_r sum_c |H_rc| equals the host CSR row-sum bound (device
+    hypot against numpy's abs: 1e-13 relative)."""
+    make_model, make_spec = CASES[name]
+    m, spec = make_model(), make_spec()
+    g = engine.lattice_gershgorin(spec)
+    assert abs(g - m.gershgorin_bound()) <= 1e-13 * m.gershgorin_bound()
