@@ -117,6 +117,15 @@ static void writeBin(const string &path, const void *data, size_t bytes){
 	fclose(f);
 }
 
+// With json=<path>, a property mode also writes the property's own JSON
+// serialization (serialize(Serializable::Mode::JSON)).
+template<typename P>
+static void writeJSON(const P &property){
+	if(!args.count("json")) return;
+	const string j = property.serialize(Serializable::Mode::JSON);
+	writeBin(arg("json"), j.data(), j.size());
+}
+
 static Solver::ChebyshevExpander::Type solverType(const string &s){
 	if(s == "Retarded") return Solver::ChebyshevExpander::Type::Retarded;
 	if(s == "Advanced") return Solver::ChebyshevExpander::Type::Advanced;
@@ -245,6 +254,7 @@ int main(int argc, char **argv){
 			for(int e = 0; e < res; e++)
 				flat.push_back(gf({t, from}, e));
 		writeBin(arg("out"), flat.data(), flat.size()*sizeof(complex<double>));
+		writeJSON(gf);
 		return 0;
 	}
 	// Thermodynamic-equilibrium properties: T= (K) and mu= (eV) set the
@@ -266,6 +276,7 @@ int main(int argc, char **argv){
 			: pe.calculateDensity(parseIndices(arg("patterns")));
 		const vector<double> &d = density.getData();
 		writeBin(arg("out"), d.data(), d.size()*sizeof(double));
+		writeJSON(density);
 		return 0;
 	}
 	if(mode == "mag"){
@@ -293,6 +304,15 @@ int main(int argc, char **argv){
 		Property::LDOS ldos = pe.calculateLDOS(parseIndices(arg("patterns")));
 		const vector<double> &d = ldos.getData();
 		writeBin(arg("out"), d.data(), d.size()*sizeof(double));
+		writeJSON(ldos);
+		return 0;
+	}
+	if(mode == "ldosr"){
+		// Ranges format: pattern= with IDX_X/IDX_Y/IDX_Z loops, ranges=.
+		Property::LDOS ldos = pe.calculateLDOS(parseIndex(arg("pattern")), parseIndex(arg("ranges")));
+		const vector<double> &d = ldos.getData();
+		writeBin(arg("out"), d.data(), d.size()*sizeof(double));
+		writeJSON(ldos);
 		return 0;
 	}
 	if(mode == "sampledos"){
@@ -304,6 +324,7 @@ int main(int argc, char **argv){
 		);
 		const vector<double> &d = dos.getData();
 		writeBin(arg("out"), d.data(), d.size()*sizeof(double));
+		writeJSON(dos);
 		return 0;
 	}
 	if(mode == "timeldos"){

@@ -9,8 +9,10 @@
 #define TBTK_AMD_INDEX_TREE
 
 #include "TBTK/Index.h"
+#include "TBTK/Serializable.h"
 
 #include <map>
+#include <string>
 #include <set>
 #include <vector>
 
@@ -19,6 +21,17 @@ namespace TBTK {
 class IndexTree {
 public:
 	enum class SearchMode { StrictMatch, MatchWildcards };
+
+	IndexTree() {}
+	/* From IndexTree::serialize output: the included indices of the tree,
+	 * in its depth-first order (Lib/src/Core/IndexTree.cpp:41-131). */
+	IndexTree(const std::string &serialization, Serializable::Mode mode);
+	/* The reference's node tree of these indices (children per subindex
+	 * value, flags, linear indices; Lib/src/Core/IndexTree.cpp:166-313,
+	 * 606-650), as JSON. Wildcard types are written with the reference's
+	 * values of IDX_ALL, IDX_SUM_ALL and IDX_SPIN
+	 * (Lib/include/Core/TBTK/Subindex.h:395-398). */
+	std::string serialize(Serializable::Mode mode) const;
 
 	void add(const Index &index);
 	void generateLinearMap();

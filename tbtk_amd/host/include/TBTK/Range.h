@@ -4,11 +4,19 @@
 #ifndef TBTK_AMD_RANGE
 #define TBTK_AMD_RANGE
 
+#include "TBTK/Serializable.h"
+
+#include <string>
+
 namespace TBTK {
 
 class Range {
 public:
 	Range(double lowerBound, double upperBound, unsigned int resolution);
+	/* From Range::serialize output (Lib/src/Utilities/Range.cpp:59-92). */
+	Range(const std::string &serialization, Serializable::Mode mode);
+	/* {"dx", "id": "Range", "resolution", "start"} (Range.cpp:94-115). */
+	std::string serialize(Serializable::Mode mode) const;
 	double operator[](unsigned int n) const { return start + n * dx; }
 	unsigned int getResolution() const { return resolution; }
 	double getLast() const { return (*this)[resolution - 1]; }

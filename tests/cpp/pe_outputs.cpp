@@ -60,6 +60,17 @@ template <typename C> static std::vector<complex<double>> flatten(const C &rows)
 	return out;
 }
 
+// A property's JSON serialization, as json_<name>.json.
+static void writeJSON(const std::string &name, const std::string &text) {
+	FILE *f = fopen((outdir + "/json_" + name + ".json").c_str(), "wb");
+	if (!f) {
+		fprintf(stderr, "cannot write %s\n", name.c_str());
+		exit(2);
+	}
+	fwrite(text.data(), 1, text.size(), f);
+	fclose(f);
+}
+
 int main(int argc, char **argv) {
 	if (argc < 2) {
 		fprintf(stderr, "usage: pe_outputs <outdir> [crand.bin]\n");
@@ -106,7 +117,14 @@ int main(int argc, char **argv) {
 			for (const Index &ti : tos)
 				for (int e = 0; e < 300; e++) out.push_back(g({ti, {15, 15, 0}}, e));
 			write(std::string("square30_spin_gf_") + names[t], out);
+			if (t == 3) writeJSON("gf_NonPrincipal", g.serialize(Serializable::Mode::JSON));
 		}
+		// JSON serializations of LDOS in the Custom and Ranges formats.
+		writeJSON("ldos_custom",
+		          pe.calculateLDOS({{15, 15, IDX_SUM_ALL}, {3, 4, IDX_ALL}})
+		              .serialize(Serializable::Mode::JSON));
+		writeJSON("ldos_ranges", pe.calculateLDOS({IDX_X, 15, IDX_SUM_ALL}, {2, 1, 2})
+		                             .serialize(Serializable::Mode::JSON));
 		// The pattern form of calculateGreensFunction must give the same.
 		Property::GreensFunction gp = pe.calculateGreensFunction(
 		    {{{15, 15, 0}, {15, 15, 0}}, {{16, 15, 0}, {15, 15, 0}}},
@@ -127,11 +145,15 @@ int main(int argc, char **argv) {
 		for (double T : {300.0, 0.0}) {
 			model.setTemperature(T);
 			const std::string tag = T == 0 ? "T0" : "T300";
-			write("thermo_density_custom_" + tag,
-			      pe.calculateDensity({{15, 15, IDX_SUM_ALL}, {3, 4, IDX_ALL}, {0, 0, 1}})
-			          .getData());
-			write("thermo_density_ranges_" + tag,
-			      pe.calculateDensity({IDX_X, 15, IDX_SUM_ALL}, {4, 1, 2}).getData());
+			const Property::Density dc =
+			    pe.calculateDensity({{15, 15, IDX_SUM_ALL}, {3, 4, IDX_ALL}, {0, 0, 1}});
+			const Property::Density dr = pe.calculateDensity({IDX_X, 15, IDX_SUM_ALL}, {4, 1, 2});
+			write("thermo_density_custom_" + tag, dc.getData());
+			write("thermo_density_ranges_" + tag, dr.getData());
+			if (T == 300) {
+				writeJSON("density_custom_T300", dc.serialize(Serializable::Mode::JSON));
+				writeJSON("density_ranges_T300", dr.serialize(Serializable::Mode::JSON));
+			}
 			write("thermo_mag_custom_" + tag,
 			      spins(pe.calculateMagnetization({{15, 15, IDX_SPIN}, {2, 3, IDX_SPIN}})
 			                .getData()));
@@ -169,7 +191,9 @@ int main(int argc, char **argv) {
 		solver.setUseLookupTable(true);
 		PropertyExtractor::ChebyshevExpander pe(solver);
 		pe.setEnergyWindow(-9, 9, 200);
-		write("periodic5_sampledos", pe.sampleDOS(7, {}, 3).getData());
+		const Property::DOS dos = pe.sampleDOS(7, {}, 3);
+		write("periodic5_sampledos", dos.getData());
+		writeJSON("sampledos", dos.serialize(Serializable::Mode::JSON));
 	}
 
 	{	// ring10 LDOS, windows +-0.9s and +-0.7s, direct contraction

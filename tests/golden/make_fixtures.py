@@ -111,6 +111,32 @@ def thermo_fixture(p1):
 
 
 
+def json_fixture(p1, pp):
+    """The reference's own JSON serializations (serialize(Serializable::
+    Mode::JSON)) of the properties tests/cpp/pe_outputs.cpp computes with the
+    same parameters: LDOS (Custom, Ranges), a NonPrincipal Green's function,
+    Density (Custom, Ranges, T = 300 K, mu = 0.3) and sampleDOS. Written as
+    tests/golden/json/<name>.json."""
+    jdir = os.path.join(OUT, "json")
+    os.makedirs(jdir, exist_ok=True)
+    base = dict(model=p1, scale=10.0, M=400, broadening=1e-6, window="-9.5,9.5,300", lookup=1)
+    out = os.path.join(TMP, "j.bin")
+    cases = [
+        ("ldos_custom", dict(mode="ldos", patterns="15,15,+;3,4,*", **base)),
+        ("ldos_ranges", dict(mode="ldosr", pattern="x,15,+", ranges="2,1,2", **base)),
+        ("gf_NonPrincipal", dict(mode="gf", type="NonPrincipal", to="15,15,0;16,15,0",
+                                 **{"from": "15,15,0"}, **base)),
+        ("density_custom_T300", dict(mode="density", patterns="15,15,+;3,4,*;0,0,1", T=300.0,
+                                     mu=0.3, **base)),
+        ("density_ranges_T300", dict(mode="density", pattern="x,15,+", ranges="4,1,2", T=300.0,
+                                     mu=0.3, **base)),
+        ("sampledos", dict(mode="sampledos", model=pp, scale=10.0, M=200, broadening=1e-6,
+                           window="-9,9,200", lookup=1, n=7, seed=3)),
+    ]
+    for name, kw in cases:
+        O.run_ref(out=out, json=os.path.join(jdir, name + ".json"), **kw)
+
+
 def main():
     if not O.ref_available():
         sys.exit("build oracle/_ref/ref_driver first (make -f oracle/Makefile.ref)")
@@ -209,6 +235,8 @@ def main():
     save("periodic5_sampledos", dict(scale=10.0, M=200, window=[-9, 9, 200], n=7,
                                      seed=3, broadening=1e-6),
          dos=np.fromfile(out, np.float64))
+
+    json_fixture(p1, pp)
 
 
 if __name__ == "__main__":

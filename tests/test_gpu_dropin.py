@@ -104,3 +104,21 @@ def test_thermodynamic_properties_bitexact(outputs):
     for name in ("splodos_custom", "splodos_ranges"):
         got = _read(outputs, f"thermo_{name}", np.complex128)
         assert np.array_equal(got, fx[name]), name
+
+
+JSON_NAMES = ["ldos_custom", "ldos_ranges", "gf_NonPrincipal", "density_custom_T300",
+              "density_ranges_T300", "sampledos"]
+
+
+@pytest.mark.parametrize("name", JSON_NAMES)
+def test_json_serialization_matches_reference(outputs, name):
+    """serialize(Serializable::Mode::JSON) of the properties computed here
+    equals the reference's serialization of the same computation
+    (tests/golden/json), except the reference's uninitialized
+    "defaultValue" (see tests/test_json.py)."""
+    import re
+    norm = lambda t: re.sub(r'"defaultValue":"[^"]*"', '"defaultValue":"<default>"', t)
+    got = (outputs / f"json_{name}.json").read_text()
+    with open(os.path.join(ROOT, "tests", "golden", "json", name + ".json")) as f:
+        ref = f.read()
+    assert norm(got) == norm(ref)
