@@ -156,6 +156,110 @@ __device__ __forceinline__ void mac(float2 a, float2 x, float &ax, float &ay) {
 	}
 }
 
+// The generic path of multi_step (Q >= 2 rows per group): tiles t, t + nwx,
+// ... of model d below tend, RPI row groups in flight per lane.
+// The generic path of multi_step (Q >= 2 rows per group): tiles t, t + nwx,
+// ... of model d below tend, RPI row groups in flight per lane. With a
+// shared window (Q W <= 64) a lane keeps only the window entry it loaded and
+// takes its row's columns from the other lanes just before issuing the
+// gathers and its row's values just before the multiply-adds, so a row
+// group costs 3 VGPRs of window besides the gathered values. SHARED: the
+// window is known to be shared (Q W <= 64) at compile time.
+template <int W, int RPI, bool SHARED, bool FIRST, bool DOTS, bool REALH>
+__device__ __forceinline__ void multi_rows(
+    const MDesc &d, int &t, int tend, int nwx, int toff_m, int G, int Q, int qc, bool act,
+    int b, int w, int64_t tr, bool shared_window, int lane, const uint32_t *__restrict__ ecol,
+    const float2 *__restrict__ ev, const float2 *__restrict__ X, float2 *__restrict__ Y,
+    int packed, double &d0x, double &d0y, double &d1x, double &d1y) {
+	const int base = qc * W;
+	if (SHARED) shared_window = true;
+	// The model's block of X (wave-uniform base) and 32-bit byte offsets
+	// into it (N_m w 8 B < 4 GB): one VGPR per gather address.
+	const char *Xm = reinterpret_cast<const char *>(X + d.vec0);
+	for (; t < tend; t += nwx) {
+		// Tile lt of the model: strip k, layer a, j-th tile of the segment.
+		const uint32_t lt = (uint32_t)(t - toff_m);
+		const uint32_t seg = lt / (uint32_t)d.tps;
+		const uint32_t j = lt - seg * (uint32_t)d.tps;
+		const uint32_t kk = seg / (uint32_t)d.A;
+		const int64_t a = seg - kk * (uint32_t)d.A;
+		const int64_t c0 = (int64_t)kk * d.S + (int64_t)j * tr;
+		int64_t rows = d.S - (int64_t)j * tr < tr ? d.S - (int64_t)j * tr : tr;
+		if (c0 + rows > d.D) rows = d.D - c0;
+		int64_t row0 = a * d.D + c0;
+		if (row0 + rows > d.N) rows = d.N - row0;
+		if (rows <= 0) {
+			rows = 0;
+			row0 = 0;
+		}
+		for (int gg = 0; gg < G; gg += RPI) {
+			int64_t row[RPI];
+			bool live[RPI];
+			uint32_t wc[RPI];
+			float2 wv[RPI];
+			float2 xv[RPI][W], yo[RPI];
+#pragma unroll
+			for (int r = 0; r < RPI; r++) {
+				const int64_t off = (int64_t)(gg + r) * Q + qc;
+				live[r] = act && off < rows;
+				row[r] = row0 + (off < rows ? off : (rows > 0 ? rows - 1 : 0));
+				if (shared_window) {
+					// The group's Q rows are consecutive, so their Q W
+					// entries are one contiguous window: lane l loads
+					// entry l.
+					const int64_t gfirst = row0 + (int64_t)(gg + r) * Q;
+					const int64_t glast = rows > 0 ? row0 + rows - 1 : row0;
+					int64_t wrow = gfirst + lane / W;
+					if (wrow > glast) wrow = glast;
+					const int64_t we = (d.erow0 + wrow) * W + lane % W;
+					wc[r] = ecol[we];
+					wv[r] = ev[we];
+				}
+				if (!FIRST) yo[r] = load_nt(&Y[d.vec0 + row[r] * w + b]);
+			}
+			// Gathers; a2 holds the values of the rows without a shared
+			// window (loaded with the columns).
+			float2 a2[RPI][W];
+#pragma unroll
+			for (int r = 0; r < RPI; r++) {
+				if (shared_window) {
+#pragma unroll
+					for (int u = 0; u < W; u++) {
+						const uint32_t c = (uint32_t)__shfl((int)wc[r], base + u);
+						xv[r][u] = *reinterpret_cast<const float2 *>(
+						    Xm + (c * (uint32_t)w + (uint32_t)b) * 8u);
+					}
+				} else {
+					const int64_t e = (d.erow0 + row[r]) * W;
+#pragma unroll
+					for (int u = 0; u < W; u++) {
+						a2[r][u] = ev[e + u];
+						xv[r][u] = *reinterpret_cast<const float2 *>(
+						    Xm + (ecol[e + u] * (uint32_t)w + (uint32_t)b) * 8u);
+					}
+				}
+			}
+#pragma unroll
+			for (int r = 0; r < RPI; r++) {
+				float ax = FIRST ? 0.0f : -yo[r].x;
+				float ay = FIRST ? 0.0f : -yo[r].y;
+#pragma unroll
+				for (int u = 0; u < W; u++) {
+					const float2 av =
+					    shared_window
+					        ? make_float2(__shfl(wv[r].x, base + u), __shfl(wv[r].y, base + u))
+					        : a2[r][u];
+					mac<REALH>(av, xv[r][u], ax, ay);
+				}
+				if (!live[r]) continue;
+				store_nt(&Y[d.vec0 + row[r] * w + b], ax, ay);
+				// Slot 0 is the row itself (diagonal first).
+				if (DOTS) dot_acc(xv[r][0].x, xv[r][0].y, ax, ay, packed, d0x, d0y, d1x, d1y);
+			}
+		}
+	}
+}
+
 // One recurrence step of every active model. Each model is traversed in
 // L2 strips (layers of D rows, strips of S rows; a tile is G row groups of
 // Q rows inside one strip segment), the models' tile lists are concatenated
@@ -176,7 +280,9 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 	const int lane = threadIdx.x & 63;
 	const int x = blockIdx.x & 7;
 	const int nwx = (gridDim.x >> 3) * kWaves;                  // wavefronts per XCD
-	const int wx = (blockIdx.x >> 3) * kWaves + (threadIdx.x >> 6);
+	// Wave-uniform (readfirstlane tells the compiler): the tile, the model
+	// and its descriptor then live in scalar registers.
+	const int wx = __builtin_amdgcn_readfirstlane((blockIdx.x >> 3) * kWaves + (threadIdx.x >> 6));
 	const int plo = xpart[x], phi = xpart[x + 1];
 	int t = plo + wx;
 	if (t >= phi) return;
@@ -258,83 +364,16 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 				}
 			}
 		}
-		for (; t < tend; t += nwx) {
-			// Tile lt of the model: strip k, layer a, j-th tile of the segment.
-			const uint32_t lt = (uint32_t)(t - toff[m]);
-			const uint32_t seg = lt / (uint32_t)d.tps;
-			const uint32_t j = lt - seg * (uint32_t)d.tps;
-			const uint32_t kk = seg / (uint32_t)d.A;
-			const int64_t a = seg - kk * (uint32_t)d.A;
-			const int64_t c0 = (int64_t)kk * d.S + (int64_t)j * tr;
-			int64_t rows = d.S - (int64_t)j * tr < tr ? d.S - (int64_t)j * tr : tr;
-			if (c0 + rows > d.D) rows = d.D - c0;
-			int64_t row0 = a * d.D + c0;
-			if (row0 + rows > d.N) rows = d.N - row0;
-			if (rows <= 0) {
-				rows = 0;
-				row0 = 0;
-			}
-			for (int gg = 0; gg < G; gg += kRpi) {
-				int64_t row[kRpi];
-				bool live[kRpi];
-				uint32_t c[kRpi][W];
-				float2 a2[kRpi][W], xv[kRpi][W], yo[kRpi], xs[kRpi];
-#pragma unroll
-				for (int r = 0; r < kRpi; r++) {
-					const int64_t off = (int64_t)(gg + r) * Q + qc;
-					live[r] = act && off < rows;
-					row[r] = row0 + (off < rows ? off : (rows > 0 ? rows - 1 : 0));
-					if (shared_window) {
-						// The group's Q rows are consecutive, so their Q W
-						// entries are one contiguous window: lane l loads
-						// entry l, and each lane takes its row's entries
-						// from the lanes that loaded them.
-						const int64_t gfirst = row0 + (int64_t)(gg + r) * Q;
-						const int64_t glast = rows > 0 ? row0 + rows - 1 : row0;
-						int64_t wrow = gfirst + lane / W;
-						if (wrow > glast) wrow = glast;
-						const int64_t we = (d.erow0 + wrow) * W + lane % W;
-						const uint32_t wc = ecol[we];
-						const float2 wv = ev[we];
-						const int base = qc * W;
-#pragma unroll
-						for (int u = 0; u < W; u++) {
-							c[r][u] = (uint32_t)__shfl((int)wc, base + u);
-							a2[r][u] = make_float2(__shfl(wv.x, base + u), __shfl(wv.y, base + u));
-						}
-					} else {
-						const int64_t e = (d.erow0 + row[r]) * W;
-#pragma unroll
-						for (int u = 0; u < W; u++) {
-							c[r][u] = ecol[e + u];
-							a2[r][u] = ev[e + u];
-						}
-					}
-					const int64_t o = d.vec0 + row[r] * w + b;
-					if (!FIRST) yo[r] = load_nt(&Y[o]);
-				}
-#pragma unroll
-				for (int r = 0; r < kRpi; r++)
-#pragma unroll
-					for (int u = 0; u < W; u++)
-						xv[r][u] = X[d.vec0 + (int64_t)c[r][u] * w + b];
-				// Slot 0 is the row itself (diagonal first).
-#pragma unroll
-				for (int r = 0; r < kRpi; r++) xs[r] = xv[r][0];
-#pragma unroll
-				for (int r = 0; r < kRpi; r++) {
-					float ax = FIRST ? 0.0f : -yo[r].x;
-					float ay = FIRST ? 0.0f : -yo[r].y;
-#pragma unroll
-					for (int u = 0; u < W; u++) mac<REALH>(a2[r][u], xv[r][u], ax, ay);
-					if (!live[r]) continue;
-					store_nt(&Y[d.vec0 + row[r] * w + b], ax, ay);
-					if (DOTS) {
-						dot_acc(xs[r].x, xs[r].y, ax, ay, packed, d0x, d0y, d1x, d1y);
-					}
-				}
-			}
-		}
+		// Wide models (Q <= 4 rows per group, so Q W <= 32: shared window)
+		// keep twice the row groups in flight.
+		if (Q <= 4)
+			multi_rows<W, 2 * kRpi, true, FIRST, DOTS, REALH>(
+			    d, t, tend, nwx, toff[m], G, Q, qc, act, b, w, tr, true, lane, ecol, ev, X, Y,
+			    packed, d0x, d0y, d1x, d1y);
+		else
+			multi_rows<W, kRpi, false, FIRST, DOTS, REALH>(
+			    d, t, tend, nwx, toff[m], G, Q, qc, act, b, w, tr, shared_window, lane, ecol, ev, X,
+			    Y, packed, d0x, d0y, d1x, d1y);
 		if (DOTS) {
 			// Column b's sum over the row slots q = 0 .. Q-1, ascending.
 			double s0x = 0.0, s0y = 0.0, s1x = 0.0, s1y = 0.0;
