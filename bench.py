@@ -463,6 +463,12 @@ def main():
         del a, b
 
     out = {"metric": METRIC}
+    if "mfma" in parts and not args.calibrate_copy:
+        # torch (the library GEMM of the MFMA part) brings its own HIP
+        # runtime, which initializes only when it comes before the engine's.
+        import torch
+        torch.zeros(1, device="cuda")
+        torch.cuda.synchronize()
     eng = X.Engine(0)
     if args.complex:
         eng.set_option(X.OPT_REAL, 0)
