@@ -27,8 +27,8 @@
 //
 // Lane mapping for a model of width w: a wavefront covers
 // Q = floor(64 / w) consecutive rows at once, lane l -> row slot q = l / w,
-// column b = l % w; lanes l >= Q w are idle (they mirror a valid row's loads
-// and store nothing), so no HBM traffic is spent on padding columns. A tile is
+// column b = l % w; lanes l >= Q w are idle (they mirror a valid row of the
+// same group), so no HBM traffic is spent on padding columns. A tile is
 // G row groups (G Q rows). Each wavefront of the persistent grid walks a
 // contiguous range of the phase's tile list, cut so that every wavefront gets
 // the same number of algorithmic bytes.
@@ -46,6 +46,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -156,87 +157,60 @@ __device__ __forceinline__ void mac(float2 a, float2 x, float &ax, float &ay) {
 	}
 }
 
-// The generic path of multi_step (Q >= 2 rows per group): tiles t, t + nwx,
-// ... of model d below tend, RPI row groups in flight per lane.
-// The generic path of multi_step (Q >= 2 rows per group): tiles t, t + nwx,
-// ... of model d below tend, RPI row groups in flight per lane. With a
-// shared window (Q W <= 64) a lane keeps only the window entry it loaded and
-// takes its row's columns from the other lanes just before issuing the
-// gathers and its row's values just before the multiply-adds, so a row
-// group costs 3 VGPRs of window besides the gathered values. SHARED: the
-// window is known to be shared (Q W <= 64) at compile time.
-template <int W, int RPI, bool SHARED, bool FIRST, bool DOTS, bool REALH>
+// Tile lt of a model (strip k, layer a, j-th tile of the segment): its first
+// row, and in rows its row count (0 for a tile past the end of the model).
+__device__ __forceinline__ int64_t tile_first_row(const MDesc &d, uint32_t lt, int64_t tr,
+                                                  int64_t &rows) {
+	const uint32_t seg = lt / (uint32_t)d.tps;
+	const uint32_t j = lt - seg * (uint32_t)d.tps;
+	const uint32_t kk = seg / (uint32_t)d.A;
+	const int64_t a = seg - kk * (uint32_t)d.A;
+	const int64_t c0 = (int64_t)kk * d.S + (int64_t)j * tr;
+	rows = d.S - (int64_t)j * tr < tr ? d.S - (int64_t)j * tr : tr;
+	if (c0 + rows > d.D) rows = d.D - c0;
+	const int64_t row0 = a * d.D + c0;
+	if (row0 + rows > d.N) rows = d.N - row0;
+	if (rows <= 0) {
+		rows = 0;
+		return 0;
+	}
+	return row0;
+}
+
+// Narrow models (Q W > 64: a row group's entries do not fit one window):
+// tiles t, t + nwx, ... of model d below tend, RPI row groups in flight per
+// lane, each lane loading its own row's entries.
+template <int W, int RPI, bool FIRST, bool DOTS, bool REALH>
 __device__ __forceinline__ void multi_rows(
     const MDesc &d, int &t, int tend, int nwx, int toff_m, int G, int Q, int qc, bool act,
-    int b, int w, int64_t tr, bool shared_window, int lane, const uint32_t *__restrict__ ecol,
+    int b, int w, int64_t tr, const uint32_t *__restrict__ ecol,
     const float2 *__restrict__ ev, const float2 *__restrict__ X, float2 *__restrict__ Y,
     int packed, double &d0x, double &d0y, double &d1x, double &d1y) {
-	const int base = qc * W;
-	if (SHARED) shared_window = true;
 	// The model's block of X (wave-uniform base) and 32-bit byte offsets
-	// into it (N_m w 8 B < 4 GB): one VGPR per gather address.
+	// into it (N_m w 8 B < 4 GiB, checked on the host).
 	const char *Xm = reinterpret_cast<const char *>(X + d.vec0);
 	for (; t < tend; t += nwx) {
-		// Tile lt of the model: strip k, layer a, j-th tile of the segment.
-		const uint32_t lt = (uint32_t)(t - toff_m);
-		const uint32_t seg = lt / (uint32_t)d.tps;
-		const uint32_t j = lt - seg * (uint32_t)d.tps;
-		const uint32_t kk = seg / (uint32_t)d.A;
-		const int64_t a = seg - kk * (uint32_t)d.A;
-		const int64_t c0 = (int64_t)kk * d.S + (int64_t)j * tr;
-		int64_t rows = d.S - (int64_t)j * tr < tr ? d.S - (int64_t)j * tr : tr;
-		if (c0 + rows > d.D) rows = d.D - c0;
-		int64_t row0 = a * d.D + c0;
-		if (row0 + rows > d.N) rows = d.N - row0;
-		if (rows <= 0) {
-			rows = 0;
-			row0 = 0;
-		}
+		int64_t rows;
+		const int64_t row0 = tile_first_row(d, (uint32_t)(t - toff_m), tr, rows);
 		for (int gg = 0; gg < G; gg += RPI) {
 			int64_t row[RPI];
 			bool live[RPI];
-			uint32_t wc[RPI];
-			float2 wv[RPI];
-			float2 xv[RPI][W], yo[RPI];
+			float2 a2[RPI][W], xv[RPI][W], yo[RPI];
 #pragma unroll
 			for (int r = 0; r < RPI; r++) {
 				const int64_t off = (int64_t)(gg + r) * Q + qc;
 				live[r] = act && off < rows;
 				row[r] = row0 + (off < rows ? off : (rows > 0 ? rows - 1 : 0));
-				if (shared_window) {
-					// The group's Q rows are consecutive, so their Q W
-					// entries are one contiguous window: lane l loads
-					// entry l.
-					const int64_t gfirst = row0 + (int64_t)(gg + r) * Q;
-					const int64_t glast = rows > 0 ? row0 + rows - 1 : row0;
-					int64_t wrow = gfirst + lane / W;
-					if (wrow > glast) wrow = glast;
-					const int64_t we = (d.erow0 + wrow) * W + lane % W;
-					wc[r] = ecol[we];
-					wv[r] = ev[we];
-				}
 				if (!FIRST) yo[r] = load_nt(&Y[d.vec0 + row[r] * w + b]);
 			}
-			// Gathers; a2 holds the values of the rows without a shared
-			// window (loaded with the columns).
-			float2 a2[RPI][W];
 #pragma unroll
 			for (int r = 0; r < RPI; r++) {
-				if (shared_window) {
+				const int64_t e = (d.erow0 + row[r]) * W;
 #pragma unroll
-					for (int u = 0; u < W; u++) {
-						const uint32_t c = (uint32_t)__shfl((int)wc[r], base + u);
-						xv[r][u] = *reinterpret_cast<const float2 *>(
-						    Xm + (c * (uint32_t)w + (uint32_t)b) * 8u);
-					}
-				} else {
-					const int64_t e = (d.erow0 + row[r]) * W;
-#pragma unroll
-					for (int u = 0; u < W; u++) {
-						a2[r][u] = ev[e + u];
-						xv[r][u] = *reinterpret_cast<const float2 *>(
-						    Xm + (ecol[e + u] * (uint32_t)w + (uint32_t)b) * 8u);
-					}
+				for (int u = 0; u < W; u++) {
+					a2[r][u] = ev[e + u];
+					xv[r][u] = *reinterpret_cast<const float2 *>(
+					    Xm + (ecol[e + u] * (uint32_t)w + (uint32_t)b) * 8u);
 				}
 			}
 #pragma unroll
@@ -244,13 +218,7 @@ __device__ __forceinline__ void multi_rows(
 				float ax = FIRST ? 0.0f : -yo[r].x;
 				float ay = FIRST ? 0.0f : -yo[r].y;
 #pragma unroll
-				for (int u = 0; u < W; u++) {
-					const float2 av =
-					    shared_window
-					        ? make_float2(__shfl(wv[r].x, base + u), __shfl(wv[r].y, base + u))
-					        : a2[r][u];
-					mac<REALH>(av, xv[r][u], ax, ay);
-				}
+				for (int u = 0; u < W; u++) mac<REALH>(a2[r][u], xv[r][u], ax, ay);
 				if (!live[r]) continue;
 				store_nt(&Y[d.vec0 + row[r] * w + b], ax, ay);
 				// Slot 0 is the row itself (diagonal first).
@@ -258,6 +226,149 @@ __device__ __forceinline__ void multi_rows(
 			}
 		}
 	}
+}
+
+// Shared-window path (Q W <= 64) with the ELL window of the next iteration
+// loaded one iteration ahead. An iteration is RPI = 4 row groups of one
+// tile; their entries come from NW = 4 / GPW windows (window k: the
+// GPW Q W consecutive entries of groups k GPW .. k GPW + GPW - 1, lane l
+// holding entry l). Per iteration, in issue order: the Phi_{n-1} rows and
+// the gathers (columns taken from the window registers), the stores of the
+// previous iteration's rows (kept in registers until now), the next
+// iteration's windows, then the wait for this iteration's loads and the
+// multiply-adds. Vector memory operations retire in issue order on one
+// counter, so that wait never covers the stores or the prefetch and an
+// iteration costs one memory round trip instead of two.
+//
+// The stores and the prefetch are issued by every lane on every iteration
+// (no branch around them, so the compiler's wait counts are exact): the
+// last iteration loads its own window again, iterations and tiles without
+// rows are skipped, and a lane whose row slot lies past the tile's last row
+// (or an idle lane, q >= Q) takes that last row, which a live lane of the
+// same iteration computes from the same entries and the same Phi_{n-1}
+// value: it stores the same value to the same address.
+template <int W, int GPW, bool FIRST, bool DOTS, bool REALH>
+__device__ __forceinline__ void multi_rows_pipe(
+    const MDesc &d, int &t, int tend, int nwx, int toff_m, int G, int Q, int qc, bool act,
+    int b, int w, int64_t tr, int lane, const uint32_t *__restrict__ ecol,
+    const float2 *__restrict__ ev, const float2 *__restrict__ X, float2 *__restrict__ Y,
+    int packed, double &d0x, double &d0y, double &d1x, double &d1y) {
+	constexpr int RPI = 4, NW = RPI / GPW;
+	// The model's blocks of X and Y (wave-uniform bases) and 32-bit byte
+	// offsets into them (N_m w 8 B < 4 GiB, checked on the host).
+	const char *Xm = reinterpret_cast<const char *>(X + d.vec0);
+	char *Ym = reinterpret_cast<char *>(Y + d.vec0);
+	const int wrow_l = lane / W, wslot = lane - wrow_l * W;
+	// The wave's first tile with rows.
+	int64_t rows = 0, row0 = 0;
+	for (; t < tend; t += nwx) {
+		row0 = tile_first_row(d, (uint32_t)(t - toff_m), tr, rows);
+		if (rows > 0) break;
+	}
+	if (t >= tend) return;
+	int gg = 0;
+	uint32_t wc[NW];
+	float2 wv[NW];
+	// Windows of the iteration at groups g0.. of the tile at r0 (nr rows).
+	auto load_windows = [&](int64_t r0, int64_t nr, int g0) {
+		const int64_t glast = r0 + nr - 1;
+#pragma unroll
+		for (int k = 0; k < NW; k++) {
+			int64_t wrow = r0 + (int64_t)(g0 + k * GPW) * Q + wrow_l;
+			if (wrow > glast) wrow = glast;
+			const int64_t we = (d.erow0 + wrow) * W + wslot;
+			wc[k] = ecol[we];
+			wv[k] = ev[we];
+		}
+	};
+	load_windows(row0, rows, 0);
+	float2 ps[RPI];      // the previous iteration's results, stored late
+	uint32_t po[RPI];
+	// One iteration; STORE: the previous iteration's rows are stored (every
+	// iteration but the first). Returns whether another iteration follows.
+	auto iteration = [&](auto store_prev) -> bool {
+		constexpr bool STORE = decltype(store_prev)::value;
+		uint32_t off[RPI];
+		bool live[RPI];
+		float2 yo[RPI], xv[RPI][W];
+#pragma unroll
+		for (int r = 0; r < RPI; r++) {
+			const int64_t o = (int64_t)(gg + r) * Q + qc;
+			live[r] = act && o < rows;
+			const int64_t row = row0 + (o < rows ? o : rows - 1);
+			off[r] = ((uint32_t)row * (uint32_t)w + (uint32_t)b) * 8u;
+			if (!FIRST) yo[r] = load_nt(reinterpret_cast<const float2 *>(Ym + off[r]));
+		}
+#pragma unroll
+		for (int r = 0; r < RPI; r++) {
+#pragma unroll
+			for (int u = 0; u < W; u++) {
+				const uint32_t c =
+				    (uint32_t)__shfl((int)wc[r / GPW], ((r % GPW) * Q + qc) * W + u);
+				xv[r][u] = *reinterpret_cast<const float2 *>(
+				    Xm + (c * (uint32_t)w + (uint32_t)b) * 8u);
+			}
+		}
+		// This iteration's values; the window registers take the next one.
+		// (The empty asm keeps the copy, so the prefetch below can load into
+		// the window registers themselves and the loop carries no copies of
+		// pending loads.)
+		float2 vc[NW];
+#pragma unroll
+		for (int k = 0; k < NW; k++) {
+			vc[k] = wv[k];
+			asm volatile("" : "+v"(vc[k].x), "+v"(vc[k].y));
+		}
+		if constexpr (STORE) {
+#pragma unroll
+			for (int r = 0; r < RPI; r++)
+				store_nt(reinterpret_cast<float2 *>(Ym + po[r]), ps[r].x, ps[r].y);
+		}
+		// Next iteration: the next RPI groups of this tile while they hold
+		// rows, else the wave's next tile with rows.
+		int tn = t, gn = gg + RPI;
+		int64_t row0n = row0, rowsn = rows;
+		if (gn >= G || (int64_t)gn * Q >= rows) {
+			gn = 0;
+			for (;;) {
+				tn += nwx;
+				if (tn >= tend) break;
+				row0n = tile_first_row(d, (uint32_t)(tn - toff_m), tr, rowsn);
+				if (rowsn > 0) break;
+			}
+		}
+		const bool more = tn < tend;
+		if (more) load_windows(row0n, rowsn, gn);
+		else load_windows(row0, rows, gg);
+#pragma unroll
+		for (int r = 0; r < RPI; r++) {
+			float ax = FIRST ? 0.0f : -yo[r].x;
+			float ay = FIRST ? 0.0f : -yo[r].y;
+#pragma unroll
+			for (int u = 0; u < W; u++) {
+				const int src = ((r % GPW) * Q + qc) * W + u;
+				const float2 av = make_float2(__shfl(vc[r / GPW].x, src),
+				                              __shfl(vc[r / GPW].y, src));
+				mac<REALH>(av, xv[r][u], ax, ay);
+			}
+			ps[r] = make_float2(ax, ay);
+			po[r] = off[r];
+			// Slot 0 is the row itself (diagonal first).
+			if (DOTS && live[r])
+				dot_acc(xv[r][0].x, xv[r][0].y, ax, ay, packed, d0x, d0y, d1x, d1y);
+		}
+		t = tn;
+		gg = gn;
+		row0 = row0n;
+		rows = rowsn;
+		return more;
+	};
+	if (iteration(std::false_type{}))
+		while (iteration(std::true_type{})) {
+		}
+#pragma unroll
+	for (int r = 0; r < RPI; r++)
+		store_nt(reinterpret_cast<float2 *>(Ym + po[r]), ps[r].x, ps[r].y);
 }
 
 // One recurrence step of every active model. Each model is traversed in
@@ -302,7 +413,6 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 		const bool act = q < Q;
 		const int qc = act ? q : Q - 1;
 		const int64_t tr = (int64_t)G * Q;   // rows per tile
-		const bool shared_window = Q * W <= 64;
 		const int tend = phi < toff[m + 1] ? phi : toff[m + 1];
 		double d0x = 0.0, d0y = 0.0, d1x = 0.0, d1y = 0.0;
 		if (Q == 1) {
@@ -311,16 +421,8 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 			// with readlane (wave-uniform columns, scalar addresses), as in
 			// the single-model step.
 			for (; t < tend; t += nwx) {
-				const uint32_t lt = (uint32_t)(t - toff[m]);
-				const uint32_t seg = lt / (uint32_t)d.tps;
-				const uint32_t j = lt - seg * (uint32_t)d.tps;
-				const uint32_t kk = seg / (uint32_t)d.A;
-				const int64_t a = seg - kk * (uint32_t)d.A;
-				const int64_t c0 = (int64_t)kk * d.S + (int64_t)j * tr;
-				int64_t rows = d.S - (int64_t)j * tr < tr ? d.S - (int64_t)j * tr : tr;
-				if (c0 + rows > d.D) rows = d.D - c0;
-				int64_t row0 = a * d.D + c0;
-				if (row0 + rows > d.N) rows = d.N - row0;
+				int64_t rows;
+				const int64_t row0 = tile_first_row(d, (uint32_t)(t - toff[m]), tr, rows);
 				if (rows <= 0) continue;
 				for (int gg = 0; gg < rows; gg += kRpi1) {
 					// Window: lane l holds entry l % W of row gg + l / W.
@@ -363,17 +465,26 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 					}
 				}
 			}
+		} else if (Q * W <= 16) {
+			// A row group's entries fit one window (Q W <= 64): the
+			// pipelined path, with as many groups per window as fit.
+			multi_rows_pipe<W, 4, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc, act,
+			                                         b, w, tr, lane, ecol, ev, X, Y, packed,
+			                                         d0x, d0y, d1x, d1y);
+		} else if (Q * W <= 32) {
+			multi_rows_pipe<W, 2, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc, act,
+			                                         b, w, tr, lane, ecol, ev, X, Y, packed,
+			                                         d0x, d0y, d1x, d1y);
+		} else if (Q * W <= 64) {
+			multi_rows_pipe<W, 1, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc, act,
+			                                         b, w, tr, lane, ecol, ev, X, Y, packed,
+			                                         d0x, d0y, d1x, d1y);
+		} else {
+			// Narrow models: each lane loads its own row's entries.
+			multi_rows<W, kRpi, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc, act, b,
+			                                        w, tr, ecol, ev, X, Y, packed, d0x, d0y,
+			                                        d1x, d1y);
 		}
-		// Wide models (Q <= 4 rows per group, so Q W <= 32: shared window)
-		// keep twice the row groups in flight.
-		if (Q <= 4)
-			multi_rows<W, 2 * kRpi, true, FIRST, DOTS, REALH>(
-			    d, t, tend, nwx, toff[m], G, Q, qc, act, b, w, tr, true, lane, ecol, ev, X, Y,
-			    packed, d0x, d0y, d1x, d1y);
-		else
-			multi_rows<W, kRpi, false, FIRST, DOTS, REALH>(
-			    d, t, tend, nwx, toff[m], G, Q, qc, act, b, w, tr, shared_window, lane, ecol, ev, X,
-			    Y, packed, d0x, d0y, d1x, d1y);
 		if (DOTS) {
 			// Column b's sum over the row slots q = 0 .. Q-1, ascending.
 			double s0x = 0.0, s0y = 0.0, s1x = 0.0, s1y = 0.0;
@@ -896,6 +1007,9 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 			TBTKX_REQUIRE(sites[ncols + b] >= 0 && sites[ncols + b] < h->mm_N[m],
 			              TBTKX_ERR_ARG, tag + "site outside the basis");
 		width[m] = packed ? (num_sites[m] + 1) / 2 : num_sites[m];
+		// The step addresses a model's vectors with 32-bit byte offsets.
+		TBTKX_REQUIRE(h->mm_N[m] * width[m] * 8 < ((int64_t)1 << 32), TBTKX_ERR_ARG,
+		              tag + "rows x columns x 8 B must stay below 4 GiB");
 		S[m] = dots ? (num_moments[m] + 1) / 2 : num_moments[m] - 1;
 		out_off[m + 1] = out_off[m] + (int64_t)num_sites[m] * num_moments[m];
 		ncols += num_sites[m];
