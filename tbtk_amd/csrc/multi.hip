@@ -25,10 +25,13 @@
 // part of column c, 2c + 1 in the imaginary part); the step is unchanged,
 // the gathers and dots read per component. Otherwise w = B_m.
 //
-// Lane mapping for a model of width w: a wavefront covers
-// Q = floor(64 / w) consecutive rows at once, lane l -> row slot q = l / w,
-// column b = l % w; lanes l >= Q w are idle (they mirror a valid row of the
-// same group), so no HBM traffic is spent on padding columns. A tile is
+// Lane mapping for a model of width w: a lane moves one column (8 B) or,
+// for wide models, two adjacent columns (16 B; an odd w is then stored with
+// one zero column more, wp = w + 1, which stays zero through the
+// recurrence). With lpr lanes per row (w, or wp / 2), a wavefront covers
+// Q = floor(64 / lpr) consecutive rows at once, lane l -> row slot
+// q = l / lpr, first column (l % lpr) times the columns per lane; lanes
+// l >= Q lpr are idle (they mirror a valid row of the same group). A tile is
 // G row groups (G Q rows). Each wavefront of the persistent grid walks a
 // contiguous range of the phase's tile list, cut so that every wavefront gets
 // the same number of algorithmic bytes.
@@ -41,7 +44,7 @@
 //
 // Algorithmic HBM bytes per step: sum over active models of
 // 12 W N_m (ELL column + value) + 24 w N_m (read Phi_n, read Phi_{n-1},
-// write Phi_{n+1}).
+// write Phi_{n+1}); a padding column adds 24 N_m that are not counted.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -55,7 +58,6 @@ namespace {
 
 constexpr int kWaves = kThreads / 64;
 constexpr int kRpi = 2;          // row groups in flight per lane
-constexpr int kRpi1 = 4;         // rows in flight per wavefront for w > 32
 constexpr int kSegSlots = 64;    // dot slots summed per group in pass 1
 constexpr int kMaxWidth = 64;    // start vectors per model
 
@@ -66,10 +68,12 @@ struct MDesc {
 	int64_t dof;     // first element of the model's D block (doubling)
 	int64_t D;       // layer length of the strip traversal (the model's band)
 	int w;           // columns (start vectors)
-	int Q;           // rows per row group, 64 / w
+	int Q;           // rows per row group: 64 / lanes per row
 	int S;           // strip rows
 	int A;           // layers, ceil(N / D)
 	int tps;         // tiles per strip segment
+	int wp;          // stored columns: w, or w rounded up to even for pairs
+	int pairs;       // 1: a lane moves two columns (16 B), else one
 	int pad;
 };
 
@@ -90,6 +94,40 @@ __device__ __forceinline__ float2 load_nt(const float2 *p) {
 __device__ __forceinline__ void store_nt(float2 *p, float x, float y) {
 	f32x2 q = {x, y};
 	__builtin_nontemporal_store(q, reinterpret_cast<f32x2 *>(p));
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// CPL consecutive complex columns of one row at byte address p (8 CPL B,
+// aligned to that): plain and non-temporal loads, non-temporal stores.
+template <int CPL>
+__device__ __forceinline__ void lane_load(const char *p, float2 (&o)[CPL]) {
+	if constexpr (CPL == 1) {
+		o[0] = *reinterpret_cast<const float2 *>(p);
+	} else {
+		const float4 v = *reinterpret_cast<const float4 *>(p);
+		o[0] = make_float2(v.x, v.y);
+		o[1] = make_float2(v.z, v.w);
+	}
+}
+template <int CPL>
+__device__ __forceinline__ void lane_load_nt(const char *p, float2 (&o)[CPL]) {
+	if constexpr (CPL == 1) {
+		o[0] = load_nt(reinterpret_cast<const float2 *>(p));
+	} else {
+		const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(p));
+		o[0] = make_float2(v.x, v.y);
+		o[1] = make_float2(v.z, v.w);
+	}
+}
+template <int CPL>
+__device__ __forceinline__ void lane_store_nt(char *p, const float2 (&v)[CPL]) {
+	if constexpr (CPL == 1) {
+		store_nt(reinterpret_cast<float2 *>(p), v[0].x, v[0].y);
+	} else {
+		f32x4 q = {v[0].x, v[0].y, v[1].x, v[1].y};
+		__builtin_nontemporal_store(q, reinterpret_cast<f32x4 *>(p));
+	}
 }
 
 inline unsigned grid_for(int64_t n) {
@@ -185,7 +223,7 @@ __device__ __forceinline__ void multi_rows(
     const MDesc &d, int &t, int tend, int nwx, int toff_m, int G, int Q, int qc, bool act,
     int b, int w, int64_t tr, const uint32_t *__restrict__ ecol,
     const float2 *__restrict__ ev, const float2 *__restrict__ X, float2 *__restrict__ Y,
-    int packed, double &d0x, double &d0y, double &d1x, double &d1y) {
+    int packed, double (&dd)[2][4]) {
 	// The model's block of X (wave-uniform base) and 32-bit byte offsets
 	// into it (N_m w 8 B < 4 GiB, checked on the host).
 	const char *Xm = reinterpret_cast<const char *>(X + d.vec0);
@@ -222,15 +260,18 @@ __device__ __forceinline__ void multi_rows(
 				if (!live[r]) continue;
 				store_nt(&Y[d.vec0 + row[r] * w + b], ax, ay);
 				// Slot 0 is the row itself (diagonal first).
-				if (DOTS) dot_acc(xv[r][0].x, xv[r][0].y, ax, ay, packed, d0x, d0y, d1x, d1y);
+				if (DOTS)
+					dot_acc(xv[r][0].x, xv[r][0].y, ax, ay, packed, dd[0][0], dd[0][1],
+					        dd[0][2], dd[0][3]);
 			}
 		}
 	}
 }
 
 // Shared-window path (Q W <= 64) with the ELL window of the next iteration
-// loaded one iteration ahead. An iteration is RPI = 4 row groups of one
-// tile; their entries come from NW = 4 / GPW windows (window k: the
+// loaded one iteration ahead. A lane moves CPL consecutive columns of its
+// row (CPL = 2: 16-byte loads and stores). An iteration is RPI row groups of
+// one tile; their entries come from NW = RPI / GPW windows (window k: the
 // GPW Q W consecutive entries of groups k GPW .. k GPW + GPW - 1, lane l
 // holding entry l). Per iteration, in issue order: the Phi_{n-1} rows and
 // the gathers (columns taken from the window registers), the stores of the
@@ -247,15 +288,16 @@ __device__ __forceinline__ void multi_rows(
 // (or an idle lane, q >= Q) takes that last row, which a live lane of the
 // same iteration computes from the same entries and the same Phi_{n-1}
 // value: it stores the same value to the same address.
-template <int W, int GPW, bool FIRST, bool DOTS, bool REALH>
+template <int W, int GPW, int RPI, int CPL, bool FIRST, bool DOTS, bool REALH>
 __device__ __forceinline__ void multi_rows_pipe(
     const MDesc &d, int &t, int tend, int nwx, int toff_m, int G, int Q, int qc, bool act,
-    int b, int w, int64_t tr, int lane, const uint32_t *__restrict__ ecol,
+    int b, int64_t tr, int lane, const uint32_t *__restrict__ ecol,
     const float2 *__restrict__ ev, const float2 *__restrict__ X, float2 *__restrict__ Y,
-    int packed, double &d0x, double &d0y, double &d1x, double &d1y) {
-	constexpr int RPI = 4, NW = RPI / GPW;
+    int packed, double (&dd)[2][4]) {
+	constexpr int NW = RPI / GPW;
+	const uint32_t wp = (uint32_t)d.wp;
 	// The model's blocks of X and Y (wave-uniform bases) and 32-bit byte
-	// offsets into them (N_m w 8 B < 4 GiB, checked on the host).
+	// offsets into them (N_m wp 8 B < 4 GiB, checked on the host).
 	const char *Xm = reinterpret_cast<const char *>(X + d.vec0);
 	char *Ym = reinterpret_cast<char *>(Y + d.vec0);
 	const int wrow_l = lane / W, wslot = lane - wrow_l * W;
@@ -282,7 +324,7 @@ __device__ __forceinline__ void multi_rows_pipe(
 		}
 	};
 	load_windows(row0, rows, 0);
-	float2 ps[RPI];      // the previous iteration's results, stored late
+	float2 ps[RPI][CPL];   // the previous iteration's results, stored late
 	uint32_t po[RPI];
 	// One iteration; STORE: the previous iteration's rows are stored (every
 	// iteration but the first). Returns whether another iteration follows.
@@ -290,14 +332,14 @@ __device__ __forceinline__ void multi_rows_pipe(
 		constexpr bool STORE = decltype(store_prev)::value;
 		uint32_t off[RPI];
 		bool live[RPI];
-		float2 yo[RPI], xv[RPI][W];
+		float2 yo[RPI][CPL], xv[RPI][W][CPL];
 #pragma unroll
 		for (int r = 0; r < RPI; r++) {
 			const int64_t o = (int64_t)(gg + r) * Q + qc;
 			live[r] = act && o < rows;
 			const int64_t row = row0 + (o < rows ? o : rows - 1);
-			off[r] = ((uint32_t)row * (uint32_t)w + (uint32_t)b) * 8u;
-			if (!FIRST) yo[r] = load_nt(reinterpret_cast<const float2 *>(Ym + off[r]));
+			off[r] = ((uint32_t)row * wp + (uint32_t)b) * 8u;
+			if (!FIRST) lane_load_nt<CPL>(Ym + off[r], yo[r]);
 		}
 #pragma unroll
 		for (int r = 0; r < RPI; r++) {
@@ -305,8 +347,7 @@ __device__ __forceinline__ void multi_rows_pipe(
 			for (int u = 0; u < W; u++) {
 				const uint32_t c =
 				    (uint32_t)__shfl((int)wc[r / GPW], ((r % GPW) * Q + qc) * W + u);
-				xv[r][u] = *reinterpret_cast<const float2 *>(
-				    Xm + (c * (uint32_t)w + (uint32_t)b) * 8u);
+				lane_load<CPL>(Xm + (c * wp + (uint32_t)b) * 8u, xv[r][u]);
 			}
 		}
 		// This iteration's values; the window registers take the next one.
@@ -321,8 +362,7 @@ __device__ __forceinline__ void multi_rows_pipe(
 		}
 		if constexpr (STORE) {
 #pragma unroll
-			for (int r = 0; r < RPI; r++)
-				store_nt(reinterpret_cast<float2 *>(Ym + po[r]), ps[r].x, ps[r].y);
+			for (int r = 0; r < RPI; r++) lane_store_nt<CPL>(Ym + po[r], ps[r]);
 		}
 		// Next iteration: the next RPI groups of this tile while they hold
 		// rows, else the wave's next tile with rows.
@@ -342,20 +382,30 @@ __device__ __forceinline__ void multi_rows_pipe(
 		else load_windows(row0, rows, gg);
 #pragma unroll
 		for (int r = 0; r < RPI; r++) {
-			float ax = FIRST ? 0.0f : -yo[r].x;
-			float ay = FIRST ? 0.0f : -yo[r].y;
+			float ax[CPL], ay[CPL];
+#pragma unroll
+			for (int j = 0; j < CPL; j++) {
+				ax[j] = FIRST ? 0.0f : -yo[r][j].x;
+				ay[j] = FIRST ? 0.0f : -yo[r][j].y;
+			}
 #pragma unroll
 			for (int u = 0; u < W; u++) {
 				const int src = ((r % GPW) * Q + qc) * W + u;
 				const float2 av = make_float2(__shfl(vc[r / GPW].x, src),
 				                              __shfl(vc[r / GPW].y, src));
-				mac<REALH>(av, xv[r][u], ax, ay);
+#pragma unroll
+				for (int j = 0; j < CPL; j++) mac<REALH>(av, xv[r][u][j], ax[j], ay[j]);
 			}
-			ps[r] = make_float2(ax, ay);
+#pragma unroll
+			for (int j = 0; j < CPL; j++) ps[r][j] = make_float2(ax[j], ay[j]);
 			po[r] = off[r];
 			// Slot 0 is the row itself (diagonal first).
-			if (DOTS && live[r])
-				dot_acc(xv[r][0].x, xv[r][0].y, ax, ay, packed, d0x, d0y, d1x, d1y);
+			if (DOTS && live[r]) {
+#pragma unroll
+				for (int j = 0; j < CPL; j++)
+					dot_acc(xv[r][0][j].x, xv[r][0][j].y, ax[j], ay[j], packed, dd[j][0],
+					        dd[j][1], dd[j][2], dd[j][3]);
+			}
 		}
 		t = tn;
 		gg = gn;
@@ -367,8 +417,7 @@ __device__ __forceinline__ void multi_rows_pipe(
 		while (iteration(std::true_type{})) {
 		}
 #pragma unroll
-	for (int r = 0; r < RPI; r++)
-		store_nt(reinterpret_cast<float2 *>(Ym + po[r]), ps[r].x, ps[r].y);
+	for (int r = 0; r < RPI; r++) lane_store_nt<CPL>(Ym + po[r], ps[r]);
 }
 
 // One recurrence step of every active model. Each model is traversed in
@@ -407,100 +456,62 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 	int m = lo;
 	for (;;) {
 		const MDesc d = md[m];
-		const int w = d.w, Q = d.Q;
-		const int q = lane / w;
-		const int b = lane - q * w;
+		// Lane l: row slot q = l / lpr of the group, columns b .. b + cpl - 1.
+		const int Q = d.Q, cpl = d.pairs ? 2 : 1;
+		const int lpr = d.wp / cpl;
+		const int q = lane / lpr;
+		const int j = lane - q * lpr;
+		const int b = j * cpl;
 		const bool act = q < Q;
 		const int qc = act ? q : Q - 1;
 		const int64_t tr = (int64_t)G * Q;   // rows per tile
 		const int tend = phi < toff[m + 1] ? phi : toff[m + 1];
-		double d0x = 0.0, d0y = 0.0, d1x = 0.0, d1y = 0.0;
-		if (Q == 1) {
-			// One row per group (w > 32): every lane needs the same ELL
-			// entries, so a tile's rows are read as one window and broadcast
-			// with readlane (wave-uniform columns, scalar addresses), as in
-			// the single-model step.
-			for (; t < tend; t += nwx) {
-				int64_t rows;
-				const int64_t row0 = tile_first_row(d, (uint32_t)(t - toff[m]), tr, rows);
-				if (rows <= 0) continue;
-				for (int gg = 0; gg < rows; gg += kRpi1) {
-					// Window: lane l holds entry l % W of row gg + l / W.
-					int64_t wrow = row0 + gg + lane / W;
-					if (wrow > row0 + rows - 1) wrow = row0 + rows - 1;
-					const int64_t we = (d.erow0 + wrow) * W + lane % W;
-					const uint32_t wc = ecol[we];
-					const float2 wv = ev[we];
-					int64_t row[kRpi1];
-					uint32_t c[kRpi1][W];
-					float2 a2[kRpi1][W], xv[kRpi1][W], yo[kRpi1];
-#pragma unroll
-					for (int r = 0; r < kRpi1; r++) {
-						const int i = gg + r < rows ? r : (int)(rows - 1 - gg);
-						row[r] = row0 + gg + i;
-#pragma unroll
-						for (int u = 0; u < W; u++) {
-							c[r][u] = (uint32_t)__builtin_amdgcn_readlane((int)wc, i * W + u);
-							a2[r][u] = make_float2(
-							    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv.x), i * W + u)),
-							    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv.y), i * W + u)));
-						}
-#pragma unroll
-						for (int u = 0; u < W; u++)
-							xv[r][u] = X[d.vec0 + (int64_t)c[r][u] * w + b];
-						if (!FIRST) yo[r] = load_nt(&Y[d.vec0 + row[r] * w + b]);
-					}
-#pragma unroll
-					for (int r = 0; r < kRpi1; r++) {
-						float ax = FIRST ? 0.0f : -yo[r].x;
-						float ay = FIRST ? 0.0f : -yo[r].y;
-#pragma unroll
-						for (int u = 0; u < W; u++) mac<REALH>(a2[r][u], xv[r][u], ax, ay);
-						if (!act || gg + r >= rows) continue;
-						store_nt(&Y[d.vec0 + row[r] * w + b], ax, ay);
-						if (DOTS) {
-							dot_acc(xv[r][0].x, xv[r][0].y, ax, ay, packed, d0x, d0y, d1x,
-							        d1y);
-						}
-					}
-				}
-			}
+		double dd[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+		// A row group's entries fit one window (Q W <= 64, guaranteed on
+		// the host for pair models): the pipelined path, with as many groups
+		// per window as fit; narrow models load their rows' entries per lane.
+		if (d.pairs) {
+			if (Q * W <= 32)
+				multi_rows_pipe<W, 2, 2, 2, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc,
+				                                               act, b, tr, lane, ecol, ev, X, Y,
+				                                               packed, dd);
+			else
+				multi_rows_pipe<W, 1, 2, 2, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc,
+				                                               act, b, tr, lane, ecol, ev, X, Y,
+				                                               packed, dd);
 		} else if (Q * W <= 16) {
-			// A row group's entries fit one window (Q W <= 64): the
-			// pipelined path, with as many groups per window as fit.
-			multi_rows_pipe<W, 4, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc, act,
-			                                         b, w, tr, lane, ecol, ev, X, Y, packed,
-			                                         d0x, d0y, d1x, d1y);
+			multi_rows_pipe<W, 4, 4, 1, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc,
+			                                               act, b, tr, lane, ecol, ev, X, Y,
+			                                               packed, dd);
 		} else if (Q * W <= 32) {
-			multi_rows_pipe<W, 2, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc, act,
-			                                         b, w, tr, lane, ecol, ev, X, Y, packed,
-			                                         d0x, d0y, d1x, d1y);
+			multi_rows_pipe<W, 2, 4, 1, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc,
+			                                               act, b, tr, lane, ecol, ev, X, Y,
+			                                               packed, dd);
 		} else if (Q * W <= 64) {
-			multi_rows_pipe<W, 1, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc, act,
-			                                         b, w, tr, lane, ecol, ev, X, Y, packed,
-			                                         d0x, d0y, d1x, d1y);
+			multi_rows_pipe<W, 1, 4, 1, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc,
+			                                               act, b, tr, lane, ecol, ev, X, Y,
+			                                               packed, dd);
 		} else {
-			// Narrow models: each lane loads its own row's entries.
 			multi_rows<W, kRpi, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc, act, b,
-			                                        w, tr, ecol, ev, X, Y, packed, d0x, d0y,
-			                                        d1x, d1y);
+			                                        d.wp, tr, ecol, ev, X, Y, packed, dd);
 		}
 		if (DOTS) {
-			// Column b's sum over the row slots q = 0 .. Q-1, ascending.
-			double s0x = 0.0, s0y = 0.0, s1x = 0.0, s1y = 0.0;
-			for (int k = 0; k < Q; k++) {
-				const int src = b + k * w;
-				s0x += __shfl(d0x, src);
-				s0y += __shfl(d0y, src);
-				s1x += __shfl(d1x, src);
-				s1y += __shfl(d1y, src);
-			}
+			// Column sums over the row slots q = 0 .. Q-1, ascending; lane
+			// j < lpr writes columns j cpl .. j cpl + cpl - 1.
 			const int pi = m * 8 + x;
 			const int s0w = (pstart[pi] - plo) % nwx;
 			const int slot = pbase[pi] + (wx - s0w + nwx) % nwx;
-			if (lane < w) {
-				part[((int64_t)slot * 2 + 0) * kMaxWidth + lane] = make_double2(s0x, s0y);
-				part[((int64_t)slot * 2 + 1) * kMaxWidth + lane] = make_double2(s1x, s1y);
+			for (int c = 0; c < cpl; c++) {
+				double s[4] = {0.0, 0.0, 0.0, 0.0};
+				for (int k = 0; k < Q; k++) {
+					const int src = j + k * lpr;
+#pragma unroll
+					for (int i = 0; i < 4; i++) s[i] += __shfl(c ? dd[1][i] : dd[0][i], src);
+				}
+				if (lane < lpr) {
+					part[((int64_t)slot * 2 + 0) * kMaxWidth + b + c] = make_double2(s[0], s[1]);
+					part[((int64_t)slot * 2 + 1) * kMaxWidth + b + c] = make_double2(s[2], s[3]);
+				}
 			}
 		}
 		if (t >= phi) break;
@@ -567,6 +578,18 @@ struct Plan {
 	int64_t vec_elems = 0, d_elems = 0;
 };
 
+// Two columns per lane (16-byte accesses, half the memory instructions per
+// byte) for models whose rows are wide enough that the padding column of an
+// odd width costs little (w even, or w >= 13) and whose row group of
+// Q = 64 / (lanes per row) rows still fits one ELL window (Q W <= 64).
+// Models wider than 32 columns always qualify.
+bool use_pairs(int w, int W) {
+	const int wp = w + (w & 1);
+	const int Q = kMaxWidth / (wp / 2);
+	if (w < 2 || Q * W > 64) return false;
+	return w > 32 || w % 2 == 0 || w >= 13;
+}
+
 // Tables of one phase: tile offsets, the byte-balanced cut of the tile list
 // into 8 XCD parts, the pieces (model x part) with the base of their dot
 // slots (model-major, so a model's slots are contiguous) and the slot groups
@@ -576,7 +599,7 @@ void plan_phase(const Plan &p, int W, int kact, int nwx, Phase &ph, std::vector<
 	std::vector<double> cost(kact), cum(kact + 1, 0.0);
 	for (int k = 0; k < kact; k++) {
 		toff[k + 1] = toff[k] + (int)p.tiles[k];
-		cost[k] = (double)p.desc[k].N * (12.0 * W + 24.0 * p.desc[k].w);
+		cost[k] = (double)p.desc[k].N * (12.0 * W + 24.0 * p.desc[k].wp);
 		cum[k + 1] = cum[k] + cost[k];
 	}
 	const int T = toff[kact];
@@ -679,14 +702,16 @@ int make_plan(tbtkx_context *h, const std::vector<int> &width, const std::vector
 		d.N = h->mm_N[m];
 		d.erow0 = h->mm_erow0[m];
 		d.w = width[m];
-		d.Q = kMaxWidth / d.w;
+		d.pairs = use_pairs(d.w, h->mm_W) ? 1 : 0;
+		d.wp = d.pairs ? d.w + (d.w & 1) : d.w;
+		d.Q = kMaxWidth / (d.pairs ? d.wp / 2 : d.wp);
 		d.vec0 = vec;
 		d.dof = dof;
 		// Strips: the rows whose Phi rows are reused one layer later span
 		// 2 S rows; about 512 KB of them per strip, at least one tile.
 		const int64_t tr = (int64_t)p.G * d.Q;
 		d.D = h->mm_band[m];
-		int64_t S = (int64_t)32768 / d.w;
+		int64_t S = (int64_t)32768 / d.wp;
 		if (S < tr) S = tr;
 		if (2 * S > d.D) S = d.D;
 		d.S = (int)S;
@@ -695,7 +720,7 @@ int make_plan(tbtkx_context *h, const std::vector<int> &width, const std::vector
 		d.pad = 0;
 		p.tiles[k] = ((d.D + S - 1) / S) * d.A * d.tps;
 		tiles_all += p.tiles[k];
-		vec += d.N * d.w;
+		vec += d.N * d.wp;
 		dof += (int64_t)steps[m] * 2 * d.w;
 		p.S[k] = steps[m];
 	}
@@ -1007,9 +1032,10 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 			TBTKX_REQUIRE(sites[ncols + b] >= 0 && sites[ncols + b] < h->mm_N[m],
 			              TBTKX_ERR_ARG, tag + "site outside the basis");
 		width[m] = packed ? (num_sites[m] + 1) / 2 : num_sites[m];
-		// The step addresses a model's vectors with 32-bit byte offsets.
-		TBTKX_REQUIRE(h->mm_N[m] * width[m] * 8 < ((int64_t)1 << 32), TBTKX_ERR_ARG,
-		              tag + "rows x columns x 8 B must stay below 4 GiB");
+		// The step addresses a model's vectors with 32-bit byte offsets
+		// (stored columns: at most one past the model's).
+		TBTKX_REQUIRE(h->mm_N[m] * (width[m] + 1) * 8 < ((int64_t)1 << 32), TBTKX_ERR_ARG,
+		              tag + "rows x (columns + 1) x 8 B must stay below 4 GiB");
 		S[m] = dots ? (num_moments[m] + 1) / 2 : num_moments[m] - 1;
 		out_off[m + 1] = out_off[m] + (int64_t)num_sites[m] * num_moments[m];
 		ncols += num_sites[m];
@@ -1054,6 +1080,7 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 			d.erow0 = h->mm_erow0[m];
 			d.vec0 = vec;
 			d.w = width[m];
+			d.wp = width[m];
 			d.Q = kMaxWidth / width[m];
 			vec += h->mm_N[m] * width[m];
 		}
@@ -1066,7 +1093,7 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 	for (int m = 0, c = 0; m < K; m++) {
 		const MDesc &d = p.desc[pos[m]];
 		for (int b = 0; b < num_sites[m]; b++, c++) {
-			cols[c].xoff = d.vec0 + sites[c] * d.w + (packed ? b >> 1 : b);
+			cols[c].xoff = d.vec0 + sites[c] * d.wp + (packed ? b >> 1 : b);
 			cols[c].out = out_off[m] + (int64_t)b * num_moments[m];
 			cols[c].M = num_moments[m];
 			cols[c].comp = packed ? 1 + (b & 1) : 0;
