@@ -232,7 +232,10 @@ def uneven_batch(eng):
     B = np.array([len(x) for x in sites], dtype=np.int64)
     M = np.array(Ms, dtype=np.int64)
     S = (M + 1) // 2
-    W = 6 if max(sp.max_row_entries() for sp in specs) > 4 else 4
+    # ELL slots per row of the batch (the longest row, diagonal included;
+    # kernels exist for 4, 5, 6 and 8 slots)
+    longest = max(sp.max_row_entries() for sp in specs)
+    W = 4 if longest <= 4 else (longest if longest <= 6 else 8)
     # 1: autotune + direct launches; 2: hipGraph capture; 3: graph replay (timed)
     t0 = time.perf_counter()
     mu = eng.multi_moments(sites, Ms, X.DOUBLING)

@@ -924,6 +924,7 @@ int launch_step(BlockRun &r, bool first, bool dots, const VT *X, VT *Y,
 	do {                                                                   \
 		if (h->ell_W == 4) TBTKX_ELL(F, D, R, 4);                      \
 		else if (h->ell_W == 6) TBTKX_ELL(F, D, R, 6);                 \
+		else if (h->ell_W == 7) TBTKX_ELL(F, D, R, 7);                 \
 		else TBTKX_ELL(F, D, R, 8);                                    \
 	} while (0)
 #define TBTKX_ELL_R(F, D)                                                      \
@@ -1006,15 +1007,18 @@ const void *ell_fn(int W, bool strip, int rmul) {
 	if (strip && rmul == 2 && kFp32Pair<VT, HT>) {
 		if (W == 4) return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 4, 2>;
 		if (W == 6) return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 6, 2>;
+		if (W == 7) return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 7, 2>;
 		return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 8, 2>;
 	}
 	if (strip) {
 		if (W == 4) return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 4, 1>;
 		if (W == 6) return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 6, 1>;
+		if (W == 7) return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 7, 1>;
 		return (const void *)spmm_step_strip<VT, HT, AT, B, false, D, R, 8, 1>;
 	}
 	if (W == 4) return (const void *)spmm_step_ell<VT, HT, AT, B, false, D, R, 4>;
 	if (W == 6) return (const void *)spmm_step_ell<VT, HT, AT, B, false, D, R, 6>;
+	if (W == 7) return (const void *)spmm_step_ell<VT, HT, AT, B, false, D, R, 7>;
 	return (const void *)spmm_step_ell<VT, HT, AT, B, false, D, R, 8>;
 }
 

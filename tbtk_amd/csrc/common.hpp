@@ -107,7 +107,8 @@ struct tbtkx_context {
 	// on-site amplitudes, row counts, per-workgroup extents, scan storage.
 	tbtkx::DevBuf lat_terms, lat_onsite, lat_count, lat_part, lat_tmp;
 	// Tile-major ELL copy of H for the batched step (ell_W = slots per row,
-	// 0 when a row is longer than 8): tiles of 64/ell_W rows.
+	// ell_slots_block; 0 when a row is longer than 8): tiles of 64/ell_W
+	// rows.
 	tbtkx::DevBuf ell_col, ell_v1f, ell_v2f, ell_v1d, ell_v2d;
 	int ell_W = 0;
 	int64_t band = 0;      // max |col - row| over the non-zeros (N if diagonal)
@@ -211,5 +212,15 @@ inline int ell_row_slots(const uint32_t *rowptr, const uint32_t *col, int64_t r,
 void scale_complex(const double *in, double *out, int64_t Z, double d);
 // Row blocks of the LDS-staged plain kernel from a host row pointer.
 std::vector<uint32_t> make_row_blocks(int64_t N, const uint32_t *rowptr);
+
+// ELL slots per row for a longest row of maxlen entries (diagonal
+// included): the single-model batched step has kernels for 4, 6, 7 and 8
+// slots, the uneven batch for 4, 5, 6 and 8; 0: longer than 8.
+inline int ell_slots_block(int64_t maxlen) {
+	return maxlen <= 4 ? 4 : maxlen <= 6 ? 6 : maxlen <= 8 ? (int)maxlen : 0;
+}
+inline int ell_slots_multi(int64_t maxlen) {
+	return maxlen <= 4 ? 4 : maxlen <= 6 ? (int)maxlen : maxlen <= 8 ? 8 : 0;
+}
 
 }  // namespace tbtkx

@@ -287,7 +287,7 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 		if (len > maxlen) maxlen = len;
 	}
 	h->band = band > 0 ? band : N;
-	h->ell_W = maxlen <= 4 ? 4 : maxlen <= 6 ? 6 : maxlen <= 8 ? 8 : 0;
+	h->ell_W = ell_slots_block(maxlen);
 	if (h->ell_W) {
 		const int W = h->ell_W, T = 64 / W;
 		const int64_t ntiles = (N + T - 1) / T, E = ntiles * T * W;

@@ -390,7 +390,7 @@ int tbtkx_set_lattice(tbtkx_handle h, const tbtkx_lattice *lat, const double *on
 	if ((rc = h->rblk.reserve(blocks.size() * 4))) return rc;
 	if ((rc = h->vec_a.reserve(N * 16))) return rc;
 	if ((rc = h->vec_b.reserve(N * 16))) return rc;
-	const int W = maxlen <= 4 ? 4 : maxlen <= 6 ? 6 : maxlen <= 8 ? 8 : 0;
+	const int W = ell_slots_block(maxlen);
 	int64_t ell_rows = 0;
 	if (W) {
 		const int T = 64 / W;
@@ -511,7 +511,7 @@ int tbtkx_multi_set_lattices(tbtkx_handle h, int K, const tbtkx_lattice *lats,
 	TBTKX_REQUIRE(maxlen <= 8, TBTKX_ERR_ARG,
 	              "tbtkx_multi_set_lattices: rows with more than 8 non-zeros (7 without a "
 	              "diagonal entry) are not supported");
-	const int W = maxlen <= 4 ? 4 : maxlen <= 6 ? 6 : 8;
+	const int W = ell_slots_multi(maxlen);
 	h->mm_graph.reset();
 	h->mm_K = 0;
 	if ((rc = h->mm_ecol.reserve((size_t)rows * W * 4))) return rc;

@@ -668,6 +668,7 @@ int grid_size(tbtkx_context *h, bool dots, int share, int *nwg) {
 	fn = dots ? (R ? step_fn<WW, true, true>() : step_fn<WW, true, false>()) \
 	          : (R ? step_fn<WW, false, true>() : step_fn<WW, false, false>())
 	if (h->mm_W == 4) TBTKX_FN(4);
+	else if (h->mm_W == 5) TBTKX_FN(5);
 	else if (h->mm_W == 6) TBTKX_FN(6);
 	else TBTKX_FN(8);
 #undef TBTKX_FN
@@ -771,16 +772,16 @@ void launch_step_w(tbtkx_context *h, const Plan &p, const Phase &ph, bool first,
 void launch_step(tbtkx_context *h, const Plan &p, const Phase &ph, bool first,
                  bool dots, const float2 *X, float2 *Y) {
 	const bool R = h->mm_real;
-	if (h->mm_W == 4) {
-		if (R) launch_step_w<4, true>(h, p, ph, first, dots, X, Y);
-		else launch_step_w<4, false>(h, p, ph, first, dots, X, Y);
-	} else if (h->mm_W == 6) {
-		if (R) launch_step_w<6, true>(h, p, ph, first, dots, X, Y);
-		else launch_step_w<6, false>(h, p, ph, first, dots, X, Y);
-	} else {
-		if (R) launch_step_w<8, true>(h, p, ph, first, dots, X, Y);
-		else launch_step_w<8, false>(h, p, ph, first, dots, X, Y);
-	}
+#define TBTKX_LS(WW)                                                  \
+	do {                                                          \
+		if (R) launch_step_w<WW, true>(h, p, ph, first, dots, X, Y);  \
+		else launch_step_w<WW, false>(h, p, ph, first, dots, X, Y);   \
+	} while (0)
+	if (h->mm_W == 4) TBTKX_LS(4);
+	else if (h->mm_W == 5) TBTKX_LS(5);
+	else if (h->mm_W == 6) TBTKX_LS(6);
+	else TBTKX_LS(8);
+#undef TBTKX_LS
 }
 
 int reserve_plan(tbtkx_context *h, const Plan &p, int ncols, int64_t mu_elems) {
@@ -940,7 +941,7 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 	int rc = set_device(h);
 	if (rc) return rc;
 	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
-	const int W = maxlen <= 4 ? 4 : maxlen <= 6 ? 6 : 8;
+	const int W = ell_slots_multi(maxlen);
 	std::vector<uint32_t> ec((size_t)rows * W);
 	std::vector<float> e1((size_t)rows * W * 2, 0.0f), e2((size_t)rows * W * 2, 0.0f);
 	h->mm_N.assign(K, 0);
