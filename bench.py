@@ -28,6 +28,7 @@ error against the FP64 path.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
                        [--parts plain,batched,uneven,dos,mfma,cpu]
                        [--skip-cpu-baseline] [--calibrate-copy] [--complex]
+                       [--no-graph]
 """
 from __future__ import annotations
 
@@ -213,7 +214,7 @@ def batched_comparison(eng):
     }
 
 
-def uneven_batch(eng):
+def uneven_batch(eng, graph=True):
     """64 independent models of uneven size (honeycomb ribbons and square
     lattices, 20k-1M states, 1-64 sites and 500-8000 moments each, seed 4) in
     one persistent launch per recurrence step: FP32 storage, FP64 moments,
@@ -245,9 +246,12 @@ def uneven_batch(eng):
     packed = eng.get_option(X.OPT_REAL) == 1
     cols = (B + 1) // 2 if packed else B
     step_bytes_total = float(((12 * W * N + 24 * cols * N) * S).sum())
-    eng.set_option(X.OPT_GRAPH, 1)
+    # graph=False (--no-graph): a second direct run is timed instead of the
+    # graph replay (for profilers that do not follow graph launches)
+    eng.set_option(X.OPT_GRAPH, 1 if graph else 0)
     try:
-        eng.multi_moments(sites, Ms, X.DOUBLING)
+        if graph:
+            eng.multi_moments(sites, Ms, X.DOUBLING)
         t0 = time.perf_counter()
         mu_g = eng.multi_moments(sites, Ms, X.DOUBLING)
         t_graph = time.perf_counter() - t0
@@ -279,6 +283,7 @@ def uneven_batch(eng):
         "recurrence_steps": int(S.max()),
         "dtype": ("fp32 storage (H and vectors), fp64 moments, doubling"
                   + (", real H: two real vectors per complex column" if packed else "")),
+        "timed_run": "hipGraph replay" if graph else "direct launches (--no-graph)",
         "sites_per_s": float(B.sum()) / (t_graph),
         "device_s": dev_s,
         "direct_device_s": ms_direct * 1e-3,
@@ -441,6 +446,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 1 GiB read + 1 GiB write elementwise kernel (torch) "
                          "whose known bytes scale the rocprofv3 memory counters")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="uneven part: time a direct run instead of the hipGraph replay")
     ap.add_argument("--complex", action="store_true",
                     help="run every recurrence in complex arithmetic, also for a real H "
                          "(TBTKX_OPT_REAL = 0), to measure the complex kernels")
@@ -549,7 +556,7 @@ def main():
         out["batched"] = batched_comparison(eng)
     if "uneven" in parts:
         log("uneven batch (64 models, one persistent launch per step)")
-        out["uneven_batch"] = uneven_batch(eng)
+        out["uneven_batch"] = uneven_batch(eng, graph=not args.no_graph)
     if "dos" in parts:
         log("stochastic DOS (128^3 cubic, R=512, M=4096)")
         out["stochastic_dos"] = stochastic_dos(eng)

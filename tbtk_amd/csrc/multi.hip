@@ -529,11 +529,20 @@ __global__ __launch_bounds__(kThreads) void multi_reduce_slots(
 	if (gw >= 2 * nseg) return;
 	const int s = gw >> 1, which = gw & 1;
 	const int lo = seg[3 * s + 1], hi = seg[3 * s + 2];
+	// Slots in ascending order; eight loads are issued before their adds.
 	double2 acc = make_double2(0.0, 0.0);
-	for (int k = lo; k < hi; k++) {
-		const double2 v = part[((int64_t)k * 2 + which) * kMaxWidth + lane];
-		acc.x += v.x;
-		acc.y += v.y;
+	for (int k0 = lo; k0 < hi; k0 += 8) {
+		double2 v[8];
+#pragma unroll
+		for (int i = 0; i < 8; i++)
+			if (k0 + i < hi) v[i] = part[((int64_t)(k0 + i) * 2 + which) * kMaxWidth + lane];
+#pragma unroll
+		for (int i = 0; i < 8; i++) {
+			if (k0 + i < hi) {
+				acc.x += v[i].x;
+				acc.y += v[i].y;
+			}
+		}
 	}
 	segsum[((int64_t)s * 2 + which) * kMaxWidth + lane] = acc;
 }
@@ -721,7 +730,9 @@ int make_plan(tbtkx_context *h, const std::vector<int> &width, const std::vector
 		d.pad = 0;
 		p.tiles[k] = ((d.D + S - 1) / S) * d.A * d.tps;
 		tiles_all += p.tiles[k];
+		// Every model's block starts on 16 bytes (two-column lanes).
 		vec += d.N * d.wp;
+		vec += vec & 1;
 		dof += (int64_t)steps[m] * 2 * d.w;
 		p.S[k] = steps[m];
 	}

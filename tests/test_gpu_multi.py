@@ -155,3 +155,19 @@ def test_multi_rejects_bad_arguments(engine):
         engine.multi_set([dense], [20.0])
     with pytest.raises(X.EngineError, match="scale factor"):
         engine.multi_set([batch[0][0].csr()], [0.0])
+
+
+def test_multi_rejects_vectors_past_4gib(engine):
+    """The step addresses a model's vectors with 32-bit byte offsets, so a
+    model whose stored vectors (rows x (columns + 1) x 8 B) reach 4 GiB is
+    refused before anything runs: 8,410,000 states with a complex H (no
+    packing) and 64 sites. The same model with 32 sites runs."""
+    spec = L.square_spec(2900, 2900)
+    spec.terms = [(d, to, frm, re, 0.25 if d[1] == 1 else (-0.25 if d[1] == -1 else im))
+                  for d, to, frm, re, im in spec.terms]
+    engine.multi_set_lattices([spec], [6.0])
+    sites = np.arange(64, dtype=np.int64) * 131_000
+    with pytest.raises(X.EngineError, match="4 GiB"):
+        engine.multi_moments([sites], [4], X.DOUBLING)
+    mu = engine.multi_moments([sites[:32]], [4], X.DOUBLING)
+    assert mu[0].shape == (32, 4) and np.all(mu[0][:, 0] == 1.0)
