@@ -233,7 +233,10 @@ int tbtkx_multi_set_lattices(tbtkx_handle handle, int num_models,
  * FP64 moments; flags may be TBTKX_DOUBLING. The launch shape (tile size,
  * grid share) is autotuned on first use unless TBTKX_OPT_AUTOTUNE is 0 or
  * TBTKX_OPT_TILE_ROWS fixes it; TBTKX_OPT_GRAPH replays the whole run as a
- * hipGraph. */
+ * hipGraph. A model's vectors are addressed with 32-bit byte offsets: a
+ * model with num_rows x (columns + 1) x 8 B >= 4 GiB is refused with
+ * TBTKX_ERR_ARG (columns: num_sites, or ceil(num_sites / 2) when every
+ * model's H is real). */
 int tbtkx_multi_moments(tbtkx_handle handle, const int *num_sites,
                         const int64_t *sites, const int *num_moments, int flags,
                         double *out);
