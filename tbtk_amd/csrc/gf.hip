@@ -35,36 +35,52 @@ __global__ __launch_bounds__(kThreads) void lut_generate(
 	                                     (ar * sn + ai * cs) / denom);
 }
 
+// One term of the contraction, in the reference's operation order.
+template <int TYPE>
+__device__ __forceinline__ void contract_term(double2 l, double2 cn, double &gr, double &gi) {
+#pragma clang fp contract(off)
+	if (TYPE == TBTKX_RETARDED) {
+		gr = gr + (l.x * cn.x - l.y * cn.y);
+		gi = gi + (l.x * cn.y + l.y * cn.x);
+	} else if (TYPE == TBTKX_ADVANCED) {
+		gr = gr + (cn.x * l.x - cn.y * (-l.y));
+		gi = gi + (cn.x * (-l.y) + cn.y * l.x);
+	} else if (TYPE == TBTKX_PRINCIPAL) {
+		gr = gr + (-cn.x) * l.x;
+		gi = gi + (-cn.y) * l.x;
+	} else {
+		// (c * i) = (cr*0 - ci*1, cr*1 + ci*0), then times Im(L)
+		const double pr = cn.x * 0.0 - cn.y * 1.0;
+		const double pi = cn.x * 1.0 + cn.y * 0.0;
+		gr = gr - pr * l.y;
+		gi = gi - pi * l.y;
+	}
+}
+
+// A thread per (energy, coefficient vector). A few vectors of few energies
+// leave little parallelism, so each thread issues kAhead table loads before
+// it adds them, in order.
+constexpr int kAhead = 16;
+
 template <int TYPE>
 __global__ __launch_bounds__(kThreads) void contract_fp64(
     const double2 *__restrict__ L, const double2 *__restrict__ C,
     double2 *__restrict__ G, int M, int E) {
-#pragma clang fp contract(off)
 	const int e = blockIdx.x * kThreads + threadIdx.x;
 	const int k = blockIdx.y;
 	if (e >= E) return;
 	const double2 *c = C + (int64_t)k * M;
+	const double2 *l = L + e;
 	double gr = 0.0, gi = 0.0;
-	for (int n = 0; n < M; n++) {
-		const double2 l = L[(int64_t)n * E + e];
-		const double2 cn = c[n];
-		if (TYPE == TBTKX_RETARDED) {
-			gr = gr + (l.x * cn.x - l.y * cn.y);
-			gi = gi + (l.x * cn.y + l.y * cn.x);
-		} else if (TYPE == TBTKX_ADVANCED) {
-			gr = gr + (cn.x * l.x - cn.y * (-l.y));
-			gi = gi + (cn.x * (-l.y) + cn.y * l.x);
-		} else if (TYPE == TBTKX_PRINCIPAL) {
-			gr = gr + (-cn.x) * l.x;
-			gi = gi + (-cn.y) * l.x;
-		} else {
-			// (c * i) = (cr*0 - ci*1, cr*1 + ci*0), then times Im(L)
-			const double pr = cn.x * 0.0 - cn.y * 1.0;
-			const double pi = cn.x * 1.0 + cn.y * 0.0;
-			gr = gr - pr * l.y;
-			gi = gi - pi * l.y;
-		}
+	int n = 0;
+	for (; n + kAhead <= M; n += kAhead) {
+		double2 lv[kAhead];
+#pragma unroll
+		for (int i = 0; i < kAhead; i++) lv[i] = l[(int64_t)(n + i) * E];
+#pragma unroll
+		for (int i = 0; i < kAhead; i++) contract_term<TYPE>(lv[i], c[n + i], gr, gi);
 	}
+	for (; n < M; n++) contract_term<TYPE>(l[(int64_t)n * E], c[n], gr, gi);
 	G[(int64_t)k * E + e] = make_double2(gr, gi);
 }
 
