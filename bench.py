@@ -233,10 +233,10 @@ def uneven_batch(eng, graph=True):
     B = np.array([len(x) for x in sites], dtype=np.int64)
     M = np.array(Ms, dtype=np.int64)
     S = (M + 1) // 2
-    # ELL slots per row of the batch (the longest row, diagonal included;
-    # kernels exist for 4, 5, 6 and 8 slots)
-    longest = max(sp.max_row_entries() for sp in specs)
-    W = 4 if longest <= 4 else (longest if longest <= 6 else 8)
+    # ELL slots per row of each model (its longest row, diagonal included,
+    # rounded up to a kernel width: 4, 5, 6 or 8)
+    W = np.array([min(w for w in (4, 5, 6, 8) if w >= sp.max_row_entries()) for sp in specs],
+                 dtype=np.int64)
     # 1: autotune + direct launches; 2: hipGraph capture; 3: graph replay (timed)
     t0 = time.perf_counter()
     mu = eng.multi_moments(sites, Ms, X.DOUBLING)
@@ -279,7 +279,7 @@ def uneven_batch(eng, graph=True):
                      "1-64 LDOS sites and 500-8000 moments per model, Gershgorin*1.05 scales"),
         "models": len(batch), "states": int(N.sum()), "sites": int(B.sum()),
 
-        "ell_slots": W,
+        "ell_slots": {str(w): int((W == w).sum()) for w in (4, 5, 6, 8) if (W == w).any()},
         "recurrence_steps": int(S.max()),
         "dtype": ("fp32 storage (H and vectors), fp64 moments, doubling"
                   + (", real H: two real vectors per complex column" if packed else "")),

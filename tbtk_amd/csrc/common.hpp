@@ -139,13 +139,15 @@ struct tbtkx_context {
 	tbtkx::GraphCache blk_graph;
 
 	// Uneven multi-model batch (tbtkx_multi_*): every model's H as row-major
-	// ELL with mm_W slots per row (FP32, scaled by 1/s_m and 2/s_m), rows of
+	// ELL with mm_Wm[m] slots per row (mm_W: the widest; FP32, scaled by
+	// 1/s_m and 2/s_m), rows of
 	// all models concatenated in upload order.
 	int mm_K = 0, mm_W = 0;
 	bool mm_real = false;
 	int mm_packed = 0;                        // current call packs real vectors
 	int64_t mm_gen = 0;                       // bumped by every tbtkx_multi_set
-	std::vector<int64_t> mm_N, mm_erow0;      // rows, first ELL row per model
+	std::vector<int64_t> mm_N, mm_e0;         // rows, first ELL entry per model
+	std::vector<int> mm_Wm;                   // ELL slots per row, per model
 	std::vector<int64_t> mm_band;             // max |col - row| per model
 	tbtkx::DevBuf mm_ecol, mm_ev1, mm_ev2;
 	tbtkx::DevBuf mm_desc, mm_tab, mm_x, mm_y, mm_part, mm_seg, mm_D, mm_cols;

@@ -499,14 +499,17 @@ int tbtkx_multi_set_lattices(tbtkx_handle h, int K, const tbtkx_lattice *lats,
 	int rc = set_device(h);
 	if (rc) return rc;
 	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
-	// Pass 1: row lengths and bands, for the common slot count W.
+	// Pass 1: row lengths and bands, for each model's slot count.
 	std::vector<int64_t> band(K);
-	int64_t maxlen = 0;
+	std::vector<int> Wm(K);
+	int64_t maxlen = 0, entries = 0;
 	for (int m = 0; m < K; m++) {
 		if ((rc = stage_lattice(h, &lats[m], onsite ? onsite[m] : nullptr, N[m]))) return rc;
 		int64_t ml = 0;
 		if ((rc = lattice_extent(h, g[m], N[m], false, &ml, &band[m]))) return rc;
 		maxlen = std::max(maxlen, ml);
+		Wm[m] = ell_slots_multi(ml);
+		entries += N[m] * Wm[m];
 	}
 	TBTKX_REQUIRE(maxlen <= 8, TBTKX_ERR_ARG,
 	              "tbtkx_multi_set_lattices: rows with more than 8 non-zeros (7 without a "
@@ -514,28 +517,29 @@ int tbtkx_multi_set_lattices(tbtkx_handle h, int K, const tbtkx_lattice *lats,
 	const int W = ell_slots_multi(maxlen);
 	h->mm_graph.reset();
 	h->mm_K = 0;
-	if ((rc = h->mm_ecol.reserve((size_t)rows * W * 4))) return rc;
-	if ((rc = h->mm_ev1.reserve((size_t)rows * W * 8))) return rc;
-	if ((rc = h->mm_ev2.reserve((size_t)rows * W * 8))) return rc;
+	if ((rc = h->mm_ecol.reserve((size_t)entries * 4))) return rc;
+	if ((rc = h->mm_ev1.reserve((size_t)entries * 8))) return rc;
+	if ((rc = h->mm_ev2.reserve((size_t)entries * 8))) return rc;
 	h->mm_N.assign(K, 0);
-	h->mm_erow0.assign(K, 0);
+	h->mm_e0.assign(K, 0);
+	h->mm_Wm = Wm;
 	h->mm_band.assign(K, 0);
 	// Pass 2: each model's ELL rows.
-	int64_t erow = 0;
+	int64_t e0 = 0;
 	for (int m = 0; m < K; m++) {
 		if ((rc = stage_lattice(h, &lats[m], onsite ? onsite[m] : nullptr, N[m]))) return rc;
 		lattice_fill_multi<<<grid_for(N[m]), kThreads, 0, h->stream>>>(
 		    g[m], h->lat_terms.as<tbtkx_lattice_term>(),
-		    g[m].has_onsite ? h->lat_onsite.as<double2>() : nullptr, N[m], 1 / scales[m], W,
-		    h->mm_ecol.as<uint32_t>() + erow * W, h->mm_ev1.as<float2>() + erow * W,
-		    h->mm_ev2.as<float2>() + erow * W);
+		    g[m].has_onsite ? h->lat_onsite.as<double2>() : nullptr, N[m], 1 / scales[m], Wm[m],
+		    h->mm_ecol.as<uint32_t>() + e0, h->mm_ev1.as<float2>() + e0,
+		    h->mm_ev2.as<float2>() + e0);
 		TBTKX_CHECK_HIP(hipGetLastError());
 		// The staging buffers are reused by the next model.
 		TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
 		h->mm_band[m] = band[m] > 0 ? band[m] : N[m];
 		h->mm_N[m] = N[m];
-		h->mm_erow0[m] = erow;
-		erow += N[m];
+		h->mm_e0[m] = e0;
+		e0 += N[m] * Wm[m];
 	}
 	h->mm_K = K;
 	h->mm_W = W;

@@ -8,7 +8,8 @@
 // that order and the launch covers a prefix of the tile list; the launch
 // shape changes only when a model finishes (at most K "phases").
 //
-// Storage: FP32 complex H (row-major ELL, W slots per row, scaled by 1/s_m
+// Storage: FP32 complex H (row-major ELL, W_m slots per row: the model's
+// longest row rounded up to 4, 5, 6 or 8; scaled by 1/s_m
 // and 2/s_m exactly as the single-model path scales its FP32 copies) and FP32
 // complex vectors, row-major per model (N_m rows x B_m columns), rows summed
 // in FP32 with the same fused multiply-add order as the single-model step, so
@@ -43,7 +44,7 @@
 // deterministic for a given launch shape.
 //
 // Algorithmic HBM bytes per step: sum over active models of
-// 12 W N_m (ELL column + value) + 24 w N_m (read Phi_n, read Phi_{n-1},
+// 12 W_m N_m (ELL column + value) + 24 w N_m (read Phi_n, read Phi_{n-1},
 // write Phi_{n+1}); a padding column adds 24 N_m that are not counted.
 #include <algorithm>
 #include <cmath>
@@ -63,7 +64,7 @@ constexpr int kMaxWidth = 64;    // start vectors per model
 
 struct MDesc {
 	int64_t N;       // rows
-	int64_t erow0;   // first row of the model in the ELL arrays
+	int64_t e0;      // first ELL entry of the model (its rows have W slots)
 	int64_t vec0;    // first element of the model's vectors in X / Y
 	int64_t dof;     // first element of the model's D block (doubling)
 	int64_t D;       // layer length of the strip traversal (the model's band)
@@ -74,7 +75,7 @@ struct MDesc {
 	int tps;         // tiles per strip segment
 	int wp;          // stored columns: w, or w rounded up to even for pairs
 	int pairs;       // 1: a lane moves two columns (16 B), else one
-	int pad;
+	int W;           // ELL slots per row of this model (4, 5, 6 or 8)
 };
 
 struct GCol {
@@ -243,7 +244,7 @@ __device__ __forceinline__ void multi_rows(
 			}
 #pragma unroll
 			for (int r = 0; r < RPI; r++) {
-				const int64_t e = (d.erow0 + row[r]) * W;
+				const int64_t e = d.e0 + row[r] * W;
 #pragma unroll
 				for (int u = 0; u < W; u++) {
 					a2[r][u] = ev[e + u];
@@ -318,7 +319,7 @@ __device__ __forceinline__ void multi_rows_pipe(
 		for (int k = 0; k < NW; k++) {
 			int64_t wrow = r0 + (int64_t)(g0 + k * GPW) * Q + wrow_l;
 			if (wrow > glast) wrow = glast;
-			const int64_t we = (d.erow0 + wrow) * W + wslot;
+			const int64_t we = d.e0 + wrow * W + wslot;
 			wc[k] = ecol[we];
 			wv[k] = ev[we];
 		}
@@ -420,6 +421,64 @@ __device__ __forceinline__ void multi_rows_pipe(
 	for (int r = 0; r < RPI; r++) lane_store_nt<CPL>(Ym + po[r], ps[r]);
 }
 
+// One model's tiles t, t + nwx, ... below tend with its ELL width W: the
+// path for its lane mode and row-group size.
+template <int W, bool FIRST, bool DOTS, bool REALH>
+__device__ __forceinline__ void model_rows_w(
+    const MDesc &d, int &t, int tend, int nwx, int toff_m, int G, int Q, int qc, bool act,
+    int b, int64_t tr, int lane, const uint32_t *__restrict__ ecol,
+    const float2 *__restrict__ ev, const float2 *__restrict__ X, float2 *__restrict__ Y,
+    int packed, double (&dd)[2][4]) {
+	// A row group's entries fit one window (Q W <= 64, guaranteed on the
+	// host for pair models): the pipelined path, with as many groups per
+	// window as fit; narrow models load their rows' entries per lane.
+#define TBTKX_PIPE(GPW, RPI, CPL)                                                        \
+	multi_rows_pipe<W, GPW, RPI, CPL, FIRST, DOTS, REALH>(d, t, tend, nwx, toff_m, G, Q, qc, \
+	                                                      act, b, tr, lane, ecol, ev, X, Y,  \
+	                                                      packed, dd)
+	if (d.pairs) {
+		if (Q * W <= 32) TBTKX_PIPE(2, 2, 2);
+		else TBTKX_PIPE(1, 2, 2);
+	} else if (Q * W <= 16) {
+		TBTKX_PIPE(4, 4, 1);
+	} else if (Q * W <= 32) {
+		TBTKX_PIPE(2, 4, 1);
+	} else if (Q * W <= 64) {
+		TBTKX_PIPE(1, 4, 1);
+	} else {
+		multi_rows<W, kRpi, FIRST, DOTS, REALH>(d, t, tend, nwx, toff_m, G, Q, qc, act, b, d.wp,
+		                                        tr, ecol, ev, X, Y, packed, dd);
+	}
+#undef TBTKX_PIPE
+}
+
+// Dispatch on the model's ELL width (4, 5, 6 or 8, at most the batch's
+// widest, WMAX, which fixes the kernel's registers).
+template <int WMAX, bool FIRST, bool DOTS, bool REALH>
+__device__ __forceinline__ void model_rows(
+    const MDesc &d, int &t, int tend, int nwx, int toff_m, int G, int Q, int qc, bool act,
+    int b, int64_t tr, int lane, const uint32_t *__restrict__ ecol,
+    const float2 *__restrict__ ev, const float2 *__restrict__ X, float2 *__restrict__ Y,
+    int packed, double (&dd)[2][4]) {
+#define TBTKX_MR(WW)                                                                 \
+	model_rows_w<WW, FIRST, DOTS, REALH>(d, t, tend, nwx, toff_m, G, Q, qc, act, b, tr, lane, \
+	                                     ecol, ev, X, Y, packed, dd)
+	if (d.W == WMAX) {
+		TBTKX_MR(WMAX);
+	} else if constexpr (WMAX > 4) {
+		if (d.W == 4) {
+			TBTKX_MR(4);
+		} else if constexpr (WMAX > 5) {
+			if (d.W == 5) {
+				TBTKX_MR(5);
+			} else if constexpr (WMAX > 6) {
+				TBTKX_MR(6);
+			}
+		}
+	}
+#undef TBTKX_MR
+}
+
 // One recurrence step of every active model. Each model is traversed in
 // L2 strips (layers of D rows, strips of S rows; a tile is G row groups of
 // Q rows inside one strip segment), the models' tile lists are concatenated
@@ -430,8 +489,12 @@ __device__ __forceinline__ void multi_rows_pipe(
 // it leaves a model, into slot pbase[k*8+x] + its rank among the
 // wavefronts that touch the model's piece of part x (pstart: the piece's
 // first tile).
-template <int W, bool FIRST, bool DOTS, bool REALH>
-__global__ __launch_bounds__(kThreads) void multi_step(
+// Batches of at most 5 ELL slots per row are held to 4 wavefronts per SIMD
+// (128 VGPRs): the doubling variants then keep 4-6 values in scratch and
+// still finish the 64-model batch 1.3% sooner than at 3 wavefronts.
+template <int WMAX, bool FIRST, bool DOTS, bool REALH>
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(WMAX <= 5 ? 4 : 1))) void multi_step(
     const MDesc *__restrict__ md, const int *__restrict__ toff, int kact,
     const int *__restrict__ xpart, const int *__restrict__ pbase,
     const int *__restrict__ pstart, int G, const uint32_t *__restrict__ ecol,
@@ -467,34 +530,8 @@ __global__ __launch_bounds__(kThreads) void multi_step(
 		const int64_t tr = (int64_t)G * Q;   // rows per tile
 		const int tend = phi < toff[m + 1] ? phi : toff[m + 1];
 		double dd[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-		// A row group's entries fit one window (Q W <= 64, guaranteed on
-		// the host for pair models): the pipelined path, with as many groups
-		// per window as fit; narrow models load their rows' entries per lane.
-		if (d.pairs) {
-			if (Q * W <= 32)
-				multi_rows_pipe<W, 2, 2, 2, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc,
-				                                               act, b, tr, lane, ecol, ev, X, Y,
-				                                               packed, dd);
-			else
-				multi_rows_pipe<W, 1, 2, 2, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc,
-				                                               act, b, tr, lane, ecol, ev, X, Y,
-				                                               packed, dd);
-		} else if (Q * W <= 16) {
-			multi_rows_pipe<W, 4, 4, 1, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc,
-			                                               act, b, tr, lane, ecol, ev, X, Y,
-			                                               packed, dd);
-		} else if (Q * W <= 32) {
-			multi_rows_pipe<W, 2, 4, 1, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc,
-			                                               act, b, tr, lane, ecol, ev, X, Y,
-			                                               packed, dd);
-		} else if (Q * W <= 64) {
-			multi_rows_pipe<W, 1, 4, 1, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc,
-			                                               act, b, tr, lane, ecol, ev, X, Y,
-			                                               packed, dd);
-		} else {
-			multi_rows<W, kRpi, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc, act, b,
-			                                        d.wp, tr, ecol, ev, X, Y, packed, dd);
-		}
+		model_rows<WMAX, FIRST, DOTS, REALH>(d, t, tend, nwx, toff[m], G, Q, qc, act, b, tr, lane,
+		                                     ecol, ev, X, Y, packed, dd);
 		if (DOTS) {
 			// Column sums over the row slots q = 0 .. Q-1, ascending; lane
 			// j < lpr writes columns j cpl .. j cpl + cpl - 1.
@@ -603,12 +640,12 @@ bool use_pairs(int w, int W) {
 // into 8 XCD parts, the pieces (model x part) with the base of their dot
 // slots (model-major, so a model's slots are contiguous) and the slot groups
 // of each model for the ordered reduction.
-void plan_phase(const Plan &p, int W, int kact, int nwx, Phase &ph, std::vector<int> &tab) {
+void plan_phase(const Plan &p, int kact, int nwx, Phase &ph, std::vector<int> &tab) {
 	std::vector<int> toff(kact + 1, 0);
 	std::vector<double> cost(kact), cum(kact + 1, 0.0);
 	for (int k = 0; k < kact; k++) {
 		toff[k + 1] = toff[k] + (int)p.tiles[k];
-		cost[k] = (double)p.desc[k].N * (12.0 * W + 24.0 * p.desc[k].wp);
+		cost[k] = (double)p.desc[k].N * (12.0 * p.desc[k].W + 24.0 * p.desc[k].wp);
 		cum[k + 1] = cum[k] + cost[k];
 	}
 	const int T = toff[kact];
@@ -710,9 +747,10 @@ int make_plan(tbtkx_context *h, const std::vector<int> &width, const std::vector
 		const int m = p.order[k];
 		MDesc &d = p.desc[k];
 		d.N = h->mm_N[m];
-		d.erow0 = h->mm_erow0[m];
+		d.e0 = h->mm_e0[m];
+		d.W = h->mm_Wm[m];
 		d.w = width[m];
-		d.pairs = use_pairs(d.w, h->mm_W) ? 1 : 0;
+		d.pairs = use_pairs(d.w, d.W) ? 1 : 0;
 		d.wp = d.pairs ? d.w + (d.w & 1) : d.w;
 		d.Q = kMaxWidth / (d.pairs ? d.wp / 2 : d.wp);
 		d.vec0 = vec;
@@ -727,7 +765,6 @@ int make_plan(tbtkx_context *h, const std::vector<int> &width, const std::vector
 		d.S = (int)S;
 		d.A = (int)((d.N + d.D - 1) / d.D);
 		d.tps = (int)((S + tr - 1) / tr);
-		d.pad = 0;
 		p.tiles[k] = ((d.D + S - 1) / S) * d.A * d.tps;
 		tiles_all += p.tiles[k];
 		// Every model's block starts on 16 bytes (two-column lanes).
@@ -753,7 +790,7 @@ int make_plan(tbtkx_context *h, const std::vector<int> &width, const std::vector
 		Phase ph;
 		ph.n_lo = n;
 		ph.n_hi = n_hi;
-		plan_phase(p, h->mm_W, kact, nwx, ph, p.tab);
+		plan_phase(p, kact, nwx, ph, p.tab);
 		p.max_slots = std::max(p.max_slots, ph.nslots);
 		p.max_seg = std::max(p.max_seg, ph.nseg);
 		p.phases.push_back(ph);
@@ -912,9 +949,10 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_multi_set: null handle");
 	TBTKX_REQUIRE(K > 0 && K <= 65536 && num_rows && nnz && rowptr && scales,
 	              TBTKX_ERR_ARG, "tbtkx_multi_set: need 1..65536 models and arrays");
-	int64_t rp0 = 0, z0 = 0, rows = 0;
+	int64_t rp0 = 0, z0 = 0, rows = 0, entries = 0;
 	uint32_t maxlen = 0;
 	bool real = true;
+	std::vector<int> Wm(K, 4);   // ELL slots per row, per model
 	for (int m = 0; m < K; m++) {
 		const int64_t N = num_rows[m], Z = nnz[m];
 		const std::string tag = "tbtkx_multi_set: model " + std::to_string(m) + ": ";
@@ -933,13 +971,17 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 			              tag + "column index out of range");
 			if (val[2 * (z0 + k) + 1] != 0.0) real = false;
 		}
+		uint32_t ml = 0;
 		for (int64_t r = 0; r < N; r++) {
 			uint32_t len = rp[r + 1] - rp[r];
 			bool diag = false;
 			for (uint32_t k = rp[r]; k < rp[r + 1]; k++)
 				if ((int64_t)col[z0 + k] == r) diag = true;
-			maxlen = std::max(maxlen, len + (diag ? 0u : 1u));
+			ml = std::max(ml, len + (diag ? 0u : 1u));
 		}
+		maxlen = std::max(maxlen, ml);
+		Wm[m] = ell_slots_multi(ml);
+		entries += N * Wm[m];
 		rp0 += N + 1;
 		z0 += Z;
 		rows += N;
@@ -953,18 +995,20 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 	if (rc) return rc;
 	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
 	const int W = ell_slots_multi(maxlen);
-	std::vector<uint32_t> ec((size_t)rows * W);
-	std::vector<float> e1((size_t)rows * W * 2, 0.0f), e2((size_t)rows * W * 2, 0.0f);
+	std::vector<uint32_t> ec((size_t)entries);
+	std::vector<float> e1((size_t)entries * 2, 0.0f), e2((size_t)entries * 2, 0.0f);
 	h->mm_N.assign(K, 0);
-	h->mm_erow0.assign(K, 0);
+	h->mm_e0.assign(K, 0);
+	h->mm_Wm = Wm;
 	h->mm_band.assign(K, 0);
 	rp0 = 0;
 	z0 = 0;
-	int64_t erow = 0;
+	int64_t e0 = 0;
 	std::vector<double> v1, v2;
 	std::vector<int64_t> slots(W + 1);
 	for (int m = 0; m < K; m++) {
 		const int64_t N = num_rows[m], Z = nnz[m];
+		const int Wmod = Wm[m];
 		const uint32_t *rp = rowptr + rp0;
 		v1.assign(2 * Z, 0.0);
 		v2.assign(2 * Z, 0.0);
@@ -973,9 +1017,9 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 		// Row slots as the single-model ELL copy (ell_row_slots: diagonal
 		// first), on the model's local column indices.
 		for (int64_t r = 0; r < N; r++) {
-			const int64_t base = (erow + r) * W;
+			const int64_t base = e0 + r * Wmod;
 			const int n = ell_row_slots(rp, col + z0, r, slots.data());
-			for (int u = 0; u < W; u++) {
+			for (int u = 0; u < Wmod; u++) {
 				const int64_t k = u < n ? slots[u] : -1;
 				if (k >= 0) {
 					ec[base + u] = col[z0 + k];
@@ -996,8 +1040,8 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 			}
 		h->mm_band[m] = band > 0 ? band : N;
 		h->mm_N[m] = N;
-		h->mm_erow0[m] = erow;
-		erow += N;
+		h->mm_e0[m] = e0;
+		e0 += N * Wmod;
 		rp0 += N + 1;
 		z0 += Z;
 	}
@@ -1089,7 +1133,8 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 			MDesc &d = p.desc[m];
 			d = MDesc{};
 			d.N = h->mm_N[m];
-			d.erow0 = h->mm_erow0[m];
+			d.e0 = h->mm_e0[m];
+			d.W = h->mm_Wm[m];
 			d.vec0 = vec;
 			d.w = width[m];
 			d.wp = width[m];
