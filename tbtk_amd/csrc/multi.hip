@@ -216,59 +216,6 @@ __device__ __forceinline__ int64_t tile_first_row(const MDesc &d, uint32_t lt, i
 	return row0;
 }
 
-// Narrow models (Q W > 64: a row group's entries do not fit one window):
-// tiles t, t + nwx, ... of model d below tend, RPI row groups in flight per
-// lane, each lane loading its own row's entries.
-template <int W, int RPI, bool FIRST, bool DOTS, bool REALH>
-__device__ __forceinline__ void multi_rows(
-    const MDesc &d, int &t, int tend, int nwx, int toff_m, int G, int Q, int qc, bool act,
-    int b, int w, int64_t tr, const uint32_t *__restrict__ ecol,
-    const float2 *__restrict__ ev, const float2 *__restrict__ X, float2 *__restrict__ Y,
-    int packed, double (&dd)[2][4]) {
-	// The model's block of X (wave-uniform base) and 32-bit byte offsets
-	// into it (N_m w 8 B < 4 GiB, checked on the host).
-	const char *Xm = reinterpret_cast<const char *>(X + d.vec0);
-	for (; t < tend; t += nwx) {
-		int64_t rows;
-		const int64_t row0 = tile_first_row(d, (uint32_t)(t - toff_m), tr, rows);
-		for (int gg = 0; gg < G; gg += RPI) {
-			int64_t row[RPI];
-			bool live[RPI];
-			float2 a2[RPI][W], xv[RPI][W], yo[RPI];
-#pragma unroll
-			for (int r = 0; r < RPI; r++) {
-				const int64_t off = (int64_t)(gg + r) * Q + qc;
-				live[r] = act && off < rows;
-				row[r] = row0 + (off < rows ? off : (rows > 0 ? rows - 1 : 0));
-				if (!FIRST) yo[r] = load_nt(&Y[d.vec0 + row[r] * w + b]);
-			}
-#pragma unroll
-			for (int r = 0; r < RPI; r++) {
-				const int64_t e = d.e0 + row[r] * W;
-#pragma unroll
-				for (int u = 0; u < W; u++) {
-					a2[r][u] = ev[e + u];
-					xv[r][u] = *reinterpret_cast<const float2 *>(
-					    Xm + (ecol[e + u] * (uint32_t)w + (uint32_t)b) * 8u);
-				}
-			}
-#pragma unroll
-			for (int r = 0; r < RPI; r++) {
-				float ax = FIRST ? 0.0f : -yo[r].x;
-				float ay = FIRST ? 0.0f : -yo[r].y;
-#pragma unroll
-				for (int u = 0; u < W; u++) mac<REALH>(a2[r][u], xv[r][u], ax, ay);
-				if (!live[r]) continue;
-				store_nt(&Y[d.vec0 + row[r] * w + b], ax, ay);
-				// Slot 0 is the row itself (diagonal first).
-				if (DOTS)
-					dot_acc(xv[r][0].x, xv[r][0].y, ax, ay, packed, dd[0][0], dd[0][1],
-					        dd[0][2], dd[0][3]);
-			}
-		}
-	}
-}
-
 // Shared-window path (Q W <= 64) with the ELL window of the next iteration
 // loaded one iteration ahead. A lane moves CPL consecutive columns of its
 // row (CPL = 2: 16-byte loads and stores). An iteration is RPI row groups of
@@ -295,7 +242,9 @@ __device__ __forceinline__ void multi_rows_pipe(
     int b, int64_t tr, int lane, const uint32_t *__restrict__ ecol,
     const float2 *__restrict__ ev, const float2 *__restrict__ X, float2 *__restrict__ Y,
     int packed, double (&dd)[2][4]) {
-	constexpr int NW = RPI / GPW;
+	// GPW = 0: each lane loads its own rows' W entries (RPI W registers).
+	constexpr bool PERLANE = GPW == 0;
+	constexpr int NW = PERLANE ? RPI * W : RPI / (PERLANE ? 1 : GPW);
 	const uint32_t wp = (uint32_t)d.wp;
 	// The model's blocks of X and Y (wave-uniform bases) and 32-bit byte
 	// offsets into them (N_m wp 8 B < 4 GiB, checked on the host).
@@ -315,13 +264,26 @@ __device__ __forceinline__ void multi_rows_pipe(
 	// Windows of the iteration at groups g0.. of the tile at r0 (nr rows).
 	auto load_windows = [&](int64_t r0, int64_t nr, int g0) {
 		const int64_t glast = r0 + nr - 1;
+		if constexpr (PERLANE) {
 #pragma unroll
-		for (int k = 0; k < NW; k++) {
-			int64_t wrow = r0 + (int64_t)(g0 + k * GPW) * Q + wrow_l;
-			if (wrow > glast) wrow = glast;
-			const int64_t we = d.e0 + wrow * W + wslot;
-			wc[k] = ecol[we];
-			wv[k] = ev[we];
+			for (int r = 0; r < RPI; r++) {
+				const int64_t o = (int64_t)(g0 + r) * Q + qc;
+				const int64_t e = d.e0 + (r0 + (o < nr ? o : nr - 1)) * W;
+#pragma unroll
+				for (int u = 0; u < W; u++) {
+					wc[r * W + u] = ecol[e + u];
+					wv[r * W + u] = ev[e + u];
+				}
+			}
+		} else {
+#pragma unroll
+			for (int k = 0; k < NW; k++) {
+				int64_t wrow = r0 + (int64_t)(g0 + k * GPW) * Q + wrow_l;
+				if (wrow > glast) wrow = glast;
+				const int64_t we = d.e0 + wrow * W + wslot;
+				wc[k] = ecol[we];
+				wv[k] = ev[we];
+			}
 		}
 	};
 	load_windows(row0, rows, 0);
@@ -346,8 +308,11 @@ __device__ __forceinline__ void multi_rows_pipe(
 		for (int r = 0; r < RPI; r++) {
 #pragma unroll
 			for (int u = 0; u < W; u++) {
-				const uint32_t c =
-				    (uint32_t)__shfl((int)wc[r / GPW], ((r % GPW) * Q + qc) * W + u);
+				uint32_t c;
+				if constexpr (PERLANE)
+					c = wc[r * W + u];
+				else
+					c = (uint32_t)__shfl((int)wc[r / GPW], ((r % GPW) * Q + qc) * W + u);
 				lane_load<CPL>(Xm + (c * wp + (uint32_t)b) * 8u, xv[r][u]);
 			}
 		}
@@ -391,9 +356,13 @@ __device__ __forceinline__ void multi_rows_pipe(
 			}
 #pragma unroll
 			for (int u = 0; u < W; u++) {
-				const int src = ((r % GPW) * Q + qc) * W + u;
-				const float2 av = make_float2(__shfl(vc[r / GPW].x, src),
-				                              __shfl(vc[r / GPW].y, src));
+				float2 av;
+				if constexpr (PERLANE) {
+					av = vc[r * W + u];
+				} else {
+					const int src = ((r % GPW) * Q + qc) * W + u;
+					av = make_float2(__shfl(vc[r / GPW].x, src), __shfl(vc[r / GPW].y, src));
+				}
 #pragma unroll
 				for (int j = 0; j < CPL; j++) mac<REALH>(av, xv[r][u][j], ax[j], ay[j]);
 			}
@@ -446,8 +415,7 @@ __device__ __forceinline__ void model_rows_w(
 	} else if (Q * W <= 64) {
 		TBTKX_PIPE(1, 4, 1);
 	} else {
-		multi_rows<W, kRpi, FIRST, DOTS, REALH>(d, t, tend, nwx, toff_m, G, Q, qc, act, b, d.wp,
-		                                        tr, ecol, ev, X, Y, packed, dd);
+		TBTKX_PIPE(0, kRpi, 1);
 	}
 #undef TBTKX_PIPE
 }
