@@ -138,6 +138,38 @@ def test_multi_graph_replays_with_eager_work_between(engine):
         engine.set_option(X.OPT_GRAPH, 0)
 
 
+@pytest.mark.parametrize("which", ["complex", "real"])
+def test_multi_mixed_ell_widths(engine, oracle_mod, which):
+    """One batch with ELL widths 4 (honeycomb ribbon), 5 (square lattice), 6
+    (Nambu s-wave) and 8 (cubic lattice, 7 entries per row): each model runs
+    on the path compiled for its own width (two-column lanes, one-column
+    pipelined, per-lane entries), bitwise equal in gather mode to the
+    single-model batched path, and within the FP32 tolerance of the FP64
+    oracle in doubling mode. 'complex' adds a square lattice with flux, which
+    makes the whole batch complex (no packing)."""
+    models = [L.honeycomb_ribbon(11, 7, disorder=0.5, seed=1),
+              L.square(13, 9, disorder=0.5, seed=2),
+              L.nambu_swave(7, 6, disorder=0.5, seed=3),
+              L.cubic(6, 5, 4, disorder=1.0, seed=4)]
+    widths = [64, 5, 27, 9]
+    Ms = [40, 57, 33, 64]
+    if which == "complex":
+        models.append(L.square(6, 5, disorder=0.5, seed=5, flux=0.13))
+        widths.append(3)
+        Ms.append(21)
+    rng = np.random.default_rng(8)
+    batch = [(m, np.sort(rng.choice(m.basis_size, b, replace=False)), M)
+             for m, b, M in zip(models, widths, Ms)]
+    csrs, scales = upload(engine, batch)
+    got = engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], 0)
+    dbl = engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], X.DOUBLING)
+    for (m, sites, M), csr, s, g, d in zip(batch, csrs, scales, got, dbl):
+        ref = single_model(engine, csr, s, sites, M, 0)
+        assert np.array_equal(g, ref), (m.name, np.abs(g - ref).max())
+        exact = np.array([oracle_mod.moments(csr, s, b, [b], M)[0] for b in sites])
+        assert np.abs(d - exact).max() < 2e-5, (m.name, np.abs(d - exact).max())
+
+
 def test_multi_rejects_bad_arguments(engine):
     batch = small_batch()[:2]
     upload(engine, batch)
