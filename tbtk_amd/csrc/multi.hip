@@ -216,12 +216,14 @@ __device__ __forceinline__ int64_t tile_first_row(const MDesc &d, uint32_t lt, i
 	return row0;
 }
 
-// Shared-window path (Q W <= 64) with the ELL window of the next iteration
-// loaded one iteration ahead. A lane moves CPL consecutive columns of its
-// row (CPL = 2: 16-byte loads and stores). An iteration is RPI row groups of
-// one tile; their entries come from NW = RPI / GPW windows (window k: the
-// GPW Q W consecutive entries of groups k GPW .. k GPW + GPW - 1, lane l
-// holding entry l). Per iteration, in issue order: the Phi_{n-1} rows and
+// Row groups with the ELL entries of the next iteration loaded one iteration
+// ahead. A lane moves CPL consecutive columns of its row (CPL = 2: 16-byte
+// loads and stores). An iteration is RPI row groups of one tile. Their
+// entries come either from NW = RPI / GPW shared windows (Q W <= 64;
+// window k: the GPW Q W consecutive entries of groups k GPW .. k GPW + GPW
+// - 1, lane l holding entry l, read by the other lanes through shuffles) or,
+// with GPW = 0 (narrow models, Q W > 64), from each lane's own loads of its
+// rows' W entries. Per iteration, in issue order: the Phi_{n-1} rows and
 // the gathers (columns taken from the window registers), the stores of the
 // previous iteration's rows (kept in registers until now), the next
 // iteration's windows, then the wait for this iteration's loads and the
