@@ -28,8 +28,14 @@ def lorentz(c: np.ndarray, broadening: float) -> np.ndarray:
         return c
     M = c.shape[1]
     lam = broadening * M
-    num = np.array([math.sinh(lam * (1 - n / float(M))) for n in range(M)])
-    return (c * num[None, :]) / math.sinh(lam)
+    num = np.array([math.sinh(lam * (1 - n / float(M))) for n in range(M)])[None, :]
+    den = math.sinh(lam)
+    # Component by component, as std::complex<double> * double and / double
+    # do (numpy's complex / real would multiply by 1/den instead).
+    out = np.empty_like(c)
+    out.real = (c.real * num) / den
+    out.imag = (c.imag * num) / den
+    return out
 
 
 def jackson(M: int) -> np.ndarray:

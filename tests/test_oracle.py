@@ -225,3 +225,19 @@ def test_uniform_stream_statistics(oracle_mod):
     assert 0 <= u.min() and u.max() < 1
     assert abs(u.mean() - 0.5) < 0.01
     assert len(np.unique(u)) == u.size
+
+
+def test_host_lorentz_bitwise_vs_oracle(oracle_mod):
+    """The host-side Lorentz damping the engine's LDOS uses (tbtk_amd.kpm)
+    is the oracle's, bit for bit: c_n sinh(lambda (1 - n/M)) / sinh(lambda)
+    per real and imaginary part, as std::complex<double> times and divided by
+    a double (Lib/src/Solver/ChebyshevExpander.cpp:190-196). A complex
+    division by the real denominator (which multiplies by its reciprocal)
+    differs in the last bit for a share of the entries."""
+    from tbtk_amd import kpm
+    rng = np.random.default_rng(5)
+    for K, M, b in ((1, 1000, 1e-6), (3, 257, 0.01), (2, 4000, 0.003)):
+        c = rng.standard_normal((K, M)) + 1j * rng.standard_normal((K, M))
+        got = kpm.lorentz(c, b)
+        ref = oracle_mod.lorentz(c, b)
+        assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), (K, M, b)
