@@ -9,6 +9,7 @@ where the reference defines it:
 """
 from __future__ import annotations
 
+import functools
 import math
 
 import numpy as np
@@ -20,16 +21,23 @@ def energy_grid(lo: float, hi: float, res: int) -> np.ndarray:
     return lo + np.arange(res) * dx
 
 
+@functools.lru_cache(maxsize=64)
+def _lorentz_factors(M: int, broadening: float):
+    """sinh(lambda (1 - n/M)) for n < M and sinh(lambda), lambda =
+    broadening*M, with the C library's sinh (math.sinh)."""
+    lam = broadening * M
+    num = np.array([math.sinh(lam * (1 - n / float(M))) for n in range(M)])[None, :]
+    num.setflags(write=False)
+    return num, math.sinh(lam)
+
+
 def lorentz(c: np.ndarray, broadening: float) -> np.ndarray:
     """c[k][n] * sinh(lambda (1 - n/M)) / sinh(lambda), lambda = broadening*M,
     evaluated with the C library's sinh in the reference's operation order."""
     c = np.array(c, dtype=np.complex128, ndmin=2)
     if broadening == 0:
         return c
-    M = c.shape[1]
-    lam = broadening * M
-    num = np.array([math.sinh(lam * (1 - n / float(M))) for n in range(M)])[None, :]
-    den = math.sinh(lam)
+    num, den = _lorentz_factors(c.shape[1], float(broadening))
     # Component by component, as std::complex<double> * double and / double
     # do (numpy's complex / real would multiply by 1/den instead).
     out = np.empty_like(c)
