@@ -154,10 +154,11 @@ def batched_comparison(eng):
     sites = np.sort(rng.choice(N, B_SITES, replace=False))
     lo, hi = -0.99 * s, 0.99 * s
     eng.lut_generate(M_BATCH, E_PTS, lo, hi, s)
-    g = kpm.jackson(M_BATCH)
+    # Jackson damping in FP64 on the device, applied by the contraction call
+    eng.set_damping(X.DAMP_JACKSON)
 
     def contract(mu):
-        return eng.contract(mu * g[None, :], X.NONPRINCIPAL).imag / np.pi
+        return eng.contract(mu, X.NONPRINCIPAL).imag / np.pi
 
     # warm-up (allocations and the shape autotuner for 256 sites), then timed
     eng.moments_block(sites, 16, X.DOUBLING)
@@ -182,6 +183,7 @@ def batched_comparison(eng):
 
     err = max(np.abs(ld_batched[i] - ld_plain[i]).max() / np.abs(ld_plain[i]).max()
               for i in range(PLAIN_CMP_SITES))
+    eng.set_damping(X.DAMP_NONE)
     B = eng.get_option(X.OPT_BLOCK_WIDTH)
     shape = {"block_width": B, "strip_rows": eng.get_option(X.OPT_STRIP_ROWS),
              "grid_share": eng.get_option(X.OPT_GRID_SHARE),
@@ -193,7 +195,8 @@ def batched_comparison(eng):
     return {
         "workload": ("nambu s-wave 512x512x4 (1,048,576 states, t=1, mu=-1, Delta=0.3, "
                      "J=0.5, disorder U[-0.5,0.5] seed 3), 256 seeded LDOS sites, "
-                     f"M={M_BATCH}, E={E_PTS}, Jackson damping, s=Gershgorin*1.05={s:.6f}"),
+                     f"M={M_BATCH}, E={E_PTS}, Jackson damping (FP64, on the device), "
+                     f"s=Gershgorin*1.05={s:.6f}"),
         "nnz": int(Z),
         "sites_per_s": sps_b,
         "plain_fp64_sites_per_s": sps_p,
@@ -265,11 +268,12 @@ def uneven_batch(eng, graph=True):
         spec, st, Mi = batch[i]
         eng.set_lattice(spec, scales[i])
         ref = eng.moments(int(st[0]), [int(st[0])], Mi)[0]
-        g = kpm.jackson(Mi)
         lo, hi = -0.99 * scales[i], 0.99 * scales[i]
         eng.lut_generate(Mi, E_PTS, lo, hi, scales[i])
-        l_ref = kpm.ldos_from_gf(eng.contract(ref * g, X.NONPRINCIPAL))
-        l_got = kpm.ldos_from_gf(eng.contract(mu[i][0] * g, X.NONPRINCIPAL))
+        eng.set_damping(X.DAMP_JACKSON)
+        l_ref = kpm.ldos_from_gf(eng.contract(ref, X.NONPRINCIPAL))
+        l_got = kpm.ldos_from_gf(eng.contract(mu[i][0], X.NONPRINCIPAL))
+        eng.set_damping(X.DAMP_NONE)
         errs.append(float(np.abs(l_got - l_ref).max() / np.abs(l_ref).max()))
     eng.lut_destroy()
     dev_s = ms_graph * 1e-3
@@ -324,10 +328,11 @@ def stochastic_dos(eng):
     Rc = 8
     a = eng.moments_trace(Rc, seed, M)
     b = eng.moments_trace(Rc, seed, M, X.VEC_FP64 | X.H_FP64)
-    g = kpm.jackson(M)
     eng.lut_generate(M, E_PTS, -0.99 * s, 0.99 * s, s)
-    dos_a = kpm.ldos_from_gf(eng.contract(a * g, X.NONPRINCIPAL))
-    dos_b = kpm.ldos_from_gf(eng.contract(b * g, X.NONPRINCIPAL))
+    eng.set_damping(X.DAMP_JACKSON)
+    dos_a = kpm.ldos_from_gf(eng.contract(a, X.NONPRINCIPAL))
+    dos_b = kpm.ldos_from_gf(eng.contract(b, X.NONPRINCIPAL))
+    eng.set_damping(X.DAMP_NONE)
     eng.lut_destroy()
     blocks = (R + Bw - 1) // Bw
     step_bytes = 12 * Z + 4 * N + 24 * Bw * N

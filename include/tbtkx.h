@@ -137,6 +137,25 @@ int tbtkx_coeffs_upload(tbtkx_handle handle, int num_vectors, int num_moments,
  * Lib/src/cuda/ChebyshevSolver.cu:650-670). */
 int tbtkx_contract_mfma(tbtkx_handle handle, int type, double *out);
 
+/* Damping of the contraction's coefficients, done on the device in FP64:
+ * once set, tbtkx_contract, tbtkx_contract_direct and tbtkx_coeffs_upload
+ * multiply the coefficients they are given by g_n before contracting.
+ * TBTKX_DAMP_LORENTZ: c_n * sinh(lambda (1 - n/M)) / sinh(lambda), lambda =
+ * broadening * M, per real and imaginary part as the reference (its CPU path,
+ * Lib/src/Solver/ChebyshevExpander.cpp:190-196, and its GPU path on the
+ * device, Lib/src/cuda/ChebyshevSolver.cu:632-645); broadening 0 leaves the
+ * coefficients unchanged, as there. TBTKX_DAMP_JACKSON: the Jackson kernel
+ * g_n = ((M - n + 1) cos(q n) + sin(q n) cot q) / (M + 1), q = pi/(M + 1)
+ * (no reference counterpart). TBTKX_DAMP_NONE (the default): coefficients
+ * are contracted as given. The factors use the device sinh/sin/cos, so they
+ * may differ from the host libm's in the last bit. */
+enum {
+	TBTKX_DAMP_NONE = 0,
+	TBTKX_DAMP_LORENTZ = 1,
+	TBTKX_DAMP_JACKSON = 2
+};
+int tbtkx_set_damping(tbtkx_handle handle, int kind, double broadening);
+
 /* Batched path: diagonal moments mu[b*M + n] = <s_b| T_n(H/s) |s_b> for B
  * unit start vectors at basis indices sites[b] (LDOS sites), run as blocks
  * of start vectors through one SpMM recurrence. FP32 complex vectors (unless

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <initializer_list>
@@ -156,6 +157,10 @@ struct tbtkx_context {
 	tbtkx::GraphCache mm_graph;
 
 	// Options.
+	// Damping of the contraction's coefficients (TBTKX_DAMP_*).
+	int damp_kind = 0;
+	double damp_broadening = 0;
+
 	int plain_kernel = 0;
 	int use_graph = 0;
 	int block_width = 0;   // 0 = autotune
@@ -191,6 +196,10 @@ __device__ __forceinline__ int xcd_remap(int hw, int n) {
 }
 
 int set_device(tbtkx_context *h);
+
+// Damping set by tbtkx_set_damping, applied in FP64 to K x M coefficients
+// in device memory on the handle's stream (gf.hip).
+int apply_damping(tbtkx_context *h, double2 *C, int K, int M);
 
 // Slot order of a row in the ELL copies of H: the diagonal entry first (k =
 // -1 when the row has none: a zero value in column r), then the other

@@ -35,6 +35,44 @@ __global__ __launch_bounds__(kThreads) void lut_generate(
 	                                     (ar * sn + ai * cs) / denom);
 }
 
+// Damping of K coefficient vectors C[k*M + n] in FP64, in place; one thread
+// per moment n computes its factor once and walks the vectors.
+//   Lorentz (Lib/src/Solver/ChebyshevExpander.cpp:190-196; the reference's
+//   GPU path damps on the device too, Lib/src/cuda/ChebyshevSolver.cu:632-645):
+//     c *= sinh(lambda (1 - n/M)); c /= sinh(lambda), lambda = broadening*M,
+//     real and imaginary part each, as std::complex<double> * and / double do.
+//   Jackson (Weisse et al., Rev. Mod. Phys. 78, 275 (2006), eq. 71):
+//     g_n = ((M - n + 1) cos(q n) + sin(q n) cos(q) / sin(q)) / (M + 1),
+//     q = pi/(M + 1); c *= g_n.
+__global__ __launch_bounds__(kThreads) void damp_coeffs(double2 *__restrict__ C, int K, int M,
+                                                        int kind, double broadening) {
+#pragma clang fp contract(off)
+	const int n = blockIdx.x * kThreads + threadIdx.x;
+	if (n >= M) return;
+	double num, den;
+	if (kind == TBTKX_DAMP_LORENTZ) {
+		const double lambda = broadening * M;
+		num = sinh(lambda * (1 - n / (double)M));
+		den = sinh(lambda);
+	} else {
+		const double q = M_PI / (M + 1);
+		num = (double)(M - n + 1) * cos(q * n) + sin(q * n) * cos(q) / sin(q);
+		den = (double)(M + 1);
+	}
+	const double g = num / den;
+	for (int k = blockIdx.y; k < K; k += gridDim.y) {
+		double2 c = C[(int64_t)k * M + n];
+		if (kind == TBTKX_DAMP_LORENTZ) {
+			c.x = (c.x * num) / den;
+			c.y = (c.y * num) / den;
+		} else {
+			c.x = c.x * g;
+			c.y = c.y * g;
+		}
+		C[(int64_t)k * M + n] = c;
+	}
+}
+
 // One term of the contraction, in the reference's operation order.
 template <int TYPE>
 __device__ __forceinline__ void contract_term(double2 l, double2 cn, double &gr, double &gi) {
@@ -153,6 +191,18 @@ __global__ __launch_bounds__(kThreads) void contract_direct(
 }
 
 }  // namespace
+
+// Applies the handle's damping (tbtkx_set_damping) to K x M coefficients
+// already in device memory, on the handle's stream; no-op without one.
+int apply_damping(tbtkx_context *h, double2 *C, int K, int M) {
+	if (h->damp_kind == TBTKX_DAMP_NONE || K == 0) return TBTKX_OK;
+	if (h->damp_kind == TBTKX_DAMP_LORENTZ && h->damp_broadening == 0) return TBTKX_OK;
+	dim3 grid((M + kThreads - 1) / kThreads, K < 1024 ? K : 1024);
+	damp_coeffs<<<grid, kThreads, 0, h->stream>>>(C, K, M, h->damp_kind, h->damp_broadening);
+	TBTKX_CHECK_HIP(hipGetLastError());
+	return TBTKX_OK;
+}
+
 }  // namespace tbtkx
 
 using namespace tbtkx;
@@ -212,6 +262,18 @@ int tbtkx_lut_destroy(tbtkx_handle h) {
 	return TBTKX_OK;
 }
 
+int tbtkx_set_damping(tbtkx_handle h, int kind, double broadening) {
+	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_set_damping: null handle");
+	TBTKX_REQUIRE(kind == TBTKX_DAMP_NONE || kind == TBTKX_DAMP_LORENTZ ||
+	                  kind == TBTKX_DAMP_JACKSON,
+	              TBTKX_ERR_ARG, "tbtkx_set_damping: unknown damping kind");
+	TBTKX_REQUIRE(std::isfinite(broadening), TBTKX_ERR_ARG,
+	              "tbtkx_set_damping: the broadening has to be finite");
+	h->damp_kind = kind;
+	h->damp_broadening = kind == TBTKX_DAMP_LORENTZ ? broadening : 0.0;
+	return TBTKX_OK;
+}
+
 int tbtkx_contract(tbtkx_handle h, int K, const double *coeffs, int type,
                    double *out) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_contract: null handle");
@@ -231,6 +293,7 @@ int tbtkx_contract(tbtkx_handle h, int K, const double *coeffs, int type,
 	if ((rc = h->gf_out.reserve((size_t)K * E * 16))) return rc;
 	TBTKX_CHECK_HIP(hipMemcpyAsync(h->gf_in.ptr, coeffs, (size_t)K * M * 16,
 	                               hipMemcpyHostToDevice, h->stream));
+	if ((rc = apply_damping(h, h->gf_in.as<double2>(), K, M))) return rc;
 	dim3 grid((E + kThreads - 1) / kThreads, K);
 	const double2 *L = h->lut.as<double2>(), *C = h->gf_in.as<double2>();
 	double2 *G = h->gf_out.as<double2>();
@@ -271,6 +334,7 @@ int tbtkx_contract_direct(tbtkx_handle h, int K, int M, const double *coeffs, in
 	if ((rc = h->gf_out.reserve((size_t)K * E * 16))) return rc;
 	TBTKX_CHECK_HIP(hipMemcpyAsync(h->gf_in.ptr, coeffs, (size_t)K * M * 16,
 	                               hipMemcpyHostToDevice, h->stream));
+	if ((rc = apply_damping(h, h->gf_in.as<double2>(), K, M))) return rc;
 	dim3 grid((E + kThreads - 1) / kThreads, K);
 	const double2 *C = h->gf_in.as<double2>();
 	double2 *G = h->gf_out.as<double2>();

@@ -403,6 +403,7 @@ int tbtkx_coeffs_upload(tbtkx_handle h, int K, int M, const double *coeffs) {
 	if ((rc = h->cplanes.reserve((size_t)4 * K * Mp * 2))) return rc;
 	TBTKX_CHECK_HIP(hipMemcpyAsync(h->gf_in.ptr, coeffs, (size_t)K * M * 16,
 	                               hipMemcpyHostToDevice, h->stream));
+	if ((rc = apply_damping(h, h->gf_in.as<double2>(), K, M))) return rc;
 	const int64_t n = (int64_t)K * Mp;
 	split_planes<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, h->stream>>>(
 	    h->gf_in.as<double2>(), h->cplanes.as<uint16_t>(), K, M, Mp, 0);

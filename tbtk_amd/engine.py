@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libtbtkx.so")
 
 ADVANCED, RETARDED, PRINCIPAL, NONPRINCIPAL = 0, 1, 2, 3
 H_FP64, DOUBLING, VEC_FP64 = 1, 2, 4
+DAMP_NONE, DAMP_LORENTZ, DAMP_JACKSON = 0, 1, 2
 (OPT_PLAIN_KERNEL, OPT_GRAPH, OPT_BLOCK_WIDTH, OPT_XCD_REMAP, OPT_TILE_ROWS, OPT_NT_STREAM,
  OPT_ELL, OPT_AUTOTUNE, OPT_MFMA_PIPE, OPT_STRIP_ROWS, OPT_GRID_SHARE,
  OPT_STRIP_CHUNK, OPT_ROWS_IN_FLIGHT, OPT_REAL) = range(1, 15)
@@ -42,6 +43,7 @@ SIGNATURES = {
     "tbtkx_contract_direct": [_P, _I, _I, _P, _I, _I, _D, _D, _D, _P],
     "tbtkx_coeffs_upload": [_P, _I, _I, _P],
     "tbtkx_contract_mfma": [_P, _I, _P],
+    "tbtkx_set_damping": [_P, _I, _D],
     "tbtkx_moments_block": [_P, _I, _P, _I, _I, _P],
     "tbtkx_moments_trace": [_P, _I, ctypes.c_uint64, _I, _I, _P],
     "tbtkx_set_lattice": [_P, _P, _P, _D],
@@ -206,6 +208,12 @@ class Engine:
         c = np.ascontiguousarray(np.atleast_2d(coeffs), dtype=np.complex128)
         _check(self._lib.tbtkx_coeffs_upload(self._h, c.shape[0], c.shape[1], _p(c)))
         self.coeffs_shape = c.shape
+
+    def set_damping(self, kind: int, broadening: float = 0.0):
+        """Damping applied on the device (FP64) to the coefficients of every
+        later contraction call: DAMP_NONE, DAMP_LORENTZ (with broadening)
+        or DAMP_JACKSON."""
+        _check(self._lib.tbtkx_set_damping(self._h, int(kind), float(broadening)))
 
     def contract_mfma(self, gtype: int, keep_on_device: bool = False):
         """Split-BF16 MFMA contraction of the uploaded coefficients."""
