@@ -142,11 +142,11 @@ def load_traffic(kernel):
 
 def batched_comparison(eng):
     """Config: 512x512 Nambu, 256 sites, M=2000; batched FP32 vs plain FP64."""
-    m = L.config_nambu(512)
-    csr = m.csr()
-    N, Z = m.basis_size, csr[1].size
-    s = kpm.gershgorin_scale(csr, 1.05)
-    eng.set_csr(csr, s)
+    # H and its Gershgorin bound (FP64) built on the device from the lattice
+    spec = L.config_nambu_spec(512)
+    s = eng.lattice_gershgorin(spec) * 1.05
+    eng.set_lattice(spec, s)
+    N, Z = eng.csr_size()
     # Launch shape (block width, L2 strip, grid share, rows in flight) from
     # the engine's autotuner, which times the candidates on this model once.
     eng.set_option(X.OPT_BLOCK_WIDTH, 0)
@@ -196,7 +196,7 @@ def batched_comparison(eng):
         "workload": ("nambu s-wave 512x512x4 (1,048,576 states, t=1, mu=-1, Delta=0.3, "
                      "J=0.5, disorder U[-0.5,0.5] seed 3), 256 seeded LDOS sites, "
                      f"M={M_BATCH}, E={E_PTS}, Jackson damping (FP64, on the device), "
-                     f"s=Gershgorin*1.05={s:.6f}"),
+                     f"s=Gershgorin*1.05={s:.6f} (H and bound built on the device)"),
         "nnz": int(Z),
         "sites_per_s": sps_b,
         "plain_fp64_sites_per_s": sps_p,
@@ -308,11 +308,10 @@ def uneven_batch(eng, graph=True):
 def stochastic_dos(eng):
     """128^3 cubic lattice (t=1, disorder U[-1,1], seed 5), R=512 random-phase
     start vectors, M=4096, FP32 storage, FP64 moment sums, doubling."""
-    m = L.config_cubic(128)
-    csr = m.csr()
-    N, Z = m.basis_size, csr[1].size
-    s = kpm.gershgorin_scale(csr, 1.05)
-    eng.set_csr(csr, s)
+    spec = L.config_cubic_spec(128)
+    s = eng.lattice_gershgorin(spec) * 1.05
+    eng.set_lattice(spec, s)
+    N, Z = eng.csr_size()
     R, M, seed = 512, 4096, 17
     eng.set_option(X.OPT_BLOCK_WIDTH, 0)
     eng.moments_trace(R, seed, 8)  # allocations and the shape autotuner
@@ -339,7 +338,8 @@ def stochastic_dos(eng):
     per_step = ms * 1e-3 / launches
     return {
         "workload": (f"cubic 128^3 (2,097,152 states, t=1, disorder U[-1,1] seed 5), R={R} "
-                     f"random-phase vectors (seed {seed}), M={M}, s=Gershgorin*1.05={s:.6f}"),
+                     f"random-phase vectors (seed {seed}), M={M}, s=Gershgorin*1.05={s:.6f} "
+                     "(H and bound built on the device)"),
         "nnz": int(Z),
         "dtype": "fp32 storage (H and vectors), fp64 moment sums, doubling",
         "seconds": ms * 1e-3,

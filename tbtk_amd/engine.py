@@ -260,6 +260,13 @@ class Engine:
             ctypes.byref(out)))
         return out.value
 
+    def csr_size(self):
+        """(rows, non-zeros) of the resident model."""
+        n = ctypes.c_int64(0)
+        z = ctypes.c_int64(0)
+        _check(self._lib.tbtkx_csr_size(self._h, ctypes.byref(n), ctypes.byref(z)))
+        return n.value, z.value
+
     def csr_copy(self):
         """(rowptr, col, H/s) of the resident model, as the device holds them."""
         n = ctypes.c_int64(0)
