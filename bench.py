@@ -178,6 +178,7 @@ def batched_comparison(eng):
         c = eng.moments(int(b), [int(b)], M_BATCH)
         plain_ms += eng.last_timing()[0]
         ref.append(c[0])
+    plain_cb = 2 if eng.get_option(X.OPT_COL16) == 1 else 4
     ld_plain = contract(np.array(ref))
     t_plain = (time.perf_counter() - t0) / PLAIN_CMP_SITES
 
@@ -212,7 +213,8 @@ def batched_comparison(eng):
         "spmm_GBps": step_bytes / per_step / 1e9,
         "spmm_hbm_frac": step_bytes / per_step / 1e9 / HBM_PEAK_GBPS,
         "plain_arithmetic": "real fp64" if plain_real else "complex fp64",
-        "plain_spmv_GBps": ((12 * Z + 28 * N) if plain_real else (20 * Z + 52 * N))
+        "plain_spmv_GBps": (((8 + plain_cb) * Z + 28 * N) if plain_real
+                            else ((16 + plain_cb) * Z + 52 * N))
         / (plain_ms * 1e-3 / (PLAIN_CMP_SITES * (M_BATCH - 1))) / 1e9,
     }
 
@@ -513,7 +515,11 @@ def main():
         value = args.steps * sites_per_step / elapsed
         # A real H runs the real FP64 recurrence (same moments, fewer bytes).
         real = eng.get_option(X.OPT_REAL) == 1
-        step_bytes = 12 * Z + 28 * N if real else 20 * Z + 52 * N
+        # 16-bit column offsets (every column within int16 of its row
+        # block's first row): 2 bytes per non-zero instead of 4.
+        col16 = eng.get_option(X.OPT_COL16) == 1
+        cb = 2 if col16 else 4
+        step_bytes = (8 + cb) * Z + 28 * N if real else (16 + cb) * Z + 52 * N
         avg_kernel_s = kernel_ms * 1e-3 / launches
         achieved = step_bytes / avg_kernel_s / 1e9
         dE = (WINDOW[1] - WINDOW[0]) / (E_PTS - 1)
@@ -540,15 +546,17 @@ def main():
             "physical_sites_per_s": value / 2,
             "spmv_GBps": achieved,
             "roofline": {
-                "kernel": ("plain_step_lds<double> (real FP64 CSR SpMV recurrence step; "
-                           "H is real)" if real else
-                           "plain_step_lds<double2> (complex FP64 CSR SpMV recurrence step)"),
+                "kernel": (("plain_step_lds<double> (real FP64 CSR SpMV recurrence step; "
+                            "H is real)" if real else
+                            "plain_step_lds<double2> (complex FP64 CSR SpMV recurrence step)")
+                           + ("; 16-bit column offsets" if col16 else "; 32-bit columns")),
                 "bound": "hbm",
                 "achieved": achieved,
                 "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBPS,
-                "traffic": load_traffic("plain_step_lds_real" if real else "plain_step_lds"),
+                "traffic": load_traffic(("plain_step_lds_real" if real else "plain_step_lds")
+                                        + ("_col16" if col16 else "")),
                 "algorithmic_bytes_per_launch": step_bytes,
                 "avg_launch_us": avg_kernel_s * 1e6,
             },

@@ -299,7 +299,7 @@ enum {
 	TBTKX_OPT_ROWS_IN_FLIGHT = 13, /* strip traversal, FP32: 1 or 2 times the
 	                           default rows a wavefront keeps in flight
 	                           (0 = autotuned, default) */
-	TBTKX_OPT_REAL = 14     /* 1 = when every value of H is real and the start
+	TBTKX_OPT_REAL = 14,    /* 1 = when every value of H is real and the start
 	                           vectors are unit vectors, run the recurrence in
 	                           real arithmetic (default): the plain path on
 	                           real FP64 vectors, the batched paths with two
@@ -308,6 +308,12 @@ enum {
 	                           vector bytes. The moments equal those of the
 	                           complex recurrence (whose imaginary parts stay
 	                           exactly zero). 0 = always complex */
+	TBTKX_OPT_COL16 = 15    /* 1 = the plain FP64 step reads its column indices
+	                           as 16-bit offsets from the first row of their row
+	                           block when every offset fits (default): 2 bytes
+	                           instead of 4 per non-zero, same arithmetic, same
+	                           moments; 0 = 32-bit columns always. Reading it
+	                           gives 1 when the last plain call used them */
 };
 int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
 

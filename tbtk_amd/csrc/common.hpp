@@ -116,6 +116,10 @@ struct tbtkx_context {
 	// Row blocks for the LDS-staged plain kernel.
 	tbtkx::DevBuf rblk;
 	int nblk = 0;
+	// Columns of the plain kernel as 16-bit offsets from the first row of
+	// their row block (col16_ok when every offset fits in int16).
+	tbtkx::DevBuf col16, col16_flag;
+	int col16_ok = 0;
 
 	// Plain path vectors and moment output.
 	tbtkx::DevBuf vec_a, vec_b, coef, to;
@@ -162,6 +166,8 @@ struct tbtkx_context {
 	double damp_broadening = 0;
 
 	int plain_kernel = 0;
+	int use_col16 = 1;
+	int last_col16 = 0;   // the last plain call used 16-bit columns
 	int use_graph = 0;
 	int block_width = 0;   // 0 = autotune
 	int xcd_remap = 0;     // measured faster without for both step kernels
@@ -200,6 +206,10 @@ int set_device(tbtkx_context *h);
 // Damping set by tbtkx_set_damping, applied in FP64 to K x M coefficients
 // in device memory on the handle's stream (gf.hip).
 int apply_damping(tbtkx_context *h, double2 *C, int K, int M);
+
+// Builds the 16-bit column offsets of the resident CSR (plain.hip); sets
+// col16_ok. Needs rowptr, col, rblk and nblk in place.
+int build_col16(tbtkx_context *h);
 
 // Slot order of a row in the ELL copies of H: the diagonal entry first (k =
 // -1 when the row has none: a zero value in column r), then the other

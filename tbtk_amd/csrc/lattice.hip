@@ -431,7 +431,7 @@ int tbtkx_set_lattice(tbtkx_handle h, const tbtkx_lattice *lat, const double *on
 	h->csr_gen++;
 	h->h_real = real;
 	h->nblk = (int)blocks.size() - 1;
-	return TBTKX_OK;
+	return build_col16(h);
 }
 
 int tbtkx_lattice_gershgorin(tbtkx_handle h, const tbtkx_lattice *lat, const double *onsite,

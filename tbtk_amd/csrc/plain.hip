@@ -12,7 +12,9 @@
 // column), 4 B row pointer, 16 B read of x, 16 B read and 16 B write of the
 // in-place j_{n-2} -> j_n row: 20 Z + 52 N bytes. For a real H (and the real
 // unit start vector) the recurrence runs on real FP64 values and vectors:
-// 12 Z + 28 N bytes.
+// 12 Z + 28 N bytes. With 16-bit columns (offsets from the first row of
+// the row block, used when every offset fits) a non-zero costs 2 bytes less:
+// 18 Z + 52 N complex, 10 Z + 28 N real.
 #include <cstring>
 #include <type_traits>
 
@@ -58,13 +60,37 @@ template <> struct F64<double> {
 	__device__ static double2 moment(double x) { return make_double2(x, 0.0); }
 };
 
+// Column of non-zero k in a row block starting at row r0: the 32-bit
+// column itself, or a 16-bit offset from r0.
+__device__ __forceinline__ uint32_t col_at(const uint32_t *col, uint32_t k, uint32_t) {
+	return col[k];
+}
+__device__ __forceinline__ uint32_t col_at(const int16_t *col, uint32_t k, uint32_t r0) {
+	return (uint32_t)((int32_t)r0 + (int32_t)col[k]);
+}
+
+// 16-bit column offsets: workgroup b writes col[k] - r0 for the non-zeros of
+// row block b and raises the flag if one does not fit in int16.
+__global__ __launch_bounds__(kThreads) void make_col16(
+    const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ col,
+    const uint32_t *__restrict__ rblk, int16_t *__restrict__ col16, int *overflow) {
+	const uint32_t r0 = rblk[blockIdx.x], r1 = rblk[blockIdx.x + 1];
+	const uint32_t k0 = rowptr[r0], k1 = rowptr[r1];
+	for (uint32_t k = k0 + threadIdx.x; k < k1; k += kThreads) {
+		const int64_t d = (int64_t)col[k] - (int64_t)r0;
+		const bool fits = d >= -32768 && d <= 32767;
+		col16[k] = fits ? (int16_t)d : (int16_t)0;
+		if (!fits) atomicOr(overflow, 1);
+	}
+}
+
 // Row-block kernel: the workgroup streams its block's non-zeros with
 // consecutive threads on consecutive non-zeros (coalesced value/column loads
 // and gathers of x), forms the products in registers, stages them in LDS and
 // then lets one thread per row add them up in column order.
-template <typename S, bool FIRST>
+template <typename S, bool FIRST, typename CI>
 __global__ __launch_bounds__(kThreads) void plain_step_lds(
-    const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ col,
+    const uint32_t *__restrict__ rowptr, const CI *__restrict__ col,
     const S *__restrict__ v, const S *__restrict__ x,
     S *y, const uint32_t *__restrict__ rblk, int nblk, int remap,
     const int64_t *__restrict__ to, int K, double2 *__restrict__ coef,
@@ -85,7 +111,7 @@ __global__ __launch_bounds__(kThreads) void plain_step_lds(
 
 	if (k1 - k0 <= (uint32_t)kPlainCap) {
 		for (uint32_t k = k0 + tid; k < k1; k += kThreads)
-			prod[k - k0] = A::mul(v[k], x[col[k]]);
+			prod[k - k0] = A::mul(v[k], x[col_at(col, k, r0)]);
 		__syncthreads();
 		const uint32_t row = r0 + tid;
 		if (row < r1) {
@@ -104,7 +130,7 @@ __global__ __launch_bounds__(kThreads) void plain_step_lds(
 	for (uint32_t base = k0; base < k1; base += kPlainCap) {
 		const uint32_t end = base + kPlainCap < k1 ? base + kPlainCap : k1;
 		for (uint32_t k = base + tid; k < end; k += kThreads)
-			prod[k - base] = A::mul(v[k], x[col[k]]);
+			prod[k - base] = A::mul(v[k], x[col_at(col, k, r0)]);
 		__syncthreads();
 		if (tid == 0)
 			for (uint32_t k = 0; k < end - base; k++) acc = A::add(acc, prod[k]);
@@ -151,6 +177,20 @@ inline unsigned grid_for(int64_t n) {
 	return (unsigned)((n + kThreads - 1) / kThreads);
 }
 
+template <typename S, typename CI>
+void launch_lds(tbtkx_context *h, bool first, const CI *col, const S *v, const S *x, S *y,
+                const int64_t *to, int K, double2 *coef, int M, int nprev) {
+	const unsigned grid = (unsigned)h->nblk;
+	if (first)
+		plain_step_lds<S, true, CI><<<grid, kThreads, 0, h->stream>>>(
+		    h->rowptr.as<uint32_t>(), col, v, x, y, h->rblk.as<uint32_t>(), h->nblk,
+		    h->xcd_remap, to, K, coef, M, nprev);
+	else
+		plain_step_lds<S, false, CI><<<grid, kThreads, 0, h->stream>>>(
+		    h->rowptr.as<uint32_t>(), col, v, x, y, h->rblk.as<uint32_t>(), h->nblk,
+		    h->xcd_remap, to, K, coef, M, nprev);
+}
+
 // Enqueue the step launches n = 1 .. M-1 and the final gather; S = double2
 // (complex) or double (real H and start vector: v1r/v2r, vectors of N
 // doubles in the same buffers).
@@ -165,6 +205,7 @@ int enqueue_steps(tbtkx_context *h, int K, int M) {
 	const uint32_t *rp = h->rowptr.as<uint32_t>(), *cl = h->col.as<uint32_t>();
 	const int remap = h->xcd_remap;
 	const bool lds = h->plain_kernel == 0;
+	const bool c16 = lds && h->use_col16 && h->col16_ok;
 	const int nwg_row = (int)grid_for(h->N);
 	const unsigned grid = lds ? (unsigned)h->nblk : (unsigned)nwg_row;
 	// The moment gather rides on the step launches; their grid covers
@@ -174,14 +215,11 @@ int enqueue_steps(tbtkx_context *h, int K, int M) {
 		S *y = (n & 1) ? b : a;
 		const S *v = n == 1 ? v1 : v2;
 		if (lds) {
-			if (n == 1)
-				plain_step_lds<S, true><<<grid, kThreads, 0, h->stream>>>(
-				    rp, cl, v, x, y, h->rblk.as<uint32_t>(), h->nblk,
-				    remap, to, K, coef, M, n - 1);
+			if (c16)
+				launch_lds<S, int16_t>(h, n == 1, h->col16.as<int16_t>(), v, x, y, to, K,
+				                       coef, M, n - 1);
 			else
-				plain_step_lds<S, false><<<grid, kThreads, 0, h->stream>>>(
-				    rp, cl, v, x, y, h->rblk.as<uint32_t>(), h->nblk,
-				    remap, to, K, coef, M, n - 1);
+				launch_lds<S, uint32_t>(h, n == 1, cl, v, x, y, to, K, coef, M, n - 1);
 		} else {
 			if (n == 1)
 				plain_step_row<S, true><<<grid, kThreads, 0, h->stream>>>(
@@ -200,6 +238,26 @@ int enqueue_steps(tbtkx_context *h, int K, int M) {
 }
 
 }  // namespace
+
+int build_col16(tbtkx_context *h) {
+	h->col16_ok = 0;
+	if (h->nblk <= 0) return TBTKX_OK;
+	int rc;
+	if ((rc = h->col16.reserve((size_t)(h->Z + 64) * 2))) return rc;
+	if ((rc = h->col16_flag.reserve(4))) return rc;
+	TBTKX_CHECK_HIP(hipMemsetAsync(h->col16_flag.ptr, 0, 4, h->stream));
+	make_col16<<<(unsigned)h->nblk, kThreads, 0, h->stream>>>(
+	    h->rowptr.as<uint32_t>(), h->col.as<uint32_t>(), h->rblk.as<uint32_t>(),
+	    h->col16.as<int16_t>(), h->col16_flag.as<int>());
+	TBTKX_CHECK_HIP(hipGetLastError());
+	int overflow = 1;
+	TBTKX_CHECK_HIP(hipMemcpyAsync(&overflow, h->col16_flag.ptr, 4, hipMemcpyDeviceToHost,
+	                               h->stream));
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	h->col16_ok = overflow == 0;
+	return TBTKX_OK;
+}
+
 }  // namespace tbtkx
 
 using namespace tbtkx;
@@ -242,6 +300,7 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 	// Real arithmetic when every value of H is real (the start vector is a
 	// unit vector).
 	const bool real = h->real_path && h->h_real;
+	const bool c16 = h->plain_kernel == 0 && h->use_col16 && h->col16_ok;
 	if (real)
 		init_unit<double><<<grid_for(h->N), kThreads, 0, h->stream>>>(
 		    h->vec_a.as<double>(), h->N, from);
@@ -256,9 +315,9 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 		GraphCache &gc = h->plain_graph;
 		int64_t key[GraphCache::kKey];
 		const bool key_ok = GraphCache::make_key(
-		    key, {U, M, h->plain_kernel, h->xcd_remap, h->N, real},
+		    key, {U, M, h->plain_kernel, h->xcd_remap, h->N, real, c16},
 		    {h->vec_a.ptr, h->vec_b.ptr, h->coef.ptr, h->to.ptr, h->rowptr.ptr, h->col.ptr,
-		     h->v1.ptr, h->v2.ptr, h->v1r.ptr, h->v2r.ptr, h->rblk.ptr});
+		     h->v1.ptr, h->v2.ptr, h->v1r.ptr, h->v2r.ptr, h->rblk.ptr, h->col16.ptr});
 		TBTKX_REQUIRE(key_ok, TBTKX_ERR_STATE, "tbtkx_moments: graph key too long");
 		if (!gc.matches(key)) {
 			gc.reset();
@@ -287,6 +346,7 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 	h->last_ms = ms;
 	h->last_launches = M > 1 ? M - 1 : 0;
 	h->last_real = real;
+	h->last_col16 = c16;
 
 	for (int k = 0; k < K; k++) {
 		double *o = out + (size_t)k * M * 2;
