@@ -27,7 +27,7 @@ error against the FP64 path.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
                        [--parts plain,batched,uneven,dos,mfma,cpu]
-                       [--skip-cpu-baseline] [--calibrate-copy] [--complex]
+                       [--skip-cpu-baseline] [--calibrate-copy] [--complex] [--palette]
                        [--no-graph]
 """
 from __future__ import annotations
@@ -179,6 +179,7 @@ def batched_comparison(eng):
         plain_ms += eng.last_timing()[0]
         ref.append(c[0])
     plain_cb = 2 if eng.get_option(X.OPT_COL16) == 1 else 4
+    plain_pal = eng.get_option(X.OPT_PALETTE) == 1
     ld_plain = contract(np.array(ref))
     t_plain = (time.perf_counter() - t0) / PLAIN_CMP_SITES
 
@@ -213,7 +214,8 @@ def batched_comparison(eng):
         "spmm_GBps": step_bytes / per_step / 1e9,
         "spmm_hbm_frac": step_bytes / per_step / 1e9 / HBM_PEAK_GBPS,
         "plain_arithmetic": "real fp64" if plain_real else "complex fp64",
-        "plain_spmv_GBps": (((8 + plain_cb) * Z + 28 * N) if plain_real
+        "plain_spmv_GBps": ((3 * Z + (36 if plain_real else 68) * N) if plain_pal
+                            else ((8 + plain_cb) * Z + 28 * N) if plain_real
                             else ((16 + plain_cb) * Z + 52 * N))
         / (plain_ms * 1e-3 / (PLAIN_CMP_SITES * (M_BATCH - 1))) / 1e9,
     }
@@ -455,6 +457,8 @@ def main():
                          "whose known bytes scale the rocprofv3 memory counters")
     ap.add_argument("--no-graph", action="store_true",
                     help="uneven part: time a direct run instead of the hipGraph replay")
+    ap.add_argument("--palette", action="store_true",
+                    help="plain step with the value palette (off by default)")
     ap.add_argument("--complex", action="store_true",
                     help="run every recurrence in complex arithmetic, also for a real H "
                          "(TBTKX_OPT_REAL = 0), to measure the complex kernels")
@@ -489,6 +493,8 @@ def main():
     eng = X.Engine(0)
     if args.complex:
         eng.set_option(X.OPT_REAL, 0)
+    if args.palette:
+        eng.set_option(X.OPT_PALETTE, 1)
     model = None
     if "plain" in parts or "cpu" in parts:
         log("building 1024x1024 spin lattice")
@@ -518,8 +524,14 @@ def main():
         # 16-bit column offsets (every column within int16 of its row
         # block's first row): 2 bytes per non-zero instead of 4.
         col16 = eng.get_option(X.OPT_COL16) == 1
+        # Value palette (off-diagonal values as 1-byte codes, the diagonal
+        # from a per-row array): 3 bytes per non-zero, 8 or 16 per row.
+        palette = eng.get_option(X.OPT_PALETTE) == 1
         cb = 2 if col16 else 4
-        step_bytes = (8 + cb) * Z + 28 * N if real else (16 + cb) * Z + 52 * N
+        if palette:
+            step_bytes = 3 * Z + 36 * N if real else 3 * Z + 68 * N
+        else:
+            step_bytes = (8 + cb) * Z + 28 * N if real else (16 + cb) * Z + 52 * N
         avg_kernel_s = kernel_ms * 1e-3 / launches
         achieved = step_bytes / avg_kernel_s / 1e9
         dE = (WINDOW[1] - WINDOW[0]) / (E_PTS - 1)
@@ -549,14 +561,17 @@ def main():
                 "kernel": (("plain_step_lds<double> (real FP64 CSR SpMV recurrence step; "
                             "H is real)" if real else
                             "plain_step_lds<double2> (complex FP64 CSR SpMV recurrence step)")
-                           + ("; 16-bit column offsets" if col16 else "; 32-bit columns")),
+                           + ("; 16-bit column offsets" if col16 else "; 32-bit columns")
+                           + ("; 1-byte value codes (palette) + diagonal array"
+                              if palette else "")),
                 "bound": "hbm",
                 "achieved": achieved,
                 "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBPS,
                 "traffic": load_traffic(("plain_step_lds_real" if real else "plain_step_lds")
-                                        + ("_col16" if col16 else "")),
+                                        + ("_col16" if col16 else "")
+                                        + ("_palette" if palette else "")),
                 "algorithmic_bytes_per_launch": step_bytes,
                 "avg_launch_us": avg_kernel_s * 1e6,
             },

@@ -313,7 +313,18 @@ enum {
 	                           block when every offset fits (default): 2 bytes
 	                           instead of 4 per non-zero, same arithmetic, same
 	                           moments; 0 = 32-bit columns always. Reading it
-	                           gives 1 when the last plain call used them */
+	                           gives 1 when the last plain call used them */,
+	TBTKX_OPT_PALETTE = 16  /* 1 = with 16-bit columns, the plain FP64 step
+	                           reads each off-diagonal value as a 1-byte code
+	                           into a palette of the model's distinct values
+	                           and each diagonal value from a per-row array,
+	                           when the off-diagonal values take at most 255
+	                           distinct values: same values, same arithmetic,
+	                           same moments, a third less HBM traffic on a
+	                           square lattice but, measured, no faster (the
+	                           step turns latency-bound); 0 = full values
+	                           (default). Reading it gives 1 when the last
+	                           plain call used the palette */
 };
 int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
 

@@ -120,6 +120,11 @@ struct tbtkx_context {
 	// their row block (col16_ok when every offset fits in int16).
 	tbtkx::DevBuf col16, col16_flag;
 	int col16_ok = 0;
+	// Value palette of the plain kernel (palette_ok): one code per non-zero,
+	// 255 = the row's diagonal (pdiag: H/s then 2H/s, N each), else an index
+	// into pal (256 values of H/s, then 256 of 2H/s); palr/pdiagr: real parts.
+	tbtkx::DevBuf pcode, pal, palr, pdiag, pdiagr;
+	int palette_ok = 0;
 
 	// Plain path vectors and moment output.
 	tbtkx::DevBuf vec_a, vec_b, coef, to;
@@ -168,6 +173,8 @@ struct tbtkx_context {
 	int plain_kernel = 0;
 	int use_col16 = 1;
 	int last_col16 = 0;   // the last plain call used 16-bit columns
+	int use_palette = 0;
+	int last_palette = 0; // the last plain call used the value palette
 	int use_graph = 0;
 	int block_width = 0;   // 0 = autotune
 	int xcd_remap = 0;     // measured faster without for both step kernels
@@ -210,6 +217,12 @@ int apply_damping(tbtkx_context *h, double2 *C, int K, int M);
 // Builds the 16-bit column offsets of the resident CSR (plain.hip); sets
 // col16_ok. Needs rowptr, col, rblk and nblk in place.
 int build_col16(tbtkx_context *h);
+
+// Builds the plain kernel's value palette from host copies of the CSR
+// (v1 = H/s, v2 = 2H/s, interleaved complex); sets palette_ok when the
+// off-diagonal values take at most 255 distinct (H/s, 2H/s) pairs.
+int build_palette(tbtkx_context *h, int64_t N, int64_t Z, const uint32_t *rowptr,
+                  const uint32_t *col, const double *v1, const double *v2);
 
 // Slot order of a row in the ELL copies of H: the diagonal entry first (k =
 // -1 when the row has none: a zero value in column r), then the other

@@ -179,7 +179,8 @@ int tbtkx_destroy(tbtkx_handle h) {
 	h->blk_graph.reset();
 	for (DevBuf *b : {&h->rowptr, &h->col, &h->v1, &h->v2, &h->v1f, &h->v2f,
 	                  &h->v1r, &h->v2r, &h->lat_terms, &h->lat_onsite, &h->lat_count,
-	                  &h->lat_part, &h->lat_tmp, &h->rblk, &h->col16, &h->col16_flag, &h->vec_a, &h->vec_b, &h->coef, &h->to,
+	                  &h->lat_part, &h->lat_tmp, &h->rblk, &h->col16, &h->col16_flag,
+	                  &h->pcode, &h->pal, &h->palr, &h->pdiag, &h->pdiagr, &h->vec_a, &h->vec_b, &h->coef, &h->to,
 	                  &h->lut, &h->gf_in, &h->gf_out, &h->blk_x, &h->blk_y,
 	                  &h->blk_mu, &h->blk_sites, &h->blk_part, &h->cplanes,
 	                  &h->lplanes, &h->gf_mfma_out, &h->ell_col, &h->ell_v1f,
@@ -329,7 +330,8 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 	h->blk_graph.reset();
 	h->h_real = real;
 	h->nblk = (int)blocks.size() - 1;
-	return build_col16(h);
+	if ((rc = build_col16(h))) return rc;
+	return build_palette(h, N, Z, rowptr, col, v1.data(), v2.data());
 }
 
 int tbtkx_last_timing(tbtkx_handle h, double *ms, int64_t *launches) {
@@ -414,6 +416,12 @@ int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
 		h->use_col16 = (int)value;
 		h->plain_graph.reset();
 		return TBTKX_OK;
+	case TBTKX_OPT_PALETTE:
+		TBTKX_REQUIRE(value == 0 || value == 1, TBTKX_ERR_ARG,
+		              "tbtkx_set_option: palette option must be 0 or 1");
+		h->use_palette = (int)value;
+		h->plain_graph.reset();
+		return TBTKX_OK;
 	default:
 		set_error("tbtkx_set_option: unknown option " + std::to_string(option));
 		return TBTKX_ERR_ARG;
@@ -451,6 +459,9 @@ int tbtkx_get_option(tbtkx_handle h, int option, int64_t *value) {
 		return TBTKX_OK;
 	case TBTKX_OPT_COL16:
 		*value = h->last_col16;
+		return TBTKX_OK;
+	case TBTKX_OPT_PALETTE:
+		*value = h->last_palette;
 		return TBTKX_OK;
 	default:
 		set_error("tbtkx_get_option: unknown option " + std::to_string(option));

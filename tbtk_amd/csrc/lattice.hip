@@ -431,7 +431,17 @@ int tbtkx_set_lattice(tbtkx_handle h, const tbtkx_lattice *lat, const double *on
 	h->csr_gen++;
 	h->h_real = real;
 	h->nblk = (int)blocks.size() - 1;
-	return build_col16(h);
+	if ((rc = build_col16(h))) return rc;
+	// The value palette is built from host copies of the device CSR.
+	// (hrp, the row pointer, is already on the host.)
+	std::vector<uint32_t> hcol(Z > 0 ? Z : 1);
+	std::vector<double> hv1(2 * (Z > 0 ? Z : 1)), hv2(2 * (Z > 0 ? Z : 1));
+	if (Z > 0) {
+		TBTKX_CHECK_HIP(hipMemcpy(hcol.data(), h->col.ptr, Z * 4, hipMemcpyDeviceToHost));
+		TBTKX_CHECK_HIP(hipMemcpy(hv1.data(), h->v1.ptr, Z * 16, hipMemcpyDeviceToHost));
+		TBTKX_CHECK_HIP(hipMemcpy(hv2.data(), h->v2.ptr, Z * 16, hipMemcpyDeviceToHost));
+	}
+	return build_palette(h, N, Z, hrp.data(), hcol.data(), hv1.data(), hv2.data());
 }
 
 int tbtkx_lattice_gershgorin(tbtkx_handle h, const tbtkx_lattice *lat, const double *onsite,

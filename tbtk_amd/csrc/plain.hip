@@ -14,9 +14,13 @@
 // unit start vector) the recurrence runs on real FP64 values and vectors:
 // 12 Z + 28 N bytes. With 16-bit columns (offsets from the first row of
 // the row block, used when every offset fits) a non-zero costs 2 bytes less:
-// 18 Z + 52 N complex, 10 Z + 28 N real.
+// 18 Z + 52 N complex, 10 Z + 28 N real. With the value palette as well
+// (a 1-byte code per non-zero: an index into at most 255 distinct
+// off-diagonal values, or 255 for the row's diagonal, read from a per-row
+// array): 3 Z + 68 N complex, 3 Z + 36 N real.
 #include <cstring>
 #include <type_traits>
+#include <unordered_map>
 
 #include "common.hpp"
 
@@ -88,17 +92,28 @@ __global__ __launch_bounds__(kThreads) void make_col16(
 // consecutive threads on consecutive non-zeros (coalesced value/column loads
 // and gathers of x), forms the products in registers, stages them in LDS and
 // then lets one thread per row add them up in column order.
-template <typename S, bool FIRST, typename CI>
+// PAL: values from the palette (code < 255) or the per-row diagonal array
+// (code 255, where the column is the row itself); else from v.
+template <typename S, bool FIRST, typename CI, bool PAL>
 __global__ __launch_bounds__(kThreads) void plain_step_lds(
     const uint32_t *__restrict__ rowptr, const CI *__restrict__ col,
     const S *__restrict__ v, const S *__restrict__ x,
     S *y, const uint32_t *__restrict__ rblk, int nblk, int remap,
     const int64_t *__restrict__ to, int K, double2 *__restrict__ coef,
-    int M, int nprev) {
+    int M, int nprev, const uint8_t *__restrict__ code, const S *__restrict__ pal,
+    const S *__restrict__ diag) {
 #pragma clang fp contract(off)
 	typedef F64<S> A;
 	__shared__ S prod[kPlainCap];
 	const int tid = threadIdx.x;
+	// The palette (at most 255 values) is read from global memory, where it
+	// stays cache-resident; staging it in LDS would put a load and a barrier
+	// in front of every workgroup's first product.
+	auto value = [&](uint32_t k, uint32_t c) -> S {
+		if (!PAL) return v[k];
+		const uint32_t q = code[k];
+		return q == 255 ? diag[c] : pal[q];
+	};
 
 	// Moments of the previous vector (x = j_{nprev}); x is not written in
 	// this launch, so any workgroup may read it.
@@ -110,8 +125,10 @@ __global__ __launch_bounds__(kThreads) void plain_step_lds(
 	const uint32_t k0 = rowptr[r0], k1 = rowptr[r1];
 
 	if (k1 - k0 <= (uint32_t)kPlainCap) {
-		for (uint32_t k = k0 + tid; k < k1; k += kThreads)
-			prod[k - k0] = A::mul(v[k], x[col_at(col, k, r0)]);
+		for (uint32_t k = k0 + tid; k < k1; k += kThreads) {
+			const uint32_t c = col_at(col, k, r0);
+			prod[k - k0] = A::mul(value(k, c), x[c]);
+		}
 		__syncthreads();
 		const uint32_t row = r0 + tid;
 		if (row < r1) {
@@ -129,8 +146,10 @@ __global__ __launch_bounds__(kThreads) void plain_step_lds(
 	if (tid == 0 && !FIRST) acc = A::neg(y[r0]);
 	for (uint32_t base = k0; base < k1; base += kPlainCap) {
 		const uint32_t end = base + kPlainCap < k1 ? base + kPlainCap : k1;
-		for (uint32_t k = base + tid; k < end; k += kThreads)
-			prod[k - base] = A::mul(v[k], x[col_at(col, k, r0)]);
+		for (uint32_t k = base + tid; k < end; k += kThreads) {
+			const uint32_t c = col_at(col, k, r0);
+			prod[k - base] = A::mul(value(k, c), x[c]);
+		}
 		__syncthreads();
 		if (tid == 0)
 			for (uint32_t k = 0; k < end - base; k++) acc = A::add(acc, prod[k]);
@@ -177,18 +196,20 @@ inline unsigned grid_for(int64_t n) {
 	return (unsigned)((n + kThreads - 1) / kThreads);
 }
 
-template <typename S, typename CI>
+template <typename S, typename CI, bool PAL>
 void launch_lds(tbtkx_context *h, bool first, const CI *col, const S *v, const S *x, S *y,
-                const int64_t *to, int K, double2 *coef, int M, int nprev) {
+                const int64_t *to, int K, double2 *coef, int M, int nprev,
+                const S *pal, const S *diag) {
 	const unsigned grid = (unsigned)h->nblk;
+	const uint8_t *code = PAL ? h->pcode.as<uint8_t>() : nullptr;
 	if (first)
-		plain_step_lds<S, true, CI><<<grid, kThreads, 0, h->stream>>>(
+		plain_step_lds<S, true, CI, PAL><<<grid, kThreads, 0, h->stream>>>(
 		    h->rowptr.as<uint32_t>(), col, v, x, y, h->rblk.as<uint32_t>(), h->nblk,
-		    h->xcd_remap, to, K, coef, M, nprev);
+		    h->xcd_remap, to, K, coef, M, nprev, code, pal, diag);
 	else
-		plain_step_lds<S, false, CI><<<grid, kThreads, 0, h->stream>>>(
+		plain_step_lds<S, false, CI, PAL><<<grid, kThreads, 0, h->stream>>>(
 		    h->rowptr.as<uint32_t>(), col, v, x, y, h->rblk.as<uint32_t>(), h->nblk,
-		    h->xcd_remap, to, K, coef, M, nprev);
+		    h->xcd_remap, to, K, coef, M, nprev, code, pal, diag);
 }
 
 // Enqueue the step launches n = 1 .. M-1 and the final gather; S = double2
@@ -206,6 +227,9 @@ int enqueue_steps(tbtkx_context *h, int K, int M) {
 	const int remap = h->xcd_remap;
 	const bool lds = h->plain_kernel == 0;
 	const bool c16 = lds && h->use_col16 && h->col16_ok;
+	const bool usepal = c16 && h->use_palette && h->palette_ok;
+	const S *palp = real ? h->palr.as<S>() : h->pal.as<S>();
+	const S *diagp = real ? h->pdiagr.as<S>() : h->pdiag.as<S>();
 	const int nwg_row = (int)grid_for(h->N);
 	const unsigned grid = lds ? (unsigned)h->nblk : (unsigned)nwg_row;
 	// The moment gather rides on the step launches; their grid covers
@@ -215,11 +239,16 @@ int enqueue_steps(tbtkx_context *h, int K, int M) {
 		S *y = (n & 1) ? b : a;
 		const S *v = n == 1 ? v1 : v2;
 		if (lds) {
-			if (c16)
-				launch_lds<S, int16_t>(h, n == 1, h->col16.as<int16_t>(), v, x, y, to, K,
-				                       coef, M, n - 1);
+			if (usepal)
+				launch_lds<S, int16_t, true>(h, n == 1, h->col16.as<int16_t>(), v, x, y, to,
+				                             K, coef, M, n - 1, palp + (n == 1 ? 0 : 256),
+				                             diagp + (n == 1 ? 0 : h->N));
+			else if (c16)
+				launch_lds<S, int16_t, false>(h, n == 1, h->col16.as<int16_t>(), v, x, y, to,
+				                              K, coef, M, n - 1, nullptr, nullptr);
 			else
-				launch_lds<S, uint32_t>(h, n == 1, cl, v, x, y, to, K, coef, M, n - 1);
+				launch_lds<S, uint32_t, false>(h, n == 1, cl, v, x, y, to, K, coef, M, n - 1,
+				                               nullptr, nullptr);
 		} else {
 			if (n == 1)
 				plain_step_row<S, true><<<grid, kThreads, 0, h->stream>>>(
@@ -255,6 +284,74 @@ int build_col16(tbtkx_context *h) {
 	                               h->stream));
 	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
 	h->col16_ok = overflow == 0;
+	return TBTKX_OK;
+}
+
+int build_palette(tbtkx_context *h, int64_t N, int64_t Z, const uint32_t *rowptr,
+                  const uint32_t *col, const double *v1, const double *v2) {
+	h->palette_ok = 0;
+	struct Key {
+		uint64_t w[4];
+		bool operator==(const Key &o) const {
+			return w[0] == o.w[0] && w[1] == o.w[1] && w[2] == o.w[2] && w[3] == o.w[3];
+		}
+	};
+	struct KeyHash {
+		size_t operator()(const Key &k) const {
+			uint64_t x = k.w[0] * 0x9E3779B97F4A7C15ull;
+			for (int i = 1; i < 4; i++) x = (x ^ (x >> 29)) * 0xBF58476D1CE4E5B9ull + k.w[i];
+			return (size_t)(x ^ (x >> 32));
+		}
+	};
+	std::unordered_map<Key, int, KeyHash> index;
+	std::vector<uint8_t> code((size_t)Z + kCsrPad, 0);
+	// pal: [H/s][256] then [2H/s][256] complex; diag: [H/s][N] then [2H/s][N]
+	std::vector<double> pal(2 * 256 * 2, 0.0), diag((size_t)2 * N * 2, 0.0);
+	for (int64_t r = 0; r < N; r++) {
+		for (uint32_t k = rowptr[r]; k < rowptr[r + 1]; k++) {
+			if ((int64_t)col[k] == r) {
+				code[k] = 255;
+				diag[2 * r] = v1[2 * k];
+				diag[2 * r + 1] = v1[2 * k + 1];
+				diag[2 * (N + r)] = v2[2 * k];
+				diag[2 * (N + r) + 1] = v2[2 * k + 1];
+				continue;
+			}
+			Key key;
+			memcpy(&key.w[0], &v1[2 * k], 16);
+			memcpy(&key.w[2], &v2[2 * k], 16);
+			auto it = index.find(key);
+			int q;
+			if (it != index.end()) {
+				q = it->second;
+			} else {
+				if (index.size() >= 255) return TBTKX_OK;   // too many values
+				q = (int)index.size();
+				index.emplace(key, q);
+				pal[2 * q] = v1[2 * k];
+				pal[2 * q + 1] = v1[2 * k + 1];
+				pal[2 * (256 + q)] = v2[2 * k];
+				pal[2 * (256 + q) + 1] = v2[2 * k + 1];
+			}
+			code[k] = (uint8_t)q;
+		}
+	}
+	std::vector<double> palr(2 * 256), diagr((size_t)2 * N);
+	for (size_t i = 0; i < palr.size(); i++) palr[i] = pal[2 * i];
+	for (size_t i = 0; i < diagr.size(); i++) diagr[i] = diag[2 * i];
+	int rc;
+	if ((rc = h->pcode.reserve(code.size()))) return rc;
+	if ((rc = h->pal.reserve(pal.size() * 8))) return rc;
+	if ((rc = h->palr.reserve(palr.size() * 8))) return rc;
+	if ((rc = h->pdiag.reserve(diag.size() * 8))) return rc;
+	if ((rc = h->pdiagr.reserve(diagr.size() * 8))) return rc;
+	TBTKX_CHECK_HIP(hipMemcpy(h->pcode.ptr, code.data(), code.size(), hipMemcpyHostToDevice));
+	TBTKX_CHECK_HIP(hipMemcpy(h->pal.ptr, pal.data(), pal.size() * 8, hipMemcpyHostToDevice));
+	TBTKX_CHECK_HIP(hipMemcpy(h->palr.ptr, palr.data(), palr.size() * 8, hipMemcpyHostToDevice));
+	TBTKX_CHECK_HIP(hipMemcpy(h->pdiag.ptr, diag.data(), diag.size() * 8, hipMemcpyHostToDevice));
+	TBTKX_CHECK_HIP(hipMemcpy(h->pdiagr.ptr, diagr.data(), diagr.size() * 8,
+	                          hipMemcpyHostToDevice));
+	h->palette_ok = 1;
 	return TBTKX_OK;
 }
 
@@ -301,6 +398,7 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 	// unit vector).
 	const bool real = h->real_path && h->h_real;
 	const bool c16 = h->plain_kernel == 0 && h->use_col16 && h->col16_ok;
+	const bool usepal = c16 && h->use_palette && h->palette_ok;
 	if (real)
 		init_unit<double><<<grid_for(h->N), kThreads, 0, h->stream>>>(
 		    h->vec_a.as<double>(), h->N, from);
@@ -315,9 +413,10 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 		GraphCache &gc = h->plain_graph;
 		int64_t key[GraphCache::kKey];
 		const bool key_ok = GraphCache::make_key(
-		    key, {U, M, h->plain_kernel, h->xcd_remap, h->N, real, c16},
+		    key, {U, M, h->plain_kernel, h->xcd_remap, h->N, real, c16, usepal},
 		    {h->vec_a.ptr, h->vec_b.ptr, h->coef.ptr, h->to.ptr, h->rowptr.ptr, h->col.ptr,
-		     h->v1.ptr, h->v2.ptr, h->v1r.ptr, h->v2r.ptr, h->rblk.ptr, h->col16.ptr});
+		     h->v1.ptr, h->v2.ptr, h->v1r.ptr, h->v2r.ptr, h->rblk.ptr, h->col16.ptr,
+		     h->pcode.ptr, h->pal.ptr, h->palr.ptr, h->pdiag.ptr, h->pdiagr.ptr});
 		TBTKX_REQUIRE(key_ok, TBTKX_ERR_STATE, "tbtkx_moments: graph key too long");
 		if (!gc.matches(key)) {
 			gc.reset();
@@ -347,6 +446,7 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 	h->last_launches = M > 1 ? M - 1 : 0;
 	h->last_real = real;
 	h->last_col16 = c16;
+	h->last_palette = usepal;
 
 	for (int k = 0; k < K; k++) {
 		double *o = out + (size_t)k * M * 2;

@@ -20,7 +20,7 @@ H_FP64, DOUBLING, VEC_FP64 = 1, 2, 4
 DAMP_NONE, DAMP_LORENTZ, DAMP_JACKSON = 0, 1, 2
 (OPT_PLAIN_KERNEL, OPT_GRAPH, OPT_BLOCK_WIDTH, OPT_XCD_REMAP, OPT_TILE_ROWS, OPT_NT_STREAM,
  OPT_ELL, OPT_AUTOTUNE, OPT_MFMA_PIPE, OPT_STRIP_ROWS, OPT_GRID_SHARE,
- OPT_STRIP_CHUNK, OPT_ROWS_IN_FLIGHT, OPT_REAL, OPT_COL16) = range(1, 16)
+ OPT_STRIP_CHUNK, OPT_ROWS_IN_FLIGHT, OPT_REAL, OPT_COL16, OPT_PALETTE) = range(1, 17)
 
 # (name, argtypes) of every exported function, in header order.
 _P = ctypes.c_void_p

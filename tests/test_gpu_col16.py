@@ -1,9 +1,13 @@
-"""GPU tests of the plain FP64 step with 16-bit column offsets
-(TBTKX_OPT_COL16): the step reads each column as an offset from the first
-row of its row block when every offset of the model fits in int16, else the
-32-bit columns. Same products in the same order either way, so the moments
-are bitwise equal to the 32-bit path's and to the oracle's (the reference's
-CPU recurrence, Lib/src/Solver/ChebyshevExpander.cpp:65-82, 157-186)."""
+"""GPU tests of the plain FP64 step's compressed CSR forms: 16-bit column
+offsets (TBTKX_OPT_COL16: each column as an offset from the first row of its
+row block, when every offset of the model fits in int16) and the value
+palette (TBTKX_OPT_PALETTE, off by default: a 1-byte code per non-zero into
+the model's at most 255 distinct off-diagonal values, the diagonal from a
+per-row array).
+The step forms the same products from the same values in the same order in
+every form, so the moments are bitwise equal to the 32-bit, full-value
+path's and to the oracle's (the reference's CPU recurrence,
+Lib/src/Solver/ChebyshevExpander.cpp:65-82, 157-186)."""
 import numpy as np
 import pytest
 
@@ -18,6 +22,7 @@ pytestmark = pytest.mark.gpu
 def col16(engine):
     yield engine
     engine.set_option(X.OPT_COL16, 1)
+    engine.set_option(X.OPT_PALETTE, 0)
     engine.set_option(X.OPT_GRAPH, 0)
 
 
@@ -87,3 +92,66 @@ def test_col16_graph_replay_switches_format(col16, oracle_mod):
         got = col16.moments(77, [77, 3], 129)
         assert col16.get_option(X.OPT_COL16) == use
         assert np.array_equal(got, ref), use
+
+
+@pytest.mark.parametrize("name", sorted(MODELS))
+def test_palette_bitwise_equal_to_full_values_and_oracle(col16, oracle_mod, name):
+    m = MODELS[name]()
+    csr = m.csr()
+    s = kpm.gershgorin_scale(csr, 1.05)
+    col16.set_csr(csr, s)
+    frm = m.basis_size // 3
+    to = [frm, 1, m.basis_size - 2]
+    M = 257
+    ref = oracle_mod.moments(csr, s, frm, to, M)
+    out = {}
+    for pal in (1, 0):
+        col16.set_option(X.OPT_PALETTE, pal)
+        out[pal] = col16.moments(frm, to, M)
+        assert col16.get_option(X.OPT_PALETTE) == pal
+        assert col16.get_option(X.OPT_COL16) == 1
+    assert np.array_equal(out[1], out[0])
+    assert np.array_equal(out[1], ref)
+
+
+def random_chain(n, seed):
+    """Open chain with a different random hopping on every bond (more than
+    255 distinct off-diagonal values) and random on-site energies."""
+    rng = np.random.default_rng(seed)
+    t = rng.uniform(0.5, 1.5, n - 1)
+    eps = rng.uniform(-1, 1, n)
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j, v in ((i - 1, t[i - 1] if i > 0 else None), (i, eps[i]),
+                     (i + 1, t[i] if i < n - 1 else None)):
+            if v is not None:
+                rows.append(i)
+                cols.append(j)
+                vals.append(v)
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    np.add.at(rp, np.array(rows) + 1, 1)
+    return (np.cumsum(rp).astype(np.uint32), np.array(cols, dtype=np.uint32),
+            np.array(vals, dtype=np.complex128))
+
+
+def test_palette_falls_back_on_many_values(col16, oracle_mod):
+    csr = random_chain(600, 11)
+    col16.set_option(X.OPT_PALETTE, 1)
+    col16.set_csr(csr, 4.0)
+    got = col16.moments(300, [300, 0, 599], 120)
+    assert col16.get_option(X.OPT_COL16) == 1
+    assert col16.get_option(X.OPT_PALETTE) == 0
+    assert np.array_equal(got, oracle_mod.moments(csr, 4.0, 300, [300, 0, 599], 120))
+
+
+def test_palette_device_built_lattice(col16):
+    m = L.square_spin(24, 20, disorder=1.0, seed=6)
+    spec = L.square_spin_spec(24, 20, disorder=1.0, seed=6)
+    s = kpm.gershgorin_scale(m.csr(), 1.05)
+    col16.set_option(X.OPT_PALETTE, 1)
+    col16.set_csr(m.csr(), s)
+    ref = col16.moments(480, [480, 7], 300)
+    col16.set_lattice(spec, s)
+    got = col16.moments(480, [480, 7], 300)
+    assert col16.get_option(X.OPT_PALETTE) == 1
+    assert np.array_equal(got, ref)
