@@ -27,7 +27,7 @@ error against the FP64 path.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
                        [--parts plain,batched,uneven,dos,mfma,cpu]
-                       [--skip-cpu-baseline] [--calibrate-copy] [--complex] [--palette]
+                       [--skip-cpu-baseline] [--calibrate-copy] [--complex] [--no-palette]
                        [--no-graph]
 """
 from __future__ import annotations
@@ -457,8 +457,8 @@ def main():
                          "whose known bytes scale the rocprofv3 memory counters")
     ap.add_argument("--no-graph", action="store_true",
                     help="uneven part: time a direct run instead of the hipGraph replay")
-    ap.add_argument("--palette", action="store_true",
-                    help="plain step with the value palette (off by default)")
+    ap.add_argument("--no-palette", action="store_true",
+                    help="plain step with full values instead of the value palette")
     ap.add_argument("--complex", action="store_true",
                     help="run every recurrence in complex arithmetic, also for a real H "
                          "(TBTKX_OPT_REAL = 0), to measure the complex kernels")
@@ -493,8 +493,8 @@ def main():
     eng = X.Engine(0)
     if args.complex:
         eng.set_option(X.OPT_REAL, 0)
-    if args.palette:
-        eng.set_option(X.OPT_PALETTE, 1)
+    if args.no_palette:
+        eng.set_option(X.OPT_PALETTE, 0)
     model = None
     if "plain" in parts or "cpu" in parts:
         log("building 1024x1024 spin lattice")

@@ -319,12 +319,11 @@ enum {
 	                           into a palette of the model's distinct values
 	                           and each diagonal value from a per-row array,
 	                           when the off-diagonal values take at most 255
-	                           distinct values: same values, same arithmetic,
-	                           same moments, a third less HBM traffic on a
-	                           square lattice but, measured, no faster (the
-	                           step turns latency-bound); 0 = full values
-	                           (default). Reading it gives 1 when the last
-	                           plain call used the palette */
+	                           distinct values (default): same values, same
+	                           arithmetic, same moments, a third less HBM
+	                           traffic on a square lattice; 0 = full values.
+	                           Reading it gives 1 when the last plain call
+	                           used the palette */
 };
 int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
 

@@ -173,7 +173,7 @@ struct tbtkx_context {
 	int plain_kernel = 0;
 	int use_col16 = 1;
 	int last_col16 = 0;   // the last plain call used 16-bit columns
-	int use_palette = 0;
+	int use_palette = 1;
 	int last_palette = 0; // the last plain call used the value palette
 	int use_graph = 0;
 	int block_width = 0;   // 0 = autotune

@@ -125,7 +125,22 @@ __global__ __launch_bounds__(kThreads) void plain_step_lds(
 	const uint32_t k0 = rowptr[r0], k1 = rowptr[r1];
 
 	if (k1 - k0 <= (uint32_t)kPlainCap) {
-		for (uint32_t k = k0 + tid; k < k1; k += kThreads) {
+		// Four non-zeros per iteration, their loads issued together.
+		uint32_t k = k0 + tid;
+		for (; k + 3 * kThreads < k1; k += 4 * kThreads) {
+			uint32_t c[4];
+			S vv[4], xx[4];
+#pragma unroll
+			for (int i = 0; i < 4; i++) c[i] = col_at(col, k + i * kThreads, r0);
+#pragma unroll
+			for (int i = 0; i < 4; i++) {
+				vv[i] = value(k + i * kThreads, c[i]);
+				xx[i] = x[c[i]];
+			}
+#pragma unroll
+			for (int i = 0; i < 4; i++) prod[k + i * kThreads - k0] = A::mul(vv[i], xx[i]);
+		}
+		for (; k < k1; k += kThreads) {
 			const uint32_t c = col_at(col, k, r0);
 			prod[k - k0] = A::mul(value(k, c), x[c]);
 		}

@@ -1,9 +1,8 @@
 """GPU tests of the plain FP64 step's compressed CSR forms: 16-bit column
 offsets (TBTKX_OPT_COL16: each column as an offset from the first row of its
 row block, when every offset of the model fits in int16) and the value
-palette (TBTKX_OPT_PALETTE, off by default: a 1-byte code per non-zero into
-the model's at most 255 distinct off-diagonal values, the diagonal from a
-per-row array).
+palette (TBTKX_OPT_PALETTE: a 1-byte code per non-zero into the model's at
+most 255 distinct off-diagonal values, the diagonal from a per-row array).
 The step forms the same products from the same values in the same order in
 every form, so the moments are bitwise equal to the 32-bit, full-value
 path's and to the oracle's (the reference's CPU recurrence,
@@ -22,7 +21,7 @@ pytestmark = pytest.mark.gpu
 def col16(engine):
     yield engine
     engine.set_option(X.OPT_COL16, 1)
-    engine.set_option(X.OPT_PALETTE, 0)
+    engine.set_option(X.OPT_PALETTE, 1)
     engine.set_option(X.OPT_GRAPH, 0)
 
 
