@@ -125,24 +125,51 @@ __global__ __launch_bounds__(kThreads) void plain_step_lds(
 	const uint32_t k0 = rowptr[r0], k1 = rowptr[r1];
 
 	if (k1 - k0 <= (uint32_t)kPlainCap) {
-		// Four non-zeros per iteration, their loads issued together.
-		uint32_t k = k0 + tid;
-		for (; k + 3 * kThreads < k1; k += 4 * kThreads) {
-			uint32_t c[4];
-			S vv[4], xx[4];
+		// Loads are issued for several non-zeros at once (measured fastest
+		// per form): with the value palette four per iteration; with full
+		// values all of a thread's non-zeros (at most kPlainCap / kThreads)
+		// in one batch, indices past the block's last non-zero clamped to it
+		// so every load is issued without a branch, and only the products
+		// of real non-zeros stored.
+		constexpr int U = (kPlainCap + kThreads - 1) / kThreads;
+		if (PAL) {
+			uint32_t k = k0 + tid;
+			for (; k + 3 * kThreads < k1; k += 4 * kThreads) {
+				uint32_t c[4];
+				S vv[4], xx[4];
 #pragma unroll
-			for (int i = 0; i < 4; i++) c[i] = col_at(col, k + i * kThreads, r0);
+				for (int i = 0; i < 4; i++) c[i] = col_at(col, k + i * kThreads, r0);
 #pragma unroll
-			for (int i = 0; i < 4; i++) {
-				vv[i] = value(k + i * kThreads, c[i]);
+				for (int i = 0; i < 4; i++) {
+					vv[i] = value(k + i * kThreads, c[i]);
+					xx[i] = x[c[i]];
+				}
+#pragma unroll
+				for (int i = 0; i < 4; i++) prod[k + i * kThreads - k0] = A::mul(vv[i], xx[i]);
+			}
+			for (; k < k1; k += kThreads) {
+				const uint32_t c = col_at(col, k, r0);
+				prod[k - k0] = A::mul(value(k, c), x[c]);
+			}
+		} else if (k1 > k0) {
+			uint32_t c[U];
+			S vv[U], xx[U];
+#pragma unroll
+			for (int i = 0; i < U; i++) {
+				const uint32_t k = k0 + tid + i * kThreads;
+				c[i] = col_at(col, k < k1 ? k : k1 - 1, r0);
+			}
+#pragma unroll
+			for (int i = 0; i < U; i++) {
+				const uint32_t k = k0 + tid + i * kThreads;
+				vv[i] = value(k < k1 ? k : k1 - 1, c[i]);
 				xx[i] = x[c[i]];
 			}
 #pragma unroll
-			for (int i = 0; i < 4; i++) prod[k + i * kThreads - k0] = A::mul(vv[i], xx[i]);
-		}
-		for (; k < k1; k += kThreads) {
-			const uint32_t c = col_at(col, k, r0);
-			prod[k - k0] = A::mul(value(k, c), x[c]);
+			for (int i = 0; i < U; i++) {
+				const uint32_t k = k0 + tid + i * kThreads;
+				if (k < k1) prod[k - k0] = A::mul(vv[i], xx[i]);
+			}
 		}
 		__syncthreads();
 		const uint32_t row = r0 + tid;
