@@ -44,14 +44,24 @@ def _run(cmd):
     return r
 
 
-def build_engine(jobs: int = 8) -> str:
+def _stale(src: str, obj: str, deps) -> bool:
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in [src, *deps])
+
+
+def build_engine(jobs: int = 8, changed_only: bool = False) -> str:
+    """Compile every .hip source (changed_only: only those newer than their
+    object or than a shared header) and link libtbtkx.so."""
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIB, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     objs = [os.path.join(OBJ, os.path.basename(s) + ".o") for s in srcs]
+    deps = glob.glob(os.path.join(CSRC, "*.hpp")) + [os.path.join(ROOT, "include", "tbtkx.h")]
+    todo = [(s, o) for s, o in zip(srcs, objs) if not changed_only or _stale(s, o, deps)]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        list(ex.map(_run, [[HIPCC, *HIP_FLAGS, "-c", s, "-o", o]
-                           for s, o in zip(srcs, objs)]))
+        list(ex.map(_run, [[HIPCC, *HIP_FLAGS, "-c", s, "-o", o] for s, o in todo]))
     tmp = ENGINE + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, ENGINE)
@@ -91,13 +101,14 @@ def build_cpp_tests(jobs: int = 8) -> list:
     return outs
 
 
-def build_all() -> None:
-    build_engine()
+def build_all(changed_only: bool = False) -> None:
+    build_engine(changed_only=changed_only)
     build_host()
     build_cpp_tests()
 
 
 if __name__ == "__main__":
-    build_all()
+    # --changed: recompile only the HIP sources changed since their object
+    build_all(changed_only="--changed" in sys.argv[1:])
     print("built", ENGINE, HOSTLIB if os.path.exists(HOSTLIB) else "")
     sys.exit(0)

@@ -350,8 +350,13 @@ int build_palette(tbtkx_context *h, int64_t N, int64_t Z, const uint32_t *rowptr
 	// pal: [H/s][256] then [2H/s][256] complex; diag: [H/s][N] then [2H/s][N]
 	std::vector<double> pal(2 * 256 * 2, 0.0), diag((size_t)2 * N * 2, 0.0);
 	for (int64_t r = 0; r < N; r++) {
+		bool have_diag = false;
 		for (uint32_t k = rowptr[r]; k < rowptr[r + 1]; k++) {
 			if ((int64_t)col[k] == r) {
+				// One diagonal value per row: a row that lists its diagonal
+				// twice (a CSR with unmerged entries) keeps full values.
+				if (have_diag) return TBTKX_OK;
+				have_diag = true;
 				code[k] = 255;
 				diag[2 * r] = v1[2 * k];
 				diag[2 * r + 1] = v1[2 * k + 1];

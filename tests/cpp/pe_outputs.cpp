@@ -196,6 +196,33 @@ int main(int argc, char **argv) {
 		writeJSON("sampledos", dos.serialize(Serializable::Mode::JSON));
 	}
 
+	{	// periodic6 (mu = 0.3): calculateDOS, exact (plain and FP64 block
+		// recurrences) and stochastic (FP64 and FP32 blocks), M = 200,
+		// (-5.9, 5.9) x 500, lookup table
+		Model model;
+		periodicSquare(model, 6, 0.3, 1);
+		for (int block = 0; block < 2; block++) {
+			Solver::ChebyshevExpander solver;
+			gpu(solver);
+			solver.setModel(model);
+			solver.setScaleFactor(6);
+			solver.setNumCoefficients(200);
+			solver.setUseLookupTable(true);
+			solver.setUseBlockRecurrence(block == 1);
+			solver.setBlockFlags(4 | 1 | 2);   // FP64 vectors and H, doubling
+			PropertyExtractor::ChebyshevExpander pe(solver);
+			pe.setEnergyWindow(-5.9, 5.9, 500);
+			write(std::string("periodic6_dos_exact_") + (block ? "block" : "plain"),
+			      pe.calculateDOS().getData());
+			if (block) {
+				pe.setStochasticVectors(50, 7);
+				write("periodic6_dos_stochastic_fp64", pe.calculateDOS().getData());
+				solver.setBlockFlags(2);   // FP32 vectors and H, doubling
+				write("periodic6_dos_stochastic_fp32", pe.calculateDOS().getData());
+			}
+		}
+	}
+
 	{	// ring10 LDOS, windows +-0.9s and +-0.7s, direct contraction
 		Model model;
 		model.setVerbose(false);

@@ -57,12 +57,26 @@ def save(name, meta, **arrays):
     print("wrote", name, {k: v.shape for k, v in arrays.items()})
 
 
+def csr_digest(rp, col, val) -> str:
+    """SHA-256 of a CSR's bytes: rowptr (u32), col (u32), val (complex128),
+    little endian, concatenated."""
+    import hashlib
+    h = hashlib.sha256()
+    for a, dt in ((rp, "<u4"), (col, "<u4"), (val, "<c16")):
+        h.update(np.ascontiguousarray(a, dtype=dt).tobytes())
+    return h.hexdigest()
+
+
 def moments_fixture(name, model, scale, M, frm, tos, broadenings=(0.0,),
                     keep_csr=True):
+    """Moments of the reference for the given queries; the reference's CSR
+    itself for small models, else its SHA-256 digest (csr_sha256)."""
     path, csr = ref_csr(model, name)
     arrays = {}
     if keep_csr:
         arrays.update(rowptr=csr[0], col=csr[1], val=csr[2])
+    else:
+        arrays["csr_sha256"] = np.array(csr_digest(*csr))
     for b in broadenings:
         arrays[f"coef_b{b:g}"] = ref_coef(path, scale, M, b, frm, tos)
     meta = dict(model=model.name, params=model.params, shape=list(model.shape),
@@ -137,7 +151,15 @@ def json_fixture(p1, pp):
         O.run_ref(out=out, json=os.path.join(jdir, name + ".json"), **kw)
 
 
-def main():
+def main(only=()):
+    """Write every fixture, or only those named in `only`."""
+    if only:
+        global save
+        _save = save
+
+        def save(name, meta, **arrays):  # noqa: F811
+            if name in only:
+                _save(name, meta, **arrays)
     if not O.ref_available():
         sys.exit("build oracle/_ref/ref_driver first (make -f oracle/Makefile.ref)")
 
@@ -236,8 +258,9 @@ def main():
                                      seed=3, broadening=1e-6),
          dos=np.fromfile(out, np.float64))
 
-    json_fixture(p1, pp)
+    if not only or "json" in only:
+        json_fixture(p1, pp)
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))

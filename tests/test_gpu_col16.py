@@ -154,3 +154,32 @@ def test_palette_device_built_lattice(col16):
     got = col16.moments(480, [480, 7], 300)
     assert col16.get_option(X.OPT_PALETTE) == 1
     assert np.array_equal(got, ref)
+
+
+def test_palette_duplicated_diagonal_keeps_full_values(col16, oracle_mod):
+    """A raw CSR that lists a row's diagonal twice (unmerged entries, which
+    the plain step adds one after the other) must not go through the
+    palette, which holds one diagonal value per row."""
+    n = 50
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        if i > 0:
+            rows.append(i); cols.append(i - 1); vals.append(-1.0)
+        rows.append(i); cols.append(i); vals.append(0.25 * (i % 3))
+        if i % 7 == 0:
+            rows.append(i); cols.append(i); vals.append(0.5)
+        if i < n - 1:
+            rows.append(i); cols.append(i + 1); vals.append(-1.0)
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    np.add.at(rp, np.array(rows) + 1, 1)
+    csr = (np.cumsum(rp).astype(np.uint32), np.array(cols, dtype=np.uint32),
+           np.array(vals, dtype=np.complex128))
+    col16.set_csr(csr, 3.0)
+    ref = oracle_mod.moments(csr, 3.0, 21, [21, 0, 14], 90)
+    out = {}
+    for pal in (1, 0):
+        col16.set_option(X.OPT_PALETTE, pal)
+        out[pal] = col16.moments(21, [21, 0, 14], 90)
+        assert col16.get_option(X.OPT_PALETTE) == 0
+    assert np.array_equal(out[1], ref)
+    assert np.array_equal(out[0], ref)

@@ -90,6 +90,52 @@ def test_ring_ldos_and_sampledos(outputs):
     assert np.array_equal(_read(outputs, "periodic5_sampledos", np.float64), fx["dos"])
 
 
+def _oracle_dos_inputs(O):
+    from tbtk_amd import lattice as L
+    m = L.periodic_square(6, t=1.0, mu=0.3)
+    s, M, lo, hi, E = 6.0, 200, -5.9, 5.9, 500
+    return m.csr(), s, M, O.lookup_table(M, E, lo, hi, s), m.basis_size
+
+
+def test_calculate_dos_exact_vs_oracle(outputs, oracle_mod):
+    """PE::calculateDOS without stochastic vectors is the sum of the LDOS of
+    every basis state (Tr delta(E - H)): the oracle's moments, Lorentz
+    damping (broadening 1e-6), lookup table and NonPrincipal contraction
+    (Lib/src/Solver/ChebyshevExpander.cpp:103-199, 201-270, 324-370), summed
+    over the basis in order. The plain recurrence is bitwise the oracle's,
+    the FP64 block recurrence (doubling identities) within 1e-12."""
+    O = oracle_mod
+    csr, s, M, T, N = _oracle_dos_inputs(O)
+    ref = np.zeros(T.shape[1])
+    for b in range(N):
+        c = O.lorentz(O.moments(csr, s, b, [b], M), 1e-6)[0]
+        ref += O.gf_lookup(O.NONPRINCIPAL, T, c).imag / np.pi
+    plain = _read(outputs, "periodic6_dos_exact_plain", np.float64)
+    assert np.array_equal(plain, ref)
+    block = _read(outputs, "periodic6_dos_exact_block", np.float64)
+    assert np.abs(block - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert abs(ref.sum() * (11.8 / 499) - N) < 0.05 * N
+
+
+def test_calculate_dos_stochastic_vs_oracle(outputs, oracle_mod):
+    """With setStochasticVectors(R, seed) the DOS comes from the stochastic
+    trace sum_r <r|T_n|r> / R over R random-phase vectors exp(2 pi i u),
+    u = the splitmix64 uniform of (seed, r N + i): the oracle's trace
+    recurrence on the same vectors, damped and contracted as above. FP64
+    block within 1e-10 of max DOS, FP32 block within 3e-5."""
+    O = oracle_mod
+    csr, s, M, T, N = _oracle_dos_inputs(O)
+    R, seed = 50, 7
+    vecs = np.exp(2j * np.pi * O.uniform(seed, np.arange(R * N)).reshape(R, N))
+    mu = O.moments_trace(csr, s, vecs, M) / R
+    ref = O.gf_lookup(O.NONPRINCIPAL, T, O.lorentz(mu, 1e-6)[0]).imag / np.pi
+    peak = np.abs(ref).max()
+    got = _read(outputs, "periodic6_dos_stochastic_fp64", np.float64)
+    assert np.abs(got - ref).max() <= 1e-10 * peak
+    got = _read(outputs, "periodic6_dos_stochastic_fp32", np.float64)
+    assert np.abs(got - ref).max() <= 3e-5 * peak
+
+
 def test_thermodynamic_properties_bitexact(outputs):
     """Density, Magnetization, SpinPolarizedLDOS (Custom and Ranges formats)
     and expectation values at T = 300 K and T = 0 equal the reference's bit

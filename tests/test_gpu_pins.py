@@ -1,0 +1,127 @@
+"""Direct oracle pins of the headline kernels at their real shapes.
+
+* The plain FP64 step (`plain_step_lds` with 16-bit column offsets and the
+  value palette, the forms the 1024x1024 spin lattice runs in) on a 16x1024
+  strip of the same lattice: its column offsets from the first row of a
+  256-row block span -2048 ... 2303, the full range of the 1024x1024 model,
+  and the moments must equal the oracle's bit for bit
+  (Lib/src/Solver/ChebyshevExpander.cpp:65-82, 157-186).
+* The full 1024x1024 spin lattice (2,097,152 states) at 32 moments, bitwise
+  against the oracle.
+* The split-BF16 MFMA contraction, all four types, against the oracle's FP64
+  lookup contraction (Lib/src/Solver/ChebyshevExpander.cpp:324-370) on the
+  oracle's table (:201-270), on 96 coefficient vectors of a real model:
+  within 3e-5 of max|G| (about 16 significant bits per operand).
+"""
+import numpy as np
+import pytest
+
+from tbtk_amd import engine as X
+from tbtk_amd import kpm
+from tbtk_amd import lattice as L
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def plain(engine):
+    yield engine
+    for opt, val in ((X.OPT_COL16, 1), (X.OPT_PALETTE, 1), (X.OPT_GRAPH, 0),
+                     (X.OPT_PLAIN_KERNEL, 0)):
+        engine.set_option(opt, val)
+
+
+def block_offsets(csr, rows=256, cap=1536):
+    """Column offsets from the first row of each plain-kernel row block
+    (blocks of at most `rows` rows and `cap` non-zeros), as min and max."""
+    rp = np.asarray(csr[0], dtype=np.int64)
+    col = np.asarray(csr[1], dtype=np.int64)
+    N = rp.size - 1
+    lo, hi, r = 0, 0, 0
+    while r < N:
+        start, nnz = r, 0
+        while r < N and r - start < rows and nnz + rp[r + 1] - rp[r] <= cap:
+            nnz += rp[r + 1] - rp[r]
+            r += 1
+        if r == start:
+            r += 1
+        d = col[rp[start]:rp[r]] - start
+        if d.size:
+            lo, hi = min(lo, int(d.min())), max(hi, int(d.max()))
+    return lo, hi
+
+
+def test_plain_step_full_offset_range_bitwise(plain, oracle_mod):
+    m = L.square_spin(16, 1024, t=1.0, J=0.5, disorder=1.0, seed=2)
+    csr = m.csr()
+    lo, hi = block_offsets(csr)
+    assert (lo, hi) == (-2048, 2303), (lo, hi)
+    s, M = 12.0, 400
+    centre = m.basis_index((8, 512, 0))
+    to = [centre, centre + 1, 0, m.basis_size - 1, centre - 2048, centre + 2049]
+    plain.set_csr(csr, s)
+    ref = oracle_mod.moments(csr, s, centre, to, M)
+    got = plain.moments(centre, to, M)
+    assert plain.get_option(X.OPT_COL16) == 1
+    assert plain.get_option(X.OPT_PALETTE) == 1
+    assert plain.get_option(X.OPT_REAL) == 1
+    assert np.array_equal(got, ref)
+    # the 32-bit, full-value form of the same step gives the same bits
+    plain.set_option(X.OPT_COL16, 0)
+    assert np.array_equal(plain.moments(centre, to, M), ref)
+    assert plain.get_option(X.OPT_COL16) == 0
+
+
+def test_plain_full_size_bitwise(plain, oracle_mod):
+    """The 1024x1024 spin lattice with disorder U[-1, 1] (seed 2), scale
+    factor 12, from the centre site: 32 moments equal the oracle's."""
+    m = L.config_plain_large()
+    csr = m.csr()
+    s, M = 12.0, 32
+    frm = m.basis_index((512, 512, 0))
+    to = [frm, m.basis_index((512, 512, 1)), m.basis_index((513, 512, 0)),
+          m.basis_index((512, 513, 0)), m.basis_index((500, 520, 0))]
+    plain.set_csr(csr, s)
+    got = plain.moments(frm, to, M)
+    assert plain.get_option(X.OPT_COL16) == 1
+    assert plain.get_option(X.OPT_PALETTE) == 1
+    ref = oracle_mod.moments(csr, s, frm, to, M)
+    assert np.array_equal(got, ref)
+    # the moments reach the far to-index only after 12 + 8 steps
+    assert np.count_nonzero(ref[4]) > 0
+
+
+@pytest.fixture(scope="module")
+def chebyshev_coefficients():
+    """96 Jackson-damped moment vectors (600 moments) of a Nambu model, from
+    the oracle's FP64 recurrence: 48 diagonal and 48 off-diagonal."""
+    from oracle import oracle as O
+    m = L.nambu_swave(8, 7, disorder=0.5, seed=3)
+    s = kpm.gershgorin_scale(m.csr(), 1.05)
+    M = 600
+    rows = []
+    for b in range(0, m.basis_size, m.basis_size // 24)[:24]:
+        c = O.moments(m.csr(), s, b, [b, (b + 5) % m.basis_size, (b + 1) % m.basis_size,
+                                      (b + 17) % m.basis_size], M)
+        rows.extend(c)
+    c = np.array(rows[:96]) * kpm.jackson(M)[None, :]
+    return c, s
+
+
+@pytest.mark.parametrize("table", ["device", "uploaded"])
+@pytest.mark.parametrize("gtype", [X.RETARDED, X.ADVANCED, X.PRINCIPAL, X.NONPRINCIPAL])
+def test_mfma_contraction_vs_oracle(engine, oracle_mod, chebyshev_coefficients, gtype, table):
+    c, s = chebyshev_coefficients
+    K, M = c.shape
+    E, lo, hi = 1000, -0.99 * s, 0.99 * s
+    T = oracle_mod.lookup_table(M, E, lo, hi, s)
+    if table == "device":
+        engine.lut_generate(M, E, lo, hi, s)
+    else:
+        engine.lut_upload(T)
+    engine.coeffs_upload(c)
+    got = engine.contract_mfma(gtype)
+    ref = np.array([oracle_mod.gf_lookup(gtype, T, row) for row in c])
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 3e-5, err
+    engine.lut_destroy()
