@@ -13,8 +13,9 @@
  *    otherwise; tbtkx_last_error() gives the message. Nothing throws across
  *    the ABI.
  *  - A handle owns one HIP stream on one device and all device state (the
- *    Hamiltonian, recurrence vectors, the lookup table). Calls on one handle
- *    must not overlap; different handles are independent.
+ *    Hamiltonian, recurrence vectors, the lookup table). Each call holds the
+ *    handle's lock, so threads sharing a handle run their calls one after
+ *    another; different handles are independent and run concurrently.
  *  - Host buffers are owned by the caller.
  */
 #ifndef TBTKX_H
@@ -68,6 +69,12 @@ int tbtkx_device_count(int *count);
 int tbtkx_device_acquire(int *device);
 int tbtkx_device_release(int device);
 
+/* A handle on `device`, or with TBTKX_ANY_DEVICE on the device that holds
+ * the fewest live handles (the lowest id on a tie), so solvers created by
+ * different threads spread over the GPUs the way the reference hands each
+ * thread its own device (GPUResourceManager::allocateDevice,
+ * Lib/src/Utilities/GPUResourceManager.cpp:44-72). */
+enum { TBTKX_ANY_DEVICE = -1 };
 int tbtkx_create(int device, tbtkx_handle *handle);
 int tbtkx_destroy(tbtkx_handle handle);
 
@@ -117,8 +124,13 @@ int tbtkx_contract(tbtkx_handle handle, int num_vectors, const double *coeffs,
 /* FP64 contraction without a lookup table: the direct evaluation of
  * generateGreensFunctionCPU (Lib/src/Solver/ChebyshevExpander.cpp:371-432)
  * over the energies e0 + e*de, e < E, for K vectors of M coefficients. The
- * reference has no GPU counterpart (its GPU path requires the table,
- * Lib/include/TBTK/Solver/ChebyshevExpander.h:555-565). */
+ * trigonometric factors are computed on the host with the C library's acos,
+ * complex exp, sin and cos, as the reference's CPU path computes them, and
+ * kept on the device for the next call with the same type class, M and
+ * grid; the sums run on the device in the reference's order, so the result
+ * is bitwise the reference's. The reference has no GPU counterpart (its GPU
+ * path requires the table, Lib/include/TBTK/Solver/ChebyshevExpander.h:
+ * 555-565). */
 int tbtkx_contract_direct(tbtkx_handle handle, int num_vectors, int num_moments,
                           const double *coeffs, int type, int resolution,
                           double e0, double de, double scale, double *out);

@@ -1386,6 +1386,7 @@ extern "C" {
 int tbtkx_moments_block(tbtkx_handle h, int num_sites, const int64_t *sites,
                         int M, int flags, double *out) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_moments_block: null handle");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(h->N > 0, TBTKX_ERR_STATE, "tbtkx_moments_block: no model set");
 	TBTKX_REQUIRE(M > 0, TBTKX_ERR_ARG,
 	              "tbtkx_moments_block: numCoefficients has to be larger than 0");
@@ -1407,6 +1408,7 @@ int tbtkx_moments_block(tbtkx_handle h, int num_sites, const int64_t *sites,
 int tbtkx_moments_trace(tbtkx_handle h, int R, uint64_t seed, int M, int flags,
                         double *out) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_moments_trace: null handle");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(h->N > 0, TBTKX_ERR_STATE, "tbtkx_moments_trace: no model set");
 	TBTKX_REQUIRE(M > 0 && R > 0 && out, TBTKX_ERR_ARG,
 	              "tbtkx_moments_trace: need M > 0, R > 0 and an output array");

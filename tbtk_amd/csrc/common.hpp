@@ -8,6 +8,7 @@
 #include <cstring>
 #include <initializer_list>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -39,6 +40,11 @@ void set_error(const std::string &msg);
 			return (code);                                         \
 		}                                                              \
 	} while (0)
+
+// Entry points that take a handle hold its lock for the whole call, so one
+// handle may be shared by several host threads (their calls run one after
+// another). Recursive: an entry point may call another on the same handle.
+#define TBTKX_LOCK(h) std::lock_guard<std::recursive_mutex> tbtkx_call_lock_((h)->call_mutex)
 
 // ------------------------------------------------------------ device mem --
 
@@ -91,6 +97,7 @@ struct GraphCache {
 }  // namespace tbtkx
 
 struct tbtkx_context {
+	std::recursive_mutex call_mutex;   // held by every entry point (TBTKX_LOCK)
 	int device = 0;
 	hipStream_t stream = nullptr;
 	hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -133,6 +140,11 @@ struct tbtkx_context {
 	tbtkx::DevBuf lut;
 	int lut_M = 0, lut_E = 0;
 	tbtkx::DevBuf gf_in, gf_out;
+	// Factors of the direct contraction (host libm values for one Green's
+	// function kind, M moments and one energy grid) and sqrt(1 - E~^2).
+	tbtkx::DevBuf dfac, dden;
+	int dfac_kind = -1, dfac_M = 0, dfac_E = 0;
+	double dfac_e0 = 0, dfac_de = 0, dfac_s = 0;
 	// Split-BF16 planes for the MFMA contraction.
 	tbtkx::DevBuf cplanes, lplanes, gf_mfma_out;
 	int cplanes_K = 0, cplanes_M = 0;

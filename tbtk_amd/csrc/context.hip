@@ -51,6 +51,7 @@ namespace {
 std::mutex g_pool_mutex;
 std::condition_variable g_pool_cv;
 std::vector<char> g_pool_busy;
+std::vector<int> g_pool_handles;   // live handles per device
 bool g_pool_ready = false;
 }  // namespace
 
@@ -63,6 +64,7 @@ static int pool_init() {
 		return TBTKX_ERR_NODEVICE;
 	}
 	g_pool_busy.assign(n, 0);
+	g_pool_handles.assign(n, 0);
 	g_pool_ready = true;
 	return TBTKX_OK;
 }
@@ -152,9 +154,23 @@ int tbtkx_create(int device, tbtkx_handle *handle) {
 		set_error("tbtkx_create: no HIP device available");
 		return TBTKX_ERR_NODEVICE;
 	}
-	TBTKX_REQUIRE(device >= 0 && device < n, TBTKX_ERR_ARG,
+	TBTKX_REQUIRE(device == TBTKX_ANY_DEVICE || (device >= 0 && device < n), TBTKX_ERR_ARG,
 	              "tbtkx_create: device " + std::to_string(device) +
 	                  " out of range (" + std::to_string(n) + " devices)");
+	{
+		// Count live handles per device; TBTKX_ANY_DEVICE takes the device
+		// with the fewest (the lowest id on a tie), so solvers created by
+		// different threads spread over the devices.
+		std::lock_guard<std::mutex> lock(g_pool_mutex);
+		int rc = pool_init();
+		if (rc) return rc;
+		if (device == TBTKX_ANY_DEVICE) {
+			device = 0;
+			for (int d = 1; d < (int)g_pool_handles.size(); d++)
+				if (g_pool_handles[d] < g_pool_handles[device]) device = d;
+		}
+		g_pool_handles[device]++;
+	}
 	auto *h = new tbtkx_context();
 	h->device = device;
 	hipError_t e = hipSetDevice(device);
@@ -181,7 +197,7 @@ int tbtkx_destroy(tbtkx_handle h) {
 	                  &h->v1r, &h->v2r, &h->lat_terms, &h->lat_onsite, &h->lat_count,
 	                  &h->lat_part, &h->lat_tmp, &h->rblk, &h->col16, &h->col16_flag,
 	                  &h->pcode, &h->pal, &h->palr, &h->pdiag, &h->pdiagr, &h->vec_a, &h->vec_b, &h->coef, &h->to,
-	                  &h->lut, &h->gf_in, &h->gf_out, &h->blk_x, &h->blk_y,
+	                  &h->lut, &h->gf_in, &h->gf_out, &h->dfac, &h->dden, &h->blk_x, &h->blk_y,
 	                  &h->blk_mu, &h->blk_sites, &h->blk_part, &h->cplanes,
 	                  &h->lplanes, &h->gf_mfma_out, &h->ell_col, &h->ell_v1f,
 	                  &h->ell_v2f, &h->ell_v1d, &h->ell_v2d,
@@ -192,6 +208,11 @@ int tbtkx_destroy(tbtkx_handle h) {
 	if (h->ev_start) (void)hipEventDestroy(h->ev_start);
 	if (h->ev_stop) (void)hipEventDestroy(h->ev_stop);
 	if (h->stream) (void)hipStreamDestroy(h->stream);
+	{
+		std::lock_guard<std::mutex> lock(g_pool_mutex);
+		if (h->device < (int)g_pool_handles.size() && g_pool_handles[h->device] > 0)
+			g_pool_handles[h->device]--;
+	}
 	delete h;
 	return TBTKX_OK;
 }
@@ -199,6 +220,7 @@ int tbtkx_destroy(tbtkx_handle h) {
 int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
                   const uint32_t *col, const double *val, double scale) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_set_csr: null handle");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(N > 0 && N < (int64_t)1 << 31, TBTKX_ERR_ARG,
 	              "tbtkx_set_csr: num_rows must be in [1, 2^31)");
 	TBTKX_REQUIRE(Z >= 0 && Z < ((int64_t)1 << 32), TBTKX_ERR_ARG,
@@ -337,6 +359,7 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 int tbtkx_last_timing(tbtkx_handle h, double *ms, int64_t *launches) {
 	TBTKX_REQUIRE(h && ms && launches, TBTKX_ERR_ARG,
 	              "tbtkx_last_timing: null pointer");
+	TBTKX_LOCK(h);
 	*ms = h->last_ms;
 	*launches = h->last_launches;
 	return TBTKX_OK;
@@ -344,6 +367,7 @@ int tbtkx_last_timing(tbtkx_handle h, double *ms, int64_t *launches) {
 
 int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_set_option: null handle");
+	TBTKX_LOCK(h);
 	switch (option) {
 	case TBTKX_OPT_PLAIN_KERNEL:
 		TBTKX_REQUIRE(value == 0 || value == 1, TBTKX_ERR_ARG,
@@ -430,6 +454,7 @@ int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
 
 int tbtkx_get_option(tbtkx_handle h, int option, int64_t *value) {
 	TBTKX_REQUIRE(h && value, TBTKX_ERR_ARG, "tbtkx_get_option: null pointer");
+	TBTKX_LOCK(h);
 	switch (option) {
 	case TBTKX_OPT_PLAIN_KERNEL: *value = h->plain_kernel; return TBTKX_OK;
 	case TBTKX_OPT_GRAPH: *value = h->use_graph; return TBTKX_OK;

@@ -12,6 +12,10 @@
 //
 // Traffic: the table is read once per coefficient vector by a thread per
 // energy (16 B per (n, e)); it is an FP64 GEMV/GEMM of 8 K M E flops.
+#include <algorithm>
+#include <complex>
+#include <thread>
+
 #include "common.hpp"
 
 namespace tbtkx {
@@ -129,15 +133,39 @@ __global__ __launch_bounds__(kThreads) void contract_fp64(
 //   Principal    (sum_n c_n sin(n theta)) * (2/s)
 //   NonPrincipal (c0/2 + sum_n c_n cos(n theta)) * (2i/s)
 // all divided by sqrt(1 - E~^2); energies e0 + e*de as Range::operator[].
+// The trigonometric factors come from the host's libm (direct_factors), the
+// functions the reference calls, so the sums below reproduce its results bit
+// for bit: X holds e^{-in theta} as (re, im) pairs (Retarded, Advanced),
+// sin(n theta) (Principal) or cos(n theta) (NonPrincipal) at X[(n-1)*E + e],
+// and den[e] = sqrt(1 - E~^2).
+template <int TYPE>
+__device__ __forceinline__ void direct_term(double2 cn, double2 x, double &gr, double &gi) {
+#pragma clang fp contract(off)
+	if (TYPE == TBTKX_RETARDED || TYPE == TBTKX_ADVANCED) {
+		const double pr = cn.x * x.x - cn.y * x.y;
+		const double pi = cn.x * x.y + cn.y * x.x;
+		gr = gr + pr;
+		gi = TYPE == TBTKX_RETARDED ? gi + pi : gi + (-pi);
+	} else {
+		gr = gr + cn.x * x.x;
+		gi = gi + cn.y * x.x;
+	}
+}
+
 template <int TYPE>
 __global__ __launch_bounds__(kThreads) void contract_direct(
-    const double2 *__restrict__ C, double2 *__restrict__ G, int M, int E,
-    double e0, double de, double s) {
+    const double2 *__restrict__ C, const double *__restrict__ X,
+    const double *__restrict__ den, double2 *__restrict__ G, int M, int E, double s) {
 #pragma clang fp contract(off)
+	constexpr bool kPair = TYPE == TBTKX_RETARDED || TYPE == TBTKX_ADVANCED;
 	const int e = blockIdx.x * kThreads + threadIdx.x;
 	const int k = blockIdx.y;
 	if (e >= E) return;
 	const double2 *c = C + (int64_t)k * M;
+	auto factor = [&](int n) {
+		const int64_t at = (int64_t)(n - 1) * E + e;
+		return kPair ? reinterpret_cast<const double2 *>(X)[at] : make_double2(X[at], 0.0);
+	};
 	double gr, gi;
 	if (TYPE == TBTKX_PRINCIPAL) {
 		gr = 0.0;
@@ -146,27 +174,15 @@ __global__ __launch_bounds__(kThreads) void contract_direct(
 		gr = c[0].x / 2.;
 		gi = c[0].y / 2.;
 	}
-	const double En = (e0 + e * de) / s;
-	const double th = acos(En);
-	for (int n = 1; n < M; n++) {
-		const double2 cn = c[n];
-		if (TYPE == TBTKX_RETARDED || TYPE == TBTKX_ADVANCED) {
-			double sn, cs;
-			sincos((-1.0 * (double)n) * th, &sn, &cs);
-			const double pr = cn.x * cs - cn.y * sn;
-			const double pi = cn.x * sn + cn.y * cs;
-			gr = gr + pr;
-			gi = TYPE == TBTKX_RETARDED ? gi + pi : gi + (-pi);
-		} else if (TYPE == TBTKX_PRINCIPAL) {
-			const double sn = sin(((double)n) * th);
-			gr = gr + cn.x * sn;
-			gi = gi + cn.y * sn;
-		} else {
-			const double cs = cos(((double)n) * th);
-			gr = gr + cn.x * cs;
-			gi = gi + cn.y * cs;
-		}
+	int n = 1;
+	for (; n + kAhead <= M; n += kAhead) {
+		double2 xv[kAhead];
+#pragma unroll
+		for (int i = 0; i < kAhead; i++) xv[i] = factor(n + i);
+#pragma unroll
+		for (int i = 0; i < kAhead; i++) direct_term<TYPE>(c[n + i], xv[i], gr, gi);
 	}
+	for (; n < M; n++) direct_term<TYPE>(c[n], factor(n), gr, gi);
 	double fr, fi;   // the type's prefactor
 	if (TYPE == TBTKX_RETARDED) {
 		fr = -2. * 0.0 / s;
@@ -186,7 +202,7 @@ __global__ __launch_bounds__(kThreads) void contract_direct(
 		hr = gr * fr - gi * fi;
 		hi = gr * fi + gi * fr;
 	}
-	const double r = sqrt(1 - En * En);
+	const double r = den[e];
 	G[(int64_t)k * E + e] = make_double2(hr / r, hi / r);
 }
 
@@ -203,6 +219,69 @@ int apply_damping(tbtkx_context *h, double2 *C, int K, int M) {
 	return TBTKX_OK;
 }
 
+// The factors of the direct contraction for one Green's function kind (0:
+// e^{-in theta} for Retarded and Advanced, 1: sin(n theta), 2: cos(n theta))
+// over n = 1 .. M-1 and the energies e0 + e*de, computed on the host with the
+// expressions of generateGreensFunctionCPU (Lib/src/Solver/
+// ChebyshevExpander.cpp:386-430: acos, std::exp of a complex, sin, cos,
+// sqrt), so the device sums use the reference's values. Kept on the device
+// until another kind, order or grid is asked for.
+static int direct_factors(tbtkx_context *h, int kind, int M, int E, double e0, double de,
+                          double s) {
+	if (h->dfac_kind == kind && h->dfac_M == M && h->dfac_E == E && h->dfac_e0 == e0 &&
+	    h->dfac_de == de && h->dfac_s == s)
+		return TBTKX_OK;
+	h->dfac_kind = -1;
+	std::vector<double> acosE(E), den(E);
+	for (int e = 0; e < E; e++) {
+		const double En = (e0 + e * de) / s;
+		acosE[e] = std::acos(En);
+		den[e] = std::sqrt(1 - En * En);
+	}
+	const size_t per = kind == 0 ? 2 : 1;
+	const size_t rows = M > 1 ? (size_t)(M - 1) : 0;
+	std::vector<double> fac(rows * E * per + per);
+	auto fill = [&](int first, int stride) {
+		const std::complex<double> i(0, 1);
+		for (int n = 1 + first; n < M; n += stride) {
+			double *row = fac.data() + (size_t)(n - 1) * E * per;
+			for (int e = 0; e < E; e++) {
+				if (kind == 0) {
+					const std::complex<double> v = std::exp(-i * ((double)n) * acosE[e]);
+					row[2 * e] = v.real();
+					row[2 * e + 1] = v.imag();
+				} else if (kind == 1) {
+					row[e] = std::sin(((double)n) * acosE[e]);
+				} else {
+					row[e] = std::cos(((double)n) * acosE[e]);
+				}
+			}
+		}
+	};
+	const int threads = (int)std::min<size_t>(
+	    std::max(1u, std::min(16u, std::thread::hardware_concurrency())),
+	    rows / 64 + 1);
+	std::vector<std::thread> pool;
+	for (int t = 1; t < threads; t++) pool.emplace_back(fill, t, threads);
+	fill(0, threads);
+	for (auto &t : pool) t.join();
+	int rc;
+	if ((rc = h->dfac.reserve(fac.size() * 8))) return rc;
+	if ((rc = h->dden.reserve((size_t)E * 8))) return rc;
+	TBTKX_CHECK_HIP(hipMemcpyAsync(h->dfac.ptr, fac.data(), fac.size() * 8,
+	                               hipMemcpyHostToDevice, h->stream));
+	TBTKX_CHECK_HIP(hipMemcpyAsync(h->dden.ptr, den.data(), (size_t)E * 8,
+	                               hipMemcpyHostToDevice, h->stream));
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	h->dfac_kind = kind;
+	h->dfac_M = M;
+	h->dfac_E = E;
+	h->dfac_e0 = e0;
+	h->dfac_de = de;
+	h->dfac_s = s;
+	return TBTKX_OK;
+}
+
 }  // namespace tbtkx
 
 using namespace tbtkx;
@@ -211,6 +290,7 @@ extern "C" {
 
 int tbtkx_lut_upload(tbtkx_handle h, int M, int E, const double *table) {
 	TBTKX_REQUIRE(h && table, TBTKX_ERR_ARG, "tbtkx_lut_upload: null pointer");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(M > 0 && E > 0, TBTKX_ERR_ARG,
 	              "tbtkx_lut_upload: table dimensions must be positive");
 	int rc = set_device(h);
@@ -228,6 +308,7 @@ int tbtkx_lut_upload(tbtkx_handle h, int M, int E, const double *table) {
 int tbtkx_lut_generate(tbtkx_handle h, int M, int E, double lo, double hi,
                        double s) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_lut_generate: null handle");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(M > 0, TBTKX_ERR_ARG,
 	              "tbtkx_lut_generate: numCoefficients has to be larger than 0");
 	TBTKX_REQUIRE(E > 1, TBTKX_ERR_ARG,
@@ -253,6 +334,7 @@ int tbtkx_lut_generate(tbtkx_handle h, int M, int E, double lo, double hi,
 
 int tbtkx_lut_destroy(tbtkx_handle h) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_lut_destroy: null handle");
+	TBTKX_LOCK(h);
 	int rc = set_device(h);
 	if (rc) return rc;
 	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
@@ -264,6 +346,7 @@ int tbtkx_lut_destroy(tbtkx_handle h) {
 
 int tbtkx_set_damping(tbtkx_handle h, int kind, double broadening) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_set_damping: null handle");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(kind == TBTKX_DAMP_NONE || kind == TBTKX_DAMP_LORENTZ ||
 	                  kind == TBTKX_DAMP_JACKSON,
 	              TBTKX_ERR_ARG, "tbtkx_set_damping: unknown damping kind");
@@ -277,6 +360,7 @@ int tbtkx_set_damping(tbtkx_handle h, int kind, double broadening) {
 int tbtkx_contract(tbtkx_handle h, int K, const double *coeffs, int type,
                    double *out) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_contract: null handle");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(h->lut_M > 0, TBTKX_ERR_STATE,
 	              "tbtkx_contract: no lookup table loaded");
 	TBTKX_REQUIRE(K >= 0 && (K == 0 || (coeffs && out)), TBTKX_ERR_ARG,
@@ -321,6 +405,7 @@ int tbtkx_contract(tbtkx_handle h, int K, const double *coeffs, int type,
 int tbtkx_contract_direct(tbtkx_handle h, int K, int M, const double *coeffs, int type,
                           int E, double e0, double de, double s, double *out) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_contract_direct: null handle");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(K >= 0 && K <= 65535 && M > 0 && E > 0 && (K == 0 || (coeffs && out)),
 	              TBTKX_ERR_ARG, "tbtkx_contract_direct: bad sizes or arrays");
 	TBTKX_REQUIRE(type >= TBTKX_ADVANCED && type <= TBTKX_NONPRINCIPAL, TBTKX_ERR_ARG,
@@ -335,21 +420,24 @@ int tbtkx_contract_direct(tbtkx_handle h, int K, int M, const double *coeffs, in
 	TBTKX_CHECK_HIP(hipMemcpyAsync(h->gf_in.ptr, coeffs, (size_t)K * M * 16,
 	                               hipMemcpyHostToDevice, h->stream));
 	if ((rc = apply_damping(h, h->gf_in.as<double2>(), K, M))) return rc;
+	const int kind = type == TBTKX_PRINCIPAL ? 1 : type == TBTKX_NONPRINCIPAL ? 2 : 0;
+	if ((rc = direct_factors(h, kind, M, E, e0, de, s))) return rc;
 	dim3 grid((E + kThreads - 1) / kThreads, K);
 	const double2 *C = h->gf_in.as<double2>();
+	const double *X = h->dfac.as<double>(), *D = h->dden.as<double>();
 	double2 *G = h->gf_out.as<double2>();
 	switch (type) {
 	case TBTKX_RETARDED:
-		contract_direct<TBTKX_RETARDED><<<grid, kThreads, 0, h->stream>>>(C, G, M, E, e0, de, s);
+		contract_direct<TBTKX_RETARDED><<<grid, kThreads, 0, h->stream>>>(C, X, D, G, M, E, s);
 		break;
 	case TBTKX_ADVANCED:
-		contract_direct<TBTKX_ADVANCED><<<grid, kThreads, 0, h->stream>>>(C, G, M, E, e0, de, s);
+		contract_direct<TBTKX_ADVANCED><<<grid, kThreads, 0, h->stream>>>(C, X, D, G, M, E, s);
 		break;
 	case TBTKX_PRINCIPAL:
-		contract_direct<TBTKX_PRINCIPAL><<<grid, kThreads, 0, h->stream>>>(C, G, M, E, e0, de, s);
+		contract_direct<TBTKX_PRINCIPAL><<<grid, kThreads, 0, h->stream>>>(C, X, D, G, M, E, s);
 		break;
 	default:
-		contract_direct<TBTKX_NONPRINCIPAL><<<grid, kThreads, 0, h->stream>>>(C, G, M, E, e0, de, s);
+		contract_direct<TBTKX_NONPRINCIPAL><<<grid, kThreads, 0, h->stream>>>(C, X, D, G, M, E, s);
 		break;
 	}
 	TBTKX_CHECK_HIP(hipGetLastError());

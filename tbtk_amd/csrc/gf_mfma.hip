@@ -395,6 +395,7 @@ extern "C" {
 
 int tbtkx_coeffs_upload(tbtkx_handle h, int K, int M, const double *coeffs) {
 	TBTKX_REQUIRE(h && coeffs, TBTKX_ERR_ARG, "tbtkx_coeffs_upload: null pointer");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(K > 0 && M > 0, TBTKX_ERR_ARG, "tbtkx_coeffs_upload: K and M must be positive");
 	int rc = set_device(h);
 	if (rc) return rc;
@@ -416,6 +417,7 @@ int tbtkx_coeffs_upload(tbtkx_handle h, int K, int M, const double *coeffs) {
 
 int tbtkx_contract_mfma(tbtkx_handle h, int type, double *out) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_contract_mfma: null handle");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(h->lut_M > 0, TBTKX_ERR_STATE, "tbtkx_contract_mfma: no lookup table loaded");
 	TBTKX_REQUIRE(h->cplanes_K > 0, TBTKX_ERR_STATE,
 	              "tbtkx_contract_mfma: no coefficients uploaded (tbtkx_coeffs_upload)");

@@ -340,6 +340,7 @@ extern "C" {
 int tbtkx_set_lattice(tbtkx_handle h, const tbtkx_lattice *lat, const double *onsite,
                       double scale) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_set_lattice: null handle");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(scale > 0, TBTKX_ERR_ARG,
 	              "tbtkx_set_lattice: the scale factor has to be larger than 0");
 	LatDev g{};
@@ -447,6 +448,7 @@ int tbtkx_set_lattice(tbtkx_handle h, const tbtkx_lattice *lat, const double *on
 int tbtkx_lattice_gershgorin(tbtkx_handle h, const tbtkx_lattice *lat, const double *onsite,
                              double *bound) {
 	TBTKX_REQUIRE(h && bound, TBTKX_ERR_ARG, "tbtkx_lattice_gershgorin: null pointer");
+	TBTKX_LOCK(h);
 	LatDev g{};
 	int64_t N = 0;
 	int max_row = 0;
@@ -462,6 +464,7 @@ int tbtkx_lattice_gershgorin(tbtkx_handle h, const tbtkx_lattice *lat, const dou
 
 int tbtkx_csr_size(tbtkx_handle h, int64_t *num_rows, int64_t *nnz) {
 	TBTKX_REQUIRE(h && num_rows && nnz, TBTKX_ERR_ARG, "tbtkx_csr_size: null pointer");
+	TBTKX_LOCK(h);
 	*num_rows = h->N;
 	*nnz = h->Z;
 	return TBTKX_OK;
@@ -470,6 +473,7 @@ int tbtkx_csr_size(tbtkx_handle h, int64_t *num_rows, int64_t *nnz) {
 int tbtkx_csr_copy(tbtkx_handle h, uint32_t *rowptr, uint32_t *col, double *values) {
 	TBTKX_REQUIRE(h && rowptr && (h->Z == 0 || (col && values)), TBTKX_ERR_ARG,
 	              "tbtkx_csr_copy: null pointer");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(h->N > 0, TBTKX_ERR_STATE, "tbtkx_csr_copy: no model set");
 	int rc = set_device(h);
 	if (rc) return rc;
@@ -486,6 +490,7 @@ int tbtkx_csr_copy(tbtkx_handle h, uint32_t *rowptr, uint32_t *col, double *valu
 int tbtkx_multi_set_lattices(tbtkx_handle h, int K, const tbtkx_lattice *lats,
                              const double *const *onsite, const double *scales) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_multi_set_lattices: null handle");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(K > 0 && K <= 65536 && lats && scales, TBTKX_ERR_ARG,
 	              "tbtkx_multi_set_lattices: need 1..65536 models and arrays");
 	std::vector<LatDev> g(K);

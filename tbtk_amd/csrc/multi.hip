@@ -917,6 +917,7 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
                     const int64_t *nnz, const uint32_t *rowptr, const uint32_t *col,
                     const double *val, const double *scales) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_multi_set: null handle");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(K > 0 && K <= 65536 && num_rows && nnz && rowptr && scales,
 	              TBTKX_ERR_ARG, "tbtkx_multi_set: need 1..65536 models and arrays");
 	int64_t rp0 = 0, z0 = 0, rows = 0, entries = 0;
@@ -1034,6 +1035,7 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sites,
                         const int *num_moments, int flags, double *out) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_multi_moments: null handle");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(h->mm_K > 0, TBTKX_ERR_STATE, "tbtkx_multi_moments: no models set");
 	TBTKX_REQUIRE(num_sites && sites && num_moments && out, TBTKX_ERR_ARG,
 	              "tbtkx_multi_moments: null array");

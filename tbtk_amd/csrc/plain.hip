@@ -409,6 +409,7 @@ using namespace tbtkx;
 extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
                              const int64_t *to, int M, double *out) {
 	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_moments: null handle");
+	TBTKX_LOCK(h);
 	TBTKX_REQUIRE(h->N > 0, TBTKX_ERR_STATE, "tbtkx_moments: no model set");
 	TBTKX_REQUIRE(M > 0, TBTKX_ERR_ARG,
 	              "tbtkx_moments: numCoefficients has to be larger than 0");

@@ -6,13 +6,22 @@
  * flags and lookup table off. The GPU members that the reference builds from
  * Lib/src/cuda/ChebyshevSolver.cu call the tbtkx engine (include/tbtkx.h).
  *
- * This build computes on the GPU only. With setCalculateCoefficientsOnGPU
- * (false) or setGenerateGreensFunctionsOnGPU(false) the corresponding calls
- * exit with code 1 and say so, the mirror image of the reference's nocuda
- * stubs (Lib/src/nocuda/ChebyshevSolver.cpp:32-79). The GPU side is a
- * superset of the reference's: all four Green's function types (the
- * reference's GPU path supports Retarded only) and the direct evaluation
- * without a lookup table.
+ * Both settings of setCalculateCoefficientsOnGPU and
+ * setGenerateGreensFunctionsOnGPU run on the GPU, and both give the
+ * reference CPU path's results bit for bit: the FP64 recurrence of
+ * calculateCoefficientsCPU and, for the Green's functions, the lookup-table
+ * or direct evaluation of generateGreensFunctionCPU (Lib/src/Solver/
+ * ChebyshevExpander.cpp:103-199, 277-435). So the reference's callers, which
+ * set both flags false (Templates/BasicChebyshev/src/main.cpp:100-101,
+ * doc/examples/Solver/ChebyshevExpander.cpp:28-29), run unchanged. As in the
+ * reference, setGenerateGreensFunctionsOnGPU(true) requires the lookup table;
+ * unlike its GPU path it supports all four Green's function types. There is
+ * no CPU compute path: without a GPU the calls exit with code 1.
+ *
+ * Threads: each solver owns one engine handle, created on the device with
+ * the fewest live handles; calls on one solver from several threads run one
+ * after another (the handle is locked per call), calls on different solvers
+ * run concurrently.
  *
  * Additions (no reference counterpart):
  *  - setUseBlockRecurrence: diagonal queries of the property extractor
@@ -31,6 +40,7 @@
 
 #include <complex>
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 struct tbtkx_context;
@@ -131,6 +141,9 @@ private:
 	int lookupTableResolution = 0;
 
 	tbtkx_context *handle = nullptr;
+	/* Held by every computing call: threads sharing the solver run their
+	 * calls one after another, and the lookup table is built once. */
+	std::recursive_mutex callMutex;
 	const Model *uploadedModel = nullptr;
 	uint64_t uploadedGeneration = 0;
 	double uploadedScale = 0;

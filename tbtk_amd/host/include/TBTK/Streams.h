@@ -1,19 +1,21 @@
 /* Output streams with the muting switches the reference's tests use
- * (Streams::setStdMuteOut / setStdMuteErr, Lib/include/Utilities/TBTK/
- * Streams.h:67-163), reduced to what the Chebyshev path prints. */
+ * (Lib/include/Utilities/TBTK/Streams.h:67-163): Streams::out and
+ * Streams::err are std::ostreams writing to stdout and stderr until
+ * setStdMuteOut / setStdMuteErr is called (no log file). */
 #ifndef TBTK_AMD_STREAMS
 #define TBTK_AMD_STREAMS
 
+#include <ostream>
 #include <string>
 
 namespace TBTK {
 
 class Streams {
 public:
+	static std::ostream out;
+	static std::ostream err;
 	static void setStdMuteOut();
 	static void setStdMuteErr();
-	static void out(const std::string &s);
-	static void err(const std::string &s);
 };
 
 /* Global verbosity of Communicator (Lib/include/Utilities/TBTK/

@@ -17,7 +17,7 @@
 		tbtk_ss_ << "Error in " << function << "\n\t" << message     \
 		         << "\n\tHint: " << hint << "\n\tWhere: "            \
 		         << TBTKWhere << "\n";                               \
-		::TBTK::Streams::err(tbtk_ss_.str());                         \
+		::TBTK::Streams::err << tbtk_ss_.str() << std::flush;         \
 		std::exit(1);                                                 \
 	} while (0)
 

@@ -16,17 +16,37 @@ namespace TBTK {
 // ------------------------------------------------------------- Streams --
 
 namespace {
-bool g_muteOut = false, g_muteErr = false, g_globalVerbose = true;
-}
+bool g_globalVerbose = true;
 
-void Streams::setStdMuteOut() { g_muteOut = true; }
-void Streams::setStdMuteErr() { g_muteErr = true; }
-void Streams::out(const std::string &s) {
-	if (!g_muteOut) fputs(s.c_str(), stdout);
-}
-void Streams::err(const std::string &s) {
-	if (!g_muteErr) fputs(s.c_str(), stderr);
-}
+// Forwards to a C stream until muted.
+class MutableBuffer : public std::streambuf {
+public:
+	explicit MutableBuffer(FILE *target) : target(target) {}
+	bool muted = false;
+
+protected:
+	int overflow(int c) override {
+		if (c != EOF && !muted) fputc(c, target);
+		return c == EOF ? 0 : c;
+	}
+	std::streamsize xsputn(const char *s, std::streamsize n) override {
+		if (!muted) fwrite(s, 1, (size_t)n, target);
+		return n;
+	}
+	int sync() override { return fflush(target) == 0 ? 0 : -1; }
+
+private:
+	FILE *target;
+};
+
+MutableBuffer g_outBuffer(stdout), g_errBuffer(stderr);
+}  // namespace
+
+std::ostream Streams::out(&g_outBuffer);
+std::ostream Streams::err(&g_errBuffer);
+
+void Streams::setStdMuteOut() { g_outBuffer.muted = true; }
+void Streams::setStdMuteErr() { g_errBuffer.muted = true; }
 void Communicator::setGlobalVerbose(bool v) { g_globalVerbose = v; }
 bool Communicator::getGlobalVerbose() { return g_globalVerbose; }
 

@@ -76,13 +76,10 @@ void ChebyshevExpander::ensureHandle() {
 	tbtkx_device_count(&count);
 	TBTKAssert(count > 0, "Solver::ChebyshevExpander", "No GPU available.",
 	           "This build runs the Chebyshev path on an MI355X.");
-	// Take a free device from the pool for the handle, then return it: the
-	// handle keeps its own stream and state on that device.
-	int device = 0;
-	check(tbtkx_device_acquire(&device), "Solver::ChebyshevExpander");
-	const int rc = tbtkx_create(device, &handle);
-	tbtkx_device_release(device);
-	check(rc, "Solver::ChebyshevExpander");
+	// The handle lives on the device with the fewest live handles, so
+	// solvers of different threads spread over the GPUs; its own stream and
+	// state stay there for the solver's lifetime.
+	check(tbtkx_create(TBTKX_ANY_DEVICE, &handle), "Solver::ChebyshevExpander");
 }
 
 void ChebyshevExpander::ensureModelOnDevice() {
@@ -121,12 +118,14 @@ void ChebyshevExpander::applyDamping(std::vector<std::complex<double>> &c) const
 		c[n] = c[n] * std::sinh(lambda * (1 - n / (double)numCoefficients)) / std::sinh(lambda);
 }
 
+// Both settings of setCalculateCoefficientsOnGPU run the engine's FP64
+// recurrence, whose moments are bitwise those of calculateCoefficientsCPU
+// (Lib/src/Solver/ChebyshevExpander.cpp:103-199): the reference's dispatch
+// (Lib/include/TBTK/Solver/ChebyshevExpander.h:491-527) picks its CPU or GPU
+// member, here both are the same device path.
 std::vector<std::vector<std::complex<double>>> ChebyshevExpander::calculateCoefficients(
     std::vector<Index> &to, Index from) {
-	if (!calculateCoefficientsOnGPU)
-		TBTKExit("Solver::ChebyshevExpander::calculateCoefficients()",
-		         "This build computes Chebyshev coefficients on the GPU only.",
-		         "Call Solver::ChebyshevExpander::setCalculateCoefficientsOnGPU(true).");
+	std::lock_guard<std::recursive_mutex> lock(callMutex);
 	TBTKAssert(scaleFactor > 0, "ChebyshevExpander::calculateCoefficientsGPU()",
 	           "Scale factor must be larger than zero.",
 	           "Use ChebyshevExpander::setScaleFactor() to set scale factor.");
@@ -162,6 +161,7 @@ std::vector<std::complex<double>> ChebyshevExpander::calculateCoefficients(Index
 
 std::vector<std::vector<std::complex<double>>> ChebyshevExpander::calculateCoefficientsBlock(
     const std::vector<Index> &indices) {
+	std::lock_guard<std::recursive_mutex> lock(callMutex);
 	TBTKAssert(numCoefficients > 0, "ChebyshevExpander::calculateCoefficientsBlock()",
 	           "numCoefficients has to be larger than zero.", "");
 	ensureModelOnDevice();
@@ -186,6 +186,7 @@ std::vector<std::vector<std::complex<double>>> ChebyshevExpander::calculateCoeff
 
 std::vector<std::complex<double>> ChebyshevExpander::calculateTraceCoefficients(int numVectors,
                                                                                uint64_t seed) {
+	std::lock_guard<std::recursive_mutex> lock(callMutex);
 	TBTKAssert(numVectors > 0 && numCoefficients > 0,
 	           "ChebyshevExpander::calculateTraceCoefficients()",
 	           "numVectors and numCoefficients have to be larger than zero.", "");
@@ -265,12 +266,24 @@ int ChebyshevExpander::toTbtkxType(Type type) const {
 	         "Unkown Green's function type.", "This should never happen.");
 }
 
+// generateGreensFunction (Lib/include/TBTK/Solver/ChebyshevExpander.h:
+// 536-545) for many coefficient vectors. Either setting of
+// setGenerateGreensFunctionsOnGPU contracts on the device, bitwise as
+// generateGreensFunctionCPU (Lib/src/Solver/ChebyshevExpander.cpp:277-435):
+// with the lookup table through the host-built table, without it through
+// the direct evaluation (factors from the host's libm). As in the
+// reference, the GPU setting requires the lookup table
+// (ensureLookupTableIsReady, ChebyshevExpander.h:548-565), and it supports
+// all four types (the reference's GPU path only Retarded).
 std::vector<std::vector<std::complex<double>>> ChebyshevExpander::generateGreensFunctions(
     const std::vector<std::vector<std::complex<double>>> &coefficients, Type type) {
-	if (!generateGreensFunctionsOnGPU)
-		TBTKExit("Solver::ChebyshevExpander::generateGreensFunction()",
-		         "This build generates Green's functions on the GPU only.",
-		         "Call Solver::ChebyshevExpander::setGenerateGreensFunctionsOnGPU(true).");
+	std::lock_guard<std::recursive_mutex> lock(callMutex);
+	if (generateGreensFunctionsOnGPU && !useLookupTable)
+		TBTKExit("Solver::ChebyshevSolver::ensureLookupTableIsReady()",
+		         "Green's functions can only be generated on GPU using lookup tables.",
+		         "Use Solver::ChebyshevExpander::setGenerateGreensFunctionOnGPU() and "
+		         "Solver::ChebyshevExpander::setUseLookupTable() to configure the "
+		         "Solver::ChebyshevExpander.");
 	// Asserts of generateGreensFunctionCPU, ChebyshevExpander.cpp:281-317.
 	TBTKAssert(numCoefficients > 0, "ChebyshevExpander::generateGreensFunction()",
 	           "numCoefficients has to be larger than 0.", "");

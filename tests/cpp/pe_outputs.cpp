@@ -3,7 +3,13 @@
 // ... + HC), and writes the raw outputs to <outdir>/<name>.bin for
 // comparison with tests/golden/*.npz (written by the compiled reference).
 //
-// Usage: pe_outputs <outdir> [<crand.bin>]
+// Usage: pe_outputs <outdir> [<crand.bin>] [gpu]
+//
+// By default both GPU flags stay false, the reference's default and the
+// setting of its own callers; with "gpu" the coefficients run with
+// setCalculateCoefficientsOnGPU(true) and, where a lookup table is used,
+// setGenerateGreensFunctionsOnGPU(true). Both must give the reference's
+// outputs.
 #include "TBTK/Model.h"
 #include "TBTK/PropertyExtractor/ChebyshevExpander.h"
 #include "TBTK/Solver/ChebyshevExpander.h"
@@ -17,6 +23,7 @@ using namespace TBTK;
 using std::complex;
 
 static std::string outdir;
+static bool gpuFlags = false;
 
 template <typename T> static void write(const std::string &name, const std::vector<T> &v) {
 	FILE *f = fopen((outdir + "/" + name + ".bin").c_str(), "wb");
@@ -26,8 +33,15 @@ template <typename T> static void write(const std::string &name, const std::vect
 
 static void gpu(Solver::ChebyshevExpander &s) {
 	s.setVerbose(false);
-	s.setCalculateCoefficientsOnGPU(true);
-	s.setGenerateGreensFunctionsOnGPU(true);
+	s.setCalculateCoefficientsOnGPU(gpuFlags);
+	s.setGenerateGreensFunctionsOnGPU(false);
+}
+
+// The lookup-table setting, and with it the Green's-function flag (the
+// reference evaluates Green's functions on the GPU only through the table).
+static void lookup(Solver::ChebyshevExpander &s, bool useLookupTable) {
+	s.setUseLookupTable(useLookupTable);
+	s.setGenerateGreensFunctionsOnGPU(gpuFlags && useLookupTable);
 }
 
 // 30x30 with spin, README idiom: -t hopping, Zeeman -J*2*(1/2 - s).
@@ -77,6 +91,8 @@ int main(int argc, char **argv) {
 		return 2;
 	}
 	outdir = argv[1];
+	for (int a = 2; a < argc; a++)
+		if (std::string(argv[a]) == "gpu") gpuFlags = true;
 
 	{	// square30_spin: coefficients, broadening 0 and 1e-6
 		Model model;
@@ -93,15 +109,15 @@ int main(int argc, char **argv) {
 		write("square30_spin_coef_b1e-06", flatten(solver.calculateCoefficients(to, {15, 15, 0})));
 
 		// LDOS at (15,15,IDX_SUM_ALL) on [-9.99, 9.99] x 1000
-		for (int lookup = 0; lookup < 2; lookup++) {
-			solver.setUseLookupTable(lookup);
+		for (int table = 0; table < 2; table++) {
+			lookup(solver, table);
 			PropertyExtractor::ChebyshevExpander pe(solver);
 			pe.setEnergyWindow(-9.99, 9.99, 1000);
 			Property::LDOS ldos = pe.calculateLDOS({{15, 15, IDX_SUM_ALL}});
-			write("square30_spin_ldos_lookup" + std::to_string(lookup), ldos.getData());
+			write("square30_spin_ldos_lookup" + std::to_string(table), ldos.getData());
 		}
 		// Green's functions, 4 types, M = 400, (-9.5, 9.5) x 300, lookup
-		solver.setUseLookupTable(true);
+		lookup(solver, true);
 		solver.setNumCoefficients(400);
 		PropertyExtractor::ChebyshevExpander pe(solver);
 		pe.setEnergyWindow(-9.5, 9.5, 300);
@@ -188,7 +204,7 @@ int main(int argc, char **argv) {
 		// sampleDOS: 7 samples, seed 3, M = 200, (-9, 9) x 200
 		solver.setNumCoefficients(200);
 		solver.setBroadening(1e-6);
-		solver.setUseLookupTable(true);
+		lookup(solver, true);
 		PropertyExtractor::ChebyshevExpander pe(solver);
 		pe.setEnergyWindow(-9, 9, 200);
 		const Property::DOS dos = pe.sampleDOS(7, {}, 3);
@@ -207,7 +223,7 @@ int main(int argc, char **argv) {
 			solver.setModel(model);
 			solver.setScaleFactor(6);
 			solver.setNumCoefficients(200);
-			solver.setUseLookupTable(true);
+			lookup(solver, true);
 			solver.setUseBlockRecurrence(block == 1);
 			solver.setBlockFlags(4 | 1 | 2);   // FP64 vectors and H, doubling
 			PropertyExtractor::ChebyshevExpander pe(solver);
@@ -240,9 +256,43 @@ int main(int argc, char **argv) {
 		}
 	}
 
-	if (argc > 2) {	// Green's functions of given coefficients, 4 types
+	{	// The calls of Templates/BasicChebyshev/src/main.cpp:91-116 on its
+		// model (100x100 with spin, mu = -1, t = 1, open edges) with the
+		// window [-4.99, 4.99] (the template's [-5, 5] at s = 5 exits in the
+		// reference and here alike).
+		const int SIZE_X = 100, SIZE_Y = 100;
+		const complex<double> mu = -1.0, t = 1.0;
+		Model model;
+		model.setVerbose(false);
+		for (int x = 0; x < SIZE_X; x++)
+			for (int y = 0; y < SIZE_Y; y++)
+				for (int s = 0; s < 2; s++) {
+					model << HoppingAmplitude(-mu, {x, y, s}, {x, y, s});
+					if (x + 1 < SIZE_X) model << HoppingAmplitude(-t, {x + 1, y, s}, {x, y, s}) + HC;
+					if (y + 1 < SIZE_Y) model << HoppingAmplitude(-t, {x, y + 1, s}, {x, y, s}) + HC;
+				}
+		model.construct();
+		Solver::ChebyshevExpander solver;
+		solver.setVerbose(false);
+		solver.setModel(model);
+		solver.setScaleFactor(5.);
+		solver.setCalculateCoefficientsOnGPU(gpuFlags);
+		solver.setGenerateGreensFunctionsOnGPU(gpuFlags);
+		solver.setUseLookupTable(true);
+		solver.setNumCoefficients(500);
+		PropertyExtractor::ChebyshevExpander propertyExtractor(solver);
+		propertyExtractor.setEnergyWindow(-4.99, 4.99, 1000);
+		Property::LDOS ldos = propertyExtractor.calculateLDOS({{_a_, SIZE_Y / 2, IDX_SUM_ALL}});
+		write("basic_chebyshev_template_ldos", ldos.getData());
+	}
+
+	if (argc > 2 && std::string(argv[2]) != "gpu") {	// Green's functions of given coefficients, 4 types
 		std::vector<complex<double>> crand;
 		FILE *f = fopen(argv[2], "rb");
+		if (!f) {
+			fprintf(stderr, "cannot read %s\n", argv[2]);
+			return 2;
+		}
 		double v[2];
 		while (fread(v, 8, 2, f) == 2) crand.push_back(complex<double>(v[0], v[1]));
 		fclose(f);
@@ -256,16 +306,16 @@ int main(int argc, char **argv) {
 		typedef Solver::ChebyshevExpander::Type T;
 		const T types[4] = {T::Advanced, T::Retarded, T::Principal, T::NonPrincipal};
 		for (const Case &cs : cases)
-			for (int lookup = 0; lookup < 2; lookup++)
+			for (int table = 0; table < 2; table++)
 				for (int t = 0; t < 4; t++) {
 					Solver::ChebyshevExpander solver;
 					gpu(solver);
 					solver.setScaleFactor(10);
 					solver.setNumCoefficients((int)cs.c.size());
-					solver.setUseLookupTable(lookup);
+					lookup(solver, table);
 					solver.setEnergyWindow(Range(cs.lo, cs.hi, cs.E));
 					write(std::string("gf_types_") + cs.name + "_" + names[t] + "_lookup" +
-					          std::to_string(lookup),
+					          std::to_string(table),
 					      solver.generateGreensFunction(cs.c, types[t]));
 				}
 	}

@@ -14,6 +14,7 @@
 
 #include <complex>
 #include <limits>
+#include <thread>
 
 using namespace TBTK;
 using std::complex;
@@ -103,22 +104,46 @@ static Model periodicSquare(int SIZE, double mu, double t) {
 	return model;
 }
 
-// With the GPU flags off this build exits, as the reference's nocuda build
-// does for the GPU calls.
-TEST(Api, cpuPathExits) {
-	Model model = periodicSquare(5, -2, 1);
+// As in the reference (ensureLookupTableIsReady,
+// Lib/include/TBTK/Solver/ChebyshevExpander.h:548-565), Green's functions on
+// the GPU need the lookup table. The check comes before any device work.
+TEST(Api, gpuGreensFunctionNeedsLookupTable) {
 	Solver::ChebyshevExpander solver;
-	solver.setModel(model);
 	solver.setScaleFactor(10);
-	EXPECT_EXIT_CODE({
-		Streams::setStdMuteErr();
-		solver.calculateCoefficients({0, 0}, {0, 0});
-	}, 1);
-	std::vector<complex<double>> c = {1, 2, 3};
 	solver.setNumCoefficients(3);
+	solver.setGenerateGreensFunctionsOnGPU(true);
+	std::vector<complex<double>> c = {1, 2, 3};
 	EXPECT_EXIT_CODE({
 		Streams::setStdMuteErr();
 		solver.generateGreensFunction(c);
+	}, 1);
+}
+
+// Templates/BasicChebyshev/src/main.cpp:91-110 as written: the window
+// [-5, 5] at scale factor 5 is not strictly inside (-s, s), so the
+// reference exits there (Lib/include/TBTK/Solver/ChebyshevExpander.h:
+// 434-451), before any computation; so does the drop-in.
+TEST(Api, basicChebyshevTemplateWindow) {
+	Model model;
+	model.setVerbose(false);
+	for (int x = 0; x < 4; x++)
+		for (int s = 0; s < 2; s++) {
+			model << HoppingAmplitude(1.0, {x, 0, s}, {x, 0, s});
+			if (x + 1 < 4) model << HoppingAmplitude(-1.0, {x + 1, 0, s}, {x, 0, s}) + HC;
+		}
+	model.construct();
+	const double SCALE_FACTOR = 5.;
+	Solver::ChebyshevExpander solver;
+	solver.setModel(model);
+	solver.setScaleFactor(SCALE_FACTOR);
+	solver.setCalculateCoefficientsOnGPU(false);
+	solver.setGenerateGreensFunctionsOnGPU(false);
+	solver.setUseLookupTable(true);
+	solver.setNumCoefficients(500);
+	PropertyExtractor::ChebyshevExpander propertyExtractor(solver);
+	EXPECT_EXIT_CODE({
+		Streams::setStdMuteErr();
+		propertyExtractor.setEnergyWindow(-SCALE_FACTOR, SCALE_FACTOR, 1000);
 	}, 1);
 }
 
@@ -217,10 +242,13 @@ TEST(Api, occupationFunctions) {
 
 // ------------------------------------------------------------------ Gpu --
 
+// The reference's own tests run with its default flags (CPU path); here
+// the coefficient flag is set, the Green's-function flag left false so that
+// both the table and the direct evaluation are reachable.
 static void gpuSolver(Solver::ChebyshevExpander &solver) {
 	solver.setVerbose(false);
 	solver.setCalculateCoefficientsOnGPU(true);
-	solver.setGenerateGreensFunctionsOnGPU(true);
+	solver.setGenerateGreensFunctionsOnGPU(false);
 }
 
 TEST(Gpu, calculateCoefficients) {
@@ -376,6 +404,118 @@ TEST(Gpu, calculateDOS) {
 		peak = std::max(peak, exact(n));
 	}
 	EXPECT_TRUE(diff < 0.1 * peak);
+}
+
+// The reference's callers leave both GPU flags false
+// (Templates/BasicChebyshev/src/main.cpp:100-101,
+// doc/examples/Solver/ChebyshevExpander.cpp:28-29). Their results equal
+// those of the GPU flags bit for bit.
+TEST(Gpu, referenceDefaultFlags) {
+	Model model = periodicSquare(7, 0.4, 1);
+	typedef Solver::ChebyshevExpander::Type T;
+	const T types[4] = {T::Advanced, T::Retarded, T::Principal, T::NonPrincipal};
+	std::vector<std::vector<complex<double>>> c[2];
+	std::vector<complex<double>> g[2][4], direct[4];
+	for (int flags = 0; flags < 2; flags++) {
+		Solver::ChebyshevExpander solver;
+		solver.setVerbose(false);
+		solver.setModel(model);
+		solver.setScaleFactor(6);
+		solver.setNumCoefficients(150);
+		solver.setEnergyWindow(Range(-5.5, 5.5, 120));
+		if (flags) {
+			solver.setCalculateCoefficientsOnGPU(true);
+			solver.setGenerateGreensFunctionsOnGPU(true);
+		} else {
+			EXPECT_FALSE(solver.getCalculateCoefficientsOnGPU());
+			EXPECT_FALSE(solver.getGenerateGreensFunctionsOnGPU());
+			for (int t = 0; t < 4; t++)   // default: no lookup table
+				direct[t] = solver.generateGreensFunction(
+				    solver.calculateCoefficients({3, 3}, {3, 3}), types[t]);
+		}
+		solver.setUseLookupTable(true);
+		std::vector<Index> to = {{3, 3}, {4, 3}, {0, 6}};
+		c[flags] = solver.calculateCoefficients(to, {3, 3});
+		for (int t = 0; t < 4; t++) g[flags][t] = solver.generateGreensFunction(c[flags][0], types[t]);
+	}
+	for (int k = 0; k < 3; k++) EXPECT_TRUE(c[0][k] == c[1][k]);
+	for (int t = 0; t < 4; t++) {
+		EXPECT_TRUE(g[0][t] == g[1][t]);
+		// table and direct evaluation agree to rounding
+		double diff = 0, peak = 0;
+		for (size_t e = 0; e < direct[t].size(); e++) {
+			diff = std::max(diff, std::abs(direct[t][e] - g[0][t][e]));
+			peak = std::max(peak, std::abs(g[0][t][e]));
+		}
+		EXPECT_TRUE(diff <= 1e-12 * peak);
+	}
+}
+
+// The reference hands each thread its own device per GPU call
+// (Lib/src/Utilities/GPUResourceManager.cpp:44-90) and its property extractor
+// calls the solver from an OpenMP parallel loop
+// (Lib/src/PropertyExtractor/ChebyshevExpander.cpp:278). Here: solvers in
+// separate threads at once, and one solver shared by several threads, give
+// the single-threaded results bit for bit.
+static std::vector<double> threadWork(Model &model, int site) {
+	Solver::ChebyshevExpander solver;
+	gpuSolver(solver);
+	solver.setModel(model);
+	solver.setScaleFactor(6);
+	solver.setNumCoefficients(300);
+	solver.setUseLookupTable(site % 2 == 0);
+	PropertyExtractor::ChebyshevExpander pe(solver);
+	pe.setEnergyWindow(-5.5, 5.5, 200);
+	std::vector<double> out;
+	for (int rep = 0; rep < 3; rep++) {
+		Property::LDOS ldos = pe.calculateLDOS({{site, site, IDX_SUM_ALL}, {1, site, 0}});
+		out.insert(out.end(), ldos.getData().begin(), ldos.getData().end());
+	}
+	return out;
+}
+
+TEST(Gpu, threadsAndSolvers) {
+	Model model;
+	model.setVerbose(false);
+	for (int x = 0; x < 9; x++)
+		for (int y = 0; y < 9; y++)
+			for (int s = 0; s < 2; s++) {
+				if (x + 1 < 9) model << HoppingAmplitude(-1, {x + 1, y, s}, {x, y, s}) + HC;
+				if (y + 1 < 9) model << HoppingAmplitude(-1, {x, y + 1, s}, {x, y, s}) + HC;
+				model << HoppingAmplitude(0.3 * (s ? 1 : -1) + 0.01 * x, {x, y, s}, {x, y, s});
+			}
+	model.construct();
+	const int T = 4;
+	std::vector<double> single[T], multi[T];
+	for (int t = 0; t < T; t++) single[t] = threadWork(model, t + 2);
+	std::vector<std::thread> threads;
+	for (int t = 0; t < T; t++)
+		threads.emplace_back([&, t] { multi[t] = threadWork(model, t + 2); });
+	for (auto &th : threads) th.join();
+	for (int t = 0; t < T; t++) EXPECT_TRUE(single[t] == multi[t]);
+
+	// One solver, Green's functions and coefficients from several threads.
+	Solver::ChebyshevExpander solver;
+	gpuSolver(solver);
+	solver.setModel(model);
+	solver.setScaleFactor(6);
+	solver.setNumCoefficients(200);
+	solver.setUseLookupTable(true);
+	solver.setEnergyWindow(Range(-5.5, 5.5, 150));
+	typedef Solver::ChebyshevExpander::Type Ty;
+	std::vector<complex<double>> ref[T], got[T];
+	for (int t = 0; t < T; t++)
+		ref[t] = solver.generateGreensFunction(solver.calculateCoefficients({t, 4, 0}, {t, 4, 0}),
+		                                       Ty::NonPrincipal);
+	threads.clear();
+	for (int t = 0; t < T; t++)
+		threads.emplace_back([&, t] {
+			for (int rep = 0; rep < 3; rep++)
+				got[t] = solver.generateGreensFunction(
+				    solver.calculateCoefficients({t, 4, 0}, {t, 4, 0}), Ty::NonPrincipal);
+		});
+	for (auto &th : threads) th.join();
+	for (int t = 0; t < T; t++) EXPECT_TRUE(ref[t] == got[t]);
 }
 
 MINITEST_MAIN

@@ -151,8 +151,51 @@ def json_fixture(p1, pp):
         O.run_ref(out=out, json=os.path.join(jdir, name + ".json"), **kw)
 
 
+def callers_fixture():
+    """The reference's own callers of the path, which leave both GPU flags
+    false and use the lookup table.
+
+    doc_example_chebyshev.txt: stdout of doc/examples/Solver/
+    ChebyshevExpander.cpp compiled as written against the reference library
+    (oracle/_ref/doc_example_ref, built by oracle/Makefile.ref).
+
+    basic_chebyshev_template.npz: the LDOS of Templates/BasicChebyshev/src/
+    main.cpp:91-116 (100x100 with spin, mu = -1, t = 1, open edges; s = 5,
+    500 coefficients, 1000 energies, {_a_, 50, IDX_SUM_ALL}). As written the
+    template sets the window [-5, 5] at s = 5, which the reference rejects
+    (exit 1), so the fixture uses [-4.99, 4.99]. Stored as the SHA-256 of
+    the LDOS bytes (float64, little endian) and the rows x = 0, 50, 99."""
+    import hashlib
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "doc_example_ref")
+    r = subprocess.run([exe], check=True, capture_output=True, text=True,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+    with open(os.path.join(OUT, "doc_example_chebyshev.txt"), "w") as f:
+        f.write(r.stdout)
+    print("wrote doc_example_chebyshev.txt")
+    m = L.square_spin(100, 100, t=1.0, J=0.0, mu=-1.0)
+    path = os.path.join(TMP, "template.bin")
+    L.write_amplitudes(m, path)
+    out = os.path.join(TMP, "template_ldos.bin")
+    O.run_ref(mode="ldos", model=path, scale=5.0, M=500, broadening=1e-6,
+              window="-4.99,4.99,1000", lookup=1, patterns="*,50,+", out=out)
+    ldos = np.fromfile(out, np.float64)
+    assert ldos.size == 100 * 1000, ldos.size
+    rows = ldos.reshape(100, 1000)[[0, 50, 99]]
+    save("basic_chebyshev_template",
+         dict(model="square_spin(100, 100, t=1, J=0, mu=-1)", scale=5.0, M=500,
+              broadening=1e-6, window=[-4.99, 4.99, 1000], pattern="_a_,50,IDX_SUM_ALL",
+              rows=[0, 50, 99]),
+         sha256=np.array(hashlib.sha256(ldos.astype("<f8").tobytes()).hexdigest()),
+         rows=rows)
+
+
 def main(only=()):
-    """Write every fixture, or only those named in `only`."""
+    """Write every fixture, or only those named in `only` ("callers" alone
+    writes just the reference-caller fixtures)."""
+    if tuple(only) == ("callers",):
+        callers_fixture()
+        return
     if only:
         global save
         _save = save
@@ -260,6 +303,8 @@ def main(only=()):
 
     if not only or "json" in only:
         json_fixture(p1, pp)
+    if not only or "callers" in only:
+        callers_fixture()
 
 
 if __name__ == "__main__":

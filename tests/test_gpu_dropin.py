@@ -3,12 +3,15 @@ ChebyshevExpander on the tbtkx engine).
 
 tests/cpp/pe_outputs builds the fixture models with the public API (the
 loops of the reference's README and tests) and writes coefficients, LDOS,
-Green's functions of all four types (multi-to and pattern forms) and a
-sampleDOS; they must equal the reference's outputs in tests/golden bit for
-bit. The direct (no table) contraction runs on the device's acos/sincos, so
-those outputs are checked to 1e-13 relative instead. test_solver Gpu.* holds
-the reference's known-answer and closed-form tests.
+Green's functions of all four types (multi-to and pattern forms, with the
+lookup table and the direct evaluation) and a sampleDOS; they must equal the
+reference's outputs in tests/golden bit for bit, both with the reference's
+default flags (both GPU flags false, as its own callers set them) and with
+the GPU flags on. test_solver Gpu.* holds the reference's known-answer and
+closed-form tests; the reference's documentation example, compiled as
+written against the drop-in, must print the reference's text.
 """
+import hashlib
 import os
 import subprocess
 
@@ -23,13 +26,16 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tests", "cpp", "bin")
 
 
-@pytest.fixture(scope="module")
-def outputs(tmp_path_factory):
-    d = tmp_path_factory.mktemp("pe")
+@pytest.fixture(scope="module", params=["default_flags", "gpu_flags"])
+def outputs(request, tmp_path_factory):
+    d = tmp_path_factory.mktemp("pe_" + request.param)
     crand = d / "crand.bin"
     load_fixture("gf_types")["crand_coef"].tofile(crand)
-    r = subprocess.run([os.path.join(BIN, "pe_outputs"), str(d), str(crand)],
-                       capture_output=True, text=True, timeout=300)
+    cmd = [os.path.join(BIN, "pe_outputs"), str(d), str(crand)]
+    if request.param == "gpu_flags":
+        cmd.append("gpu")
+    print("running", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     return d
 
@@ -58,9 +64,8 @@ def test_ldos_and_gf_bitexact(outputs):
     fx = load_fixture("square30_spin_ldos")
     assert np.array_equal(_read(outputs, "square30_spin_ldos_lookup1", np.float64),
                           fx["ldos_lookup1"])
-    direct = _read(outputs, "square30_spin_ldos_lookup0", np.float64)
-    ref = fx["ldos_lookup0"]
-    assert np.abs(direct - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert np.array_equal(_read(outputs, "square30_spin_ldos_lookup0", np.float64),
+                          fx["ldos_lookup0"])
     for t in ("Advanced", "Retarded", "Principal", "NonPrincipal"):
         got = _read(outputs, f"square30_spin_gf_{t}", np.complex128).reshape(2, 300)
         assert np.array_equal(got, fx[f"gf_{t}"]), t
@@ -75,19 +80,45 @@ def test_gf_types_from_coefficients(outputs):
             ref1 = fx[f"{c}_{t}_lookup1"]
             assert np.array_equal(_read(outputs, f"gf_types_{c}_{t}_lookup1", np.complex128),
                                   ref1), (c, t)
-            ref0 = fx[f"{c}_{t}_lookup0"]
-            got0 = _read(outputs, f"gf_types_{c}_{t}_lookup0", np.complex128)
-            assert np.abs(got0 - ref0).max() <= 1e-13 * np.abs(ref0).max(), (c, t)
+            assert np.array_equal(_read(outputs, f"gf_types_{c}_{t}_lookup0", np.complex128),
+                                  fx[f"{c}_{t}_lookup0"]), (c, t)
 
 
 def test_ring_ldos_and_sampledos(outputs):
     fx = load_fixture("ring10_ldos")
     for f in ("0.9", "0.7"):
-        got = _read(outputs, f"ring10_ldos_{f}", np.float64)
-        ref = fx[f"ldos_{f}"]
-        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), f
+        assert np.array_equal(_read(outputs, f"ring10_ldos_{f}", np.float64),
+                              fx[f"ldos_{f}"]), f
     fx = load_fixture("periodic5_sampledos")
     assert np.array_equal(_read(outputs, "periodic5_sampledos", np.float64), fx["dos"])
+
+
+def test_basic_chebyshev_template(outputs):
+    """The calls of Templates/BasicChebyshev/src/main.cpp:91-116 (both GPU
+    flags false, lookup table, 100x100 with spin, 500 coefficients, LDOS
+    along y = 50 summed over spin) give the reference's LDOS bit for bit
+    (tests/golden/basic_chebyshev_template.npz: SHA-256 of all 100 x 1000
+    values and three rows)."""
+    fx = load_fixture("basic_chebyshev_template")
+    got = _read(outputs, "basic_chebyshev_template_ldos", np.float64)
+    assert got.size == 100 * 1000
+    rows = got.reshape(100, 1000)[[0, 50, 99]]
+    assert np.array_equal(rows, fx["rows"]), np.abs(rows - fx["rows"]).max()
+    assert hashlib.sha256(got.astype("<f8").tobytes()).hexdigest() == str(fx["sha256"])
+
+
+def test_documentation_example_as_written():
+    """doc/examples/Solver/ChebyshevExpander.cpp, compiled unchanged against
+    the drop-in (oracle/_ref/doc_example_dropin, built by oracle/
+    Makefile.ref), prints the text the same source prints when compiled
+    against the reference library (tests/golden/doc_example_chebyshev.txt)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "doc_example_dropin")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/doc_example_dropin not built (needs the reference sources)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    with open(os.path.join(ROOT, "tests", "golden", "doc_example_chebyshev.txt")) as f:
+        assert r.stdout == f.read()
 
 
 def _oracle_dos_inputs(O):

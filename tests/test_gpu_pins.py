@@ -125,3 +125,22 @@ def test_mfma_contraction_vs_oracle(engine, oracle_mod, chebyshev_coefficients, 
     err = np.abs(got - ref).max() / np.abs(ref).max()
     assert err < 3e-5, err
     engine.lut_destroy()
+
+
+@pytest.mark.parametrize("gtype", [X.RETARDED, X.ADVANCED, X.PRINCIPAL, X.NONPRINCIPAL])
+def test_direct_contraction_vs_oracle_bitwise(engine, oracle_mod, chebyshev_coefficients, gtype):
+    """The direct evaluation without a table (Lib/src/Solver/
+    ChebyshevExpander.cpp:371-432): factors from the host's libm, sums on
+    the device in the reference's order, equal to the oracle's bit for bit
+    for 40 vectors; then a second grid and a shorter moment count (the cached
+    factors are replaced), and the first again."""
+    c, s = chebyshev_coefficients
+    c = c[:40]
+    for (E, lo, hi, M) in ((700, -0.97 * s, 0.96 * s, c.shape[1]),
+                           (333, -0.5 * s, 0.8 * s, 157),
+                           (700, -0.97 * s, 0.96 * s, c.shape[1])):
+        de = (hi - lo) / (E - 1)
+        cm = np.ascontiguousarray(c[:, :M])
+        got = engine.contract_direct(cm, gtype, E, lo, de, s)
+        ref = np.array([oracle_mod.gf_direct(gtype, row, lo, hi, E, s) for row in cm])
+        assert np.array_equal(got, ref), (E, M, np.abs(got - ref).max())
