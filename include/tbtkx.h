@@ -178,6 +178,24 @@ int tbtkx_moments_block(tbtkx_handle handle, int num_sites,
                         const int64_t *sites, int num_moments, int flags,
                         double *out);
 
+/* Moments of many start vectors with many to-indices each:
+ * out[k*M + n] = <to_k| T_n(H/s) |from_j> for to_offsets[j] <= k <
+ * to_offsets[j+1] (to_offsets[0] = 0, num_from + 1 entries), bitwise equal
+ * to tbtkx_moments(from_j, its to-list) for every j, including its rule for
+ * a repeated to-index. The engine runs either one plain recurrence per
+ * from-index or exact FP64 blocks of 64-256 start vectors through one SpMM
+ * per step (CSR order, no fused multiply-add: every column reproduces the
+ * plain recurrence bit for bit; two real vectors per column for a real H),
+ * whichever moves fewer bytes (TBTKX_OPT_MULTI). This batches the
+ * reference's sequential per-index recurrences of
+ * PropertyExtractor::ChebyshevExpander::calculateGreensFunction (pattern
+ * form) and calculateLDOS (Lib/src/PropertyExtractor/ChebyshevExpander.cpp:
+ * 107-232, 475-505), each of which calls calculateCoefficients once per
+ * from-index (Lib/src/Solver/ChebyshevExpander.cpp:103-199). */
+int tbtkx_moments_multi(tbtkx_handle handle, int num_from, const int64_t *from,
+                        const int *to_offsets, const int64_t *to, int num_moments,
+                        double *out);
+
 /* Stochastic trace: mu[n] = sum_r <r| T_n(H/s) |r> over R random-phase
  * vectors r_i = exp(2*pi*i*u), u = the 53-bit splitmix64 uniform of
  * (seed, r*N + i). No reference counterpart (the reference's sampleDOS,
@@ -335,7 +353,12 @@ enum {
 	                           arithmetic, same moments, a third less HBM
 	                           traffic on a square lattice; 0 = full values.
 	                           Reading it gives 1 when the last plain call
-	                           used the palette */
+	                           used the palette */,
+	TBTKX_OPT_MULTI = 17    /* tbtkx_moments_multi: 0 = plain recurrences or
+	                           exact FP64 blocks, whichever moves fewer bytes
+	                           (default); 1 = plain recurrences; 2 = exact
+	                           blocks. Reading it gives what the last call
+	                           ran (1 plain, 2 blocks) */
 };
 int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
 

@@ -5,7 +5,7 @@
 // baseline of bench.py. It is never part of the product path.
 //
 // Usage: ref_driver mode=<mode> key=value ...
-//   modes: csr | coef | gf | gfcoef | ldos | sampledos | timeldos |
+//   modes: csr | coef | gf | gfpat | gfcoef | ldos | sampledos | timeldos |
 //          density | mag | splodos | expval
 //   model=<file>        binary HoppingAmplitude list (format in
 //                       tbtk_amd/lattice.py: write_amplitudes)
@@ -255,6 +255,23 @@ int main(int argc, char **argv){
 				flat.push_back(gf({t, from}, e));
 		writeBin(arg("out"), flat.data(), flat.size()*sizeof(complex<double>));
 		writeJSON(gf);
+		return 0;
+	}
+	// Pattern form: to=<to pattern> from=<from pattern>; written for every
+	// concrete (to, from) pair, to-index outer, from-index inner, in the
+	// model's index order of each pattern.
+	if(mode == "gfpat"){
+		const HoppingAmplitudeSet &has = model.getHoppingAmplitudeSet();
+		Index toPattern = parseIndex(arg("to")), fromPattern = parseIndex(arg("from"));
+		Property::GreensFunction gf = pe.calculateGreensFunction(
+			{{toPattern, fromPattern}}, propertyType(arg("type"))
+		);
+		vector<complex<double>> flat;
+		for(const Index &t : has.getIndexList(vector<Index>{toPattern}))
+			for(const Index &f : has.getIndexList(vector<Index>{fromPattern}))
+				for(int e = 0; e < res; e++)
+					flat.push_back(gf({t, f}, e));
+		writeBin(arg("out"), flat.data(), flat.size()*sizeof(complex<double>));
 		return 0;
 	}
 	// Thermodynamic-equilibrium properties: T= (K) and mu= (eV) set the

@@ -52,18 +52,27 @@ def _stale(src: str, obj: str, deps) -> bool:
 
 
 def build_engine(jobs: int = 8, changed_only: bool = False) -> str:
-    """Compile every .hip source (changed_only: only those newer than their
-    object or than a shared header) and link libtbtkx.so."""
+    """Compile every .hip source of csrc/ with hipcc and every .cpp source
+    with the host compiler (changed_only: only those newer than their object
+    or than a shared header) and link libtbtkx.so."""
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIB, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     objs = [os.path.join(OBJ, os.path.basename(s) + ".o") for s in srcs]
     deps = glob.glob(os.path.join(CSRC, "*.hpp")) + [os.path.join(ROOT, "include", "tbtkx.h")]
     todo = [(s, o) for s, o in zip(srcs, objs) if not changed_only or _stale(s, o, deps)]
+    # Host-only sources (host_math.cpp) go through the host C++ compiler,
+    # not HIP mode (see host_math.hpp).
+    csrcs = sorted(glob.glob(os.path.join(CSRC, "*.cpp")))
+    cobjs = [os.path.join(OBJ, os.path.basename(s) + ".o") for s in csrcs]
+    ctodo = [(s, o) for s, o in zip(csrcs, cobjs) if not changed_only or _stale(s, o, deps)]
+    cmds = [[HIPCC, *HIP_FLAGS, "-c", s, "-o", o] for s, o in todo]
+    cmds += [["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-c", s, "-o", o] for s, o in ctodo]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        list(ex.map(_run, [[HIPCC, *HIP_FLAGS, "-c", s, "-o", o] for s, o in todo]))
+        list(ex.map(_run, cmds))
     tmp = ENGINE + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, *cobjs, "-lpthread",
+          "-o", tmp])
     os.replace(tmp, ENGINE)
     return ENGINE
 

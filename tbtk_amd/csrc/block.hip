@@ -23,6 +23,7 @@
 //   12 Z (value + column) + 4 N (row pointer) + 24 B N (read Phi_n, read
 //   Phi_{n-1}, write Phi_{n+1}, 8 B each per element).
 #include <cmath>
+#include <map>
 #include <cstring>
 #include <type_traits>
 
@@ -264,17 +265,32 @@ __device__ __forceinline__ void gather_chunk(
 		Row<VT, CPL>::load(X + (int64_t)cc[u] * B + lane * CPL, xv[u]);
 }
 
-template <typename VT, typename AT, int CPL, int U, bool REALH>
+// EXACT: the reference's operation sequence instead of fused multiply-adds,
+// acc + (a.re x.re - a.im x.im, a.re x.im + a.im x.re) with every product and
+// sum rounded (for a real H: acc + a.re x, per component), non-zeros in CSR
+// order; with FP64 H and vectors each column then reproduces the plain
+// recurrence (and the reference's) bit for bit. Padding entries past a row's
+// end add (+-0).
+template <typename VT, typename AT, int CPL, int U, bool REALH, bool EXACT = false>
 __device__ __forceinline__ void accumulate_chunk(const CA<AT> (&aa)[U],
                                                  const VT (&xv)[U][CPL],
                                                  CA<AT> (&acc)[CPL]) {
+#pragma clang fp contract(off)
 #pragma unroll
 	for (int u = 0; u < U; u++) {
 #pragma unroll
 		for (int j = 0; j < CPL; j++) {
 			const CA<AT> x = to_acc<AT>(xv[u][j]);
 			const CA<AT> a = aa[u];
-			if (REALH) {
+			if (EXACT && REALH) {
+				acc[j].x = acc[j].x + a.x * x.x;
+				acc[j].y = acc[j].y + a.x * x.y;
+			} else if (EXACT) {
+				const AT pr = a.x * x.x - a.y * x.y;
+				const AT pi = a.x * x.y + a.y * x.x;
+				acc[j].x = acc[j].x + pr;
+				acc[j].y = acc[j].y + pi;
+			} else if (REALH) {
 				acc[j].x = fma(a.x, x.x, acc[j].x);
 				acc[j].y = fma(a.x, x.y, acc[j].y);
 			} else {
@@ -293,7 +309,7 @@ __device__ __forceinline__ void accumulate_chunk(const CA<AT> (&aa)[U],
 // to mu[b*M + nprev]. REALH: every value of H is real (2 FMAs per complex
 // product instead of 4).
 template <typename VT, typename HT, typename AT, int B, bool FIRST, bool DOTS,
-          bool REALH>
+          bool REALH, bool EXACT = false>
 __global__ __launch_bounds__(kThreads) void spmm_step(
     const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ col,
     const HT *__restrict__ v, const VT *__restrict__ X, VT *Y, int64_t N,
@@ -394,13 +410,13 @@ __global__ __launch_bounds__(kThreads) void spmm_step(
 						acc[j] = CA<AT>{-p.x, -p.y};
 					}
 				}
-				accumulate_chunk<VT, AT, CPL, kUnroll, REALH>(aa[r], xv[r], acc);
+				accumulate_chunk<VT, AT, CPL, kUnroll, REALH, EXACT>(aa[r], xv[r], acc);
 				// Rows longer than kUnroll (none in nearest-neighbour
 				// lattices) continue chunk by chunk.
 				for (uint32_t k0 = kb[r] + kUnroll; k0 < ke[r]; k0 += kUnroll) {
 					gather_chunk<VT, HT, AT, CPL, kUnroll>(col, v, X, kb0, wcol, wval,
 					                                       k0, ke[r], lane, aa[r], xv[r]);
-					accumulate_chunk<VT, AT, CPL, kUnroll, REALH>(aa[r], xv[r], acc);
+					accumulate_chunk<VT, AT, CPL, kUnroll, REALH, EXACT>(aa[r], xv[r], acc);
 				}
 				if (i0 + r >= nrows) continue;
 				VT st[CPL];
@@ -844,6 +860,26 @@ __global__ __launch_bounds__(kThreads) void gather_block(
 	if (g < live) mu[(int64_t)g * M + n] = site_moment<VT>(X, sites, g, B, packed);
 }
 
+// General moment gather: mu[l*M + n] = component of Phi_n at (row[l],
+// vector vec[l]) for every entry l < L (vector v in column v, or with packed
+// real vectors in the real (v even) or imaginary (v odd) part of column v/2).
+template <typename VT>
+__global__ __launch_bounds__(kThreads) void gather_list(
+    const VT *__restrict__ X, const int64_t *__restrict__ row, const int *__restrict__ vec,
+    int L, int B, double2 *__restrict__ mu, int M, int n, int packed) {
+	const int l = blockIdx.x * kThreads + threadIdx.x;
+	if (l >= L) return;
+	const int v = vec[l];
+	double2 x;
+	if (packed) {
+		const double2 c = C2<VT>::load(X[row[l] * B + (v >> 1)]);
+		x = make_double2((v & 1) ? c.y : c.x, 0.0);
+	} else {
+		x = C2<VT>::load(X[row[l] * B + v]);
+	}
+	mu[(int64_t)l * M + n] = x;
+}
+
 inline unsigned grid_for(int64_t n) {
 	return (unsigned)((n + kThreads - 1) / kThreads);
 }
@@ -859,12 +895,16 @@ struct BlockRun {
 	int share = 1;     // grid = occupancy x CUs / share
 	int rmul = 1;      // rows in flight multiplier of the strip step (1 or 2)
 	StripMap sm{};
+	bool exact = false;   // CSR step in the reference's operation order
 };
 
 // The strip step with doubled rows in flight is built for FP32 H and
 // vectors only (the batched default).
 template <typename VT, typename HT>
 constexpr bool kFp32Pair = std::is_same<VT, float2>::value && std::is_same<HT, float2>::value;
+// The exact step exists for FP64 H and vectors only.
+template <typename VT, typename HT>
+constexpr bool kFp64Pair = std::is_same<VT, double2>::value && std::is_same<HT, double2>::value;
 
 // Strip map for S rows per strip on the resident model (ELL layout), tiles
 // of T rows (the step kernel's rows in flight).
@@ -895,7 +935,7 @@ int launch_step(BlockRun &r, bool first, bool dots, const VT *X, VT *Y,
 	using AT = typename std::conditional<std::is_same<HT, float2>::value &&
 	                                         std::is_same<VT, float2>::value,
 	                                     float, double>::type;
-	if (h->ell_W > 0 && h->use_ell) {
+	if (h->ell_W > 0 && h->use_ell && !r.exact) {
 		const uint32_t *ec = h->ell_col.as<uint32_t>();
 		const HT *ev;
 		if (std::is_same<HT, float2>::value)
@@ -943,9 +983,21 @@ int launch_step(BlockRun &r, bool first, bool dots, const VT *X, VT *Y,
 		return TBTKX_OK;
 	}
 #define TBTKX_STEP(F, D, R)                                                    \
-	spmm_step<VT, HT, AT, B, F, D, R><<<r.nwg, kThreads, 0, h->stream>>>(  \
-	    rp, cl, v, X, Y, h->N, r.tile, r.nwg, h->xcd_remap, h->nt_stream,   \
-	    part, sites, live, mu, M, nprev, packed)
+	do {                                                                   \
+		if constexpr (kFp64Pair<VT, HT>) {                             \
+			if (r.exact) {                                         \
+				spmm_step<VT, HT, AT, B, F, D, R, true>        \
+				    <<<r.nwg, kThreads, 0, h->stream>>>(       \
+				        rp, cl, v, X, Y, h->N, r.tile, r.nwg,  \
+				        h->xcd_remap, h->nt_stream, part, sites, \
+				        live, mu, M, nprev, packed);           \
+				break;                                         \
+			}                                                      \
+		}                                                              \
+		spmm_step<VT, HT, AT, B, F, D, R><<<r.nwg, kThreads, 0, h->stream>>>( \
+		    rp, cl, v, X, Y, h->N, r.tile, r.nwg, h->xcd_remap, h->nt_stream, \
+		    part, sites, live, mu, M, nprev, packed);                  \
+	} while (0)
 #define TBTKX_STEP_R(F, D)                                                     \
 	do {                                                                   \
 		if (h->h_real) TBTKX_STEP(F, D, true);                         \
@@ -1034,7 +1086,7 @@ int persistent_grid(tbtkx_context *h, bool dots, BlockRun &r) {
 	                                         std::is_same<VT, float2>::value,
 	                                     float, double>::type;
 	const void *fn;
-	const bool ell = h->ell_W > 0 && h->use_ell;
+	const bool ell = h->ell_W > 0 && h->use_ell && !r.exact;
 	if (!ell) r.S = -1;
 	if (ell) {
 		// The occupancy of the steady-state ELL variant.
@@ -1055,6 +1107,12 @@ int persistent_grid(tbtkx_context *h, bool dots, BlockRun &r) {
 		else
 			fn = h->h_real ? ell_fn<VT, HT, AT, B, false, true>(h->ell_W, strip, r.rmul)
 			               : ell_fn<VT, HT, AT, B, false, false>(h->ell_W, strip, r.rmul);
+	} else if (r.exact) {
+		if constexpr (kFp64Pair<VT, HT>)
+			fn = h->h_real ? (const void *)spmm_step<VT, HT, AT, B, false, false, true, true>
+			               : (const void *)spmm_step<VT, HT, AT, B, false, false, false, true>;
+		else
+			return TBTKX_ERR_STATE;
 	} else if (dots) {
 		fn = h->h_real ? (const void *)spmm_step<VT, HT, AT, B, false, true, true>
 		               : (const void *)spmm_step<VT, HT, AT, B, false, true, false>;
@@ -1319,6 +1377,111 @@ int dispatch_width(tbtkx_context *h, int B, int total, const int64_t *sites,
 	}
 }
 
+// Exact FP64 blocks for many unit start vectors (tbtkx_moments_multi): each
+// column (or packed real half-column) runs the plain recurrence of its start
+// vector bit for bit (spmm_step<..., EXACT>: CSR order, no fused
+// multiply-add), and after every step a gather list copies the requested
+// components of Phi_n into the moments. Vectors are taken vpb at a time
+// (B columns, two real vectors per column when packed); entries of the
+// gather list are ordered by vector, so every block gathers one contiguous
+// range of them.
+template <int B>
+int run_exact(tbtkx_context *h, int nvec, const int64_t *from, const std::vector<int64_t> &grow,
+              const std::vector<int> &gvec, int M, int packed, std::vector<double2> &mu) {
+	constexpr int kRpi = B >= 256 ? 1 : 256 / B;
+	int tile = h->tile_rows > 0 ? h->tile_rows : 4;
+	tile = (tile + kRpi - 1) / kRpi * kRpi;
+	if (tile > 32) tile = 32;
+	BlockRun r{h, B, TBTKX_VEC_FP64 | TBTKX_H_FP64, 0, tile, 0};
+	r.exact = true;
+	int rc = persistent_grid<double2, double2, B>(h, false, r);
+	if (rc) return rc;
+	const int vpb = packed ? 2 * B : B;
+	const int nblocks = (nvec + vpb - 1) / vpb;
+	const int L = (int)grow.size();
+	// first gather entry of every block (entries are ordered by vector)
+	std::vector<int> first(nblocks + 1, L);
+	for (int l = L - 1; l >= 0; l--) first[gvec[l] / vpb] = l;
+	for (int b = nblocks - 1; b >= 0; b--)
+		if (first[b] > first[b + 1]) first[b] = first[b + 1];
+	std::vector<int> lvec(L);
+	for (int l = 0; l < L; l++) lvec[l] = gvec[l] % vpb;
+	if ((rc = h->blk_x.reserve((size_t)h->N * B * 16))) return rc;
+	if ((rc = h->blk_y.reserve((size_t)h->N * B * 16))) return rc;
+	if ((rc = h->blk_mu.reserve((size_t)(L > 0 ? L : 1) * M * 16))) return rc;
+	if ((rc = h->blk_sites.reserve((size_t)nvec * 8))) return rc;
+	if ((rc = h->blk_grow.reserve((size_t)(L > 0 ? L : 1) * 8))) return rc;
+	if ((rc = h->blk_gvec.reserve((size_t)(L > 0 ? L : 1) * 4))) return rc;
+	TBTKX_CHECK_HIP(hipMemcpyAsync(h->blk_sites.ptr, from, (size_t)nvec * 8,
+	                               hipMemcpyHostToDevice, h->stream));
+	if (L > 0) {
+		TBTKX_CHECK_HIP(hipMemcpyAsync(h->blk_grow.ptr, grow.data(), (size_t)L * 8,
+		                               hipMemcpyHostToDevice, h->stream));
+		TBTKX_CHECK_HIP(hipMemcpyAsync(h->blk_gvec.ptr, lvec.data(), (size_t)L * 4,
+		                               hipMemcpyHostToDevice, h->stream));
+	}
+	double2 *X0 = h->blk_x.as<double2>(), *Y0 = h->blk_y.as<double2>();
+	double2 *mu_dev = h->blk_mu.as<double2>();
+	const int64_t *grow_dev = h->blk_grow.as<int64_t>();
+	const int *gvec_dev = h->blk_gvec.as<int>();
+	TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
+	for (int blk = 0; blk < nblocks; blk++) {
+		const int live = nvec - blk * vpb < vpb ? nvec - blk * vpb : vpb;
+		const int g0 = first[blk], gl = first[blk + 1] - first[blk];
+		double2 *X = X0, *Y = Y0;
+		block_init<double2><<<grid_for(h->N * B), kThreads, 0, h->stream>>>(
+		    X, h->N, B, live, 0, h->blk_sites.as<int64_t>() + (int64_t)blk * vpb, 0, 0, packed);
+		for (int n = 0; n < M; n++) {
+			if (gl > 0)
+				gather_list<double2><<<grid_for(gl), kThreads, 0, h->stream>>>(
+				    X, grow_dev + g0, gvec_dev + g0, gl, B, mu_dev + (int64_t)g0 * M, M, n,
+				    packed);
+			if (n == M - 1) break;
+			// Phi_{n+1} from Phi_n (X) and Phi_{n-1} (Y), into Y.
+			if ((rc = launch_step<double2, double2, B>(r, n == 0, false, X, Y, nullptr,
+			                                           nullptr, 0, nullptr, M, n, packed)))
+				return rc;
+			double2 *t = X;
+			X = Y;
+			Y = t;
+		}
+		TBTKX_CHECK_HIP(hipGetLastError());
+	}
+	TBTKX_CHECK_HIP(hipEventRecord(h->ev_stop, h->stream));
+	mu.assign((size_t)L * M, make_double2(0, 0));
+	if (L > 0)
+		TBTKX_CHECK_HIP(hipMemcpyAsync(mu.data(), mu_dev, (size_t)L * M * 16,
+		                               hipMemcpyDeviceToHost, h->stream));
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	float ms = 0;
+	TBTKX_CHECK_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
+	h->last_ms = ms;
+	h->last_launches = (int64_t)nblocks * (M - 1);
+	h->last_block_width = B;
+	h->last_real = packed;
+	return TBTKX_OK;
+}
+
+// Algorithmic HBM bytes of the whole call: plain recurrences (one per start
+// vector, in the plain kernel's current form) or exact FP64 blocks of width
+// B (20 Z + 4 N for FP64 H and row pointers, 48 B N for the three block
+// passes, per step and block).
+double plain_call_bytes(const tbtkx_context *h, int nvec) {
+	const double N = (double)h->N, Z = (double)h->Z;
+	const bool real = h->real_path && h->h_real;
+	const bool c16 = h->plain_kernel == 0 && h->use_col16 && h->col16_ok;
+	const bool pal = c16 && h->use_palette && h->palette_ok;
+	const double per = real ? (pal ? 3 * Z + 36 * N : c16 ? 10 * Z + 28 * N : 12 * Z + 28 * N)
+	                        : (pal ? 3 * Z + 68 * N : c16 ? 18 * Z + 52 * N : 20 * Z + 52 * N);
+	return per * nvec;
+}
+double block_call_bytes(const tbtkx_context *h, int nvec, int B, int packed) {
+	const double N = (double)h->N, Z = (double)h->Z;
+	const int cols = packed ? (nvec + 1) / 2 : nvec;
+	const int blocks = (cols + B - 1) / B;
+	return blocks * (20 * Z + 4 * N + 48.0 * B * N);
+}
+
 int block_entry(tbtkx_context *h, int total, const int64_t *sites, int mode,
                 uint64_t seed, int M, int flags, std::vector<double2> &mu) {
 	int rc = set_device(h);
@@ -1402,6 +1565,76 @@ int tbtkx_moments_block(tbtkx_handle h, int num_sites, const int64_t *sites,
 	int rc = block_entry(h, num_sites, sites, 0, 0, M, flags, mu);
 	if (rc) return rc;
 	memcpy(out, mu.data(), mu.size() * 16);
+	return TBTKX_OK;
+}
+
+int tbtkx_moments_multi(tbtkx_handle h, int num_from, const int64_t *from,
+                        const int *to_offsets, const int64_t *to, int M, double *out) {
+	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_moments_multi: null handle");
+	TBTKX_LOCK(h);
+	TBTKX_REQUIRE(h->N > 0, TBTKX_ERR_STATE, "tbtkx_moments_multi: no model set");
+	TBTKX_REQUIRE(M > 0, TBTKX_ERR_ARG,
+	              "tbtkx_moments_multi: numCoefficients has to be larger than 0");
+	TBTKX_REQUIRE(num_from >= 0 && (num_from == 0 || (from && to_offsets)), TBTKX_ERR_ARG,
+	              "tbtkx_moments_multi: bad from-index list");
+	if (num_from == 0) return TBTKX_OK;
+	TBTKX_REQUIRE(to_offsets[0] == 0, TBTKX_ERR_ARG,
+	              "tbtkx_moments_multi: to_offsets[0] must be 0");
+	for (int j = 0; j < num_from; j++) {
+		TBTKX_REQUIRE(from[j] >= 0 && from[j] < h->N, TBTKX_ERR_ARG,
+		              "tbtkx_moments_multi: from-index outside the basis");
+		TBTKX_REQUIRE(to_offsets[j + 1] >= to_offsets[j], TBTKX_ERR_ARG,
+		              "tbtkx_moments_multi: to_offsets not ascending");
+	}
+	const int T = to_offsets[num_from];
+	TBTKX_REQUIRE(T == 0 || (to && out), TBTKX_ERR_ARG, "tbtkx_moments_multi: null array");
+	for (int k = 0; k < T; k++)
+		TBTKX_REQUIRE(to[k] >= 0 && to[k] < h->N, TBTKX_ERR_ARG,
+		              "tbtkx_moments_multi: to-index outside the basis");
+	if (T == 0) return TBTKX_OK;
+	int rc = set_device(h);
+	if (rc) return rc;
+	const int packed = h->real_path && h->h_real;
+	int B = 64;
+	for (int b : {128, 256})
+		if (block_call_bytes(h, num_from, b, packed) < block_call_bytes(h, num_from, B, packed))
+			B = b;
+	bool blocks = h->multi_mode == 2 ||
+	              (h->multi_mode == 0 &&
+	               block_call_bytes(h, num_from, B, packed) < plain_call_bytes(h, num_from));
+	if (!blocks) {
+		for (int j = 0; j < num_from; j++) {
+			const int k0 = to_offsets[j], k1 = to_offsets[j + 1];
+			if (k1 == k0) continue;
+			if ((rc = tbtkx_moments(h, from[j], k1 - k0, to + k0, M, out + (size_t)k0 * M * 2)))
+				return rc;
+		}
+		h->last_multi_blocks = 0;
+		return TBTKX_OK;
+	}
+	// Gather entries: per from-index its distinct to-indices (a repeated
+	// to-index keeps only its last position, as tbtkx_moments), ordered by
+	// from-index.
+	std::vector<int64_t> grow;
+	std::vector<int> gvec, gslot;
+	for (int j = 0; j < num_from; j++) {
+		std::map<int64_t, int> last;
+		for (int k = to_offsets[j]; k < to_offsets[j + 1]; k++) last[to[k]] = k;
+		for (auto &kv : last) {
+			grow.push_back(kv.first);
+			gvec.push_back(j);
+			gslot.push_back(kv.second);
+		}
+	}
+	std::vector<double2> mu;
+	if (B == 64) rc = run_exact<64>(h, num_from, from, grow, gvec, M, packed, mu);
+	else if (B == 128) rc = run_exact<128>(h, num_from, from, grow, gvec, M, packed, mu);
+	else rc = run_exact<256>(h, num_from, from, grow, gvec, M, packed, mu);
+	if (rc) return rc;
+	memset(out, 0, (size_t)T * M * 16);
+	for (size_t l = 0; l < grow.size(); l++)
+		memcpy(out + (size_t)gslot[l] * M * 2, mu.data() + l * M, (size_t)M * 16);
+	h->last_multi_blocks = 1;
 	return TBTKX_OK;
 }
 

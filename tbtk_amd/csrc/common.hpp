@@ -152,6 +152,9 @@ struct tbtkx_context {
 
 	// Batched path buffers.
 	tbtkx::DevBuf blk_x, blk_y, blk_mu, blk_sites, blk_part;
+	// Gather list of the exact blocks (tbtkx_moments_multi): rows and
+	// vectors within the block.
+	tbtkx::DevBuf blk_grow, blk_gvec;
 	int64_t csr_gen = 0;                          // bumped by every tbtkx_set_csr
 	int64_t blk_tune_key[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
 	int blk_tune_B = 0, blk_tune_S = -1, blk_tune_share = 1, blk_tune_rmul = 1;
@@ -201,6 +204,8 @@ struct tbtkx_context {
 	int strip_chunk = 0;   // consecutive strip tiles per workgroup (multiple
 	                       // of 4; 0 = 4)
 	int rows_in_flight = 0;   // strip step rows-in-flight multiplier (0 = tuned)
+	int multi_mode = 0;    // tbtkx_moments_multi: 0 by cost, 1 plain, 2 exact blocks
+	int last_multi_blocks = 0;   // the last tbtkx_moments_multi ran exact blocks
 	int real_path = 1;     // real arithmetic for real H and real start vectors
 	int last_real = 0;     // the last recurrence call ran in real arithmetic
 

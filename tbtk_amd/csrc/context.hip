@@ -198,7 +198,7 @@ int tbtkx_destroy(tbtkx_handle h) {
 	                  &h->lat_part, &h->lat_tmp, &h->rblk, &h->col16, &h->col16_flag,
 	                  &h->pcode, &h->pal, &h->palr, &h->pdiag, &h->pdiagr, &h->vec_a, &h->vec_b, &h->coef, &h->to,
 	                  &h->lut, &h->gf_in, &h->gf_out, &h->dfac, &h->dden, &h->blk_x, &h->blk_y,
-	                  &h->blk_mu, &h->blk_sites, &h->blk_part, &h->cplanes,
+	                  &h->blk_mu, &h->blk_sites, &h->blk_part, &h->blk_grow, &h->blk_gvec, &h->cplanes,
 	                  &h->lplanes, &h->gf_mfma_out, &h->ell_col, &h->ell_v1f,
 	                  &h->ell_v2f, &h->ell_v1d, &h->ell_v2d,
 	                  &h->mm_ecol, &h->mm_ev1, &h->mm_ev2, &h->mm_desc, &h->mm_tab,
@@ -446,6 +446,11 @@ int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
 		h->use_palette = (int)value;
 		h->plain_graph.reset();
 		return TBTKX_OK;
+	case TBTKX_OPT_MULTI:
+		TBTKX_REQUIRE(value >= 0 && value <= 2, TBTKX_ERR_ARG,
+		              "tbtkx_set_option: multi-vector mode must be 0, 1 or 2");
+		h->multi_mode = (int)value;
+		return TBTKX_OK;
 	default:
 		set_error("tbtkx_set_option: unknown option " + std::to_string(option));
 		return TBTKX_ERR_ARG;
@@ -467,6 +472,7 @@ int tbtkx_get_option(tbtkx_handle h, int option, int64_t *value) {
 	case TBTKX_OPT_ELL: *value = h->use_ell && h->ell_W > 0; return TBTKX_OK;
 	case TBTKX_OPT_AUTOTUNE: *value = h->autotune; return TBTKX_OK;
 	case TBTKX_OPT_MFMA_PIPE: *value = h->mfma_pipe; return TBTKX_OK;
+	case TBTKX_OPT_MULTI: *value = h->last_multi_blocks ? 2 : 1; return TBTKX_OK;
 	case TBTKX_OPT_STRIP_ROWS:
 		*value = h->strip_rows ? h->strip_rows : h->last_strip;
 		return TBTKX_OK;

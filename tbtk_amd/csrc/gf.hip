@@ -12,11 +12,8 @@
 //
 // Traffic: the table is read once per coefficient vector by a thread per
 // energy (16 B per (n, e)); it is an FP64 GEMV/GEMM of 8 K M E flops.
-#include <algorithm>
-#include <complex>
-#include <thread>
-
 #include "common.hpp"
+#include "host_math.hpp"
 
 namespace tbtkx {
 namespace {
@@ -222,49 +219,19 @@ int apply_damping(tbtkx_context *h, double2 *C, int K, int M) {
 // The factors of the direct contraction for one Green's function kind (0:
 // e^{-in theta} for Retarded and Advanced, 1: sin(n theta), 2: cos(n theta))
 // over n = 1 .. M-1 and the energies e0 + e*de, computed on the host with the
-// expressions of generateGreensFunctionCPU (Lib/src/Solver/
-// ChebyshevExpander.cpp:386-430: acos, std::exp of a complex, sin, cos,
-// sqrt), so the device sums use the reference's values. Kept on the device
-// until another kind, order or grid is asked for.
+// expressions of generateGreensFunctionCPU (host_math.cpp), so the device
+// sums use the reference's values. Kept on the device until another kind,
+// order or grid is asked for.
 static int direct_factors(tbtkx_context *h, int kind, int M, int E, double e0, double de,
                           double s) {
 	if (h->dfac_kind == kind && h->dfac_M == M && h->dfac_E == E && h->dfac_e0 == e0 &&
 	    h->dfac_de == de && h->dfac_s == s)
 		return TBTKX_OK;
 	h->dfac_kind = -1;
-	std::vector<double> acosE(E), den(E);
-	for (int e = 0; e < E; e++) {
-		const double En = (e0 + e * de) / s;
-		acosE[e] = std::acos(En);
-		den[e] = std::sqrt(1 - En * En);
-	}
 	const size_t per = kind == 0 ? 2 : 1;
 	const size_t rows = M > 1 ? (size_t)(M - 1) : 0;
-	std::vector<double> fac(rows * E * per + per);
-	auto fill = [&](int first, int stride) {
-		const std::complex<double> i(0, 1);
-		for (int n = 1 + first; n < M; n += stride) {
-			double *row = fac.data() + (size_t)(n - 1) * E * per;
-			for (int e = 0; e < E; e++) {
-				if (kind == 0) {
-					const std::complex<double> v = std::exp(-i * ((double)n) * acosE[e]);
-					row[2 * e] = v.real();
-					row[2 * e + 1] = v.imag();
-				} else if (kind == 1) {
-					row[e] = std::sin(((double)n) * acosE[e]);
-				} else {
-					row[e] = std::cos(((double)n) * acosE[e]);
-				}
-			}
-		}
-	};
-	const int threads = (int)std::min<size_t>(
-	    std::max(1u, std::min(16u, std::thread::hardware_concurrency())),
-	    rows / 64 + 1);
-	std::vector<std::thread> pool;
-	for (int t = 1; t < threads; t++) pool.emplace_back(fill, t, threads);
-	fill(0, threads);
-	for (auto &t : pool) t.join();
+	std::vector<double> fac(rows * E * per + per), den(E);
+	direct_factor_table(kind, M, E, e0, de, s, fac.data(), den.data());
 	int rc;
 	if ((rc = h->dfac.reserve(fac.size() * 8))) return rc;
 	if ((rc = h->dden.reserve((size_t)E * 8))) return rc;

@@ -20,7 +20,9 @@ H_FP64, DOUBLING, VEC_FP64 = 1, 2, 4
 DAMP_NONE, DAMP_LORENTZ, DAMP_JACKSON = 0, 1, 2
 (OPT_PLAIN_KERNEL, OPT_GRAPH, OPT_BLOCK_WIDTH, OPT_XCD_REMAP, OPT_TILE_ROWS, OPT_NT_STREAM,
  OPT_ELL, OPT_AUTOTUNE, OPT_MFMA_PIPE, OPT_STRIP_ROWS, OPT_GRID_SHARE,
- OPT_STRIP_CHUNK, OPT_ROWS_IN_FLIGHT, OPT_REAL, OPT_COL16, OPT_PALETTE) = range(1, 17)
+ OPT_STRIP_CHUNK, OPT_ROWS_IN_FLIGHT, OPT_REAL, OPT_COL16, OPT_PALETTE,
+ OPT_MULTI) = range(1, 18)
+ANY_DEVICE = -1
 
 # (name, argtypes) of every exported function, in header order.
 _P = ctypes.c_void_p
@@ -45,6 +47,7 @@ SIGNATURES = {
     "tbtkx_contract_mfma": [_P, _I, _P],
     "tbtkx_set_damping": [_P, _I, _D],
     "tbtkx_moments_block": [_P, _I, _P, _I, _I, _P],
+    "tbtkx_moments_multi": [_P, _I, _P, _P, _P, _I, _P],
     "tbtkx_moments_trace": [_P, _I, ctypes.c_uint64, _I, _I, _P],
     "tbtkx_set_lattice": [_P, _P, _P, _D],
     "tbtkx_lattice_gershgorin": [_P, _P, _P, _P],
@@ -230,6 +233,19 @@ class Engine:
         out = np.zeros((sites.size, M), dtype=np.complex128)
         _check(self._lib.tbtkx_moments_block(self._h, sites.size, _p(sites), M, flags, _p(out)))
         return out
+
+    def moments_multi(self, froms, tos, M: int) -> list:
+        """Moments <to|T_n|from> for every from-index and its to-list
+        (tbtkx_moments_multi): a list of (len(to_j), M) arrays."""
+        froms = np.ascontiguousarray(np.atleast_1d(froms), dtype=np.int64)
+        offs = np.zeros(froms.size + 1, dtype=np.int32)
+        offs[1:] = np.cumsum([len(t) for t in tos])
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(t, dtype=np.int64) for t in tos])
+                                    if len(tos) else np.zeros(0, np.int64), dtype=np.int64)
+        out = np.zeros((max(int(offs[-1]), 1), M), dtype=np.complex128)
+        _check(self._lib.tbtkx_moments_multi(self._h, froms.size, _p(froms), _p(offs), _p(flat),
+                                             M, _p(out)))
+        return [out[offs[j]:offs[j + 1]] for j in range(froms.size)]
 
     def moments_trace(self, R: int, seed: int, M: int, flags: int = 0) -> np.ndarray:
         out = np.zeros(M, dtype=np.complex128)

@@ -10,11 +10,15 @@
  * vectors. sampleDOS keeps the reference's srand/rand site sampling
  * (Lib/src/PropertyExtractor/PropertyExtractor.cpp:234-290).
  *
- * Differences in execution, not in results: all to-indices of one
- * from-index are contracted in one GPU call (the reference loops them under
- * an OpenMP parallel for that also races on the lookup table, F9), and with
- * Solver::ChebyshevExpander::setUseBlockRecurrence(true) the diagonal
- * elements of calculateLDOS run as blocks of start vectors.
+ * Differences in execution, not in results: the reference runs one
+ * recurrence per from-index in sequence and contracts the to-indices under
+ * an OpenMP parallel for (which also races on the lookup table); here every
+ * query (the pattern form of calculateGreensFunction, the LDOS, DOS,
+ * Density, Magnetization and SpinPolarizedLDOS sites) is one batched
+ * recurrence call, which the engine runs as exact FP64 blocks of start
+ * vectors when that moves fewer bytes (bitwise the same moments), and one
+ * contraction call. With Solver::ChebyshevExpander::setUseBlockRecurrence
+ * (true) the diagonal queries use the approximate FP32 blocks instead.
  */
 #ifndef TBTK_AMD_PROPERTY_EXTRACTOR_CHEBYSHEV_EXPANDER
 #define TBTK_AMD_PROPERTY_EXTRACTOR_CHEBYSHEV_EXPANDER
@@ -22,6 +26,7 @@
 #include "TBTK/Property/Property.h"
 #include "TBTK/Solver/ChebyshevExpander.h"
 
+#include <array>
 #include <complex>
 #include <vector>
 
@@ -123,17 +128,21 @@ private:
 	                          std::vector<Index> &indices, std::vector<int> &offsets);
 	static std::vector<int> loopRanges(Index pattern, Index ranges);
 	double occupation(double energy) const;
+	std::vector<std::vector<std::complex<double>>> diagonalCoefficients(
+	    const std::vector<Index> &indices);
 	std::vector<std::vector<std::complex<double>>> diagonalGreensFunctions(
 	    const std::vector<Index> &indices);
-	void spinGreensFunctions(const Index &index, int spinIndex,
-	                         std::vector<std::complex<double>> (&g)[4]);
+	std::vector<std::array<std::vector<std::complex<double>>, 4>> spinGreensFunctions(
+	    const std::vector<Index> &indices, const std::vector<int> &spinIndices);
 	void accumulateDensity(const std::vector<Index> &indices,
 	                       const std::vector<int> &offsets, Property::Density &density);
 	void accumulateMagnetization(const std::vector<Index> &indices,
-	                             const std::vector<int> &offsets, int spinIndex,
+	                             const std::vector<int> &offsets,
+	                             const std::vector<int> &spinIndices,
 	                             Property::Magnetization &magnetization);
 	void accumulateSpinPolarizedLDOS(const std::vector<Index> &indices,
-	                                 const std::vector<int> &offsets, int spinIndex,
+	                                 const std::vector<int> &offsets,
+	                                 const std::vector<int> &spinIndices,
 	                                 Property::SpinPolarizedLDOS &splodos);
 };
 

@@ -98,12 +98,22 @@ public:
 	void setDamping(Damping d) { damping = d; }
 	Damping getDamping() const { return damping; }
 
+	/* Diagonal queries of the property extractor (LDOS, DOS, Density) as
+	 * FP32 blocks with the doubling identities (approximate, ~1e-6) instead
+	 * of the exact path (off by default). */
 	void setUseBlockRecurrence(bool v) { useBlockRecurrence = v; }
 	bool getUseBlockRecurrence() const { return useBlockRecurrence; }
 	/* tbtkx flags of the block recurrence (TBTKX_DOUBLING by default). */
 	void setBlockFlags(int flags) { blockFlags = flags; }
 	int getBlockFlags() const { return blockFlags; }
 
+	/* Damped coefficients for many from-indices, each with its own
+	 * to-list: result[j][k] equals calculateCoefficients(to[j], from[j])[k]
+	 * bit for bit. One engine call: exact FP64 blocks of start vectors
+	 * through one SpMM per step when that moves fewer bytes than one
+	 * recurrence per from-index (tbtkx_moments_multi). */
+	std::vector<std::vector<std::vector<std::complex<double>>>> calculateCoefficientsMulti(
+	    const std::vector<Index> &from, const std::vector<std::vector<Index>> &to);
 	/* Damped diagonal coefficients <i|T_n|i> for every index, computed as
 	 * blocks of unit start vectors. */
 	std::vector<std::vector<std::complex<double>>> calculateCoefficientsBlock(

@@ -207,6 +207,10 @@ Property::GreensFunction ChebyshevExpander::calculateGreensFunction(
 	fromIndices.generateLinearMap();
 	Property::GreensFunction gf(layout, type, getEnergyWindow());
 	const unsigned int E = getEnergyWindow().getResolution();
+	// Every from-index with its to-indices (in the layout's order), all
+	// recurrences in one engine call, all contractions in one more.
+	std::vector<Index> froms;
+	std::vector<std::vector<Index>> tos;
 	for (const Index &from : fromIndices) {
 		std::vector<Index> toIndices;
 		for (unsigned int size : toSizes) {
@@ -218,34 +222,52 @@ Property::GreensFunction ChebyshevExpander::calculateGreensFunction(
 				toIndices.push_back(t);
 			}
 		}
-		Property::GreensFunction part = calculateGreensFunctions(toIndices, from, type);
-		for (const Index &t : toIndices) {
-			const Index compound({t, from});
-			const int o = gf.getOffset(compound), op = part.getOffset(compound);
-			for (unsigned int c = 0; c < E; c++) gf.getDataRW()[o + c] = part.getData()[op + c];
-		}
+		froms.push_back(from);
+		tos.push_back(toIndices);
 	}
+	std::vector<std::vector<std::vector<std::complex<double>>>> c =
+	    solver->calculateCoefficientsMulti(froms, tos);
+	std::vector<std::vector<std::complex<double>>> flat;
+	for (auto &perFrom : c)
+		for (auto &v : perFrom) flat.push_back(std::move(v));
+	std::vector<std::vector<std::complex<double>>> g =
+	    solver->generateGreensFunctions(flat, toSolverType(type));
+	size_t n = 0;
+	for (size_t j = 0; j < froms.size(); j++)
+		for (const Index &t : tos[j]) {
+			const int o = gf.getOffset(Index({t, froms[j]}));
+			for (unsigned int e = 0; e < E; e++) gf.getDataRW()[o + e] = g[n][e];
+			n++;
+		}
 	return gf;
+}
+
+// Damped diagonal coefficients <i|T_n|i> of the given indices: exact (the
+// plain recurrence's bits) through calculateCoefficientsMulti, or the
+// approximate FP32 blocks with setUseBlockRecurrence(true).
+std::vector<std::vector<std::complex<double>>> ChebyshevExpander::diagonalCoefficients(
+    const std::vector<Index> &indices) {
+	if (solver->getUseBlockRecurrence()) return solver->calculateCoefficientsBlock(indices);
+	std::vector<std::vector<Index>> tos;
+	tos.reserve(indices.size());
+	for (const Index &idx : indices) tos.push_back({idx});
+	std::vector<std::vector<std::vector<std::complex<double>>>> c =
+	    solver->calculateCoefficientsMulti(indices, tos);
+	std::vector<std::vector<std::complex<double>>> out;
+	out.reserve(indices.size());
+	for (auto &perFrom : c) out.push_back(std::move(perFrom[0]));
+	return out;
 }
 
 // The LDOS callback, Lib/src/PropertyExtractor/ChebyshevExpander.cpp:661-684:
 // data[offset + n] += Im(G_NonPrincipal(index, index)[n])/pi, in the order of
-// the indices. With the block recurrence all diagonal moments come from one
-// batched run.
+// the indices. All diagonal moments come from one batched call.
 void ChebyshevExpander::accumulateLDOS(const std::vector<Index> &indices,
                                        const std::vector<int> &offsets,
                                        Property::LDOS &ldos) {
 	const unsigned int E = getEnergyWindow().getResolution();
-	std::vector<std::vector<std::complex<double>>> coefficients;
-	if (solver->getUseBlockRecurrence()) {
-		coefficients = solver->calculateCoefficientsBlock(indices);
-	} else {
-		coefficients.reserve(indices.size());
-		for (const Index &idx : indices)
-			coefficients.push_back(solver->calculateCoefficients(idx, idx));
-	}
 	std::vector<std::vector<std::complex<double>>> g = solver->generateGreensFunctions(
-	    coefficients, Solver::ChebyshevExpander::Type::NonPrincipal);
+	    diagonalCoefficients(indices), Solver::ChebyshevExpander::Type::NonPrincipal);
 	std::vector<double> &data = ldos.getDataRW();
 	for (size_t i = 0; i < indices.size(); i++)
 		for (unsigned int n = 0; n < E; n++) data[offsets[i] + n] += std::imag(g[i][n]) / M_PI;
@@ -354,13 +376,8 @@ Property::DOS ChebyshevExpander::calculateDOS() {
 	for (int b0 = 0; b0 < N; b0 += chunk) {
 		std::vector<Index> idx;
 		for (int b = b0; b < N && b < b0 + chunk; b++) idx.push_back(has.getPhysicalIndex(b));
-		std::vector<std::vector<std::complex<double>>> c =
-		    solver->getUseBlockRecurrence() ? solver->calculateCoefficientsBlock(idx)
-		                                    : std::vector<std::vector<std::complex<double>>>();
-		if (!solver->getUseBlockRecurrence())
-			for (const Index &i : idx) c.push_back(solver->calculateCoefficients(i, i));
 		std::vector<std::vector<std::complex<double>>> g = solver->generateGreensFunctions(
-		    c, Solver::ChebyshevExpander::Type::NonPrincipal);
+		    diagonalCoefficients(idx), Solver::ChebyshevExpander::Type::NonPrincipal);
 		for (const auto &gi : g)
 			for (unsigned int n = 0; n < E; n++) data[n] += std::imag(gi[n]) / M_PI;
 	}
@@ -385,40 +402,48 @@ double ChebyshevExpander::occupation(double energy) const {
 	         "Unknown statistics.", "This should never happen, contact the developer.");
 }
 
-// Diagonal NonPrincipal Green's functions of the given indices (block
-// recurrence when enabled, as accumulateLDOS).
+// Diagonal NonPrincipal Green's functions of the given indices.
 std::vector<std::vector<std::complex<double>>> ChebyshevExpander::diagonalGreensFunctions(
     const std::vector<Index> &indices) {
-	std::vector<std::vector<std::complex<double>>> coefficients;
-	if (solver->getUseBlockRecurrence()) {
-		coefficients = solver->calculateCoefficientsBlock(indices);
-	} else {
-		coefficients.reserve(indices.size());
-		for (const Index &idx : indices)
-			coefficients.push_back(solver->calculateCoefficients(idx, idx));
-	}
-	return solver->generateGreensFunctions(coefficients,
+	return solver->generateGreensFunctions(diagonalCoefficients(indices),
 	                                       Solver::ChebyshevExpander::Type::NonPrincipal);
 }
 
-// The four NonPrincipal Green's functions G(to spin n/2, from spin n%2) of an
-// index, n = 0..3 (up-up, up-down, down-up, down-down). One recurrence per
-// from-spin gathers both to-spins; the moments of each pair are the ones
-// separate calls give.
-void ChebyshevExpander::spinGreensFunctions(const Index &index, int spinIndex,
-                                            std::vector<std::complex<double>> (&g)[4]) {
-	for (int c = 0; c < 2; c++) {
-		Index from(index);
-		from.at(spinIndex) = c;
-		std::vector<Index> to = {Index(index), Index(index)};
-		to[0].at(spinIndex) = 0;
-		to[1].at(spinIndex) = 1;
-		std::vector<std::vector<std::complex<double>>> gf = solver->generateGreensFunctions(
-		    solver->calculateCoefficients(to, from),
-		    Solver::ChebyshevExpander::Type::NonPrincipal);
-		g[0 * 2 + c] = gf[0];
-		g[1 * 2 + c] = gf[1];
-	}
+// The four NonPrincipal Green's functions G(to spin n/2, from spin n%2) of
+// every index, n = 0..3 (up-up, up-down, down-up, down-down): one start
+// vector per (index, from-spin) gathering both to-spins, all in one batched
+// call; the moments are the ones separate calls give.
+std::vector<std::array<std::vector<std::complex<double>>, 4>>
+ChebyshevExpander::spinGreensFunctions(const std::vector<Index> &indices,
+                                       const std::vector<int> &spinIndices) {
+	std::vector<Index> froms;
+	std::vector<std::vector<Index>> tos;
+	for (size_t k = 0; k < indices.size(); k++)
+		for (int c = 0; c < 2; c++) {
+			const Index &index = indices[k];
+			const int spinIndex = spinIndices[k];
+			Index from(index);
+			from.at(spinIndex) = c;
+			std::vector<Index> to = {Index(index), Index(index)};
+			to[0].at(spinIndex) = 0;
+			to[1].at(spinIndex) = 1;
+			froms.push_back(from);
+			tos.push_back(to);
+		}
+	std::vector<std::vector<std::vector<std::complex<double>>>> c =
+	    solver->calculateCoefficientsMulti(froms, tos);
+	std::vector<std::vector<std::complex<double>>> flat;
+	for (auto &perFrom : c)
+		for (auto &v : perFrom) flat.push_back(std::move(v));
+	std::vector<std::vector<std::complex<double>>> gf = solver->generateGreensFunctions(
+	    flat, Solver::ChebyshevExpander::Type::NonPrincipal);
+	std::vector<std::array<std::vector<std::complex<double>>, 4>> g(indices.size());
+	for (size_t k = 0; k < indices.size(); k++)
+		for (int c = 0; c < 2; c++) {
+			g[k][0 * 2 + c] = gf[4 * k + 2 * c];
+			g[k][1 * 2 + c] = gf[4 * k + 2 * c + 1];
+		}
+	return g;
 }
 
 // Lib/src/PropertyExtractor/ChebyshevExpander.cpp:297-327.
@@ -483,15 +508,15 @@ Property::Density ChebyshevExpander::calculateDensity(std::vector<Index> pattern
 // cpp:609-647: data[offset].at(n/2, n%2) += f(E) (-i) G_n(E)/pi dE.
 void ChebyshevExpander::accumulateMagnetization(const std::vector<Index> &indices,
                                                 const std::vector<int> &offsets,
-                                                int spinIndex,
+                                                const std::vector<int> &spinIndices,
                                                 Property::Magnetization &magnetization) {
 	const std::complex<double> i(0, 1);
 	std::vector<SpinMatrix> &data = magnetization.getDataRW();
 	const Range &energyWindow = getEnergyWindow();
 	const double dE = energyWindow[1] - energyWindow[0];
+	const auto gs = spinGreensFunctions(indices, spinIndices);
 	for (size_t k = 0; k < indices.size(); k++) {
-		std::vector<std::complex<double>> g[4];
-		spinGreensFunctions(indices[k], spinIndex, g);
+		const auto &g = gs[k];
 		for (int n = 0; n < 4; n++)
 			for (unsigned int e = 0; e < energyWindow.getResolution(); e++) {
 				const double weight = occupation(energyWindow[e]);
@@ -504,14 +529,14 @@ void ChebyshevExpander::accumulateMagnetization(const std::vector<Index> &indice
 // ChebyshevExpander.cpp:686-717: data[offset + e].at(n/2, n%2) += -i G_n(E)/pi.
 void ChebyshevExpander::accumulateSpinPolarizedLDOS(const std::vector<Index> &indices,
                                                     const std::vector<int> &offsets,
-                                                    int spinIndex,
+                                                    const std::vector<int> &spinIndices,
                                                     Property::SpinPolarizedLDOS &splodos) {
 	const std::complex<double> i(0, 1);
 	std::vector<SpinMatrix> &data = splodos.getDataRW();
 	const unsigned int E = getEnergyWindow().getResolution();
+	const auto gs = spinGreensFunctions(indices, spinIndices);
 	for (size_t k = 0; k < indices.size(); k++) {
-		std::vector<std::complex<double>> g[4];
-		spinGreensFunctions(indices[k], spinIndex, g);
+		const auto &g = gs[k];
 		for (int n = 0; n < 4; n++)
 			for (unsigned int e = 0; e < E; e++)
 				data[offsets[k] + e].at(n / 2, n % 2) += -i * g[n][e] / M_PI;
@@ -556,7 +581,8 @@ Property::Magnetization ChebyshevExpander::calculateMagnetization(Index pattern,
 	std::vector<Index> indices;
 	std::vector<int> offsets;
 	rangesIndices(pattern, ranges, 1, indices, offsets);
-	accumulateMagnetization(indices, offsets, spinIndex, magnetization);
+	accumulateMagnetization(indices, offsets, std::vector<int>(indices.size(), spinIndex),
+	                        magnetization);
 	return magnetization;
 }
 
@@ -565,12 +591,15 @@ Property::Magnetization ChebyshevExpander::calculateMagnetization(std::vector<In
 	IndexTree allIndices = generateIndexTree(patterns, has, false, true);
 	IndexTree layout = generateIndexTree(patterns, has, true, true);
 	Property::Magnetization magnetization(layout);
+	std::vector<Index> indices;
+	std::vector<int> offsets, spins;
 	for (const Index &idx : allIndices) {
-		const int spinIndex =
-		    spinPosition(idx, "PropertyExtractor::ChebyshevExpander::calculateMagnetization()");
-		accumulateMagnetization({idx}, {magnetization.getOffset(idx)}, spinIndex,
-		                        magnetization);
+		spins.push_back(
+		    spinPosition(idx, "PropertyExtractor::ChebyshevExpander::calculateMagnetization()"));
+		indices.push_back(idx);
+		offsets.push_back(magnetization.getOffset(idx));
 	}
+	accumulateMagnetization(indices, offsets, spins, magnetization);
 	return magnetization;
 }
 
@@ -582,7 +611,8 @@ Property::SpinPolarizedLDOS ChebyshevExpander::calculateSpinPolarizedLDOS(Index 
 	std::vector<Index> indices;
 	std::vector<int> offsets;
 	rangesIndices(pattern, ranges, (int)getEnergyWindow().getResolution(), indices, offsets);
-	accumulateSpinPolarizedLDOS(indices, offsets, spinIndex, splodos);
+	accumulateSpinPolarizedLDOS(indices, offsets, std::vector<int>(indices.size(), spinIndex),
+	                            splodos);
 	return splodos;
 }
 
@@ -592,11 +622,15 @@ Property::SpinPolarizedLDOS ChebyshevExpander::calculateSpinPolarizedLDOS(
 	IndexTree allIndices = generateIndexTree(patterns, has, false, true);
 	IndexTree layout = generateIndexTree(patterns, has, true, true);
 	Property::SpinPolarizedLDOS splodos(layout, getEnergyWindow());
+	std::vector<Index> indices;
+	std::vector<int> offsets, spins;
 	for (const Index &idx : allIndices) {
-		const int spinIndex = spinPosition(
-		    idx, "PropertyExtractor::ChebsyhevExpander::calculateSpinPolarizedLDOS()");
-		accumulateSpinPolarizedLDOS({idx}, {splodos.getOffset(idx)}, spinIndex, splodos);
+		spins.push_back(spinPosition(
+		    idx, "PropertyExtractor::ChebsyhevExpander::calculateSpinPolarizedLDOS()"));
+		indices.push_back(idx);
+		offsets.push_back(splodos.getOffset(idx));
 	}
+	accumulateSpinPolarizedLDOS(indices, offsets, spins, splodos);
 	return splodos;
 }
 

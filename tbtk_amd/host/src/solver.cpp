@@ -159,6 +159,53 @@ std::vector<std::complex<double>> ChebyshevExpander::calculateCoefficients(Index
 	return calculateCoefficients(tos, from)[0];
 }
 
+std::vector<std::vector<std::vector<std::complex<double>>>>
+ChebyshevExpander::calculateCoefficientsMulti(const std::vector<Index> &from,
+                                              const std::vector<std::vector<Index>> &to) {
+	std::lock_guard<std::recursive_mutex> lock(callMutex);
+	TBTKAssert(from.size() == to.size(), "ChebyshevExpander::calculateCoefficientsMulti()",
+	           "Expected one to-list per from-index.", "");
+	TBTKAssert(scaleFactor > 0, "ChebyshevExpander::calculateCoefficientsGPU()",
+	           "Scale factor must be larger than zero.",
+	           "Use ChebyshevExpander::setScaleFactor() to set scale factor.");
+	TBTKAssert(numCoefficients > 0, "ChebyshevExpander::calculateCoefficients()",
+	           "numCoefficients has to be larger than zero.", "");
+	std::vector<std::vector<std::vector<std::complex<double>>>> out(from.size());
+	if (from.empty()) return out;
+	ensureModelOnDevice();
+	const HoppingAmplitudeSet &has = getModel().getHoppingAmplitudeSet();
+	std::vector<int64_t> fromBasis(from.size()), toBasis;
+	std::vector<int> offsets(1, 0);
+	for (size_t j = 0; j < from.size(); j++) {
+		fromBasis[j] = has.getBasisIndex(from[j]);
+		TBTKAssert(fromBasis[j] >= 0, "ChebyshevExpander::calculateCoefficients()",
+		           "The from-Index " << from[j].toString() << " is not in the model.", "");
+		for (const Index &t : to[j]) {
+			const int b = has.getBasisIndex(t);
+			TBTKAssert(b >= 0, "ChebyshevExpander::calculateCoefficients()",
+			           "The to-Index " << t.toString() << " is not in the model.", "");
+			toBasis.push_back(b);
+		}
+		TBTKAssert(toBasis.size() < (size_t)1 << 31, "ChebyshevExpander::calculateCoefficientsMulti()",
+		           "Too many to-indices in one call.", "");
+		offsets.push_back((int)toBasis.size());
+	}
+	std::vector<std::complex<double>> raw(toBasis.size() * (size_t)numCoefficients);
+	check(tbtkx_moments_multi(handle, (int)from.size(), fromBasis.data(), offsets.data(),
+	                          toBasis.data(), numCoefficients,
+	                          reinterpret_cast<double *>(raw.data())),
+	      "ChebyshevExpander::calculateCoefficients()");
+	for (size_t j = 0; j < from.size(); j++) {
+		out[j].resize(to[j].size());
+		for (size_t k = 0; k < to[j].size(); k++) {
+			const size_t at = ((size_t)offsets[j] + k) * numCoefficients;
+			out[j][k].assign(raw.begin() + at, raw.begin() + at + numCoefficients);
+			applyDamping(out[j][k]);
+		}
+	}
+	return out;
+}
+
 std::vector<std::vector<std::complex<double>>> ChebyshevExpander::calculateCoefficientsBlock(
     const std::vector<Index> &indices) {
 	std::lock_guard<std::recursive_mutex> lock(callMutex);

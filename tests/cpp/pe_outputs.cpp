@@ -149,6 +149,16 @@ int main(int argc, char **argv) {
 		for (const Index &ti : std::vector<Index>{{15, 15, 0}, {16, 15, 0}})
 			for (int e = 0; e < 300; e++) out.push_back(gp({ti, {15, 15, 0}}, e));
 		write("square30_spin_gfpattern_NonPrincipal", out);
+		// Pattern form with 900 from-indices: G((15,15,0), (x,y,0)), Retarded.
+		{
+			const Index toIdx = {15, 15, 0}, fromPattern = {IDX_ALL, IDX_ALL, 0};
+			Property::GreensFunction g900 = pe.calculateGreensFunction(
+			    {{toIdx, fromPattern}}, Property::GreensFunction::Type::Retarded);
+			std::vector<complex<double>> flat;
+			for (const Index &f : model.getHoppingAmplitudeSet().getIndexList({fromPattern}))
+				for (int e = 0; e < 300; e++) flat.push_back(g900({toIdx, f}, e));
+			write("square30_spin_gfpattern900", flat);
+		}
 
 		// Thermodynamic-equilibrium properties at mu = 0.3, T = 300 K and 0.
 		auto spins = [](const std::vector<SpinMatrix> &d) {

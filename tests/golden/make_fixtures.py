@@ -182,6 +182,23 @@ def callers_fixture():
     ldos = np.fromfile(out, np.float64)
     assert ldos.size == 100 * 1000, ldos.size
     rows = ldos.reshape(100, 1000)[[0, 50, 99]]
+    # The pattern form of calculateGreensFunction with 900 from-indices
+    # (Lib/src/PropertyExtractor/ChebyshevExpander.cpp:107-232): G((15,15,0),
+    # (x,y,0)) for every x, y on the 30x30 spin lattice, Retarded, M = 400,
+    # lookup table; 900 x 300 values as SHA-256 plus the first 4 from-indices.
+    p1 = os.path.join(TMP, "sq30.bin")
+    L.write_amplitudes(L.config_plain_small(), p1)
+    out = os.path.join(TMP, "gfpat.bin")
+    O.run_ref(mode="gfpat", model=p1, scale=10.0, M=400, broadening=1e-6,
+              window="-9.5,9.5,300", lookup=1, type="Retarded", to="15,15,0", **{"from": "*,*,0"},
+              out=out)
+    gp = np.fromfile(out, np.complex128)
+    assert gp.size == 900 * 300, gp.size
+    save("square30_spin_gfpattern900",
+         dict(scale=10.0, M=400, broadening=1e-6, window=[-9.5, 9.5, 300], lookup=1,
+              type="Retarded", to="15,15,0", frm="*,*,0"),
+         sha256=np.array(hashlib.sha256(gp.astype("<c16").tobytes()).hexdigest()),
+         first=gp.reshape(900, 300)[:4])
     save("basic_chebyshev_template",
          dict(model="square_spin(100, 100, t=1, J=0, mu=-1)", scale=5.0, M=500,
               broadening=1e-6, window=[-4.99, 4.99, 1000], pattern="_a_,50,IDX_SUM_ALL",

@@ -73,6 +73,21 @@ def test_ldos_and_gf_bitexact(outputs):
     assert np.array_equal(pat, fx["gf_NonPrincipal"])
 
 
+def test_pattern_gf_many_from_indices(outputs):
+    """The pattern form of calculateGreensFunction with 900 from-indices
+    (G((15,15,0), (x,y,0)), Retarded, M = 400) runs as one batched call,
+    as exact FP64 blocks of start vectors, and equals the reference's 900
+    sequential recurrences bit for bit (tests/golden/
+    square30_spin_gfpattern900.npz: SHA-256 of all values, the first four
+    from-indices in full)."""
+    fx = load_fixture("square30_spin_gfpattern900")
+    got = _read(outputs, "square30_spin_gfpattern900", np.complex128)
+    assert got.size == 900 * 300
+    first = got.reshape(900, 300)[:4]
+    assert np.array_equal(first, fx["first"]), np.abs(first - fx["first"]).max()
+    assert hashlib.sha256(got.astype("<c16").tobytes()).hexdigest() == str(fx["sha256"])
+
+
 def test_gf_types_from_coefficients(outputs):
     fx = load_fixture("gf_types")
     for c in ("c123", "crand"):
