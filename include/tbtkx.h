@@ -97,6 +97,26 @@ int tbtkx_set_csr(tbtkx_handle handle, int64_t num_rows, int64_t nnz,
 int tbtkx_moments(tbtkx_handle handle, int64_t from, int num_to,
                   const int64_t *to, int num_moments, double *out);
 
+/* Allocation-free variant of tbtkx_moments, with no counterpart in the
+ * reference: for callers that cannot allocate device memory during the call
+ * (tight serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * results (bitwise equal to tbtkx_moments) and errors; the recurrence
+ * vectors, the moments and the to-list live in a fixed scratch array of
+ * TBTKX_NOALLOC_BYTES per GPU, held from the library's load until the
+ * process ends, and the call makes no hipMalloc, hipFree or other
+ * allocating call (the model itself is set up beforehand with
+ * tbtkx_set_csr or tbtkx_set_lattice). A call needing more scratch than
+ * that (2 vectors of N values, 8 B each for a real H and 16 B for a complex
+ * one, plus K x M x 16 B of moments and K x 8 B, each rounded up to 256 B),
+ * or more than min(N, TBTKX_NOALLOC_MAX_TO) to-indices, is rejected with
+ * TBTKX_ERR_ARG. All handles on one GPU share its scratch, which keeps the
+ * last call's contents; their calls take a per-GPU lock and run one after
+ * another (calls never overlap on one GPU). */
+#define TBTKX_NOALLOC_BYTES (64u << 20)
+#define TBTKX_NOALLOC_MAX_TO 4096
+int tbtkx_moments_noalloc(tbtkx_handle handle, int64_t from, int num_to,
+                          const int64_t *to, int num_moments, double *out);
+
 /* Load a host-generated lookup table L[n*E + e] (M x E complex) for the
  * contraction. Replaces loadLookupTableGPU
  * (Lib/src/cuda/ChebyshevSolver.cu:672-760). */

@@ -19,6 +19,7 @@
 // off-diagonal values, or 255 for the row's diagonal, read from a per-row
 // array): 3 Z + 68 N complex, 3 Z + 36 N real.
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <unordered_map>
 
@@ -258,13 +259,11 @@ void launch_lds(tbtkx_context *h, bool first, const CI *col, const S *v, const S
 // (complex) or double (real H and start vector: v1r/v2r, vectors of N
 // doubles in the same buffers).
 template <typename S>
-int enqueue_steps(tbtkx_context *h, int K, int M) {
+int enqueue_steps(tbtkx_context *h, int K, int M, S *a, S *b, double2 *coef,
+                  const int64_t *to) {
 	const bool real = std::is_same<S, double>::value;
 	const S *v1 = real ? h->v1r.as<S>() : h->v1.as<S>();
 	const S *v2 = real ? h->v2r.as<S>() : h->v2.as<S>();
-	S *a = h->vec_a.as<S>(), *b = h->vec_b.as<S>();
-	double2 *coef = h->coef.as<double2>();
-	const int64_t *to = h->to.as<int64_t>();
 	const uint32_t *rp = h->rowptr.as<uint32_t>(), *cl = h->col.as<uint32_t>();
 	const int remap = h->xcd_remap;
 	const bool lds = h->plain_kernel == 0;
@@ -402,9 +401,122 @@ int build_palette(tbtkx_context *h, int64_t N, int64_t Z, const uint32_t *rowptr
 	return TBTKX_OK;
 }
 
+// Scratch of tbtkx_moments_noalloc: the two recurrence vectors, the moment
+// rows and the to-list of one call, in a fixed device array of
+// TBTKX_NOALLOC_BYTES per GPU (allocated with the code object when the
+// library is loaded on that GPU, held until the process ends). Host code
+// reaches it only through the address hipGetSymbolAddress gives on each
+// device (noalloc_scratch); the array's name is not a device address there.
+static __device__ __align__(256) unsigned char scratch[TBTKX_NOALLOC_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices];
+// Held for a whole tbtkx_moments_noalloc call: calls on one GPU share its
+// scratch, so they run one after another.
+std::mutex g_scratch_busy[kMaxDevices];
+}  // namespace
+
+// The device address of `scratch` on the handle's device, looked up once per
+// device (with that device current) and kept.
+static int noalloc_scratch(tbtkx_context *h, unsigned char **ptr) {
+	TBTKX_REQUIRE(h->device >= 0 && h->device < kMaxDevices, TBTKX_ERR_ARG,
+	              "tbtkx_moments_noalloc: device id beyond the scratch table");
+	std::lock_guard<std::mutex> lock(g_scratch_mutex);
+	if (!g_scratch_ptr[h->device]) {
+		int rc = set_device(h);
+		if (rc) return rc;
+		void *p = nullptr;
+		TBTKX_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+		TBTKX_REQUIRE(p != nullptr, TBTKX_ERR_HIP,
+		              "tbtkx_moments_noalloc: no device address for the scratch array");
+		g_scratch_ptr[h->device] = p;
+	}
+	*ptr = static_cast<unsigned char *>(g_scratch_ptr[h->device]);
+	return TBTKX_OK;
+}
+
 }  // namespace tbtkx
 
 using namespace tbtkx;
+
+extern "C" int tbtkx_moments_noalloc(tbtkx_handle h, int64_t from, int K,
+                                     const int64_t *to, int M, double *out) {
+	TBTKX_REQUIRE(h, TBTKX_ERR_ARG, "tbtkx_moments_noalloc: null handle");
+	TBTKX_LOCK(h);
+	TBTKX_REQUIRE(h->N > 0, TBTKX_ERR_STATE, "tbtkx_moments_noalloc: no model set");
+	TBTKX_REQUIRE(M > 0, TBTKX_ERR_ARG,
+	              "tbtkx_moments_noalloc: numCoefficients has to be larger than 0");
+	TBTKX_REQUIRE(K >= 0 && (K == 0 || (to && out)), TBTKX_ERR_ARG,
+	              "tbtkx_moments_noalloc: bad to-index list");
+	TBTKX_REQUIRE(from >= 0 && from < h->N, TBTKX_ERR_ARG,
+	              "tbtkx_moments_noalloc: from-index outside the basis");
+	TBTKX_REQUIRE(K <= h->N && K <= TBTKX_NOALLOC_MAX_TO, TBTKX_ERR_ARG,
+	              "tbtkx_moments_noalloc: at most min(N, " +
+	                  std::to_string(TBTKX_NOALLOC_MAX_TO) + ") to-indices");
+	for (int k = 0; k < K; k++)
+		TBTKX_REQUIRE(to[k] >= 0 && to[k] < h->N, TBTKX_ERR_ARG,
+		              "tbtkx_moments_noalloc: to-index outside the basis");
+	if (K == 0) return TBTKX_OK;
+	const bool real = h->real_path && h->h_real;
+	const size_t elem = real ? 8 : 16;
+	// Layout: vector a, vector b, moments (K x M), to-list; each 256-aligned.
+	auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+	const size_t vbytes = up((size_t)h->N * elem), cbytes = up((size_t)K * M * 16);
+	const size_t need = 2 * vbytes + cbytes + up((size_t)K * 8);
+	TBTKX_REQUIRE(need <= (size_t)TBTKX_NOALLOC_BYTES, TBTKX_ERR_ARG,
+	              "tbtkx_moments_noalloc: the call needs " + std::to_string(need) +
+	                  " bytes of scratch (2 vectors of N, K x M moments, K to-indices), more "
+	                  "than the " + std::to_string((size_t)TBTKX_NOALLOC_BYTES) +
+	                  " available; use tbtkx_moments");
+	unsigned char *base = nullptr;
+	int rc = noalloc_scratch(h, &base);
+	if (rc) return rc;
+	std::lock_guard<std::mutex> busy(g_scratch_busy[h->device]);
+	if ((rc = set_device(h))) return rc;
+	void *va = base, *vb = base + vbytes;
+	double2 *coef = reinterpret_cast<double2 *>(base + 2 * vbytes);
+	int64_t *tod = reinterpret_cast<int64_t *>(base + 2 * vbytes + cbytes);
+	TBTKX_CHECK_HIP(hipMemcpyAsync(tod, to, (size_t)K * 8, hipMemcpyHostToDevice, h->stream));
+	// Every scratch byte the call reads is written first: the start vector
+	// by init_unit, the other vector by the first step (which does not read
+	// it), every moment by a gather.
+	const bool c16 = h->plain_kernel == 0 && h->use_col16 && h->col16_ok;
+	const bool usepal = c16 && h->use_palette && h->palette_ok;
+	TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
+	if (real) {
+		init_unit<double><<<grid_for(h->N), kThreads, 0, h->stream>>>(
+		    static_cast<double *>(va), h->N, from);
+		rc = enqueue_steps<double>(h, K, M, static_cast<double *>(va), static_cast<double *>(vb),
+		                           coef, tod);
+	} else {
+		init_unit<double2><<<grid_for(h->N), kThreads, 0, h->stream>>>(
+		    static_cast<double2 *>(va), h->N, from);
+		rc = enqueue_steps<double2>(h, K, M, static_cast<double2 *>(va),
+		                            static_cast<double2 *>(vb), coef, tod);
+	}
+	if (rc) return rc;
+	TBTKX_CHECK_HIP(hipEventRecord(h->ev_stop, h->stream));
+	TBTKX_CHECK_HIP(hipMemcpyAsync(out, coef, (size_t)K * M * 16, hipMemcpyDeviceToHost,
+	                               h->stream));
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	float ms = 0;
+	TBTKX_CHECK_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
+	h->last_ms = ms;
+	h->last_launches = M > 1 ? M - 1 : 0;
+	h->last_real = real;
+	h->last_col16 = c16;
+	h->last_palette = usepal;
+	// A repeated to-index keeps only its last position (as tbtkx_moments).
+	for (int k = 0; k < K; k++)
+		for (int l = k + 1; l < K; l++)
+			if (to[l] == to[k]) {
+				memset(out + (size_t)k * M * 2, 0, (size_t)M * 16);
+				break;
+			}
+	return TBTKX_OK;
+}
 
 extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
                              const int64_t *to, int M, double *out) {
@@ -454,7 +566,14 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 		init_unit<double2><<<grid_for(h->N), kThreads, 0, h->stream>>>(
 		    h->vec_a.as<double2>(), h->N, from);
 	TBTKX_CHECK_HIP(hipGetLastError());
-	auto enqueue = [&]() { return real ? enqueue_steps<double>(h, U, M) : enqueue_steps<double2>(h, U, M); };
+	auto enqueue = [&]() {
+		double2 *coef = h->coef.as<double2>();
+		const int64_t *tod = h->to.as<int64_t>();
+		return real ? enqueue_steps<double>(h, U, M, h->vec_a.as<double>(),
+		                                    h->vec_b.as<double>(), coef, tod)
+		            : enqueue_steps<double2>(h, U, M, h->vec_a.as<double2>(),
+		                                     h->vec_b.as<double2>(), coef, tod);
+	};
 
 	TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
 	if (h->use_graph) {

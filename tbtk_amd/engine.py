@@ -38,6 +38,7 @@ SIGNATURES = {
     "tbtkx_destroy": [_P],
     "tbtkx_set_csr": [_P, _I64, _I64, _P, _P, _P, _D],
     "tbtkx_moments": [_P, _I64, _I, _P, _I, _P],
+    "tbtkx_moments_noalloc": [_P, _I64, _I, _P, _I, _P],
     "tbtkx_lut_upload": [_P, _I, _I, _P],
     "tbtkx_lut_generate": [_P, _I, _I, _D, _D, _D],
     "tbtkx_lut_destroy": [_P],
@@ -174,6 +175,13 @@ class Engine:
         to = np.ascontiguousarray(np.atleast_1d(to), dtype=np.int64)
         out = np.zeros((to.size, M), dtype=np.complex128)
         _check(self._lib.tbtkx_moments(self._h, int(frm), to.size, _p(to), M, _p(out)))
+        return out
+
+    def moments_noalloc(self, frm: int, to, M: int) -> np.ndarray:
+        """tbtkx_moments_noalloc: tbtkx_moments on the fixed per-GPU scratch."""
+        to = np.ascontiguousarray(np.atleast_1d(to), dtype=np.int64)
+        out = np.zeros((to.size, M), dtype=np.complex128)
+        _check(self._lib.tbtkx_moments_noalloc(self._h, int(frm), to.size, _p(to), M, _p(out)))
         return out
 
     # ---------------------------------------------------------- contraction
