@@ -19,6 +19,11 @@ coefficients, FP32 storage with FP64 accumulation and the doubling identities,
 against the plain FP64 path on the same model; error = max over compared sites
 of max_E |LDOS - LDOS_FP64| / max_E |LDOS_FP64|.
 
+Every configuration carries a "cpu_baseline": the reference's own CPU path
+(compiled from its sources, oracle/_ref/ref_driver) timed on a bounded sample
+of the same workload on this host, on one core and in one process per core
+(the throughput mode), with the GPU/CPU ratios beside it.
+
 Further objects on the same line: "uneven_batch" (64 models of uneven size
 in one persistent launch per step), "stochastic_dos" (128^3 cubic, 512
 random-phase vectors, M=4096) and "contraction_mfma" (1e5 x 4096 x 8192
@@ -90,44 +95,153 @@ def plain_ldos(eng, model, site_xy, M):
     return ldos, kernel_ms, launches
 
 
-def cpu_baseline(model, patterns, scale, window, E, M_target, sites_per_call):
-    """Time the reference's own CPU path (oracle/_ref/ref_driver, built from
-    the reference sources) on a bounded sample: calculateLDOS at M=20 and
-    M=60, one thread, extrapolated linearly in M to the full moment count.
-    Falls back to the FP64 C restatement (oracle) if the reference build is
-    absent."""
+def cpu_info():
+    """CPU model and logical CPU count of this host, and the worker count the
+    throughput mode uses (the OMP_NUM_THREADS share, at most 16)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", nproc) or nproc)
+    return model, nproc, max(1, min(16, share, nproc))
+
+
+def ref_runs(procs, **kw):
+    """Start `procs` copies of the compiled reference driver (oracle/_ref/
+    ref_driver, one OpenMP thread each) with the same arguments at once and
+    return each one's JSON timing lines."""
     from oracle import oracle as O
-    with tempfile.TemporaryDirectory() as td:
-        if O.ref_available():
-            path = os.path.join(td, "model.bin")
-            L.write_amplitudes(model, path)
-            Ms = (20, 60)
-            r = O.run_ref(threads=1, mode="timeldos", model=path, scale=scale, M=Ms[0],
-                          broadening=BROADENING, window=f"{window[0]},{window[1]},{E}",
-                          lookup=1, patterns=patterns, Ms=",".join(map(str, Ms)))
-            rows = [json.loads(x) for x in r.stderr.splitlines() if x.startswith("{")]
-            t = {int(x["M"]): x["seconds"] for x in rows}
-            slope = (t[Ms[1]] - t[Ms[0]]) / (Ms[1] - Ms[0])
-            fixed = t[Ms[0]] - Ms[0] * slope
-            t_full = fixed + M_target * slope
-            return {"value": sites_per_call / t_full, "unit": "LDOS sites/s", "cores": 1,
-                    "kind": "reference",
-                    "sample": (f"reference PropertyExtractor::ChebyshevExpander::calculateLDOS"
-                               f"({patterns}) on the same model, M={Ms[0]} ({t[Ms[0]]:.2f} s) and "
-                               f"M={Ms[1]} ({t[Ms[1]]:.2f} s), 1 thread, extrapolated linearly to "
-                               f"M={M_target}: {t_full:.1f} s per call of {sites_per_call} sites"),
-                    "seconds_per_call": t_full}
-        # restatement fallback: moments only
-        O.build()
-        b = model.basis_index((*CENTRE, 0))
-        Ms = 30
-        t0 = time.perf_counter()
-        O.moments(model.csr(), scale, b, [b], Ms)
-        dt = (time.perf_counter() - t0) / Ms
-        t_full = sites_per_call * dt * M_target
-        return {"value": sites_per_call / t_full, "unit": "LDOS sites/s", "cores": 1,
-                "kind": "port", "sample": f"oracle recurrence, {Ms} steps, extrapolated",
-                "seconds_per_call": t_full}
+    args = [O.REF_DRIVER] + [f"{k}={v}" for k, v in kw.items()]
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    ps = [subprocess.Popen(args, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True,
+                           env=env) for _ in range(procs)]
+    rows = []
+    for p in ps:
+        _, err = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"reference driver failed ({p.returncode}): {err[-500:]}")
+        rows.append([json.loads(x) for x in err.splitlines() if x.startswith("{")])
+    return rows
+
+
+def ref_ldos_fit(path, scale, E, patterns, Ms, procs):
+    """calculateLDOS(patterns) of the reference, timed at the moment counts
+    Ms in each of `procs` concurrent processes: per process (fixed, per
+    moment) seconds of a linear fit."""
+    fits = []
+    for rows in ref_runs(procs, mode="timeldos", model=path, scale=scale, M=Ms[0],
+                         broadening=BROADENING, window=f"{-0.99 * scale},{0.99 * scale},{E}",
+                         lookup=1, patterns=patterns, Ms=",".join(map(str, Ms))):
+        t = {int(x["M"]): x["seconds"] for x in rows}
+        slope = (t[Ms[1]] - t[Ms[0]]) / (Ms[1] - Ms[0])
+        fits.append((t[Ms[0]] - Ms[0] * slope, slope, t))
+    return fits
+
+
+def baseline_ldos(model, scale, E, patterns, sites_per_call, M_target, label, procs,
+                  workdir, Ms=(20, 60), factor=None):
+    """Reference CPU baseline of an LDOS workload: one call of
+    PropertyExtractor::ChebyshevExpander::calculateLDOS(patterns)
+    (Lib/src/PropertyExtractor/ChebyshevExpander.cpp:455-505; one sequential
+    recurrence per site, Lib/src/Solver/ChebyshevExpander.cpp:103-199) at two
+    small moment counts, extrapolated linearly to M_target, on one core. The
+    throughput mode (one process per core) is measured by running the same
+    sample in `procs` processes at once, unless `factor` (aggregate over
+    single-core rate, measured on another configuration) is given."""
+    path = os.path.join(workdir, label + ".bin")
+    L.write_amplitudes(model, path)
+    cpu, nproc, _ = cpu_info()
+    (f1, s1, t1), = ref_ldos_fit(path, scale, E, patterns, Ms, 1)
+    t_call = f1 + M_target * s1
+    single = sites_per_call / t_call
+    if factor is None:
+        # the parallel samples at the larger moment count, against the
+        # single-process one: each process's slowdown under full load
+        par = ref_ldos_fit(path, scale, E, patterns, Ms, procs)
+        factor = sum(t1[Ms[1]] / t[Ms[1]] for _, _, t in par)
+        mode = f"{procs} processes at once running the same sample"
+    else:
+        mode = (f"{procs} processes at once: the aggregate/single-core factor {factor:.2f} "
+                "measured on the 1024x1024 spin lattice sample")
+    return {"value": single, "unit": "LDOS sites/s", "cores": 1,
+            "kind": "reference",
+            "sample": (f"reference calculateLDOS({patterns}) on the same model, "
+                       f"M={Ms[0]} ({t1[Ms[0]]:.2f} s) and M={Ms[1]} ({t1[Ms[1]]:.2f} s), "
+                       f"1 thread, extrapolated linearly to M={M_target}: {t_call:.1f} s per "
+                       f"call of {sites_per_call} site(s)"),
+            "seconds_per_call": t_call,
+            "throughput": {"value": single * factor, "unit": "LDOS sites/s", "cores": procs,
+                           "aggregate_over_single": factor, "mode": mode},
+            "cpu_model": cpu, "nproc": nproc}
+
+
+def baseline_uneven(batch, procs, workdir, factor):
+    """Reference CPU time of the uneven batch: every model's sites run as
+    the reference runs them, one calculateLDOS recurrence per site. The
+    per-call fixed time and per-moment time are measured (as in
+    baseline_ldos) on the models with the fewest and the most non-zeros and
+    interpolated linearly in the non-zero count for the others."""
+    cpu, nproc, _ = cpu_info()
+    order = sorted(range(len(batch)), key=lambda i: batch[i][0].nnz)
+    pick = [order[0], order[-1]]
+    fit = {}
+    for i in pick:
+        model, sites, _ = batch[i]
+        path = os.path.join(workdir, f"uneven{i}.bin")
+        L.write_amplitudes(model, path)
+        s = model.gershgorin_bound() * 1.05
+        idx = ",".join(map(str, model.physical_index(int(sites[0]))))
+        (f1, s1, _), = ref_ldos_fit(path, s, E_PTS, idx, (20, 60), 1)
+        fit[i] = (f1, s1)
+    (za, (fa, sa)), (zb, (fb, sb)) = [(batch[i][0].nnz, fit[i]) for i in pick]
+
+    def lin(a, b, z):
+        return a + (b - a) * (z - za) / (zb - za) if zb != za else a
+    total = 0.0
+    sites = 0
+    for model, st, M in batch:
+        z = model.nnz
+        total += len(st) * (lin(fa, fb, z) + M * lin(sa, sb, z))
+        sites += len(st)
+    return {"value": sites / total, "unit": "LDOS sites/s", "cores": 1, "kind": "reference",
+            "seconds": total,
+            "sample": (f"reference calculateLDOS of one site on the models with the fewest "
+                       f"({za}) and most ({zb}) non-zeros at M=20 and 60, 1 thread; per-call "
+                       f"and per-moment times interpolated in the non-zero count over the 64 "
+                       f"models and summed over their {sites} sites: {total:.0f} s"),
+            "throughput": {"value": sites / total * factor, "unit": "LDOS sites/s",
+                           "cores": procs, "aggregate_over_single": factor,
+                           "mode": (f"{procs} processes at once: the aggregate/single-core "
+                                    f"factor measured on the 1024x1024 spin lattice sample")},
+            "cpu_model": cpu, "nproc": nproc}
+
+
+def baseline_contraction(K, M, E, scale, procs, rows=2):
+    """Reference CPU time of the Green's-function contraction: the FP64
+    lookup-table loop of generateGreensFunctionCPU (Lib/src/Solver/
+    ChebyshevExpander.cpp:324-370; one vector per call, the table built
+    once) timed for `rows` vectors and extrapolated to K."""
+    cpu, nproc, _ = cpu_info()
+    kw = dict(mode="timegf", M=M, window=f"{-0.99 * scale},{0.99 * scale},{E}", scale=scale,
+              type="Retarded", rows=rows)
+    (r1,), = ref_runs(1, **kw)
+    per = r1["seconds"] / rows
+    par = ref_runs(procs, **kw)
+    agg = sum(rows / p[0]["seconds"] for p in par)   # vectors per second, all processes
+    return {"value": K * per, "unit": "s", "cores": 1, "kind": "reference",
+            "sample": (f"reference lookup-table contraction, {rows} vectors of M={M} on E={E} "
+                       f"({r1['seconds']:.2f} s, table built apart in "
+                       f"{r1['table_and_first_seconds']:.1f} s), 1 thread, x {K} vectors"),
+            "throughput": {"value": K / agg, "unit": "s", "cores": procs,
+                           "aggregate_over_single": agg * per,
+                           "mode": f"{procs} processes at once, one vector sequence each"},
+            "cpu_model": cpu, "nproc": nproc}
 
 
 def load_traffic(kernel):
@@ -490,13 +604,15 @@ def main():
         import torch
         torch.zeros(1, device="cuda")
         torch.cuda.synchronize()
-    eng = X.Engine(0)
-    if args.complex:
-        eng.set_option(X.OPT_REAL, 0)
-    if args.no_palette:
-        eng.set_option(X.OPT_PALETTE, 0)
+    eng = None
+    if any(p != "cpu" for p in parts):
+        eng = X.Engine(0)
+        if args.complex:
+            eng.set_option(X.OPT_REAL, 0)
+        if args.no_palette:
+            eng.set_option(X.OPT_PALETTE, 0)
     model = None
-    if "plain" in parts or "cpu" in parts:
+    if "plain" in parts:
         log("building 1024x1024 spin lattice")
         model = L.config_plain_large(seed=2)
     if "plain" in parts:
@@ -591,14 +707,94 @@ def main():
     if "mfma" in parts:
         log("MFMA contraction (1e5 x 4096 x 8192)")
         out["contraction_mfma"] = contraction_mfma(eng)
-    eng.close()
+    if eng is not None:
+        eng.close()
     if "cpu" in parts:
-        log("CPU baseline (reference CPU path, bounded sample)")
-        out["cpu_baseline"] = cpu_baseline(model, "512,512,+", SCALE, WINDOW, E_PTS,
-                                           M_PLAIN, 2)
+        cpu_baselines(out, parts, model)
     elif "plain" in parts:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+def cpu_baselines(out, parts, model):
+    """Reference CPU baselines (oracle/_ref/ref_driver, the reference's own
+    CPU path compiled from its sources) for every configuration whose GPU
+    part ran (all of them with --parts cpu alone), each on a bounded sample,
+    one core and one process per core; the GPU/CPU ratios beside them."""
+    from oracle import oracle as O
+    if not O.ref_available():
+        log("no reference build (oracle/_ref/ref_driver): CPU baselines skipped")
+        out["cpu_baseline"] = None
+        return
+    only_cpu = parts == ["cpu"]
+    procs = cpu_info()[2]
+    factor = None   # throughput mode: aggregate over single-core rate
+    with tempfile.TemporaryDirectory() as td:
+        if "plain" in parts or only_cpu:
+            log("CPU baseline: plain 1024x1024 spin lattice")
+            if model is None:
+                model = L.config_plain_large(seed=2)
+            out["cpu_baseline"] = baseline_ldos(model, SCALE, E_PTS, "512,512,+", 2, M_PLAIN,
+                                                "plain", procs, td)
+            factor = out["cpu_baseline"]["throughput"]["aggregate_over_single"]
+        if "batched" in out or only_cpu:
+            log("CPU baseline: 512x512 Nambu, one site")
+            m = L.config_nambu(512)
+            site = np.sort(np.random.default_rng(3).choice(m.basis_size, B_SITES,
+                                                           replace=False))[0]
+            idx = ",".join(map(str, m.physical_index(int(site))))
+            cb = baseline_ldos(m, m.gershgorin_bound() * 1.05, E_PTS, idx, 1, M_BATCH,
+                               "nambu", procs, td, factor=factor)
+            factor = cb["throughput"]["aggregate_over_single"]
+            del m
+            sub = out.setdefault("batched", {})
+            sub["cpu_baseline"] = cb
+            if "sites_per_s" in sub:
+                sub["vs_cpu_1core"] = sub["sites_per_s"] / cb["value"]
+                sub["vs_cpu_throughput"] = sub["sites_per_s"] / cb["throughput"]["value"]
+        if "uneven_batch" in out or only_cpu:
+            log("CPU baseline: uneven batch (two sample models)")
+            if factor is None:
+                factor = baseline_ldos(L.config_nambu(64), 6.4, E_PTS, "0,0,0", 1, 500,
+                                       "factor", procs, td)["throughput"]["aggregate_over_single"]
+            cb = baseline_uneven(L.config_uneven(seed=4), procs, td, factor)
+            sub = out.setdefault("uneven_batch", {})
+            sub["cpu_baseline"] = cb
+            if "sites_per_s" in sub:
+                sub["vs_cpu_1core"] = sub["sites_per_s"] / cb["value"]
+                sub["vs_cpu_throughput"] = sub["sites_per_s"] / cb["throughput"]["value"]
+        if "stochastic_dos" in out or only_cpu:
+            # The reference's DOS is sampleDOS (Lib/src/PropertyExtractor/
+            # PropertyExtractor.cpp:234-290): one LDOS recurrence per sampled
+            # site, so 512 vectors cost 512 single-site LDOS calls.
+            log("CPU baseline: 128^3 cubic, one site")
+            m = L.config_cubic(128)
+            cl = baseline_ldos(m, m.gershgorin_bound() * 1.05, E_PTS, "64,64,64", 1, 4096,
+                               "cubic", procs, td, factor=factor)
+            factor = cl["throughput"]["aggregate_over_single"]
+            del m
+            R = 512
+            cb = {"value": R / cl["value"], "unit": "s", "cores": 1, "kind": "reference",
+                  "sample": (cl["sample"] + f"; x {R} samples (sampleDOS sums one "
+                             "single-site LDOS per sample)"),
+                  "throughput": {"value": R / cl["throughput"]["value"], "unit": "s",
+                                 "cores": cl["throughput"]["cores"],
+                                 "aggregate_over_single": factor,
+                                 "mode": cl["throughput"]["mode"]},
+                  "cpu_model": cl["cpu_model"], "nproc": cl["nproc"]}
+            sub = out.setdefault("stochastic_dos", {})
+            sub["cpu_baseline"] = cb
+            if "seconds" in sub:
+                sub["vs_cpu_1core"] = cb["value"] / sub["seconds"]
+                sub["vs_cpu_throughput"] = cb["throughput"]["value"] / sub["seconds"]
+        if "contraction_mfma" in out or only_cpu:
+            log("CPU baseline: FP64 lookup-table contraction")
+            cb = baseline_contraction(100_000, 4096, 8192, 10.0, procs)
+            sub = out.setdefault("contraction_mfma", {})
+            sub["cpu_baseline"] = cb
+            if "ms" in sub:
+                sub["vs_cpu_1core"] = cb["value"] / (sub["ms"] * 1e-3)
+                sub["vs_cpu_throughput"] = cb["throughput"]["value"] / (sub["ms"] * 1e-3)
 
 
 if __name__ == "__main__":

@@ -5,7 +5,7 @@
 // baseline of bench.py. It is never part of the product path.
 //
 // Usage: ref_driver mode=<mode> key=value ...
-//   modes: csr | coef | gf | gfpat | gfcoef | ldos | sampledos | timeldos |
+//   modes: csr | coef | gf | gfpat | gfcoef | timegf | ldos | sampledos | timeldos |
 //          density | mag | splodos | expval
 //   model=<file>        binary HoppingAmplitude list (format in
 //                       tbtk_amd/lattice.py: write_amplitudes)
@@ -192,6 +192,41 @@ int main(int argc, char **argv){
 			c, solverType(arg("type"))
 		);
 		writeBin(arg("out"), g.data(), g.size()*sizeof(complex<double>));
+		return 0;
+	}
+
+	if(mode == "timegf"){
+		// Times the lookup-table contraction of generateGreensFunctionCPU
+		// (Lib/src/Solver/ChebyshevExpander.cpp:324-370) for rows= coefficient
+		// vectors of M= moments on window= (no model): the first call builds
+		// the table (timed apart), then rows= calls are timed. Coefficients
+		// are synthetic (the loop does not depend on their values).
+		const int M = atoi(arg("M").c_str()), R = atoi(arg("rows", "2").c_str());
+		Solver::ChebyshevExpander solver;
+		solver.setVerbose(false);
+		solver.setScaleFactor(atof(arg("scale").c_str()));
+		solver.setNumCoefficients(M);
+		solver.setGenerateGreensFunctionsOnGPU(false);
+		solver.setUseLookupTable(true);
+		double lo, hi; int res;
+		window(lo, hi, res);
+		solver.setEnergyWindow(Range(lo, hi, res));
+		vector<complex<double>> c(M);
+		for(int n = 0; n < M; n++)
+			c[n] = complex<double>(cos(0.37*n), sin(0.11*n))*(1. - n/(double)M);
+		auto t0 = std::chrono::steady_clock::now();
+		vector<complex<double>> g = solver.generateGreensFunction(c, solverType(arg("type")));
+		auto t1 = std::chrono::steady_clock::now();
+		double acc = 0;
+		for(int r = 0; r < R; r++){
+			c[r % M] += 1e-3;
+			g = solver.generateGreensFunction(c, solverType(arg("type")));
+			acc += g[r % res].real();
+		}
+		auto t2 = std::chrono::steady_clock::now();
+		fprintf(stderr, "{\"table_and_first_seconds\": %.6f, \"rows\": %d, \"seconds\": %.6f, \"check\": %.17g}\n",
+			std::chrono::duration<double>(t1 - t0).count(), R,
+			std::chrono::duration<double>(t2 - t1).count(), acc);
 		return 0;
 	}
 
