@@ -643,8 +643,15 @@ def main():
         # Value palette (off-diagonal values as 1-byte codes, the diagonal
         # from a per-row array): 3 bytes per non-zero, 8 or 16 per row.
         palette = eng.get_option(X.OPT_PALETTE) == 1
+        # Slot-major ELL step (one thread per row, W slots per row, W = the
+        # longest row, at least 4) or the row-block CSR step.
+        slot = eng.get_option(X.OPT_PLAIN_KERNEL) == 2
         cb = 2 if col16 else 4
-        if palette:
+        e = 8 if real else 16
+        if slot:
+            W = max(4, int(np.diff(csr[0]).max()))
+            step_bytes = W * N * (2 + (1 if palette else e)) + N * (3 * e + (e if palette else 0))
+        elif palette:
             step_bytes = 3 * Z + 36 * N if real else 3 * Z + 68 * N
         else:
             step_bytes = (8 + cb) * Z + 28 * N if real else (16 + cb) * Z + 52 * N
@@ -674,9 +681,13 @@ def main():
             "physical_sites_per_s": value / 2,
             "spmv_GBps": achieved,
             "roofline": {
-                "kernel": (("plain_step_lds<double> (real FP64 CSR SpMV recurrence step; "
-                            "H is real)" if real else
-                            "plain_step_lds<double2> (complex FP64 CSR SpMV recurrence step)")
+                "kernel": ((("plain_step_slot<double> (real FP64 slot-major ELL SpMV "
+                             "recurrence step, one thread per row; H is real)" if real else
+                             "plain_step_slot<double2> (complex FP64 slot-major ELL SpMV "
+                             "recurrence step)") if slot else
+                            ("plain_step_lds<double> (real FP64 CSR SpMV recurrence step; "
+                             "H is real)" if real else
+                             "plain_step_lds<double2> (complex FP64 CSR SpMV recurrence step)"))
                            + ("; 16-bit column offsets" if col16 else "; 32-bit columns")
                            + ("; 1-byte value codes (palette) + diagonal array"
                               if palette else "")),
@@ -685,8 +696,9 @@ def main():
                 "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBPS,
-                "traffic": load_traffic(("plain_step_lds_real" if real else "plain_step_lds")
-                                        + ("_col16" if col16 else "")
+                "traffic": load_traffic(("plain_step_slot" if slot else "plain_step_lds")
+                                        + ("_real" if real else "")
+                                        + ("_col16" if col16 and not slot else "")
                                         + ("_palette" if palette else "")),
                 "algorithmic_bytes_per_launch": step_bytes,
                 "avg_launch_us": avg_kernel_s * 1e6,

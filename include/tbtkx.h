@@ -316,8 +316,11 @@ int tbtkx_multi_moments(tbtkx_handle handle, const int *num_sites,
 int tbtkx_last_timing(tbtkx_handle handle, double *milliseconds,
                       int64_t *launches);
 
-/* Engine options (0 = default). TBTKX_OPT_PLAIN_KERNEL: 0 = LDS-staged
- * row blocks, 1 = one thread per row. TBTKX_OPT_GRAPH: 1 = replay the step
+/* Engine options (0 = default unless stated). TBTKX_OPT_PLAIN_KERNEL:
+ * 2 (default) = slot-major ELL (one thread per row, 16-bit offsets from the
+ * row, used when every row has at most 8 non-zeros and every offset fits;
+ * else 0), 0 = LDS-staged row blocks, 1 = one thread per row on the CSR;
+ * reading it gives the kernel the last plain call ran. TBTKX_OPT_GRAPH: 1 = replay the step
  * loop as a hipGraph. TBTKX_OPT_BLOCK_WIDTH: start vectors per SpMM block
  * (64, 128, 256; 0 = chosen by the width autotuner, which times each width on
  * the resident model once per model, precision and size class).

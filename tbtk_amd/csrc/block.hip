@@ -1467,13 +1467,7 @@ int run_exact(tbtkx_context *h, int nvec, const int64_t *from, const std::vector
 // B (20 Z + 4 N for FP64 H and row pointers, 48 B N for the three block
 // passes, per step and block).
 double plain_call_bytes(const tbtkx_context *h, int nvec) {
-	const double N = (double)h->N, Z = (double)h->Z;
-	const bool real = h->real_path && h->h_real;
-	const bool c16 = h->plain_kernel == 0 && h->use_col16 && h->col16_ok;
-	const bool pal = c16 && h->use_palette && h->palette_ok;
-	const double per = real ? (pal ? 3 * Z + 36 * N : c16 ? 10 * Z + 28 * N : 12 * Z + 28 * N)
-	                        : (pal ? 3 * Z + 68 * N : c16 ? 18 * Z + 52 * N : 20 * Z + 52 * N);
-	return per * nvec;
+	return plain_form(h).bytes * nvec;
 }
 double block_call_bytes(const tbtkx_context *h, int nvec, int B, int packed) {
 	const double N = (double)h->N, Z = (double)h->Z;

@@ -132,6 +132,13 @@ struct tbtkx_context {
 	// into pal (256 values of H/s, then 256 of 2H/s); palr/pdiagr: real parts.
 	tbtkx::DevBuf pcode, pal, palr, pdiag, pdiagr;
 	int palette_ok = 0;
+	// Slot-major ELL form of the plain kernel (slot_ok: every row has at
+	// most 8 non-zeros and every column lies within int16 of its row):
+	// slot_W slots per row, 16-bit column offsets from the row, palette codes
+	// (slot_pal) or H/s, 2H/s values (complex, and real for a real H).
+	tbtkx::DevBuf scol, scode, sv1, sv2, sv1r, sv2r;
+	int slot_ok = 0, slot_pal = 0, slot_vals = 0, slot_W = 0;
+	int last_plain_kernel = 0;   // kernel of the last plain call (0, 1, 2)
 
 	// Plain path vectors and moment output.
 	tbtkx::DevBuf vec_a, vec_b, coef, to;
@@ -185,7 +192,7 @@ struct tbtkx_context {
 	int damp_kind = 0;
 	double damp_broadening = 0;
 
-	int plain_kernel = 0;
+	int plain_kernel = 2;
 	int use_col16 = 1;
 	int last_col16 = 0;   // the last plain call used 16-bit columns
 	int use_palette = 1;
@@ -234,6 +241,38 @@ int apply_damping(tbtkx_context *h, double2 *C, int K, int M);
 // Builds the 16-bit column offsets of the resident CSR (plain.hip); sets
 // col16_ok. Needs rowptr, col, rblk and nblk in place.
 int build_col16(tbtkx_context *h);
+
+// Builds the slot-major ELL form of the plain kernel from the resident CSR
+// and palette (plain.hip); `rowptr` is a host copy of the row pointer.
+int build_slots(tbtkx_context *h, const uint32_t *rowptr);
+// Adds H/s and 2H/s values to the slots (for calls without the palette).
+int build_slot_values(tbtkx_context *h);
+
+// The form the plain recurrence step runs in: the slot-major ELL kernel
+// (TBTKX_OPT_PLAIN_KERNEL 2, needs 16-bit columns), 16-bit column offsets,
+// the value palette; and the algorithmic HBM bytes of one step in that form.
+struct PlainForm {
+	bool slot, c16, pal;
+	double bytes;
+};
+inline PlainForm plain_form(const tbtkx_context *h) {
+	const bool real = h->real_path && h->h_real;
+	PlainForm f{};
+	f.slot = h->plain_kernel == 2 && h->slot_ok && h->use_col16 &&
+	         ((h->use_palette && h->slot_pal) || h->slot_vals);
+	f.c16 = f.slot || (h->plain_kernel == 0 && h->use_col16 && h->col16_ok);
+	f.pal = f.c16 && h->use_palette && (f.slot ? h->slot_pal : h->palette_ok);
+	const double N = (double)h->N, Z = (double)h->Z, e = real ? 8 : 16;
+	if (f.slot)
+		// per slot: 2 B column + 1 B code or a value; per row: x, j read and
+		// written, and the diagonal with the palette
+		f.bytes = (double)h->slot_W * N * (2 + (f.pal ? 1 : e)) + N * (3 * e + (f.pal ? e : 0));
+	else if (f.pal)
+		f.bytes = 3 * Z + N * (4 + 4 * e);
+	else
+		f.bytes = (e + (f.c16 ? 2 : 4)) * Z + N * (4 + 3 * e);
+	return f;
+}
 
 // Builds the plain kernel's value palette from host copies of the CSR
 // (v1 = H/s, v2 = 2H/s, interleaved complex); sets palette_ok when the

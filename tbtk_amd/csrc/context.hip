@@ -196,7 +196,7 @@ int tbtkx_destroy(tbtkx_handle h) {
 	for (DevBuf *b : {&h->rowptr, &h->col, &h->v1, &h->v2, &h->v1f, &h->v2f,
 	                  &h->v1r, &h->v2r, &h->lat_terms, &h->lat_onsite, &h->lat_count,
 	                  &h->lat_part, &h->lat_tmp, &h->rblk, &h->col16, &h->col16_flag,
-	                  &h->pcode, &h->pal, &h->palr, &h->pdiag, &h->pdiagr, &h->vec_a, &h->vec_b, &h->coef, &h->to,
+	                  &h->pcode, &h->pal, &h->palr, &h->pdiag, &h->pdiagr, &h->scol, &h->scode, &h->sv1, &h->sv2, &h->sv1r, &h->sv2r, &h->vec_a, &h->vec_b, &h->coef, &h->to,
 	                  &h->lut, &h->gf_in, &h->gf_out, &h->dfac, &h->dden, &h->blk_x, &h->blk_y,
 	                  &h->blk_mu, &h->blk_sites, &h->blk_part, &h->blk_grow, &h->blk_gvec, &h->cplanes,
 	                  &h->lplanes, &h->gf_mfma_out, &h->ell_col, &h->ell_v1f,
@@ -353,7 +353,8 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 	h->h_real = real;
 	h->nblk = (int)blocks.size() - 1;
 	if ((rc = build_col16(h))) return rc;
-	return build_palette(h, N, Z, rowptr, col, v1.data(), v2.data());
+	if ((rc = build_palette(h, N, Z, rowptr, col, v1.data(), v2.data()))) return rc;
+	return build_slots(h, rowptr);
 }
 
 int tbtkx_last_timing(tbtkx_handle h, double *ms, int64_t *launches) {
@@ -370,8 +371,8 @@ int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
 	TBTKX_LOCK(h);
 	switch (option) {
 	case TBTKX_OPT_PLAIN_KERNEL:
-		TBTKX_REQUIRE(value == 0 || value == 1, TBTKX_ERR_ARG,
-		              "tbtkx_set_option: plain kernel must be 0 or 1");
+		TBTKX_REQUIRE(value >= 0 && value <= 2, TBTKX_ERR_ARG,
+		              "tbtkx_set_option: plain kernel must be 0, 1 or 2");
 		h->plain_kernel = (int)value;
 		h->plain_graph.reset();
 		return TBTKX_OK;
@@ -461,7 +462,7 @@ int tbtkx_get_option(tbtkx_handle h, int option, int64_t *value) {
 	TBTKX_REQUIRE(h && value, TBTKX_ERR_ARG, "tbtkx_get_option: null pointer");
 	TBTKX_LOCK(h);
 	switch (option) {
-	case TBTKX_OPT_PLAIN_KERNEL: *value = h->plain_kernel; return TBTKX_OK;
+	case TBTKX_OPT_PLAIN_KERNEL: *value = h->last_plain_kernel; return TBTKX_OK;
 	case TBTKX_OPT_GRAPH: *value = h->use_graph; return TBTKX_OK;
 	case TBTKX_OPT_BLOCK_WIDTH:
 		*value = h->block_width ? h->block_width : h->last_block_width;

@@ -442,7 +442,9 @@ int tbtkx_set_lattice(tbtkx_handle h, const tbtkx_lattice *lat, const double *on
 		TBTKX_CHECK_HIP(hipMemcpy(hv1.data(), h->v1.ptr, Z * 16, hipMemcpyDeviceToHost));
 		TBTKX_CHECK_HIP(hipMemcpy(hv2.data(), h->v2.ptr, Z * 16, hipMemcpyDeviceToHost));
 	}
-	return build_palette(h, N, Z, hrp.data(), hcol.data(), hv1.data(), hv2.data());
+	if ((rc = build_palette(h, N, Z, hrp.data(), hcol.data(), hv1.data(), hv2.data())))
+		return rc;
+	return build_slots(h, hrp.data());
 }
 
 int tbtkx_lattice_gershgorin(tbtkx_handle h, const tbtkx_lattice *lat, const double *onsite,

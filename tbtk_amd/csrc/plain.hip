@@ -18,6 +18,7 @@
 // (a 1-byte code per non-zero: an index into at most 255 distinct
 // off-diagonal values, or 255 for the row's diagonal, read from a per-row
 // array): 3 Z + 68 N complex, 3 Z + 36 N real.
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <type_traits>
@@ -87,6 +88,100 @@ __global__ __launch_bounds__(kThreads) void make_col16(
 		col16[k] = fits ? (int16_t)d : (int16_t)0;
 		if (!fits) atomicOr(overflow, 1);
 	}
+}
+
+// Slot-major ELL form of the plain kernel (TBTKX_OPT_PLAIN_KERNEL = 2):
+// W slots per row (W >= the longest row), slot s of row r at s*N + r, so a
+// wavefront's 64 rows read each slot with one coalesced load. A slot holds
+// the column as a 16-bit offset from the row itself (kSlotPad marks the
+// padding after a row's last non-zero) and the value as a palette code
+// (PAL) or as H/s and 2H/s. The slots of a row keep the CSR order (ascending
+// columns), so the sums are the reference's, bit for bit.
+constexpr int16_t kSlotPad = -32768;
+
+template <typename C, typename R>
+__global__ __launch_bounds__(kThreads) void make_slots(
+    const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ col,
+    const uint8_t *__restrict__ pcode, const C *__restrict__ v1, const C *__restrict__ v2,
+    const R *__restrict__ v1r, const R *__restrict__ v2r, int64_t N, int W,
+    int16_t *__restrict__ scol, uint8_t *__restrict__ scode, C *__restrict__ sv1,
+    C *__restrict__ sv2, R *__restrict__ sv1r, R *__restrict__ sv2r, int *overflow) {
+	const int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+	if (r >= N) return;
+	const uint32_t k0 = rowptr[r];
+	const int len = (int)(rowptr[r + 1] - k0);
+	for (int s = 0; s < W; s++) {
+		const int64_t i = (int64_t)s * N + r;
+		if (s < len) {
+			const uint32_t k = k0 + s;
+			const int64_t d = (int64_t)col[k] - r;
+			const bool fits = d >= -32767 && d <= 32767;
+			scol[i] = fits ? (int16_t)d : (int16_t)0;
+			if (!fits) atomicOr(overflow, 1);
+			if (scode) scode[i] = pcode[k];
+			if (sv1) {
+				sv1[i] = v1[k];
+				sv2[i] = v2[k];
+			}
+			if (sv1r) {
+				sv1r[i] = v1r[k];
+				sv2r[i] = v2r[k];
+			}
+		} else {
+			scol[i] = kSlotPad;
+			if (scode) scode[i] = 0;
+			if (sv1) {
+				sv1[i] = C{};
+				sv2[i] = C{};
+			}
+			if (sv1r) {
+				sv1r[i] = R{};
+				sv2r[i] = R{};
+			}
+		}
+	}
+}
+
+// One thread per row on the slot-major form: the row's W column offsets and
+// codes (or values), its j_{n-2} entry and diagonal value are loaded at
+// once, then its W gathers of x, then the products are added in slot order
+// from -j_{n-2} (padding slots skipped). Two dependent memory trips per
+// step and no LDS or barrier.
+template <typename S, bool FIRST, bool PAL, int W>
+__global__ __launch_bounds__(kThreads) void plain_step_slot(
+    const int16_t *__restrict__ scol, const uint8_t *__restrict__ scode,
+    const S *__restrict__ sval, const S *__restrict__ pal, const S *__restrict__ diag,
+    const S *__restrict__ x, S *y, int64_t N, const int64_t *__restrict__ to, int K,
+    double2 *__restrict__ coef, int M, int nprev) {
+#pragma clang fp contract(off)
+	typedef F64<S> A;
+	const int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+	if (r < K) coef[r * M + nprev] = A::moment(x[to[r]]);
+	if (r >= N) return;
+	int16_t c[W];
+	uint8_t q[W];
+	S v[W], xx[W];
+#pragma unroll
+	for (int s = 0; s < W; s++) {
+		c[s] = scol[(int64_t)s * N + r];
+		if (PAL) q[s] = scode[(int64_t)s * N + r];
+		else v[s] = sval[(int64_t)s * N + r];
+	}
+	const S yo = FIRST ? A::zero() : y[r];
+	const S dg = PAL ? diag[r] : A::zero();
+#pragma unroll
+	for (int s = 0; s < W; s++) xx[s] = x[r + (c[s] == kSlotPad ? 0 : (int64_t)c[s])];
+	if (PAL) {
+#pragma unroll
+		for (int s = 0; s < W; s++) v[s] = q[s] == 255 ? dg : pal[q[s]];
+	}
+	S acc = FIRST ? A::zero() : A::neg(yo);
+#pragma unroll
+	for (int s = 0; s < W; s++) {
+		const S p = A::mul(v[s], xx[s]);
+		if (c[s] != kSlotPad) acc = A::add(acc, p);
+	}
+	y[r] = acc;
 }
 
 // Row-block kernel: the workgroup streams its block's non-zeros with
@@ -255,6 +350,38 @@ void launch_lds(tbtkx_context *h, bool first, const CI *col, const S *v, const S
 		    h->xcd_remap, to, K, coef, M, nprev, code, pal, diag);
 }
 
+template <typename S, bool FIRST, bool PAL>
+void launch_slot_w(tbtkx_context *h, int W, const int16_t *scol, const uint8_t *scode,
+                   const S *sv, const S *pal, const S *diag, const S *x, S *y,
+                   const int64_t *to, int K, double2 *coef, int M, int nprev) {
+	const unsigned grid = grid_for(h->N);
+#define TBTKX_SLOT(WW)                                                                  \
+	plain_step_slot<S, FIRST, PAL, WW><<<grid, kThreads, 0, h->stream>>>(             \
+	    scol, scode, sv, pal, diag, x, y, h->N, to, K, coef, M, nprev)
+	switch (W) {
+	case 4: TBTKX_SLOT(4); break;
+	case 5: TBTKX_SLOT(5); break;
+	case 6: TBTKX_SLOT(6); break;
+	case 7: TBTKX_SLOT(7); break;
+	default: TBTKX_SLOT(8); break;
+	}
+#undef TBTKX_SLOT
+}
+
+template <typename S>
+void launch_slot(tbtkx_context *h, bool first, bool pal, int W, const int16_t *scol,
+                 const uint8_t *scode, const S *sv, const S *pp, const S *dp, const S *x, S *y,
+                 const int64_t *to, int K, double2 *coef, int M, int nprev) {
+	if (first && pal)
+		launch_slot_w<S, true, true>(h, W, scol, scode, sv, pp, dp, x, y, to, K, coef, M, nprev);
+	else if (first)
+		launch_slot_w<S, true, false>(h, W, scol, scode, sv, pp, dp, x, y, to, K, coef, M, nprev);
+	else if (pal)
+		launch_slot_w<S, false, true>(h, W, scol, scode, sv, pp, dp, x, y, to, K, coef, M, nprev);
+	else
+		launch_slot_w<S, false, false>(h, W, scol, scode, sv, pp, dp, x, y, to, K, coef, M, nprev);
+}
+
 // Enqueue the step launches n = 1 .. M-1 and the final gather; S = double2
 // (complex) or double (real H and start vector: v1r/v2r, vectors of N
 // doubles in the same buffers).
@@ -266,19 +393,32 @@ int enqueue_steps(tbtkx_context *h, int K, int M, S *a, S *b, double2 *coef,
 	const S *v2 = real ? h->v2r.as<S>() : h->v2.as<S>();
 	const uint32_t *rp = h->rowptr.as<uint32_t>(), *cl = h->col.as<uint32_t>();
 	const int remap = h->xcd_remap;
-	const bool lds = h->plain_kernel == 0;
+	const bool lds = h->plain_kernel != 1;
 	const bool c16 = lds && h->use_col16 && h->col16_ok;
 	const bool usepal = c16 && h->use_palette && h->palette_ok;
 	const S *palp = real ? h->palr.as<S>() : h->pal.as<S>();
 	const S *diagp = real ? h->pdiagr.as<S>() : h->pdiag.as<S>();
 	const int nwg_row = (int)grid_for(h->N);
 	const unsigned grid = lds ? (unsigned)h->nblk : (unsigned)nwg_row;
+	const PlainForm pf = plain_form(h);
 	// The moment gather rides on the step launches; their grid covers
 	// K <= N threads (to-indices are unique basis indices).
 	for (int n = 1; n < M; n++) {
 		const S *x = (n & 1) ? a : b;
 		S *y = (n & 1) ? b : a;
 		const S *v = n == 1 ? v1 : v2;
+		if (pf.slot) {
+			const bool first = n == 1;
+			const S *sv = nullptr;
+			if (!pf.pal)
+				sv = real ? (first ? h->sv1r.as<S>() : h->sv2r.as<S>())
+				          : (first ? h->sv1.as<S>() : h->sv2.as<S>());
+			const S *pp = pf.pal ? palp + (first ? 0 : 256) : nullptr;
+			const S *dp = pf.pal ? diagp + (first ? 0 : h->N) : nullptr;
+			launch_slot<S>(h, first, pf.pal, h->slot_W, h->scol.as<int16_t>(),
+			               h->scode.as<uint8_t>(), sv, pp, dp, x, y, to, K, coef, M, n - 1);
+			continue;
+		}
 		if (lds) {
 			if (usepal)
 				launch_lds<S, int16_t, true>(h, n == 1, h->col16.as<int16_t>(), v, x, y, to,
@@ -308,6 +448,68 @@ int enqueue_steps(tbtkx_context *h, int K, int M, S *a, S *b, double2 *coef,
 }
 
 }  // namespace
+
+// Slot columns, and palette codes when the palette exists (values are added
+// by build_slot_values when a call needs them).
+static int make_slot_arrays(tbtkx_context *h, bool codes, bool values) {
+	const int W = h->slot_W;
+	const int64_t WN = (int64_t)W * h->N;
+	int rc;
+	if ((rc = h->scol.reserve((size_t)WN * 2))) return rc;
+	if (codes && (rc = h->scode.reserve((size_t)WN))) return rc;
+	const bool realv = values && h->h_real;
+	if (values) {
+		if ((rc = h->sv1.reserve((size_t)WN * 16))) return rc;
+		if ((rc = h->sv2.reserve((size_t)WN * 16))) return rc;
+		if (realv) {
+			if ((rc = h->sv1r.reserve((size_t)WN * 8))) return rc;
+			if ((rc = h->sv2r.reserve((size_t)WN * 8))) return rc;
+		}
+	}
+	if ((rc = h->col16_flag.reserve(4))) return rc;
+	TBTKX_CHECK_HIP(hipMemsetAsync(h->col16_flag.ptr, 0, 4, h->stream));
+	make_slots<double2, double><<<grid_for(h->N), kThreads, 0, h->stream>>>(
+	    h->rowptr.as<uint32_t>(), h->col.as<uint32_t>(), codes ? h->pcode.as<uint8_t>() : nullptr,
+	    h->v1.as<double2>(), h->v2.as<double2>(), h->v1r.as<double>(), h->v2r.as<double>(),
+	    h->N, W, h->scol.as<int16_t>(), codes ? h->scode.as<uint8_t>() : nullptr,
+	    values ? h->sv1.as<double2>() : nullptr, values ? h->sv2.as<double2>() : nullptr,
+	    realv ? h->sv1r.as<double>() : nullptr, realv ? h->sv2r.as<double>() : nullptr,
+	    h->col16_flag.as<int>());
+	TBTKX_CHECK_HIP(hipGetLastError());
+	int overflow = 1;
+	TBTKX_CHECK_HIP(hipMemcpyAsync(&overflow, h->col16_flag.ptr, 4, hipMemcpyDeviceToHost,
+	                               h->stream));
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	if (overflow) h->slot_ok = 0;
+	return TBTKX_OK;
+}
+
+int build_slots(tbtkx_context *h, const uint32_t *rowptr) {
+	h->slot_ok = 0;
+	h->slot_pal = 0;
+	h->slot_vals = 0;
+	h->slot_W = 0;
+	int64_t maxrow = 0;
+	for (int64_t r = 0; r < h->N; r++)
+		maxrow = std::max<int64_t>(maxrow, rowptr[r + 1] - rowptr[r]);
+	if (h->N <= 0 || maxrow > 8) return TBTKX_OK;
+	h->slot_W = maxrow < 4 ? 4 : (int)maxrow;
+	h->slot_ok = 1;
+	const bool pal = h->palette_ok != 0;
+	int rc = make_slot_arrays(h, pal, !pal);
+	if (rc || !h->slot_ok) return rc;
+	h->slot_pal = pal;
+	h->slot_vals = !pal;
+	return TBTKX_OK;
+}
+
+int build_slot_values(tbtkx_context *h) {
+	if (!h->slot_ok || h->slot_vals) return TBTKX_OK;
+	int rc = make_slot_arrays(h, h->slot_pal, true);
+	if (rc) return rc;
+	h->slot_vals = h->slot_ok;
+	return TBTKX_OK;
+}
 
 int build_col16(tbtkx_context *h) {
 	h->col16_ok = 0;
@@ -482,8 +684,8 @@ extern "C" int tbtkx_moments_noalloc(tbtkx_handle h, int64_t from, int K,
 	// Every scratch byte the call reads is written first: the start vector
 	// by init_unit, the other vector by the first step (which does not read
 	// it), every moment by a gather.
-	const bool c16 = h->plain_kernel == 0 && h->use_col16 && h->col16_ok;
-	const bool usepal = c16 && h->use_palette && h->palette_ok;
+	const PlainForm pform = plain_form(h);
+	const bool c16 = pform.c16, usepal = pform.pal;
 	TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
 	if (real) {
 		init_unit<double><<<grid_for(h->N), kThreads, 0, h->stream>>>(
@@ -508,6 +710,7 @@ extern "C" int tbtkx_moments_noalloc(tbtkx_handle h, int64_t from, int K,
 	h->last_real = real;
 	h->last_col16 = c16;
 	h->last_palette = usepal;
+	h->last_plain_kernel = pform.slot ? 2 : h->plain_kernel == 1 ? 1 : 0;
 	// A repeated to-index keeps only its last position (as tbtkx_moments).
 	for (int k = 0; k < K; k++)
 		for (int l = k + 1; l < K; l++)
@@ -550,6 +753,10 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 	const int U = (int)uniq.size();
 	int rc = set_device(h);
 	if (rc) return rc;
+	// The slot form without the palette needs the values in slots.
+	if (h->plain_kernel == 2 && !(h->use_palette && h->slot_pal) &&
+	    (rc = build_slot_values(h)))
+		return rc;
 	if ((rc = h->to.reserve((size_t)U * 8))) return rc;
 	if ((rc = h->coef.reserve((size_t)U * M * 16))) return rc;
 	TBTKX_CHECK_HIP(hipMemcpyAsync(h->to.ptr, uniq.data(), (size_t)U * 8,
@@ -557,8 +764,8 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 	// Real arithmetic when every value of H is real (the start vector is a
 	// unit vector).
 	const bool real = h->real_path && h->h_real;
-	const bool c16 = h->plain_kernel == 0 && h->use_col16 && h->col16_ok;
-	const bool usepal = c16 && h->use_palette && h->palette_ok;
+	const PlainForm pform = plain_form(h);
+	const bool c16 = pform.c16, usepal = pform.pal;
 	if (real)
 		init_unit<double><<<grid_for(h->N), kThreads, 0, h->stream>>>(
 		    h->vec_a.as<double>(), h->N, from);
@@ -580,10 +787,11 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 		GraphCache &gc = h->plain_graph;
 		int64_t key[GraphCache::kKey];
 		const bool key_ok = GraphCache::make_key(
-		    key, {U, M, h->plain_kernel, h->xcd_remap, h->N, real, c16, usepal},
+		    key, {U, M, h->plain_kernel * 2 + pform.slot, h->xcd_remap, h->N, real, c16, usepal},
 		    {h->vec_a.ptr, h->vec_b.ptr, h->coef.ptr, h->to.ptr, h->rowptr.ptr, h->col.ptr,
 		     h->v1.ptr, h->v2.ptr, h->v1r.ptr, h->v2r.ptr, h->rblk.ptr, h->col16.ptr,
-		     h->pcode.ptr, h->pal.ptr, h->palr.ptr, h->pdiag.ptr, h->pdiagr.ptr});
+		     h->pcode.ptr, h->pal.ptr, h->palr.ptr, h->pdiag.ptr, h->pdiagr.ptr, h->scol.ptr,
+		     h->scode.ptr, h->sv1.ptr, h->sv2.ptr, h->sv1r.ptr, h->sv2r.ptr});
 		TBTKX_REQUIRE(key_ok, TBTKX_ERR_STATE, "tbtkx_moments: graph key too long");
 		if (!gc.matches(key)) {
 			gc.reset();
@@ -614,6 +822,7 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 	h->last_real = real;
 	h->last_col16 = c16;
 	h->last_palette = usepal;
+	h->last_plain_kernel = pform.slot ? 2 : h->plain_kernel == 1 ? 1 : 0;
 
 	for (int k = 0; k < K; k++) {
 		double *o = out + (size_t)k * M * 2;

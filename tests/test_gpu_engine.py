@@ -24,7 +24,7 @@ def _fixture_case(name):
     return fx, meta, m
 
 
-@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("kernel", [0, 1, 2])
 @pytest.mark.parametrize("name", sorted(MOMENT_FIXTURES))
 def test_plain_moments_bitexact(engine, oracle_mod, name, kernel):
     fx, meta, m = _fixture_case(name)
@@ -36,7 +36,7 @@ def test_plain_moments_bitexact(engine, oracle_mod, name, kernel):
     for b in meta["broadenings"]:
         got = oracle_mod.lorentz(raw, b) if b else raw
         assert np.array_equal(got, fx[f"coef_b{b:g}"]), (name, b, kernel)
-    engine.set_option(X.OPT_PLAIN_KERNEL, 0)
+    engine.set_option(X.OPT_PLAIN_KERNEL, 2)
 
 
 @pytest.mark.parametrize("graph", [0, 1])
@@ -72,7 +72,7 @@ def test_plain_long_rows_and_edge_cases(engine, oracle_mod):
             ref = oracle_mod.moments(csr, s, 0, [0, 9, 0, N - 1], M)
             assert np.array_equal(got, ref), (kernel, M)
             assert np.all(got[0] == 0)       # repeated to-index: last wins
-    engine.set_option(X.OPT_PLAIN_KERNEL, 0)
+    engine.set_option(X.OPT_PLAIN_KERNEL, 2)
 
 
 def test_plain_rejects_bad_arguments(engine):

@@ -27,7 +27,7 @@ pytestmark = pytest.mark.gpu
 def plain(engine):
     yield engine
     for opt, val in ((X.OPT_COL16, 1), (X.OPT_PALETTE, 1), (X.OPT_GRAPH, 0),
-                     (X.OPT_PLAIN_KERNEL, 0)):
+                     (X.OPT_PLAIN_KERNEL, 2)):
         engine.set_option(opt, val)
 
 

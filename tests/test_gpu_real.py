@@ -28,7 +28,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture
 def restore(engine):
     yield
-    for opt, val in ((X.OPT_REAL, 1), (X.OPT_BLOCK_WIDTH, 0), (X.OPT_PLAIN_KERNEL, 0),
+    for opt, val in ((X.OPT_REAL, 1), (X.OPT_BLOCK_WIDTH, 0), (X.OPT_PLAIN_KERNEL, 2),
                      (X.OPT_GRAPH, 0)):
         engine.set_option(opt, val)
 

@@ -1,0 +1,138 @@
+"""GPU tests of the slot-major ELL form of the plain FP64 step
+(TBTKX_OPT_PLAIN_KERNEL = 2): W slots per row, slot s of row r at s*N + r,
+16-bit column offsets from the row, palette codes or full values. The slots
+keep each row's CSR order, so the moments are bitwise equal to the oracle's
+(the reference's CPU recurrence, Lib/src/Solver/ChebyshevExpander.cpp:
+65-82, 157-186) and to the other kernels'. Models whose rows exceed 8
+non-zeros or whose offsets exceed int16 fall back to the row-block kernel.
+"""
+import numpy as np
+import pytest
+
+from tbtk_amd import engine as X
+from tbtk_amd import kpm
+from tbtk_amd import lattice as L
+from test_gpu_col16 import random_chain
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def slot(engine):
+    engine.set_option(X.OPT_PLAIN_KERNEL, 2)
+    yield engine
+    for opt, val in ((X.OPT_PLAIN_KERNEL, 2), (X.OPT_COL16, 1), (X.OPT_PALETTE, 1),
+                     (X.OPT_GRAPH, 0), (X.OPT_REAL, 1)):
+        engine.set_option(opt, val)
+
+
+MODELS = {
+    "square_spin_40x37": (lambda: L.square_spin(40, 37, disorder=1.0, seed=8), 1),
+    "nambu_12x9": (lambda: L.nambu_swave(12, 9, disorder=0.5, seed=3), 1),
+    "square_flux_20x20": (lambda: L.square(20, 20, disorder=0.3, seed=2, flux=0.07), 1),
+    "cubic_9": (lambda: L.cubic(9, 9, 9, disorder=1.0, seed=5), 1),
+    "honeycomb_7x11": (lambda: L.honeycomb_ribbon(7, 11, t=2.7, disorder=0.5, seed=4), 1),
+    "periodic_5": (lambda: L.periodic_square(5), 1),
+}
+
+
+@pytest.mark.parametrize("real", [1, 0])
+@pytest.mark.parametrize("name", sorted(MODELS))
+def test_slot_bitwise_vs_oracle_and_row_blocks(slot, oracle_mod, name, real):
+    make, palette = MODELS[name]
+    m = make()
+    csr = m.csr()
+    s = kpm.gershgorin_scale(csr, 1.05)
+    slot.set_option(X.OPT_REAL, real)
+    slot.set_csr(csr, s)
+    frm = m.basis_size // 2
+    to = [frm, 0, m.basis_size - 1, frm + 1, frm]
+    M = 300
+    ref = oracle_mod.moments(csr, s, frm, to, M)
+    print(f"{name}: N={m.basis_size} real={real}", flush=True)
+    for pal in (1, 0):
+        slot.set_option(X.OPT_PALETTE, pal)
+        got = slot.moments(frm, to, M)
+        assert slot.get_option(X.OPT_PLAIN_KERNEL) == 2
+        assert slot.get_option(X.OPT_PALETTE) == (pal and palette)
+        assert np.array_equal(got, ref), (name, pal)
+    slot.set_option(X.OPT_PLAIN_KERNEL, 0)
+    assert np.array_equal(slot.moments(frm, to, M), ref)
+    assert slot.get_option(X.OPT_PLAIN_KERNEL) == 0
+
+
+def test_slot_full_offset_range(slot, oracle_mod):
+    """The 16x1024 strip of the 1024x1024 spin lattice (offsets from the row
+    up to +-2048) with the palette, bitwise."""
+    m = L.square_spin(16, 1024, t=1.0, J=0.5, disorder=1.0, seed=2)
+    csr = m.csr()
+    s, M = 12.0, 400
+    centre = m.basis_index((8, 512, 0))
+    to = [centre, centre + 1, 0, m.basis_size - 1, centre - 2048, centre + 2049]
+    slot.set_csr(csr, s)
+    got = slot.moments(centre, to, M)
+    assert slot.get_option(X.OPT_PLAIN_KERNEL) == 2
+    assert slot.get_option(X.OPT_PALETTE) == 1
+    assert np.array_equal(got, oracle_mod.moments(csr, s, centre, to, M))
+
+
+def test_slot_full_size(slot, oracle_mod):
+    """The 1024x1024 spin lattice (2,097,152 states), 32 moments, bitwise."""
+    m = L.config_plain_large()
+    csr = m.csr()
+    s, M = 12.0, 32
+    frm = m.basis_index((512, 512, 0))
+    to = [frm, m.basis_index((512, 512, 1)), m.basis_index((513, 512, 0)),
+          m.basis_index((500, 520, 0))]
+    slot.set_csr(csr, s)
+    got = slot.moments(frm, to, M)
+    assert slot.get_option(X.OPT_PLAIN_KERNEL) == 2
+    assert np.array_equal(got, oracle_mod.moments(csr, s, frm, to, M))
+
+
+def test_slot_many_values_full_values(slot, oracle_mod):
+    """599 distinct hoppings: no palette, the slots carry full values."""
+    csr = random_chain(600, 11)
+    slot.set_csr(csr, 4.0)
+    got = slot.moments(300, [300, 0, 599], 120)
+    assert slot.get_option(X.OPT_PLAIN_KERNEL) == 2
+    assert slot.get_option(X.OPT_PALETTE) == 0
+    assert np.array_equal(got, oracle_mod.moments(csr, 4.0, 300, [300, 0, 599], 120))
+
+
+def test_slot_falls_back(slot, oracle_mod):
+    """A 70000-site ring (wrap-around offset 69999) and a star graph (a row
+    of 5000 non-zeros) run on the row-block kernel, bitwise."""
+    ring = L.ring(70000).csr()
+    slot.set_csr(ring, 2.5)
+    got = slot.moments(0, [0, 1, 69999], 80)
+    assert slot.get_option(X.OPT_PLAIN_KERNEL) == 0
+    assert np.array_equal(got, oracle_mod.moments(ring, 2.5, 0, [0, 1, 69999], 80))
+    N = 5000
+    rows = np.concatenate([np.zeros(N - 1, int), np.arange(1, N), np.arange(N)])
+    cols = np.concatenate([np.arange(1, N), np.zeros(N - 1, int), np.arange(N)])
+    vals = np.concatenate([np.full(N - 1, 0.01), np.full(N - 1, 0.01),
+                           np.linspace(-1, 1, N)]).astype(np.complex128)
+    star = L.coo_to_csr(rows, cols, vals, N)
+    slot.set_csr(star, 3.0)
+    got = slot.moments(0, [0, 9, N - 1], 50)
+    assert slot.get_option(X.OPT_PLAIN_KERNEL) == 0
+    assert np.array_equal(got, oracle_mod.moments(star, 3.0, 0, [0, 9, N - 1], 50))
+
+
+def test_slot_device_lattice_graph_and_noalloc(slot, oracle_mod):
+    """A device-built lattice, graph replays and the allocation-free entry
+    point on the slot form give the host model's oracle moments."""
+    m = L.square_spin(24, 20, disorder=1.0, seed=6)
+    spec = L.square_spin_spec(24, 20, disorder=1.0, seed=6)
+    s = kpm.gershgorin_scale(m.csr(), 1.05)
+    ref = oracle_mod.moments(m.csr(), s, 480, [480, 7], 300)
+    slot.set_lattice(spec, s)
+    assert np.array_equal(slot.moments(480, [480, 7], 300), ref)
+    assert slot.get_option(X.OPT_PLAIN_KERNEL) == 2
+    slot.set_option(X.OPT_GRAPH, 1)
+    for _ in range(3):
+        assert np.array_equal(slot.moments(480, [480, 7], 300), ref)
+    slot.set_option(X.OPT_GRAPH, 0)
+    assert np.array_equal(slot.moments_noalloc(480, [480, 7], 300), ref)
+    assert slot.get_option(X.OPT_PLAIN_KERNEL) == 2
