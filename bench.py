@@ -33,6 +33,7 @@ error against the FP64 path.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
                        [--parts plain,batched,uneven,dos,mfma,cpu]
                        [--skip-cpu-baseline] [--calibrate-copy] [--complex] [--no-palette]
+                       [--no-patterns]
                        [--no-graph]
 """
 from __future__ import annotations
@@ -571,6 +572,9 @@ def main():
                          "whose known bytes scale the rocprofv3 memory counters")
     ap.add_argument("--no-graph", action="store_true",
                     help="uneven part: time a direct run instead of the hipGraph replay")
+    ap.add_argument("--no-patterns", action="store_true",
+                    help="slot-major plain step with per-slot offsets and codes instead of "
+                         "row patterns")
     ap.add_argument("--no-palette", action="store_true",
                     help="plain step with full values instead of the value palette")
     ap.add_argument("--complex", action="store_true",
@@ -611,6 +615,8 @@ def main():
             eng.set_option(X.OPT_REAL, 0)
         if args.no_palette:
             eng.set_option(X.OPT_PALETTE, 0)
+        if args.no_patterns:
+            eng.set_option(X.OPT_ROW_PATTERNS, 0)
     model = None
     if "plain" in parts:
         log("building 1024x1024 spin lattice")
@@ -646,9 +652,14 @@ def main():
         # Slot-major ELL step (one thread per row, W slots per row, W = the
         # longest row, at least 4) or the row-block CSR step.
         slot = eng.get_option(X.OPT_PLAIN_KERNEL) == 2
+        # Row patterns (slot form with the palette): one byte per row indexes
+        # the model's distinct rows (column offsets and value codes).
+        patterns = eng.get_option(X.OPT_ROW_PATTERNS) == 1
         cb = 2 if col16 else 4
         e = 8 if real else 16
-        if slot:
+        if patterns:
+            step_bytes = N * (1 + 4 * e)
+        elif slot:
             W = max(4, int(np.diff(csr[0]).max()))
             step_bytes = W * N * (2 + (1 if palette else e)) + N * (3 * e + (e if palette else 0))
         elif palette:
@@ -688,9 +699,11 @@ def main():
                             ("plain_step_lds<double> (real FP64 CSR SpMV recurrence step; "
                              "H is real)" if real else
                              "plain_step_lds<double2> (complex FP64 CSR SpMV recurrence step)"))
-                           + ("; 16-bit column offsets" if col16 else "; 32-bit columns")
-                           + ("; 1-byte value codes (palette) + diagonal array"
-                              if palette else "")),
+                           + ("; one byte per row into the model's distinct rows (16-bit "
+                              "offsets, palette codes) + diagonal array" if patterns else
+                              (("; 16-bit column offsets" if col16 else "; 32-bit columns")
+                               + ("; 1-byte value codes (palette) + diagonal array"
+                                  if palette else "")))),
                 "bound": "hbm",
                 "achieved": achieved,
                 "peak": HBM_PEAK_GBPS,
@@ -699,7 +712,8 @@ def main():
                 "traffic": load_traffic(("plain_step_slot" if slot else "plain_step_lds")
                                         + ("_real" if real else "")
                                         + ("_col16" if col16 and not slot else "")
-                                        + ("_palette" if palette else "")),
+                                        + ("_palette" if palette else "")
+                                        + ("_patterns" if patterns else "")),
                 "algorithmic_bytes_per_launch": step_bytes,
                 "avg_launch_us": avg_kernel_s * 1e6,
             },

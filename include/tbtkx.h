@@ -324,7 +324,8 @@ int tbtkx_last_timing(tbtkx_handle handle, double *milliseconds,
  * loop as a hipGraph. TBTKX_OPT_BLOCK_WIDTH: start vectors per SpMM block
  * (64, 128, 256; 0 = chosen by the width autotuner, which times each width on
  * the resident model once per model, precision and size class).
- * TBTKX_OPT_XCD_REMAP: 1 = give each XCD a contiguous row slice. TBTKX_OPT_TILE_ROWS: rows per wavefront tile of the SpMM (1-32).
+ * TBTKX_OPT_XCD_REMAP: 1 = give each XCD a contiguous row slice (the
+ * batched steps; the slot-major plain step always does). TBTKX_OPT_TILE_ROWS: rows per wavefront tile of the SpMM (1-32).
  * TBTKX_OPT_NT_STREAM: 1 = non-temporal loads/stores for the SpMM rows that
  * are used once per step (default 1). */
 enum {
@@ -381,7 +382,18 @@ enum {
 	                           exact FP64 blocks, whichever moves fewer bytes
 	                           (default); 1 = plain recurrences; 2 = exact
 	                           blocks. Reading it gives what the last call
-	                           ran (1 plain, 2 blocks) */
+	                           ran (1 plain, 2 blocks) */,
+	TBTKX_OPT_ROW_PATTERNS = 18 /* 1 = the slot-major plain step with the
+	                           palette reads one byte per row, an index into
+	                           the model's at most 255 distinct rows (column
+	                           offsets and palette codes), when the model has
+	                           that few (default): same values, same order,
+	                           same moments; 0 = per-slot offsets and codes.
+	                           Reading it gives 1 when the last plain call
+	                           used them */,
+	TBTKX_OPT_SLOT_ROWS = 19 /* rows per thread of the slot-major plain step:
+	                           1, 2 or 4; 0 (default) = 2 with row patterns,
+	                           1 without */
 };
 int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
 

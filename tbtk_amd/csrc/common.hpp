@@ -138,6 +138,13 @@ struct tbtkx_context {
 	// (slot_pal) or H/s, 2H/s values (complex, and real for a real H).
 	tbtkx::DevBuf scol, scode, sv1, sv2, sv1r, sv2r;
 	int slot_ok = 0, slot_pal = 0, slot_vals = 0, slot_W = 0;
+	// Row patterns of the slot form (pat_ok): one byte per row (ppid) into
+	// pat_count patterns of 8 offsets (ptoff) and 8 palette codes (ptcode).
+	tbtkx::DevBuf ppid, ptoff, ptcode;
+	int pat_ok = 0, pat_count = 0;
+	int use_patterns = 1;
+	int slot_rows = 0;       // rows per thread of the slot step (1, 2, 4; 0 = by form)
+	int last_patterns = 0;   // the last plain call used the row patterns
 	int last_plain_kernel = 0;   // kernel of the last plain call (0, 1, 2)
 
 	// Plain path vectors and moment output.
@@ -252,7 +259,7 @@ int build_slot_values(tbtkx_context *h);
 // (TBTKX_OPT_PLAIN_KERNEL 2, needs 16-bit columns), 16-bit column offsets,
 // the value palette; and the algorithmic HBM bytes of one step in that form.
 struct PlainForm {
-	bool slot, c16, pal;
+	bool slot, c16, pal, pat;
 	double bytes;
 };
 inline PlainForm plain_form(const tbtkx_context *h) {
@@ -262,8 +269,12 @@ inline PlainForm plain_form(const tbtkx_context *h) {
 	         ((h->use_palette && h->slot_pal) || h->slot_vals);
 	f.c16 = f.slot || (h->plain_kernel == 0 && h->use_col16 && h->col16_ok);
 	f.pal = f.c16 && h->use_palette && (f.slot ? h->slot_pal : h->palette_ok);
+	f.pat = f.slot && f.pal && h->use_patterns && h->pat_ok;
 	const double N = (double)h->N, Z = (double)h->Z, e = real ? 8 : 16;
-	if (f.slot)
+	if (f.pat)
+		// per row: the pattern id, x, j read and written, the diagonal
+		f.bytes = N * (1 + 4 * e);
+	else if (f.slot)
 		// per slot: 2 B column + 1 B code or a value; per row: x, j read and
 		// written, and the diagonal with the palette
 		f.bytes = (double)h->slot_W * N * (2 + (f.pal ? 1 : e)) + N * (3 * e + (f.pal ? e : 0));

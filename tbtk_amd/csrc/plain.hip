@@ -147,41 +147,85 @@ __global__ __launch_bounds__(kThreads) void make_slots(
 // once, then its W gathers of x, then the products are added in slot order
 // from -j_{n-2} (padding slots skipped). Two dependent memory trips per
 // step and no LDS or barrier.
-template <typename S, bool FIRST, bool PAL, int W>
+// PAT (with PAL): the row's slots come from its row pattern instead, the
+// pattern id pid[r] indexing a table of 8 offsets (scol) and 8 codes (scode)
+// per pattern. R rows per thread (rows b*R*256 + j*256 + tid, j < R): all
+// their loads are issued before the first sum, R times the loads in flight.
+template <typename S, bool FIRST, bool PAL, int W, bool PAT = false, int R = 1>
 __global__ __launch_bounds__(kThreads) void plain_step_slot(
     const int16_t *__restrict__ scol, const uint8_t *__restrict__ scode,
     const S *__restrict__ sval, const S *__restrict__ pal, const S *__restrict__ diag,
     const S *__restrict__ x, S *y, int64_t N, const int64_t *__restrict__ to, int K,
-    double2 *__restrict__ coef, int M, int nprev) {
+    double2 *__restrict__ coef, int M, int nprev, const uint8_t *__restrict__ pid,
+    int remap) {
 #pragma clang fp contract(off)
 	typedef F64<S> A;
-	const int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-	if (r < K) coef[r * M + nprev] = A::moment(x[to[r]]);
-	if (r >= N) return;
-	int16_t c[W];
-	uint8_t q[W];
-	S v[W], xx[W];
+	for (int64_t g = (int64_t)blockIdx.x * kThreads + threadIdx.x; g < K;
+	     g += (int64_t)gridDim.x * kThreads)
+		coef[g * M + nprev] = A::moment(x[to[g]]);
+	// remap: each XCD takes one contiguous eighth of the rows, so the x
+	// entries of a row's far neighbours (a lattice stride away) are read
+	// through the same L2 as the row's own (measured on the 1024x1024 spin
+	// lattice: HBM reads equal to the algorithmic bytes, 1.34x without).
+	const int64_t b = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+	int64_t row[R];
 #pragma unroll
-	for (int s = 0; s < W; s++) {
-		c[s] = scol[(int64_t)s * N + r];
-		if (PAL) q[s] = scode[(int64_t)s * N + r];
-		else v[s] = sval[(int64_t)s * N + r];
+	for (int j = 0; j < R; j++) {
+		const int64_t rj = (b * R + j) * kThreads + threadIdx.x;
+		row[j] = rj < N ? rj : N - 1;   // rows past the end load row N-1, store nothing
 	}
-	const S yo = FIRST ? A::zero() : y[r];
-	const S dg = PAL ? diag[r] : A::zero();
+	if ((b * R) * kThreads + threadIdx.x >= N) return;
+	int16_t c[R][W];
+	uint8_t q[R][W];
+	S v[R][W], xx[R][W], yo[R], dg[R];
 #pragma unroll
-	for (int s = 0; s < W; s++) xx[s] = x[r + (c[s] == kSlotPad ? 0 : (int64_t)c[s])];
+	for (int j = 0; j < R; j++) {
+		const int64_t r = row[j];
+		if (PAT) {
+			const int p = pid[r];
+			const int4 o = *reinterpret_cast<const int4 *>(scol + p * 8);
+			const int2 e = *reinterpret_cast<const int2 *>(scode + p * 8);
+			int16_t oo[8];
+			uint8_t ee[8];
+			memcpy(oo, &o, 16);
+			memcpy(ee, &e, 8);
+#pragma unroll
+			for (int s = 0; s < W; s++) {
+				c[j][s] = oo[s];
+				q[j][s] = ee[s];
+			}
+		} else {
+#pragma unroll
+			for (int s = 0; s < W; s++) {
+				c[j][s] = scol[(int64_t)s * N + r];
+				if (PAL) q[j][s] = scode[(int64_t)s * N + r];
+				else v[j][s] = sval[(int64_t)s * N + r];
+			}
+		}
+		yo[j] = FIRST ? A::zero() : y[r];
+		dg[j] = PAL ? diag[r] : A::zero();
+	}
+#pragma unroll
+	for (int j = 0; j < R; j++)
+#pragma unroll
+		for (int s = 0; s < W; s++)
+			xx[j][s] = x[row[j] + (c[j][s] == kSlotPad ? 0 : (int64_t)c[j][s])];
 	if (PAL) {
 #pragma unroll
-		for (int s = 0; s < W; s++) v[s] = q[s] == 255 ? dg : pal[q[s]];
-	}
-	S acc = FIRST ? A::zero() : A::neg(yo);
+		for (int j = 0; j < R; j++)
 #pragma unroll
-	for (int s = 0; s < W; s++) {
-		const S p = A::mul(v[s], xx[s]);
-		if (c[s] != kSlotPad) acc = A::add(acc, p);
+			for (int s = 0; s < W; s++) v[j][s] = q[j][s] == 255 ? dg[j] : pal[q[j][s]];
 	}
-	y[r] = acc;
+#pragma unroll
+	for (int j = 0; j < R; j++) {
+		S acc = FIRST ? A::zero() : A::neg(yo[j]);
+#pragma unroll
+		for (int s = 0; s < W; s++) {
+			const S p = A::mul(v[j][s], xx[j][s]);
+			if (c[j][s] != kSlotPad) acc = A::add(acc, p);
+		}
+		if ((b * R + j) * kThreads + threadIdx.x < N) y[row[j]] = acc;
+	}
 }
 
 // Row-block kernel: the workgroup streams its block's non-zeros with
@@ -350,14 +394,31 @@ void launch_lds(tbtkx_context *h, bool first, const CI *col, const S *v, const S
 		    h->xcd_remap, to, K, coef, M, nprev, code, pal, diag);
 }
 
-template <typename S, bool FIRST, bool PAL>
+// Rows per thread of the slot step: the option, or by default 2 with row
+// patterns (one byte per row leaves the loads of one row too few to cover
+// the gathers' latency) and 1 with per-slot data (measured on the 1024x1024
+// spin lattice: 16.0 against 16.1 us with patterns, 16.4 against 16.7
+// without; 4 rows: 19.0 us).
+inline int slot_rows(const tbtkx_context *h, bool pat) {
+	return h->slot_rows > 0 ? h->slot_rows : pat ? 2 : 1;
+}
+
+template <typename S, bool FIRST, bool PAL, bool PAT>
 void launch_slot_w(tbtkx_context *h, int W, const int16_t *scol, const uint8_t *scode,
                    const S *sv, const S *pal, const S *diag, const S *x, S *y,
-                   const int64_t *to, int K, double2 *coef, int M, int nprev) {
-	const unsigned grid = grid_for(h->N);
+                   const int64_t *to, int K, double2 *coef, int M, int nprev,
+                   const uint8_t *pid) {
+	const int R = slot_rows(h, PAT);
+	const unsigned grid = (unsigned)((h->N + (int64_t)kThreads * R - 1) / ((int64_t)kThreads * R));
+#define TBTKX_SLOT_R(WW, RR)                                                            \
+	plain_step_slot<S, FIRST, PAL, WW, PAT, RR><<<grid, kThreads, 0, h->stream>>>(    \
+	    scol, scode, sv, pal, diag, x, y, h->N, to, K, coef, M, nprev, pid, 1)
 #define TBTKX_SLOT(WW)                                                                  \
-	plain_step_slot<S, FIRST, PAL, WW><<<grid, kThreads, 0, h->stream>>>(             \
-	    scol, scode, sv, pal, diag, x, y, h->N, to, K, coef, M, nprev)
+	do {                                                                            \
+		if (R == 4) TBTKX_SLOT_R(WW, 4);                                        \
+		else if (R == 2) TBTKX_SLOT_R(WW, 2);                                   \
+		else TBTKX_SLOT_R(WW, 1);                                               \
+	} while (0)
 	switch (W) {
 	case 4: TBTKX_SLOT(4); break;
 	case 5: TBTKX_SLOT(5); break;
@@ -366,20 +427,23 @@ void launch_slot_w(tbtkx_context *h, int W, const int16_t *scol, const uint8_t *
 	default: TBTKX_SLOT(8); break;
 	}
 #undef TBTKX_SLOT
+#undef TBTKX_SLOT_R
 }
 
 template <typename S>
-void launch_slot(tbtkx_context *h, bool first, bool pal, int W, const int16_t *scol,
+void launch_slot(tbtkx_context *h, bool first, bool pal, bool pat, int W, const int16_t *scol,
                  const uint8_t *scode, const S *sv, const S *pp, const S *dp, const S *x, S *y,
-                 const int64_t *to, int K, double2 *coef, int M, int nprev) {
-	if (first && pal)
-		launch_slot_w<S, true, true>(h, W, scol, scode, sv, pp, dp, x, y, to, K, coef, M, nprev);
-	else if (first)
-		launch_slot_w<S, true, false>(h, W, scol, scode, sv, pp, dp, x, y, to, K, coef, M, nprev);
-	else if (pal)
-		launch_slot_w<S, false, true>(h, W, scol, scode, sv, pp, dp, x, y, to, K, coef, M, nprev);
-	else
-		launch_slot_w<S, false, false>(h, W, scol, scode, sv, pp, dp, x, y, to, K, coef, M, nprev);
+                 const int64_t *to, int K, double2 *coef, int M, int nprev,
+                 const uint8_t *pid) {
+#define TBTKX_SLOTL(F, P, T)                                                            \
+	launch_slot_w<S, F, P, T>(h, W, scol, scode, sv, pp, dp, x, y, to, K, coef, M, nprev, pid)
+	if (pat && first) TBTKX_SLOTL(true, true, true);
+	else if (pat) TBTKX_SLOTL(false, true, true);
+	else if (first && pal) TBTKX_SLOTL(true, true, false);
+	else if (first) TBTKX_SLOTL(true, false, false);
+	else if (pal) TBTKX_SLOTL(false, true, false);
+	else TBTKX_SLOTL(false, false, false);
+#undef TBTKX_SLOTL
 }
 
 // Enqueue the step launches n = 1 .. M-1 and the final gather; S = double2
@@ -415,8 +479,14 @@ int enqueue_steps(tbtkx_context *h, int K, int M, S *a, S *b, double2 *coef,
 				          : (first ? h->sv1.as<S>() : h->sv2.as<S>());
 			const S *pp = pf.pal ? palp + (first ? 0 : 256) : nullptr;
 			const S *dp = pf.pal ? diagp + (first ? 0 : h->N) : nullptr;
-			launch_slot<S>(h, first, pf.pal, h->slot_W, h->scol.as<int16_t>(),
-			               h->scode.as<uint8_t>(), sv, pp, dp, x, y, to, K, coef, M, n - 1);
+			if (pf.pat)
+				launch_slot<S>(h, first, true, true, h->slot_W, h->ptoff.as<int16_t>(),
+				               h->ptcode.as<uint8_t>(), sv, pp, dp, x, y, to, K, coef, M, n - 1,
+				               h->ppid.as<uint8_t>());
+			else
+				launch_slot<S>(h, first, pf.pal, false, h->slot_W, h->scol.as<int16_t>(),
+				               h->scode.as<uint8_t>(), sv, pp, dp, x, y, to, K, coef, M, n - 1,
+				               nullptr);
 			continue;
 		}
 		if (lds) {
@@ -530,8 +600,80 @@ int build_col16(tbtkx_context *h) {
 	return TBTKX_OK;
 }
 
+// Row patterns: the slots of a row (column offsets from the row, palette
+// codes) as one of at most 255 distinct patterns of the model, so the step
+// reads one byte per row instead of W column offsets and W codes. A lattice
+// with uniform hoppings and an on-site term has a few dozen (its boundary
+// classes); rows whose off-diagonal values vary (bond disorder, a magnetic
+// flux) make more and keep the per-slot form.
+static int build_row_patterns(tbtkx_context *h, int64_t N, const uint32_t *rowptr,
+                              const uint32_t *col, const uint8_t *code) {
+	h->pat_ok = 0;
+	struct Pat {
+		int16_t off[8];
+		uint8_t code[8];
+		bool operator==(const Pat &o) const { return memcmp(this, &o, sizeof(Pat)) == 0; }
+	};
+	struct PatHash {
+		size_t operator()(const Pat &p) const {
+			uint64_t w[3];
+			memcpy(w, &p, sizeof(Pat));
+			uint64_t x = w[0] * 0x9E3779B97F4A7C15ull;
+			x = (x ^ (x >> 29)) * 0xBF58476D1CE4E5B9ull + w[1];
+			x = (x ^ (x >> 32)) * 0x94D049BB133111EBull + w[2];
+			return (size_t)(x ^ (x >> 31));
+		}
+	};
+	static_assert(sizeof(Pat) == 24, "pattern key layout");
+	std::unordered_map<Pat, int, PatHash> index;
+	std::vector<Pat> table;
+	std::vector<uint8_t> pid((size_t)N);
+	for (int64_t r = 0; r < N; r++) {
+		const uint32_t k0 = rowptr[r], len = rowptr[r + 1] - k0;
+		if (len > 8) return TBTKX_OK;
+		Pat p;
+		for (int s = 0; s < 8; s++) {
+			p.off[s] = kSlotPad;
+			p.code[s] = 0;
+		}
+		for (uint32_t s = 0; s < len; s++) {
+			const int64_t d = (int64_t)col[k0 + s] - r;
+			if (d < -32767 || d > 32767) return TBTKX_OK;
+			p.off[s] = (int16_t)d;
+			p.code[s] = code[k0 + s];
+		}
+		auto it = index.find(p);
+		if (it == index.end()) {
+			if (table.size() >= 255) return TBTKX_OK;   // too many patterns
+			it = index.emplace(p, (int)table.size()).first;
+			table.push_back(p);
+		}
+		pid[r] = (uint8_t)it->second;
+	}
+	const size_t P = table.size();
+	std::vector<int16_t> toff(P * 8);
+	std::vector<uint8_t> tcode(P * 8);
+	for (size_t i = 0; i < P; i++) {
+		memcpy(&toff[i * 8], table[i].off, 16);
+		memcpy(&tcode[i * 8], table[i].code, 8);
+	}
+	int rc;
+	if ((rc = h->ppid.reserve((size_t)N))) return rc;
+	if ((rc = h->ptoff.reserve(P * 16 > 0 ? P * 16 : 16))) return rc;
+	if ((rc = h->ptcode.reserve(P * 8 > 0 ? P * 8 : 8))) return rc;
+	TBTKX_CHECK_HIP(hipMemcpy(h->ppid.ptr, pid.data(), (size_t)N, hipMemcpyHostToDevice));
+	if (P) {
+		TBTKX_CHECK_HIP(hipMemcpy(h->ptoff.ptr, toff.data(), P * 16, hipMemcpyHostToDevice));
+		TBTKX_CHECK_HIP(hipMemcpy(h->ptcode.ptr, tcode.data(), P * 8, hipMemcpyHostToDevice));
+	}
+	h->pat_ok = 1;
+	h->pat_count = (int)P;
+	return TBTKX_OK;
+}
+
 int build_palette(tbtkx_context *h, int64_t N, int64_t Z, const uint32_t *rowptr,
                   const uint32_t *col, const double *v1, const double *v2) {
+	h->pat_ok = 0;
 	h->palette_ok = 0;
 	struct Key {
 		uint64_t w[4];
@@ -600,7 +742,7 @@ int build_palette(tbtkx_context *h, int64_t N, int64_t Z, const uint32_t *rowptr
 	TBTKX_CHECK_HIP(hipMemcpy(h->pdiagr.ptr, diagr.data(), diagr.size() * 8,
 	                          hipMemcpyHostToDevice));
 	h->palette_ok = 1;
-	return TBTKX_OK;
+	return build_row_patterns(h, N, rowptr, col, code.data());
 }
 
 // Scratch of tbtkx_moments_noalloc: the two recurrence vectors, the moment
@@ -711,6 +853,7 @@ extern "C" int tbtkx_moments_noalloc(tbtkx_handle h, int64_t from, int K,
 	h->last_col16 = c16;
 	h->last_palette = usepal;
 	h->last_plain_kernel = pform.slot ? 2 : h->plain_kernel == 1 ? 1 : 0;
+	h->last_patterns = pform.pat;
 	// A repeated to-index keeps only its last position (as tbtkx_moments).
 	for (int k = 0; k < K; k++)
 		for (int l = k + 1; l < K; l++)
@@ -787,11 +930,14 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 		GraphCache &gc = h->plain_graph;
 		int64_t key[GraphCache::kKey];
 		const bool key_ok = GraphCache::make_key(
-		    key, {U, M, h->plain_kernel * 2 + pform.slot, h->xcd_remap, h->N, real, c16, usepal},
+		    key, {U, M, h->plain_kernel * 4 + pform.slot * 2 + pform.pat + 16 * h->slot_rows,
+		          h->xcd_remap, h->N,
+		          real, c16, usepal},
 		    {h->vec_a.ptr, h->vec_b.ptr, h->coef.ptr, h->to.ptr, h->rowptr.ptr, h->col.ptr,
 		     h->v1.ptr, h->v2.ptr, h->v1r.ptr, h->v2r.ptr, h->rblk.ptr, h->col16.ptr,
 		     h->pcode.ptr, h->pal.ptr, h->palr.ptr, h->pdiag.ptr, h->pdiagr.ptr, h->scol.ptr,
-		     h->scode.ptr, h->sv1.ptr, h->sv2.ptr, h->sv1r.ptr, h->sv2r.ptr});
+		     h->scode.ptr, h->sv1.ptr, h->sv2.ptr, h->sv1r.ptr, h->sv2r.ptr, h->ppid.ptr,
+		     h->ptoff.ptr, h->ptcode.ptr});
 		TBTKX_REQUIRE(key_ok, TBTKX_ERR_STATE, "tbtkx_moments: graph key too long");
 		if (!gc.matches(key)) {
 			gc.reset();
@@ -823,6 +969,7 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 	h->last_col16 = c16;
 	h->last_palette = usepal;
 	h->last_plain_kernel = pform.slot ? 2 : h->plain_kernel == 1 ? 1 : 0;
+	h->last_patterns = pform.pat;
 
 	for (int k = 0; k < K; k++) {
 		double *o = out + (size_t)k * M * 2;

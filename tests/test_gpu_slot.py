@@ -22,7 +22,7 @@ def slot(engine):
     engine.set_option(X.OPT_PLAIN_KERNEL, 2)
     yield engine
     for opt, val in ((X.OPT_PLAIN_KERNEL, 2), (X.OPT_COL16, 1), (X.OPT_PALETTE, 1),
-                     (X.OPT_GRAPH, 0), (X.OPT_REAL, 1)):
+                     (X.OPT_GRAPH, 0), (X.OPT_REAL, 1), (X.OPT_ROW_PATTERNS, 1)):
         engine.set_option(opt, val)
 
 
@@ -50,15 +50,39 @@ def test_slot_bitwise_vs_oracle_and_row_blocks(slot, oracle_mod, name, real):
     M = 300
     ref = oracle_mod.moments(csr, s, frm, to, M)
     print(f"{name}: N={m.basis_size} real={real}", flush=True)
-    for pal in (1, 0):
+    for pal, pat in ((1, 1), (1, 0), (0, 1)):
         slot.set_option(X.OPT_PALETTE, pal)
+        slot.set_option(X.OPT_ROW_PATTERNS, pat)
         got = slot.moments(frm, to, M)
         assert slot.get_option(X.OPT_PLAIN_KERNEL) == 2
         assert slot.get_option(X.OPT_PALETTE) == (pal and palette)
-        assert np.array_equal(got, ref), (name, pal)
+        assert slot.get_option(X.OPT_ROW_PATTERNS) == (pal and pat and palette)
+        assert np.array_equal(got, ref), (name, pal, pat)
     slot.set_option(X.OPT_PLAIN_KERNEL, 0)
     assert np.array_equal(slot.moments(frm, to, M), ref)
     assert slot.get_option(X.OPT_PLAIN_KERNEL) == 0
+
+
+def test_row_patterns_many_rows_fall_back(slot, oracle_mod):
+    """A chain with 40 distinct hoppings in a random sequence over its 699
+    bonds keeps the palette (40 values) but has more than 255 distinct rows
+    (pairs of neighbouring hoppings), so it runs on the per-slot codes."""
+    n = 700
+    rng = np.random.default_rng(5)
+    hop = rng.choice(np.linspace(-1.2, -0.8, 40), size=n - 1)
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        if i > 0:
+            rows.append(i); cols.append(i - 1); vals.append(hop[i - 1])
+        rows.append(i); cols.append(i); vals.append(0.1 * (i % 3))
+        if i < n - 1:
+            rows.append(i); cols.append(i + 1); vals.append(hop[i])
+    csr = L.coo_to_csr(np.array(rows), np.array(cols), np.array(vals, dtype=np.complex128), n)
+    slot.set_csr(csr, 3.0)
+    got = slot.moments(350, [350, 0, 699], 100)
+    assert slot.get_option(X.OPT_PALETTE) == 1
+    assert slot.get_option(X.OPT_ROW_PATTERNS) == 0
+    assert np.array_equal(got, oracle_mod.moments(csr, 3.0, 350, [350, 0, 699], 100))
 
 
 def test_slot_full_offset_range(slot, oracle_mod):
@@ -73,6 +97,7 @@ def test_slot_full_offset_range(slot, oracle_mod):
     got = slot.moments(centre, to, M)
     assert slot.get_option(X.OPT_PLAIN_KERNEL) == 2
     assert slot.get_option(X.OPT_PALETTE) == 1
+    assert slot.get_option(X.OPT_ROW_PATTERNS) == 1
     assert np.array_equal(got, oracle_mod.moments(csr, s, centre, to, M))
 
 
@@ -87,6 +112,7 @@ def test_slot_full_size(slot, oracle_mod):
     slot.set_csr(csr, s)
     got = slot.moments(frm, to, M)
     assert slot.get_option(X.OPT_PLAIN_KERNEL) == 2
+    assert slot.get_option(X.OPT_ROW_PATTERNS) == 1
     assert np.array_equal(got, oracle_mod.moments(csr, s, frm, to, M))
 
 
