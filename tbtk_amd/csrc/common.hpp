@@ -192,7 +192,11 @@ struct tbtkx_context {
 	tbtkx::DevBuf mm_desc, mm_tab, mm_x, mm_y, mm_part, mm_seg, mm_D, mm_cols;
 	int64_t mm_tune_key[3] = {-1, -1, -1};    // (generation, flags, widths hash)
 	int mm_tune_g = 0, mm_tune_occ = 0;       // tuned tile groups, grid share
-	tbtkx::GraphCache mm_graph;
+	std::vector<tbtkx::GraphCache> mm_graphs;   // the run in bounded chunks
+	void reset_mm_graphs() {
+		for (tbtkx::GraphCache &g : mm_graphs) g.reset();
+		mm_graphs.clear();
+	}
 
 	// Options.
 	// Damping of the contraction's coefficients (TBTKX_DAMP_*).

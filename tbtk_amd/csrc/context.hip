@@ -191,7 +191,7 @@ int tbtkx_destroy(tbtkx_handle h) {
 	(void)hipSetDevice(h->device);
 	if (h->stream) (void)hipStreamSynchronize(h->stream);
 	h->plain_graph.reset();
-	h->mm_graph.reset();
+	h->reset_mm_graphs();
 	h->blk_graph.reset();
 	for (DevBuf *b : {&h->rowptr, &h->col, &h->v1, &h->v2, &h->v1f, &h->v2f,
 	                  &h->v1r, &h->v2r, &h->lat_terms, &h->lat_onsite, &h->lat_count,

@@ -532,7 +532,7 @@ int tbtkx_multi_set_lattices(tbtkx_handle h, int K, const tbtkx_lattice *lats,
 	              "tbtkx_multi_set_lattices: rows with more than 8 non-zeros (7 without a "
 	              "diagonal entry) are not supported");
 	const int W = ell_slots_multi(maxlen);
-	h->mm_graph.reset();
+	h->reset_mm_graphs();
 	h->mm_K = 0;
 	if ((rc = h->mm_ecol.reserve((size_t)entries * 4))) return rc;
 	if ((rc = h->mm_ev1.reserve((size_t)entries * 8))) return rc;

@@ -826,16 +826,25 @@ int reserve_plan(tbtkx_context *h, const Plan &p, int ncols, int64_t mu_elems) {
 }
 
 // Everything from the initial vectors to the last step, on h->stream.
-int enqueue_run(tbtkx_context *h, const Plan &p, bool dots, int ncols) {
-	float2 *X = h->mm_x.as<float2>(), *Y = h->mm_y.as<float2>();
+// Enqueue steps s_lo .. s_hi - 1 of the run (all phases' steps counted in
+// order; the vectors' roles alternate with the step's parity), with the
+// start vectors before step 0 and, without the doubling dots, the final
+// gather after the last step.
+int enqueue_run(tbtkx_context *h, const Plan &p, bool dots, int ncols, int64_t s_lo,
+                int64_t s_hi) {
 	const GCol *cols = h->mm_cols.as<GCol>();
 	double2 *D = h->mm_D.as<double2>();
 	const int *tab = h->mm_tab.as<int>();
-	multi_fill<<<2048, kThreads, 0, h->stream>>>(X, p.vec_elems);
-	multi_units<<<grid_for(ncols), kThreads, 0, h->stream>>>(X, cols, ncols);
-	int64_t launches = 0;
+	float2 *X = h->mm_x.as<float2>(), *Y = h->mm_y.as<float2>();
+	if (s_lo & 1) std::swap(X, Y);
+	if (s_lo == 0) {
+		multi_fill<<<2048, kThreads, 0, h->stream>>>(X, p.vec_elems);
+		multi_units<<<grid_for(ncols), kThreads, 0, h->stream>>>(X, cols, ncols);
+	}
+	int64_t launches = 0, step = 0;
 	for (const Phase &ph : p.phases) {
-		for (int n = ph.n_lo; n < ph.n_hi; n++) {
+		for (int n = ph.n_lo; n < ph.n_hi; n++, step++) {
+			if (step < s_lo || step >= s_hi) continue;
 			if (!dots)
 				multi_gather<<<grid_for(ncols), kThreads, 0, h->stream>>>(
 				    cols, ncols, X, D, n);
@@ -855,12 +864,23 @@ int enqueue_run(tbtkx_context *h, const Plan &p, bool dots, int ncols) {
 			Y = tmp;
 		}
 	}
-	if (!dots)
+	if (!dots && s_hi >= step)
 		multi_gather<<<grid_for(ncols), kThreads, 0, h->stream>>>(cols, ncols, X, D,
 		                                                          p.Smax);
 	TBTKX_CHECK_HIP(hipGetLastError());
 	h->last_launches = launches;
 	return TBTKX_OK;
+}
+
+// Steps per captured graph of the run (3 kernel nodes per step with the
+// doubling dots).
+constexpr int64_t kGraphSteps = 256;
+
+// Steps of the run (over all phases).
+int64_t run_steps(const Plan &p) {
+	int64_t n = 0;
+	for (const Phase &ph : p.phases) n += ph.n_hi - ph.n_lo;
+	return n;
 }
 
 // Autotuner: time a few steps of the first phase for each (tile groups, grid
@@ -1016,7 +1036,7 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 		rp0 += N + 1;
 		z0 += Z;
 	}
-	h->mm_graph.reset();
+	h->reset_mm_graphs();
 	h->mm_K = 0;
 	if ((rc = h->mm_ecol.reserve(ec.size() * 4))) return rc;
 	if ((rc = h->mm_ev1.reserve(e1.size() * 4))) return rc;
@@ -1134,31 +1154,45 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 	                               hipMemcpyHostToDevice, h->stream));
 
 	TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
+	const int64_t nsteps = run_steps(p);
 	if (h->use_graph) {
-		GraphCache &gc = h->mm_graph;
+		// The run is captured as graphs of at most kGraphSteps steps each (a
+		// few hundred kernel nodes per graph rather than one graph of three
+		// launches per step), all cached under one key and replayed in order.
+		const int64_t nchunk = nsteps > 0 ? (nsteps + kGraphSteps - 1) / kGraphSteps : 1;
 		int64_t mh = hw;
 		for (int m = 0; m < K; m++) mh = (mh ^ num_moments[m]) * 1099511628211ll;
 		int64_t gkey[GraphCache::kKey];
 		const bool key_ok = GraphCache::make_key(
-		    gkey, {h->mm_gen, flags, mh, G * 16 + share, p.nwg, packed},
+		    gkey, {h->mm_gen, flags, mh, G * 16 + share, p.nwg, packed, nchunk},
 		    {h->mm_desc.ptr, h->mm_tab.ptr, h->mm_x.ptr, h->mm_y.ptr, h->mm_part.ptr,
 		     h->mm_seg.ptr, h->mm_D.ptr, h->mm_cols.ptr, h->mm_ecol.ptr, h->mm_ev1.ptr,
 		     h->mm_ev2.ptr});
 		TBTKX_REQUIRE(key_ok, TBTKX_ERR_STATE, "tbtkx_multi_moments: graph key too long");
-		if (!gc.matches(gkey)) {
-			gc.reset();
-			TBTKX_CHECK_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-			int erc = enqueue_run(h, p, dots, ncols);
-			hipError_t ec = hipStreamEndCapture(h->stream, &gc.graph);
-			if (erc) return erc;
-			TBTKX_CHECK_HIP(ec);
-			TBTKX_CHECK_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
-			memcpy(gc.key, gkey, sizeof gkey);
+		bool cached = (int64_t)h->mm_graphs.size() == nchunk;
+		for (int64_t c = 0; cached && c < nchunk; c++)
+			cached = h->mm_graphs[c].matches(gkey);
+		if (!cached) {
+			for (GraphCache &g : h->mm_graphs) g.reset();
+			h->mm_graphs.clear();
+			h->mm_graphs.resize(nchunk);
+			for (int64_t c = 0; c < nchunk; c++) {
+				GraphCache &gc = h->mm_graphs[c];
+				TBTKX_CHECK_HIP(hipStreamBeginCapture(h->stream,
+				                                      hipStreamCaptureModeThreadLocal));
+				int erc = enqueue_run(h, p, dots, ncols, c * kGraphSteps,
+				                      std::min(nsteps, (c + 1) * kGraphSteps));
+				hipError_t ec = hipStreamEndCapture(h->stream, &gc.graph);
+				if (erc) return erc;
+				TBTKX_CHECK_HIP(ec);
+				TBTKX_CHECK_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
+				memcpy(gc.key, gkey, sizeof gkey);
+			}
 		}
-		TBTKX_CHECK_HIP(hipGraphLaunch(gc.exec, h->stream));
+		for (GraphCache &gc : h->mm_graphs) TBTKX_CHECK_HIP(hipGraphLaunch(gc.exec, h->stream));
 		h->last_launches = Smax;
 	} else {
-		if ((rc = enqueue_run(h, p, dots, ncols))) return rc;
+		if ((rc = enqueue_run(h, p, dots, ncols, 0, nsteps))) return rc;
 	}
 	TBTKX_CHECK_HIP(hipEventRecord(h->ev_stop, h->stream));
 	std::vector<double2> buf(mu_elems > 0 ? mu_elems : 1);

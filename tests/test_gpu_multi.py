@@ -203,3 +203,32 @@ def test_multi_rejects_vectors_past_4gib(engine):
         engine.multi_moments([sites], [4], X.DOUBLING)
     mu = engine.multi_moments([sites[:32]], [4], X.DOUBLING)
     assert mu[0].shape == (32, 4) and np.all(mu[0][:, 0] == 1.0)
+
+
+@pytest.mark.parametrize("flags", [0, X.DOUBLING])
+def test_multi_graph_chunks_agree_with_direct(engine, flags):
+    """A run of more steps than one captured graph holds (1099 gather steps,
+    550 with the doubling identities: 5 and 3 graphs of at most 256 steps,
+    models finishing inside and across chunk boundaries) replays to the
+    direct launches' moments, three times in a row."""
+    rng = np.random.default_rng(23)
+    specs = [(L.square(8, 9, disorder=0.5, seed=1), 3, 1100),
+             (L.honeycomb_ribbon(5, 6, disorder=0.5, seed=2), 2, 513),
+             (L.square(12, 7, disorder=0.5, seed=3), 5, 257),
+             (L.square(4, 4, disorder=0.5, seed=4), 1, 40)]
+    batch = [(m, np.sort(rng.choice(m.basis_size, B, replace=False)), M) for m, B, M in specs]
+    upload(engine, batch)
+    sites = [b[1] for b in batch]
+    Ms = [b[2] for b in batch]
+    direct = engine.multi_moments(sites, Ms, flags)
+    engine.set_option(X.OPT_GRAPH, 1)
+    try:
+        for _ in range(3):
+            got = engine.multi_moments(sites, Ms, flags)
+            for a, b in zip(got, direct):
+                if flags == 0:
+                    assert np.array_equal(a, b)
+                else:
+                    assert np.abs(a - b).max() <= 1e-12
+    finally:
+        engine.set_option(X.OPT_GRAPH, 0)
