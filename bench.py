@@ -336,15 +336,21 @@ def batched_comparison(eng):
     }
 
 
-def uneven_batch(eng, graph=True):
+def uneven_batch(eng, graph=True, moments_div=1, tile_rows=0):
     """64 independent models of uneven size (honeycomb ribbons and square
     lattices, 20k-1M states, 1-64 sites and 500-8000 moments each, seed 4) in
     one persistent launch per recurrence step: FP32 storage, FP64 moments,
-    doubling, autotuned launch shape, replayed as one hipGraph. The models
+    doubling, autotuned launch shape, replayed as hipGraphs. The models
     are built on the device from their lattice descriptions
-    (tbtkx_multi_set_lattices)."""
+    (tbtkx_multi_set_lattices). For counter runs, moments_div > 1 divides
+    every model's moment count (at least 50: the same phases, a tenth of the
+    steps at 10) and tile_rows fixes the tile (row groups per wavefront
+    tile) instead of the autotuned shape."""
     t0 = time.perf_counter()
     batch = L.config_uneven_specs(seed=4)
+    if moments_div > 1:
+        batch = [(sp, st, max(50, M // moments_div)) for sp, st, M in batch]
+    eng.set_option(X.OPT_TILE_ROWS, tile_rows)
     specs = [b[0] for b in batch]
     scales = [eng.lattice_gershgorin(sp) * 1.05 for sp in specs]
     eng.multi_set_lattices(specs, scales)
@@ -364,6 +370,8 @@ def uneven_batch(eng, graph=True):
     mu = eng.multi_moments(sites, Ms, X.DOUBLING)
     t_first = time.perf_counter() - t0
     ms_direct, launches = eng.last_timing()
+    shape = {"tile_row_groups": eng.get_option(X.OPT_TILE_ROWS),
+             "grid_share": eng.get_option(X.OPT_GRID_SHARE)}
     # complex columns per model: two real vectors per column for a real batch
     packed = eng.get_option(X.OPT_REAL) == 1
     cols = (B + 1) // 2 if packed else B
@@ -395,8 +403,12 @@ def uneven_batch(eng, graph=True):
         eng.set_damping(X.DAMP_NONE)
         errs.append(float(np.abs(l_got - l_ref).max() / np.abs(l_ref).max()))
     eng.lut_destroy()
+    eng.set_option(X.OPT_TILE_ROWS, 0)
     dev_s = ms_graph * 1e-3
-    return {
+    extra = {"launch_shape": shape}
+    if moments_div > 1:
+        extra["moments_divided_by"] = moments_div
+    return {**extra,
         "workload": ("64 models, seed 4: honeycomb ribbons (t=2.7) and square lattices (t=1), "
                      "width 20-400, length 50-2000 (<= 1e6 states), disorder U[-0.5,0.5], "
                      "1-64 LDOS sites and 500-8000 moments per model, Gershgorin*1.05 scales"),
@@ -572,6 +584,12 @@ def main():
                          "whose known bytes scale the rocprofv3 memory counters")
     ap.add_argument("--no-graph", action="store_true",
                     help="uneven part: time a direct run instead of the hipGraph replay")
+    ap.add_argument("--uneven-moments-div", type=int, default=1,
+                    help="uneven part, for counter runs: divide every model's moment count "
+                         "by this (at least 50 moments)")
+    ap.add_argument("--uneven-tile-rows", type=int, default=0,
+                    help="uneven part: fixed row groups per tile instead of the autotuned "
+                         "launch shape")
     ap.add_argument("--no-patterns", action="store_true",
                     help="slot-major plain step with per-slot offsets and codes instead of "
                          "row patterns")
@@ -726,7 +744,9 @@ def main():
         out["batched"] = batched_comparison(eng)
     if "uneven" in parts:
         log("uneven batch (64 models, one persistent launch per step)")
-        out["uneven_batch"] = uneven_batch(eng, graph=not args.no_graph)
+        out["uneven_batch"] = uneven_batch(eng, graph=not args.no_graph,
+                                           moments_div=args.uneven_moments_div,
+                                           tile_rows=args.uneven_tile_rows)
     if "dos" in parts:
         log("stochastic DOS (128^3 cubic, R=512, M=4096)")
         out["stochastic_dos"] = stochastic_dos(eng)

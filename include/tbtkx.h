@@ -399,7 +399,9 @@ int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
 
 /* Effective value of an option: TBTKX_OPT_BLOCK_WIDTH, TBTKX_OPT_STRIP_ROWS
  * and TBTKX_OPT_GRID_SHARE give what the last batched call used (the
- * autotuner's choice when they are 0); TBTKX_OPT_ELL gives 1 only when the
+ * autotuner's choice when they are 0; the uneven batch sets the grid share
+ * too); TBTKX_OPT_TILE_ROWS, when 0, gives the row groups per tile of the
+ * last uneven-batch call; TBTKX_OPT_ELL gives 1 only when the
  * ELL copy exists and is enabled; TBTKX_OPT_REAL gives 1 when the last
  * recurrence call (plain, batched or uneven batch) ran in real arithmetic. */
 int tbtkx_get_option(tbtkx_handle handle, int option, int64_t *value);

@@ -175,6 +175,7 @@ struct tbtkx_context {
 	int run_strip = -1, run_share = 1, run_rmul = 1;   // shape of the current call
 	int last_block_width = 0;                     // shape of the last batched call
 	int last_strip = -1, last_share = 1, last_rmul = 1;
+	int last_tile_groups = 0;   // row groups per tile of the last uneven-batch call
 	tbtkx::GraphCache blk_graph;
 
 	// Uneven multi-model batch (tbtkx_multi_*): every model's H as row-major

@@ -480,7 +480,9 @@ int tbtkx_get_option(tbtkx_handle h, int option, int64_t *value) {
 		*value = h->block_width ? h->block_width : h->last_block_width;
 		return TBTKX_OK;
 	case TBTKX_OPT_XCD_REMAP: *value = h->xcd_remap; return TBTKX_OK;
-	case TBTKX_OPT_TILE_ROWS: *value = h->tile_rows; return TBTKX_OK;
+	case TBTKX_OPT_TILE_ROWS:
+		*value = h->tile_rows ? h->tile_rows : h->last_tile_groups;
+		return TBTKX_OK;
 	case TBTKX_OPT_NT_STREAM: *value = h->nt_stream; return TBTKX_OK;
 	case TBTKX_OPT_ELL: *value = h->use_ell && h->ell_W > 0; return TBTKX_OK;
 	case TBTKX_OPT_AUTOTUNE: *value = h->autotune; return TBTKX_OK;

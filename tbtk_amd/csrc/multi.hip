@@ -1112,6 +1112,8 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 		G = h->mm_tune_g;
 		share = h->mm_tune_occ;
 	}
+	h->last_share = share;
+	h->last_tile_groups = G;
 	Plan p;
 	if (Smax > 0) {
 		if ((rc = make_plan(h, width, S, dots, G, share, p))) return rc;

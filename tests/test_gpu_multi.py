@@ -232,3 +232,20 @@ def test_multi_graph_chunks_agree_with_direct(engine, flags):
                     assert np.abs(a - b).max() <= 1e-12
     finally:
         engine.set_option(X.OPT_GRAPH, 0)
+
+
+def test_multi_reports_launch_shape(engine):
+    """With TBTKX_OPT_TILE_ROWS at 0 the option reads back the autotuned row
+    groups per tile of the last call, and the grid share it ran with."""
+    batch = small_batch()
+    upload(engine, batch)
+    engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], X.DOUBLING)
+    assert engine.get_option(X.OPT_TILE_ROWS) in (2, 4, 8, 16)
+    assert engine.get_option(X.OPT_GRID_SHARE) in (1, 2)
+    try:
+        engine.set_option(X.OPT_TILE_ROWS, 6)
+        engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], X.DOUBLING)
+        assert engine.get_option(X.OPT_TILE_ROWS) == 6
+        assert engine.get_option(X.OPT_GRID_SHARE) == 1
+    finally:
+        engine.set_option(X.OPT_TILE_ROWS, 0)
