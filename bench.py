@@ -33,8 +33,8 @@ error against the FP64 path.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
                        [--parts plain,batched,uneven,dos,mfma,cpu]
                        [--skip-cpu-baseline] [--calibrate-copy] [--complex] [--no-palette]
-                       [--no-patterns]
-                       [--no-graph]
+                       [--no-patterns | --row-pattern-bytes] [--slot-rows R]
+                       [--no-graph] [--uneven-moments-div D] [--uneven-tile-rows G]
 """
 from __future__ import annotations
 
@@ -593,6 +593,11 @@ def main():
     ap.add_argument("--no-patterns", action="store_true",
                     help="slot-major plain step with per-slot offsets and codes instead of "
                          "row patterns")
+    ap.add_argument("--row-pattern-bytes", action="store_true",
+                    help="row patterns read as one byte per row everywhere, without the "
+                         "wave-uniform scalar path (TBTKX_OPT_ROW_PATTERNS = 2)")
+    ap.add_argument("--slot-rows", type=int, default=0,
+                    help="rows per thread of the slot-major plain step (1, 2, 4; 0 = default)")
     ap.add_argument("--no-palette", action="store_true",
                     help="plain step with full values instead of the value palette")
     ap.add_argument("--complex", action="store_true",
@@ -635,6 +640,10 @@ def main():
             eng.set_option(X.OPT_PALETTE, 0)
         if args.no_patterns:
             eng.set_option(X.OPT_ROW_PATTERNS, 0)
+        elif args.row_pattern_bytes:
+            eng.set_option(X.OPT_ROW_PATTERNS, 2)
+        if args.slot_rows:
+            eng.set_option(X.OPT_SLOT_ROWS, args.slot_rows)
     model = None
     if "plain" in parts:
         log("building 1024x1024 spin lattice")
@@ -673,10 +682,15 @@ def main():
         # Row patterns (slot form with the palette): one byte per row indexes
         # the model's distinct rows (column offsets and value codes).
         patterns = eng.get_option(X.OPT_ROW_PATTERNS) == 1
+        # Wave-uniform patterns (the default): a wavefront whose 64-row
+        # chunks share one pattern reads it from a byte per chunk, so only
+        # the rows outside such chunks read their pattern byte.
+        uniform = patterns and not args.row_pattern_bytes
+        byte_rows = eng.get_option(X.OPT_PATTERN_BYTE_ROWS) if uniform else N
         cb = 2 if col16 else 4
         e = 8 if real else 16
         if patterns:
-            step_bytes = N * (1 + 4 * e)
+            step_bytes = 4 * e * N + byte_rows + (N // 64 if uniform else 0)
         elif slot:
             W = max(4, int(np.diff(csr[0]).max()))
             step_bytes = W * N * (2 + (1 if palette else e)) + N * (3 * e + (e if palette else 0))
@@ -717,8 +731,12 @@ def main():
                             ("plain_step_lds<double> (real FP64 CSR SpMV recurrence step; "
                              "H is real)" if real else
                              "plain_step_lds<double2> (complex FP64 CSR SpMV recurrence step)"))
-                           + ("; one byte per row into the model's distinct rows (16-bit "
-                              "offsets, palette codes) + diagonal array" if patterns else
+                           + (("; row patterns (the model's distinct rows: 16-bit offsets, "
+                               "palette codes): wavefronts whose 64-row chunks share one read "
+                               "it through scalar loads, the other rows one byte per row; "
+                               "diagonal array" if uniform else
+                               "; one byte per row into the model's distinct rows (16-bit "
+                               "offsets, palette codes) + diagonal array") if patterns else
                               (("; 16-bit column offsets" if col16 else "; 32-bit columns")
                                + ("; 1-byte value codes (palette) + diagonal array"
                                   if palette else "")))),
@@ -731,8 +749,10 @@ def main():
                                         + ("_real" if real else "")
                                         + ("_col16" if col16 and not slot else "")
                                         + ("_palette" if palette else "")
-                                        + ("_patterns" if patterns else "")),
+                                        + ("_patterns" if patterns else "")
+                                        + ("_uniform" if uniform else "")),
                 "algorithmic_bytes_per_launch": step_bytes,
+                **({"pattern_byte_rows": byte_rows} if patterns else {}),
                 "avg_launch_us": avg_kernel_s * 1e6,
             },
             "checks": {"ldos_integral": float(ldos.sum() * dE),

@@ -387,13 +387,22 @@ enum {
 	                           palette reads one byte per row, an index into
 	                           the model's at most 255 distinct rows (column
 	                           offsets and palette codes), when the model has
-	                           that few (default): same values, same order,
-	                           same moments; 0 = per-slot offsets and codes.
-	                           Reading it gives 1 when the last plain call
-	                           used them */,
+	                           that few (default), and a wavefront whose rows
+	                           all share one pattern reads none: it takes the
+	                           pattern from a byte per 64-row chunk through
+	                           scalar loads; 2 = pattern bytes for every row;
+	                           all: same values, same order, same moments;
+	                           0 = per-slot offsets and codes. Reading it
+	                           gives 1 when the last plain call used patterns */,
 	TBTKX_OPT_SLOT_ROWS = 19 /* rows per thread of the slot-major plain step:
-	                           1, 2 or 4; 0 (default) = 2 with row patterns,
-	                           1 without */
+	                           1, 2 or 4; 0 (default) = 2 when every row reads
+	                           its pattern byte (TBTKX_OPT_ROW_PATTERNS = 2),
+	                           else 1 */,
+	TBTKX_OPT_PATTERN_BYTE_ROWS = 20 /* read-only: rows of the current model
+	                           outside 64-row chunks of one shared pattern
+	                           (the rows whose pattern byte the step reads
+	                           with TBTKX_OPT_ROW_PATTERNS = 1); -1 when the
+	                           model has no row patterns */
 };
 int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
 

@@ -141,7 +141,9 @@ struct tbtkx_context {
 	// Row patterns of the slot form (pat_ok): one byte per row (ppid) into
 	// pat_count patterns of 8 offsets (ptoff) and 8 palette codes (ptcode).
 	tbtkx::DevBuf ppid, ptoff, ptcode;
+	tbtkx::DevBuf pchunk;   // per 64-row chunk: its rows' common pattern, or 255
 	int pat_ok = 0, pat_count = 0;
+	int64_t pat_byte_rows = 0;   // rows outside uniform 64-row chunks
 	int use_patterns = 1;
 	int slot_rows = 0;       // rows per thread of the slot step (1, 2, 4; 0 = by form)
 	int last_patterns = 0;   // the last plain call used the row patterns

@@ -196,7 +196,7 @@ int tbtkx_destroy(tbtkx_handle h) {
 	for (DevBuf *b : {&h->rowptr, &h->col, &h->v1, &h->v2, &h->v1f, &h->v2f,
 	                  &h->v1r, &h->v2r, &h->lat_terms, &h->lat_onsite, &h->lat_count,
 	                  &h->lat_part, &h->lat_tmp, &h->rblk, &h->col16, &h->col16_flag,
-	                  &h->pcode, &h->pal, &h->palr, &h->pdiag, &h->pdiagr, &h->scol, &h->scode, &h->sv1, &h->sv2, &h->sv1r, &h->sv2r, &h->ppid, &h->ptoff, &h->ptcode, &h->vec_a, &h->vec_b, &h->coef, &h->to,
+	                  &h->pcode, &h->pal, &h->palr, &h->pdiag, &h->pdiagr, &h->scol, &h->scode, &h->sv1, &h->sv2, &h->sv1r, &h->sv2r, &h->ppid, &h->ptoff, &h->ptcode, &h->pchunk, &h->vec_a, &h->vec_b, &h->coef, &h->to,
 	                  &h->lut, &h->gf_in, &h->gf_out, &h->dfac, &h->dden, &h->blk_x, &h->blk_y,
 	                  &h->blk_mu, &h->blk_sites, &h->blk_part, &h->blk_grow, &h->blk_gvec, &h->cplanes,
 	                  &h->lplanes, &h->gf_mfma_out, &h->ell_col, &h->ell_v1f,
@@ -447,6 +447,9 @@ int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
 		h->use_palette = (int)value;
 		h->plain_graph.reset();
 		return TBTKX_OK;
+	case TBTKX_OPT_PATTERN_BYTE_ROWS:
+		TBTKX_REQUIRE(false, TBTKX_ERR_ARG,
+		              "tbtkx_set_option: TBTKX_OPT_PATTERN_BYTE_ROWS is read-only");
 	case TBTKX_OPT_SLOT_ROWS:
 		TBTKX_REQUIRE(value == 0 || value == 1 || value == 2 || value == 4, TBTKX_ERR_ARG,
 		              "tbtkx_set_option: slot rows per thread must be 1, 2 or 4 (0 = by form)");
@@ -454,8 +457,8 @@ int tbtkx_set_option(tbtkx_handle h, int option, int64_t value) {
 		h->plain_graph.reset();
 		return TBTKX_OK;
 	case TBTKX_OPT_ROW_PATTERNS:
-		TBTKX_REQUIRE(value == 0 || value == 1, TBTKX_ERR_ARG,
-		              "tbtkx_set_option: row pattern option must be 0 or 1");
+		TBTKX_REQUIRE(value >= 0 && value <= 2, TBTKX_ERR_ARG,
+		              "tbtkx_set_option: row pattern option must be 0, 1 or 2");
 		h->use_patterns = (int)value;
 		h->plain_graph.reset();
 		return TBTKX_OK;
@@ -490,6 +493,7 @@ int tbtkx_get_option(tbtkx_handle h, int option, int64_t *value) {
 	case TBTKX_OPT_MULTI: *value = h->last_multi_blocks ? 2 : 1; return TBTKX_OK;
 	case TBTKX_OPT_ROW_PATTERNS: *value = h->last_patterns; return TBTKX_OK;
 	case TBTKX_OPT_SLOT_ROWS: *value = h->slot_rows; return TBTKX_OK;
+	case TBTKX_OPT_PATTERN_BYTE_ROWS: *value = h->pat_ok ? h->pat_byte_rows : -1; return TBTKX_OK;
 	case TBTKX_OPT_STRIP_ROWS:
 		*value = h->strip_rows ? h->strip_rows : h->last_strip;
 		return TBTKX_OK;

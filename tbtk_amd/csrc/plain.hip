@@ -151,13 +151,18 @@ __global__ __launch_bounds__(kThreads) void make_slots(
 // pattern id pid[r] indexing a table of 8 offsets (scol) and 8 codes (scode)
 // per pattern. R rows per thread (rows b*R*256 + j*256 + tid, j < R): all
 // their loads are issued before the first sum, R times the loads in flight.
+// With pchunk (one byte per 64-row chunk: the pattern all its rows share, or
+// 255), a wavefront whose R chunks share one pattern reads its offsets,
+// codes and palette values through scalar loads at wave-uniform addresses
+// and no per-row pattern ids: the gathers wait for a cache hit instead of a
+// memory round trip. Same values, same order, same results.
 template <typename S, bool FIRST, bool PAL, int W, bool PAT = false, int R = 1>
 __global__ __launch_bounds__(kThreads) void plain_step_slot(
     const int16_t *__restrict__ scol, const uint8_t *__restrict__ scode,
     const S *__restrict__ sval, const S *__restrict__ pal, const S *__restrict__ diag,
     const S *__restrict__ x, S *y, int64_t N, const int64_t *__restrict__ to, int K,
     double2 *__restrict__ coef, int M, int nprev, const uint8_t *__restrict__ pid,
-    int remap) {
+    int remap, const uint8_t *__restrict__ pchunk) {
 #pragma clang fp contract(off)
 	typedef F64<S> A;
 	for (int64_t g = (int64_t)blockIdx.x * kThreads + threadIdx.x; g < K;
@@ -175,9 +180,79 @@ __global__ __launch_bounds__(kThreads) void plain_step_slot(
 		row[j] = rj < N ? rj : N - 1;   // rows past the end load row N-1, store nothing
 	}
 	if ((b * R) * kThreads + threadIdx.x >= N) return;
+	// j_{n-2} and the diagonal value, which do not depend on the row's slots.
+	// Vector loads complete in issue order, so they are issued after the
+	// loads the gathers' addresses wait for.
+	S yo[R], dg[R];
+	auto load_row_values = [&]() {
+#pragma unroll
+		for (int j = 0; j < R; j++) {
+			yo[j] = FIRST ? A::zero() : y[row[j]];
+			dg[j] = PAL ? diag[row[j]] : A::zero();
+		}
+	};
+	if (PAT && pchunk) {
+		// The wavefront's R chunks of 64 rows are chunks (b R + j) 4 + w:
+		// byte w of 32-bit word b R + j.
+		const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+		const uint32_t *pc = reinterpret_cast<const uint32_t *>(pchunk);
+		int up = (int)((pc[b * R] >> (8 * w)) & 255u);
+#pragma unroll
+		for (int j = 1; j < R; j++)
+			if ((int)((pc[b * R + j] >> (8 * w)) & 255u) != up) up = 255;
+		up = __builtin_amdgcn_readfirstlane(up);
+		if (up != 255) {
+			// Every row of the wavefront is below N (a chunk past the end is
+			// never uniform), so no store is guarded.
+			load_row_values();
+			// Whole words through readfirstlane, so that the compiler keeps
+			// them as scalar loads instead of narrowing one half-word to a
+			// vector load (which would wait on the row loads above).
+			const int4 o = *reinterpret_cast<const int4 *>(scol + up * 8);
+			const int2 e = *reinterpret_cast<const int2 *>(scode + up * 8);
+			const uint32_t ow[4] = {(uint32_t)__builtin_amdgcn_readfirstlane(o.x),
+			                        (uint32_t)__builtin_amdgcn_readfirstlane(o.y),
+			                        (uint32_t)__builtin_amdgcn_readfirstlane(o.z),
+			                        (uint32_t)__builtin_amdgcn_readfirstlane(o.w)};
+			const uint32_t ew[2] = {(uint32_t)__builtin_amdgcn_readfirstlane(e.x),
+			                        (uint32_t)__builtin_amdgcn_readfirstlane(e.y)};
+			int16_t oo[8];
+			uint8_t ee[8];
+#pragma unroll
+			for (int s = 0; s < 8; s++) {
+				oo[s] = (int16_t)(ow[s >> 1] >> (16 * (s & 1)));
+				ee[s] = (uint8_t)(ew[s >> 2] >> (8 * (s & 3)));
+			}
+			S pv[W], xu[R][W];
+			// Unconditional loads (entry 0 stands in for the diagonal code;
+			// each palette section has 256 entries), so they issue together.
+#pragma unroll
+			for (int s = 0; s < W; s++) pv[s] = pal[ee[s] == 255 ? 0 : ee[s]];
+#pragma unroll
+			for (int j = 0; j < R; j++)
+#pragma unroll
+				for (int s = 0; s < W; s++)
+					xu[j][s] = x[row[j] + (oo[s] == kSlotPad ? 0 : (int64_t)oo[s])];
+			// Every row's gathers are in flight before the first product (the
+			// scheduler would otherwise issue row j + 1's gathers only after
+			// row j's sum, one memory round trip per row).
+			__builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+			for (int j = 0; j < R; j++) {
+				S acc = FIRST ? A::zero() : A::neg(yo[j]);
+#pragma unroll
+				for (int s = 0; s < W; s++) {
+					const S p = A::mul(ee[s] == 255 ? dg[j] : pv[s], xu[j][s]);
+					if (oo[s] != kSlotPad) acc = A::add(acc, p);
+				}
+				y[row[j]] = acc;
+			}
+			return;
+		}
+	}
 	int16_t c[R][W];
 	uint8_t q[R][W];
-	S v[R][W], xx[R][W], yo[R], dg[R];
+	S v[R][W], xx[R][W];
 #pragma unroll
 	for (int j = 0; j < R; j++) {
 		const int64_t r = row[j];
@@ -202,9 +277,8 @@ __global__ __launch_bounds__(kThreads) void plain_step_slot(
 				else v[j][s] = sval[(int64_t)s * N + r];
 			}
 		}
-		yo[j] = FIRST ? A::zero() : y[r];
-		dg[j] = PAL ? diag[r] : A::zero();
 	}
+	load_row_values();
 #pragma unroll
 	for (int j = 0; j < R; j++)
 #pragma unroll
@@ -394,13 +468,15 @@ void launch_lds(tbtkx_context *h, bool first, const CI *col, const S *v, const S
 		    h->xcd_remap, to, K, coef, M, nprev, code, pal, diag);
 }
 
-// Rows per thread of the slot step: the option, or by default 2 with row
-// patterns (one byte per row leaves the loads of one row too few to cover
-// the gathers' latency) and 1 with per-slot data (measured on the 1024x1024
-// spin lattice: 16.0 against 16.1 us with patterns, 16.4 against 16.7
-// without; 4 rows: 19.0 us).
+// Rows per thread of the slot step: the option, or by default 1, and 2
+// when every row reads its pattern byte (TBTKX_OPT_ROW_PATTERNS = 2: one
+// byte per row leaves the loads of one row too few to cover the gathers'
+// latency). Measured on the 1024x1024 spin lattice: wave-uniform patterns
+// 12.05 us with 1 row per thread, 13.3 with 2, 13.6 with 4; pattern bytes
+// for every row 15.9 with 2, 16.1 with 1; per-slot data 16.4 with 1,
+// 16.7 with 2.
 inline int slot_rows(const tbtkx_context *h, bool pat) {
-	return h->slot_rows > 0 ? h->slot_rows : pat ? 2 : 1;
+	return h->slot_rows > 0 ? h->slot_rows : pat && h->use_patterns == 2 ? 2 : 1;
 }
 
 template <typename S, bool FIRST, bool PAL, bool PAT>
@@ -410,9 +486,10 @@ void launch_slot_w(tbtkx_context *h, int W, const int16_t *scol, const uint8_t *
                    const uint8_t *pid) {
 	const int R = slot_rows(h, PAT);
 	const unsigned grid = (unsigned)((h->N + (int64_t)kThreads * R - 1) / ((int64_t)kThreads * R));
+	const uint8_t *pchunk = PAT && h->use_patterns == 1 ? h->pchunk.as<uint8_t>() : nullptr;
 #define TBTKX_SLOT_R(WW, RR)                                                            \
 	plain_step_slot<S, FIRST, PAL, WW, PAT, RR><<<grid, kThreads, 0, h->stream>>>(    \
-	    scol, scode, sv, pal, diag, x, y, h->N, to, K, coef, M, nprev, pid, 1)
+	    scol, scode, sv, pal, diag, x, y, h->N, to, K, coef, M, nprev, pid, 1, pchunk)
 #define TBTKX_SLOT(WW)                                                                  \
 	do {                                                                            \
 		if (R == 4) TBTKX_SLOT_R(WW, 4);                                        \
@@ -657,17 +734,36 @@ static int build_row_patterns(tbtkx_context *h, int64_t N, const uint32_t *rowpt
 		memcpy(&toff[i * 8], table[i].off, 16);
 		memcpy(&tcode[i * 8], table[i].code, 8);
 	}
+	// 64-row chunks: the pattern of all the chunk's rows, or 255 (mixed, or
+	// a chunk running past row N - 1). Sized for the chunk indices of every
+	// rows-per-thread setting (1, 2, 4): ceil(N / 1024) x 16.
+	const int64_t nchunk = (N + 1023) / 1024 * 16;
+	std::vector<uint8_t> chunk((size_t)nchunk, 255);
+	int64_t uniform_rows = 0;
+	for (int64_t c = 0; c < nchunk && (c + 1) * 64 <= N; c++) {
+		const uint8_t p0 = pid[(size_t)(c * 64)];
+		bool same = true;
+		for (int64_t r = c * 64 + 1; r < (c + 1) * 64 && same; r++) same = pid[(size_t)r] == p0;
+		if (same) {
+			chunk[(size_t)c] = p0;
+			uniform_rows += 64;
+		}
+	}
 	int rc;
 	if ((rc = h->ppid.reserve((size_t)N))) return rc;
+	if ((rc = h->pchunk.reserve((size_t)nchunk))) return rc;
 	if ((rc = h->ptoff.reserve(P * 16 > 0 ? P * 16 : 16))) return rc;
 	if ((rc = h->ptcode.reserve(P * 8 > 0 ? P * 8 : 8))) return rc;
 	TBTKX_CHECK_HIP(hipMemcpy(h->ppid.ptr, pid.data(), (size_t)N, hipMemcpyHostToDevice));
+	TBTKX_CHECK_HIP(hipMemcpy(h->pchunk.ptr, chunk.data(), (size_t)nchunk,
+	                          hipMemcpyHostToDevice));
 	if (P) {
 		TBTKX_CHECK_HIP(hipMemcpy(h->ptoff.ptr, toff.data(), P * 16, hipMemcpyHostToDevice));
 		TBTKX_CHECK_HIP(hipMemcpy(h->ptcode.ptr, tcode.data(), P * 8, hipMemcpyHostToDevice));
 	}
 	h->pat_ok = 1;
 	h->pat_count = (int)P;
+	h->pat_byte_rows = N - uniform_rows;
 	return TBTKX_OK;
 }
 
@@ -930,13 +1026,15 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 		GraphCache &gc = h->plain_graph;
 		int64_t key[GraphCache::kKey];
 		const bool key_ok = GraphCache::make_key(
-		    key, {U, M, h->plain_kernel * 4 + pform.slot * 2 + pform.pat + 16 * h->slot_rows,
+		    key, {U, M, h->plain_kernel * 4 + pform.slot * 2 + pform.pat + 16 * h->slot_rows +
+		                    256 * h->use_patterns,
 		          h->xcd_remap, h->N,
 		          real, c16, usepal},
 		    {h->vec_a.ptr, h->vec_b.ptr, h->coef.ptr, h->to.ptr, h->rowptr.ptr, h->col.ptr,
 		     h->v1.ptr, h->v2.ptr, h->v1r.ptr, h->v2r.ptr, h->rblk.ptr, h->col16.ptr,
 		     h->pcode.ptr, h->pal.ptr, h->palr.ptr, h->pdiag.ptr, h->pdiagr.ptr, h->scol.ptr,
 		     h->scode.ptr, h->sv1.ptr, h->sv2.ptr, h->sv1r.ptr, h->sv2r.ptr, h->ppid.ptr,
+		     h->pchunk.ptr,
 		     h->ptoff.ptr, h->ptcode.ptr});
 		TBTKX_REQUIRE(key_ok, TBTKX_ERR_STATE, "tbtkx_moments: graph key too long");
 		if (!gc.matches(key)) {

@@ -21,7 +21,7 @@ DAMP_NONE, DAMP_LORENTZ, DAMP_JACKSON = 0, 1, 2
 (OPT_PLAIN_KERNEL, OPT_GRAPH, OPT_BLOCK_WIDTH, OPT_XCD_REMAP, OPT_TILE_ROWS, OPT_NT_STREAM,
  OPT_ELL, OPT_AUTOTUNE, OPT_MFMA_PIPE, OPT_STRIP_ROWS, OPT_GRID_SHARE,
  OPT_STRIP_CHUNK, OPT_ROWS_IN_FLIGHT, OPT_REAL, OPT_COL16, OPT_PALETTE,
- OPT_MULTI, OPT_ROW_PATTERNS, OPT_SLOT_ROWS) = range(1, 20)
+ OPT_MULTI, OPT_ROW_PATTERNS, OPT_SLOT_ROWS, OPT_PATTERN_BYTE_ROWS) = range(1, 21)
 ANY_DEVICE = -1
 
 # (name, argtypes) of every exported function, in header order.

@@ -50,7 +50,7 @@ def test_slot_bitwise_vs_oracle_and_row_blocks(slot, oracle_mod, name, real):
     M = 300
     ref = oracle_mod.moments(csr, s, frm, to, M)
     print(f"{name}: N={m.basis_size} real={real}", flush=True)
-    for pal, pat in ((1, 1), (1, 0), (0, 1)):
+    for pal, pat in ((1, 1), (1, 2), (1, 0), (0, 1)):
         slot.set_option(X.OPT_PALETTE, pal)
         slot.set_option(X.OPT_ROW_PATTERNS, pat)
         got = slot.moments(frm, to, M)
@@ -162,3 +162,44 @@ def test_slot_device_lattice_graph_and_noalloc(slot, oracle_mod):
     slot.set_option(X.OPT_GRAPH, 0)
     assert np.array_equal(slot.moments_noalloc(480, [480, 7], 300), ref)
     assert slot.get_option(X.OPT_PLAIN_KERNEL) == 2
+
+
+@pytest.mark.parametrize("rows", [1, 2, 4])
+def test_row_patterns_uniform_wavefronts(slot, oracle_mod, rows):
+    """Wavefronts whose 64-row chunks share one pattern read it through
+    scalar loads (TBTKX_OPT_ROW_PATTERNS = 1); with 2 every row reads its
+    pattern byte. Both are bitwise the oracle's, for 1, 2 and 4 rows per
+    thread, on a lattice with long uniform runs (800 rows per column of the
+    20x400 spin lattice), one whose runs are shorter than a chunk (cubic 17,
+    4913 rows, not a multiple of 64) and a 1000-site open chain (every
+    chunk uniform but the ends), and in a graph replay."""
+    models = [L.square_spin(20, 400, disorder=1.0, seed=21),
+              L.cubic(17, 17, 17, disorder=1.0, seed=22),
+              L.square(1000, 1, disorder=0.5, seed=23)]
+    try:
+        slot.set_option(X.OPT_SLOT_ROWS, rows)
+        for m in models:
+            csr = m.csr()
+            s = kpm.gershgorin_scale(csr, 1.05)
+            slot.set_csr(csr, s)
+            frm = m.basis_size // 2 + 1
+            to = [frm, 0, m.basis_size - 1, 63, 64, frm - 64]
+            M = 200
+            ref = oracle_mod.moments(csr, s, frm, to, M)
+            print(f"{m.name}: N={m.basis_size} rows={rows}", flush=True)
+            for pat in (1, 2):
+                slot.set_option(X.OPT_ROW_PATTERNS, pat)
+                got = slot.moments(frm, to, M)
+                assert slot.get_option(X.OPT_ROW_PATTERNS) == 1
+                assert np.array_equal(got, ref), (m.name, rows, pat)
+            if m.basis_size == 1000:
+                # chunk 0 holds the end row 0; rows 960-999 fill no chunk
+                assert slot.get_option(X.OPT_PATTERN_BYTE_ROWS) == 64 + 40
+            slot.set_option(X.OPT_ROW_PATTERNS, 1)
+            slot.set_option(X.OPT_GRAPH, 1)
+            for _ in range(2):
+                assert np.array_equal(slot.moments(frm, to, M), ref), (m.name, rows, "graph")
+            slot.set_option(X.OPT_GRAPH, 0)
+    finally:
+        slot.set_option(X.OPT_SLOT_ROWS, 0)
+        slot.set_option(X.OPT_GRAPH, 0)
