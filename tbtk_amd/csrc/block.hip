@@ -1587,6 +1587,7 @@ int tbtkx_moments_multi(tbtkx_handle h, int num_from, const int64_t *from,
 		              "tbtkx_moments_multi: to-index outside the basis");
 	if (T == 0) return TBTKX_OK;
 	int rc = set_device(h);
+	if (!rc) rc = build_plain_forms(h);
 	if (rc) return rc;
 	const int packed = h->real_path && h->h_real;
 	int B = 64;

@@ -143,6 +143,9 @@ struct tbtkx_context {
 	tbtkx::DevBuf ppid, ptoff, ptcode;
 	tbtkx::DevBuf pchunk;   // per 64-row chunk: its rows' common pattern, or 255
 	int pat_ok = 0, pat_count = 0;
+	// Device-built models: palette, patterns and slots not built yet (they
+	// are built from a host copy of the CSR on the first plain recurrence).
+	int forms_pending = 0;
 	int64_t pat_byte_rows = 0;   // rows outside uniform 64-row chunks
 	int use_patterns = 1;
 	int slot_rows = 0;       // rows per thread of the slot step (1, 2, 4; 0 = by form)
@@ -261,6 +264,9 @@ int build_col16(tbtkx_context *h);
 int build_slots(tbtkx_context *h, const uint32_t *rowptr);
 // Adds H/s and 2H/s values to the slots (for calls without the palette).
 int build_slot_values(tbtkx_context *h);
+// Builds the palette, the row patterns and the slot form of a device-built
+// model from a host copy of its CSR, once (forms_pending).
+int build_plain_forms(tbtkx_context *h);
 
 // The form the plain recurrence step runs in: the slot-major ELL kernel
 // (TBTKX_OPT_PLAIN_KERNEL 2, needs 16-bit columns), 16-bit column offsets,

@@ -353,6 +353,7 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 	h->h_real = real;
 	h->nblk = (int)blocks.size() - 1;
 	if ((rc = build_col16(h))) return rc;
+	h->forms_pending = 0;
 	if ((rc = build_palette(h, N, Z, rowptr, col, v1.data(), v2.data()))) return rc;
 	return build_slots(h, rowptr);
 }

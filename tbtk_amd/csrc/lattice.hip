@@ -433,18 +433,17 @@ int tbtkx_set_lattice(tbtkx_handle h, const tbtkx_lattice *lat, const double *on
 	h->h_real = real;
 	h->nblk = (int)blocks.size() - 1;
 	if ((rc = build_col16(h))) return rc;
-	// The value palette is built from host copies of the device CSR.
-	// (hrp, the row pointer, is already on the host.)
-	std::vector<uint32_t> hcol(Z > 0 ? Z : 1);
-	std::vector<double> hv1(2 * (Z > 0 ? Z : 1)), hv2(2 * (Z > 0 ? Z : 1));
-	if (Z > 0) {
-		TBTKX_CHECK_HIP(hipMemcpy(hcol.data(), h->col.ptr, Z * 4, hipMemcpyDeviceToHost));
-		TBTKX_CHECK_HIP(hipMemcpy(hv1.data(), h->v1.ptr, Z * 16, hipMemcpyDeviceToHost));
-		TBTKX_CHECK_HIP(hipMemcpy(hv2.data(), h->v2.ptr, Z * 16, hipMemcpyDeviceToHost));
-	}
-	if ((rc = build_palette(h, N, Z, hrp.data(), hcol.data(), hv1.data(), hv2.data())))
-		return rc;
-	return build_slots(h, hrp.data());
+	// The value palette, the row patterns and the slot form of the plain
+	// step come from a host copy of the device CSR; they are built on the
+	// first plain recurrence (build_plain_forms), so a model used only by
+	// the batched paths never copies its CSR back.
+	h->palette_ok = 0;
+	h->pat_ok = 0;
+	h->slot_ok = 0;
+	h->slot_pal = 0;
+	h->slot_vals = 0;
+	h->forms_pending = 1;
+	return TBTKX_OK;
 }
 
 int tbtkx_lattice_gershgorin(tbtkx_handle h, const tbtkx_lattice *lat, const double *onsite,

@@ -650,6 +650,25 @@ int build_slots(tbtkx_context *h, const uint32_t *rowptr) {
 	return TBTKX_OK;
 }
 
+int build_plain_forms(tbtkx_context *h) {
+	if (!h->forms_pending) return TBTKX_OK;
+	const int64_t N = h->N, Z = h->Z;
+	std::vector<uint32_t> hrp(N + 1), hcol(Z > 0 ? Z : 1);
+	std::vector<double> hv1(2 * (Z > 0 ? Z : 1)), hv2(2 * (Z > 0 ? Z : 1));
+	TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
+	TBTKX_CHECK_HIP(hipMemcpy(hrp.data(), h->rowptr.ptr, (N + 1) * 4, hipMemcpyDeviceToHost));
+	if (Z > 0) {
+		TBTKX_CHECK_HIP(hipMemcpy(hcol.data(), h->col.ptr, Z * 4, hipMemcpyDeviceToHost));
+		TBTKX_CHECK_HIP(hipMemcpy(hv1.data(), h->v1.ptr, Z * 16, hipMemcpyDeviceToHost));
+		TBTKX_CHECK_HIP(hipMemcpy(hv2.data(), h->v2.ptr, Z * 16, hipMemcpyDeviceToHost));
+	}
+	int rc = build_palette(h, N, Z, hrp.data(), hcol.data(), hv1.data(), hv2.data());
+	if (rc) return rc;
+	if ((rc = build_slots(h, hrp.data()))) return rc;
+	h->forms_pending = 0;
+	return TBTKX_OK;
+}
+
 int build_slot_values(tbtkx_context *h) {
 	if (!h->slot_ok || h->slot_vals) return TBTKX_OK;
 	int rc = make_slot_arrays(h, h->slot_pal, true);
@@ -992,6 +1011,8 @@ extern "C" int tbtkx_moments(tbtkx_handle h, int64_t from, int K,
 	const int U = (int)uniq.size();
 	int rc = set_device(h);
 	if (rc) return rc;
+	// A device-built model gets its palette, patterns and slots now.
+	if ((rc = build_plain_forms(h))) return rc;
 	// The slot form without the palette needs the values in slots.
 	if (h->plain_kernel == 2 && !(h->use_palette && h->slot_pal) &&
 	    (rc = build_slot_values(h)))

@@ -203,3 +203,29 @@ def test_row_patterns_uniform_wavefronts(slot, oracle_mod, rows):
     finally:
         slot.set_option(X.OPT_SLOT_ROWS, 0)
         slot.set_option(X.OPT_GRAPH, 0)
+
+
+def test_device_lattice_builds_plain_forms_on_first_plain_call(slot, oracle_mod):
+    """A device-built model copies its CSR back for the palette, the row
+    patterns and the slots only on its first plain recurrence: a batched
+    call leaves them unbuilt, the allocation-free entry point runs the
+    row-block form meanwhile, and every form gives the oracle's moments."""
+    m = L.square_spin(6, 300, disorder=1.0, seed=9)      # columns of 600 rows
+    spec = L.square_spin_spec(6, 300, disorder=1.0, seed=9)
+    N = m.basis_size
+    s = kpm.gershgorin_scale(m.csr(), 1.05)
+    to = [N // 2 + 1, 5, N - 1]
+    ref = oracle_mod.moments(m.csr(), s, N // 2 + 1, to, 150)
+    slot.set_lattice(spec, s)
+    assert slot.get_option(X.OPT_PATTERN_BYTE_ROWS) == -1
+    slot.moments_block([N // 2 + 1, 5], 20, 0)
+    assert slot.get_option(X.OPT_PATTERN_BYTE_ROWS) == -1
+    assert np.array_equal(slot.moments_noalloc(N // 2 + 1, to, 150), ref)
+    assert slot.get_option(X.OPT_PLAIN_KERNEL) == 0
+    assert np.array_equal(slot.moments(N // 2 + 1, to, 150), ref)
+    assert slot.get_option(X.OPT_PLAIN_KERNEL) == 2
+    assert slot.get_option(X.OPT_ROW_PATTERNS) == 1
+    # most 64-row chunks lie inside one 600-row column: uniform
+    assert 0 < slot.get_option(X.OPT_PATTERN_BYTE_ROWS) < N // 2
+    assert np.array_equal(slot.moments_noalloc(N // 2 + 1, to, 150), ref)
+    assert slot.get_option(X.OPT_PLAIN_KERNEL) == 2
