@@ -255,6 +255,36 @@ def load_traffic(kernel):
     return None if entry is None else entry.get("hbm_bytes_per_launch")
 
 
+def plain_step_bytes(eng, N, Z, csr=None, row_pattern_bytes=False):
+    """Algorithmic HBM bytes of one launch of the plain FP64 step in the
+    form the engine's last plain call ran (options read back from the
+    engine), and the flags that name the form."""
+    real = eng.get_option(X.OPT_REAL) == 1
+    col16 = eng.get_option(X.OPT_COL16) == 1
+    palette = eng.get_option(X.OPT_PALETTE) == 1
+    slot = eng.get_option(X.OPT_PLAIN_KERNEL) == 2
+    patterns = eng.get_option(X.OPT_ROW_PATTERNS) == 1
+    # Wave-uniform patterns (the default): only the rows outside 64-row
+    # chunks of one shared pattern read their pattern byte.
+    uniform = patterns and not row_pattern_bytes
+    byte_rows = eng.get_option(X.OPT_PATTERN_BYTE_ROWS) if uniform else N
+    cb = 2 if col16 else 4
+    e = 8 if real else 16
+    if patterns:
+        step_bytes = 4 * e * N + byte_rows + (N // 64 if uniform else 0)
+    elif slot:
+        rowptr = (csr if csr is not None else eng.csr_copy())[0]
+        W = max(4, int(np.diff(rowptr).max()))
+        step_bytes = W * N * (2 + (1 if palette else e)) + N * (3 * e + (e if palette else 0))
+    elif palette:
+        step_bytes = 3 * Z + 36 * N if real else 3 * Z + 68 * N
+    else:
+        step_bytes = (8 + cb) * Z + 28 * N if real else (16 + cb) * Z + 52 * N
+    form = {"real": real, "col16": col16, "palette": palette, "slot": slot,
+            "patterns": patterns, "uniform": uniform, "byte_rows": byte_rows}
+    return step_bytes, form
+
+
 def batched_comparison(eng):
     """Config: 512x512 Nambu, 256 sites, M=2000; batched FP32 vs plain FP64."""
     # H and its Gershgorin bound (FP64) built on the device from the lattice
@@ -293,8 +323,7 @@ def batched_comparison(eng):
         c = eng.moments(int(b), [int(b)], M_BATCH)
         plain_ms += eng.last_timing()[0]
         ref.append(c[0])
-    plain_cb = 2 if eng.get_option(X.OPT_COL16) == 1 else 4
-    plain_pal = eng.get_option(X.OPT_PALETTE) == 1
+    plain_bytes, plain_form = plain_step_bytes(eng, N, Z)
     ld_plain = contract(np.array(ref))
     t_plain = (time.perf_counter() - t0) / PLAIN_CMP_SITES
 
@@ -329,9 +358,11 @@ def batched_comparison(eng):
         "spmm_GBps": step_bytes / per_step / 1e9,
         "spmm_hbm_frac": step_bytes / per_step / 1e9 / HBM_PEAK_GBPS,
         "plain_arithmetic": "real fp64" if plain_real else "complex fp64",
-        "plain_spmv_GBps": ((3 * Z + (36 if plain_real else 68) * N) if plain_pal
-                            else ((8 + plain_cb) * Z + 28 * N) if plain_real
-                            else ((16 + plain_cb) * Z + 52 * N))
+        "plain_form": ("slot-major, wave-uniform row patterns" if plain_form["uniform"] else
+                       "slot-major, row patterns" if plain_form["patterns"] else
+                       "slot-major" if plain_form["slot"] else "row blocks"),
+        "plain_step_us": plain_ms * 1e3 / (PLAIN_CMP_SITES * (M_BATCH - 1)),
+        "plain_spmv_GBps": plain_bytes
         / (plain_ms * 1e-3 / (PLAIN_CMP_SITES * (M_BATCH - 1))) / 1e9,
     }
 
@@ -668,36 +699,13 @@ def main():
         eng.lut_destroy()
         sites_per_step = 2
         value = args.steps * sites_per_step / elapsed
-        # A real H runs the real FP64 recurrence (same moments, fewer bytes).
-        real = eng.get_option(X.OPT_REAL) == 1
-        # 16-bit column offsets (every column within int16 of its row
-        # block's first row): 2 bytes per non-zero instead of 4.
-        col16 = eng.get_option(X.OPT_COL16) == 1
-        # Value palette (off-diagonal values as 1-byte codes, the diagonal
-        # from a per-row array): 3 bytes per non-zero, 8 or 16 per row.
-        palette = eng.get_option(X.OPT_PALETTE) == 1
-        # Slot-major ELL step (one thread per row, W slots per row, W = the
-        # longest row, at least 4) or the row-block CSR step.
-        slot = eng.get_option(X.OPT_PLAIN_KERNEL) == 2
-        # Row patterns (slot form with the palette): one byte per row indexes
-        # the model's distinct rows (column offsets and value codes).
-        patterns = eng.get_option(X.OPT_ROW_PATTERNS) == 1
-        # Wave-uniform patterns (the default): a wavefront whose 64-row
-        # chunks share one pattern reads it from a byte per chunk, so only
-        # the rows outside such chunks read their pattern byte.
-        uniform = patterns and not args.row_pattern_bytes
-        byte_rows = eng.get_option(X.OPT_PATTERN_BYTE_ROWS) if uniform else N
-        cb = 2 if col16 else 4
-        e = 8 if real else 16
-        if patterns:
-            step_bytes = 4 * e * N + byte_rows + (N // 64 if uniform else 0)
-        elif slot:
-            W = max(4, int(np.diff(csr[0]).max()))
-            step_bytes = W * N * (2 + (1 if palette else e)) + N * (3 * e + (e if palette else 0))
-        elif palette:
-            step_bytes = 3 * Z + 36 * N if real else 3 * Z + 68 * N
-        else:
-            step_bytes = (8 + cb) * Z + 28 * N if real else (16 + cb) * Z + 52 * N
+        # Bytes of the form the calls ran: real or complex FP64, 16- or
+        # 32-bit columns, palette codes or values, slot-major or row blocks,
+        # row patterns (wave-uniform or a byte for every row).
+        step_bytes, form = plain_step_bytes(eng, N, Z, csr, args.row_pattern_bytes)
+        real, col16, palette = form["real"], form["col16"], form["palette"]
+        slot, patterns, uniform = form["slot"], form["patterns"], form["uniform"]
+        byte_rows = form["byte_rows"]
         avg_kernel_s = kernel_ms * 1e-3 / launches
         achieved = step_bytes / avg_kernel_s / 1e9
         dE = (WINDOW[1] - WINDOW[0]) / (E_PTS - 1)
