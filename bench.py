@@ -70,7 +70,7 @@ CENTRE = (512, 512)
 # Batched comparison
 M_BATCH = 2000
 B_SITES = 256
-PLAIN_CMP_SITES = 4
+PLAIN_CMP_SITES = 16
 
 PARTS = ("plain", "batched", "uneven", "dos", "mfma", "cpu")
 
@@ -420,19 +420,22 @@ def uneven_batch(eng, graph=True, moments_div=1, tile_rows=0):
     finally:
         eng.set_option(X.OPT_GRAPH, 0)
     graph_diff = max(float(np.abs(a - b).max()) for a, b in zip(mu, mu_g))
-    # error: the two smallest models, first site, against the plain FP64 path
+    # error against the plain FP64 path: the two smallest and the two
+    # largest models, their first and last sites
     errs = []
-    for i in np.argsort(N)[:2]:
+    order = np.argsort(N)
+    for i in list(order[:2]) + list(order[-2:]):
         spec, st, Mi = batch[i]
         eng.set_lattice(spec, scales[i])
-        ref = eng.moments(int(st[0]), [int(st[0])], Mi)[0]
-        lo, hi = -0.99 * scales[i], 0.99 * scales[i]
-        eng.lut_generate(Mi, E_PTS, lo, hi, scales[i])
-        eng.set_damping(X.DAMP_JACKSON)
-        l_ref = kpm.ldos_from_gf(eng.contract(ref, X.NONPRINCIPAL))
-        l_got = kpm.ldos_from_gf(eng.contract(mu[i][0], X.NONPRINCIPAL))
-        eng.set_damping(X.DAMP_NONE)
-        errs.append(float(np.abs(l_got - l_ref).max() / np.abs(l_ref).max()))
+        for k in sorted({0, len(st) - 1}):
+            ref = eng.moments(int(st[k]), [int(st[k])], Mi)[0]
+            lo, hi = -0.99 * scales[i], 0.99 * scales[i]
+            eng.lut_generate(Mi, E_PTS, lo, hi, scales[i])
+            eng.set_damping(X.DAMP_JACKSON)
+            l_ref = kpm.ldos_from_gf(eng.contract(ref, X.NONPRINCIPAL))
+            l_got = kpm.ldos_from_gf(eng.contract(mu[i][k], X.NONPRINCIPAL))
+            eng.set_damping(X.DAMP_NONE)
+            errs.append(float(np.abs(l_got - l_ref).max() / np.abs(l_ref).max()))
     eng.lut_destroy()
     eng.set_option(X.OPT_TILE_ROWS, 0)
     dev_s = ms_graph * 1e-3

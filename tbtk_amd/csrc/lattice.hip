@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kThreads) void lattice_fill(
 __global__ __launch_bounds__(kThreads) void lattice_fill_multi(
     LatDev g, const tbtkx_lattice_term *__restrict__ terms, const double2 *__restrict__ onsite,
     int64_t N, double inv_s, int W, uint32_t *__restrict__ ecol, float2 *__restrict__ e1,
-    float2 *__restrict__ e2) {
+    float2 *__restrict__ e2, int realh) {
 	const int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x;
 	if (r >= N) return;
 	uint32_t cols[kMaxRow];
@@ -217,12 +217,19 @@ __global__ __launch_bounds__(kThreads) void lattice_fill_multi(
 	for (int k = 0; k < n; k++)
 		if ((int64_t)cols[k] == r && diag < 0) diag = k;
 	int u = 0;
+	// realh: every model of the batch is real; an entry holds one FP64
+	// value in the 8 bytes of an FP32 complex value.
 	auto put = [&](uint32_t c, double2 x) {
 		const double2 x2 = cscale(x, 2.0);
 		const int64_t e = r * W + u++;
 		ecol[e] = c;
-		e1[e] = make_float2((float)x.x, (float)x.y);
-		e2[e] = make_float2((float)x2.x, (float)x2.y);
+		if (realh) {
+			reinterpret_cast<double *>(e1)[e] = x.x;
+			reinterpret_cast<double *>(e2)[e] = x2.x;
+		} else {
+			e1[e] = make_float2((float)x.x, (float)x.y);
+			e2[e] = make_float2((float)x2.x, (float)x2.y);
+		}
 	};
 	if (diag >= 0) put(cols[diag], cscale(vals[diag], inv_s));
 	else {
@@ -548,7 +555,7 @@ int tbtkx_multi_set_lattices(tbtkx_handle h, int K, const tbtkx_lattice *lats,
 		    g[m], h->lat_terms.as<tbtkx_lattice_term>(),
 		    g[m].has_onsite ? h->lat_onsite.as<double2>() : nullptr, N[m], 1 / scales[m], Wm[m],
 		    h->mm_ecol.as<uint32_t>() + e0, h->mm_ev1.as<float2>() + e0,
-		    h->mm_ev2.as<float2>() + e0);
+		    h->mm_ev2.as<float2>() + e0, real ? 1 : 0);
 		TBTKX_CHECK_HIP(hipGetLastError());
 		// The staging buffers are reused by the next model.
 		TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));

@@ -8,13 +8,17 @@
 // that order and the launch covers a prefix of the tile list; the launch
 // shape changes only when a model finishes (at most K "phases").
 //
-// Storage: FP32 complex H (row-major ELL, W_m slots per row: the model's
-// longest row rounded up to 4, 5, 6 or 8; scaled by 1/s_m
-// and 2/s_m exactly as the single-model path scales its FP32 copies) and FP32
-// complex vectors, row-major per model (N_m rows x B_m columns), rows summed
-// in FP32 with the same fused multiply-add order as the single-model step, so
-// the recurrence vectors are the ones tbtkx_moments_block produces for the
-// same model and sites. Moments are FP64: gather mode reads
+// Storage: H in row-major ELL (W_m slots per row: the model's longest row
+// rounded up to 4, 5, 6 or 8; scaled by 1/s_m and 2/s_m exactly as the
+// single-model path scales its copies), 8 bytes per entry: FP32 complex
+// values, or, when every model of the batch is real, one FP64 value (FP32
+// rounding of H moves its eigenvalues by ~1e-7 of the band, which thousands
+// of moments resolve). FP32 complex vectors, row-major per model (N_m rows x
+// B_m columns). Rows are summed with the same fused multiply-add order as
+// the single-model step, in FP32 (complex H) or FP64 (real H, rounded once
+// to FP32 on store), so the recurrence vectors are the ones
+// tbtkx_moments_block produces for the same model and sites with FP32 H,
+// respectively FP64 H (TBTKX_H_FP64). Moments are FP64: gather mode reads
 // mu_n[b] = Phi_n[site_b][b] each step; doubling mode fuses
 // D0_n = <Phi_n|Phi_n> and D1_n = <Phi_n+1|Phi_n> into the step (FP64 per
 // lane, then ordered reductions) and uses mu_2n = 2 D0_n - mu_0,
@@ -195,6 +199,24 @@ __device__ __forceinline__ void mac(float2 a, float2 x, float &ax, float &ay) {
 		ay = fmaf(a.x, x.y, fmaf(a.y, x.x, ay));
 	}
 }
+// Real H: one FP64 value per ELL entry (the 8 bytes of an FP32 complex
+// value), the row summed in FP64 and rounded once to FP32 on store.
+template <bool REALH>
+__device__ __forceinline__ void mac(double a, float2 x, double &ax, double &ay) {
+	ax = fma(a, (double)x.x, ax);
+	ay = fma(a, (double)x.y, ay);
+}
+
+// ELL value and row accumulator types: FP32 complex values and FP32 sums,
+// or (real H) FP64 values and FP64 sums.
+template <bool REALH> struct EllT {
+	typedef float2 V;
+	typedef float A;
+};
+template <> struct EllT<true> {
+	typedef double V;
+	typedef double A;
+};
 
 // Tile lt of a model (strip k, layer a, j-th tile of the segment): its first
 // row, and in rows its row count (0 for a tile past the end of the model).
@@ -247,6 +269,9 @@ __device__ __forceinline__ void multi_rows_pipe(
 	// GPW = 0: each lane loads its own rows' W entries (RPI W registers).
 	constexpr bool PERLANE = GPW == 0;
 	constexpr int NW = PERLANE ? RPI * W : RPI / (PERLANE ? 1 : GPW);
+	typedef typename EllT<REALH>::V EV;
+	typedef typename EllT<REALH>::A AC;
+	const EV *__restrict__ evv = reinterpret_cast<const EV *>(ev);
 	const uint32_t wp = (uint32_t)d.wp;
 	// The model's blocks of X and Y (wave-uniform bases) and 32-bit byte
 	// offsets into them (N_m wp 8 B < 4 GiB, checked on the host).
@@ -262,7 +287,7 @@ __device__ __forceinline__ void multi_rows_pipe(
 	if (t >= tend) return;
 	int gg = 0;
 	uint32_t wc[NW];
-	float2 wv[NW];
+	EV wv[NW];
 	// Windows of the iteration at groups g0.. of the tile at r0 (nr rows).
 	auto load_windows = [&](int64_t r0, int64_t nr, int g0) {
 		const int64_t glast = r0 + nr - 1;
@@ -274,7 +299,7 @@ __device__ __forceinline__ void multi_rows_pipe(
 #pragma unroll
 				for (int u = 0; u < W; u++) {
 					wc[r * W + u] = ecol[e + u];
-					wv[r * W + u] = ev[e + u];
+					wv[r * W + u] = evv[e + u];
 				}
 			}
 		} else {
@@ -284,7 +309,7 @@ __device__ __forceinline__ void multi_rows_pipe(
 				if (wrow > glast) wrow = glast;
 				const int64_t we = d.e0 + wrow * W + wslot;
 				wc[k] = ecol[we];
-				wv[k] = ev[we];
+				wv[k] = evv[we];
 			}
 		}
 	};
@@ -322,11 +347,12 @@ __device__ __forceinline__ void multi_rows_pipe(
 		// (The empty asm keeps the copy, so the prefetch below can load into
 		// the window registers themselves and the loop carries no copies of
 		// pending loads.)
-		float2 vc[NW];
+		EV vc[NW];
 #pragma unroll
 		for (int k = 0; k < NW; k++) {
 			vc[k] = wv[k];
-			asm volatile("" : "+v"(vc[k].x), "+v"(vc[k].y));
+			if constexpr (REALH) asm volatile("" : "+v"(vc[k]));
+			else asm volatile("" : "+v"(vc[k].x), "+v"(vc[k].y));
 		}
 		if constexpr (STORE) {
 #pragma unroll
@@ -350,33 +376,38 @@ __device__ __forceinline__ void multi_rows_pipe(
 		else load_windows(row0, rows, gg);
 #pragma unroll
 		for (int r = 0; r < RPI; r++) {
-			float ax[CPL], ay[CPL];
+			AC ax[CPL], ay[CPL];
 #pragma unroll
 			for (int j = 0; j < CPL; j++) {
-				ax[j] = FIRST ? 0.0f : -yo[r][j].x;
-				ay[j] = FIRST ? 0.0f : -yo[r][j].y;
+				ax[j] = FIRST ? (AC)0 : -(AC)yo[r][j].x;
+				ay[j] = FIRST ? (AC)0 : -(AC)yo[r][j].y;
 			}
 #pragma unroll
 			for (int u = 0; u < W; u++) {
-				float2 av;
+				EV av;
 				if constexpr (PERLANE) {
 					av = vc[r * W + u];
 				} else {
 					const int src = ((r % GPW) * Q + qc) * W + u;
-					av = make_float2(__shfl(vc[r / GPW].x, src), __shfl(vc[r / GPW].y, src));
+					if constexpr (REALH)
+						av = __shfl(vc[r / GPW], src);
+					else
+						av = make_float2(__shfl(vc[r / GPW].x, src),
+						                 __shfl(vc[r / GPW].y, src));
 				}
 #pragma unroll
 				for (int j = 0; j < CPL; j++) mac<REALH>(av, xv[r][u][j], ax[j], ay[j]);
 			}
 #pragma unroll
-			for (int j = 0; j < CPL; j++) ps[r][j] = make_float2(ax[j], ay[j]);
+			for (int j = 0; j < CPL; j++) ps[r][j] = make_float2((float)ax[j], (float)ay[j]);
 			po[r] = off[r];
-			// Slot 0 is the row itself (diagonal first).
+			// Slot 0 is the row itself (diagonal first); the dots use the
+			// stored (FP32) values, as a gather of the vectors would.
 			if (DOTS && live[r]) {
 #pragma unroll
 				for (int j = 0; j < CPL; j++)
-					dot_acc(xv[r][0][j].x, xv[r][0][j].y, ax[j], ay[j], packed, dd[j][0],
-					        dd[j][1], dd[j][2], dd[j][3]);
+					dot_acc(xv[r][0][j].x, xv[r][0][j].y, ps[r][j].x, ps[r][j].y, packed,
+					        dd[j][0], dd[j][1], dd[j][2], dd[j][3]);
 			}
 		}
 		t = tn;
@@ -1014,10 +1045,16 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 				const int64_t k = u < n ? slots[u] : -1;
 				if (k >= 0) {
 					ec[base + u] = col[z0 + k];
-					e1[2 * (base + u)] = (float)v1[2 * k];
-					e1[2 * (base + u) + 1] = (float)v1[2 * k + 1];
-					e2[2 * (base + u)] = (float)v2[2 * k];
-					e2[2 * (base + u) + 1] = (float)v2[2 * k + 1];
+					if (real) {
+						// One FP64 value in the entry's 8 bytes.
+						memcpy(&e1[2 * (base + u)], &v1[2 * k], 8);
+						memcpy(&e2[2 * (base + u)], &v2[2 * k], 8);
+					} else {
+						e1[2 * (base + u)] = (float)v1[2 * k];
+						e1[2 * (base + u) + 1] = (float)v1[2 * k + 1];
+						e2[2 * (base + u)] = (float)v2[2 * k];
+						e2[2 * (base + u) + 1] = (float)v2[2 * k + 1];
+					}
 				} else {
 					ec[base + u] = (uint32_t)r;
 				}

@@ -2,8 +2,11 @@
 
 Checks, per model of a small uneven batch:
 - gather mode is bitwise equal to the single-model batched path
-  (tbtkx_moments_block, same FP32 recurrence arithmetic; mu_n is read from the
-  vectors directly);
+  (tbtkx_moments_block, same recurrence arithmetic; mu_n is read from the
+  vectors directly): FP32 H and FP32 row sums for a batch with a complex H,
+  FP64 H and FP64 row sums (TBTKX_H_FP64 on the single-model path) for an
+  all-real batch, whose ELL entries hold one FP64 value each; FP32 vectors
+  in both;
 - doubling mode agrees with the single-model doubling path to FP64 rounding of
   the dot-product sums (1e-12 relative);
 - both agree with the FP64 oracle (oracle/kpm_oracle.c, kpm_moments) within
@@ -15,6 +18,7 @@ import numpy as np
 import pytest
 
 from tbtk_amd import engine as X
+from tbtk_amd import kpm
 from tbtk_amd import lattice as L
 
 pytestmark = pytest.mark.gpu
@@ -53,9 +57,11 @@ def upload(engine, batch):
     return csrs, scales
 
 
-def single_model(engine, csr, s, sites, M, flags):
+def single_model(engine, csr, s, sites, M, flags, real_batch=False):
+    """The single-model batched path with the uneven batch's arithmetic: FP64
+    H for an all-real batch, FP32 H otherwise."""
     engine.set_csr(csr, s)
-    return engine.moments_block(sites, M, flags)
+    return engine.moments_block(sites, M, flags | (X.H_FP64 if real_batch else 0))
 
 
 @pytest.mark.parametrize("which", ["complex", "real"])
@@ -64,7 +70,7 @@ def test_multi_gather_bitwise_vs_block(engine, which):
     csrs, scales = upload(engine, batch)
     got = engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], 0)
     for (m, sites, M), csr, s, g in zip(batch, csrs, scales, got):
-        ref = single_model(engine, csr, s, sites, M, 0)
+        ref = single_model(engine, csr, s, sites, M, 0, which == "real")
         assert g.shape == ref.shape
         assert np.array_equal(g, ref), (m.name, m.basis_size, np.abs(g - ref).max())
 
@@ -75,7 +81,7 @@ def test_multi_doubling_vs_block_and_oracle(engine, oracle_mod, which):
     csrs, scales = upload(engine, batch)
     got = engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], X.DOUBLING)
     for (m, sites, M), csr, s, g in zip(batch, csrs, scales, got):
-        ref = single_model(engine, csr, s, sites, M, X.DOUBLING)
+        ref = single_model(engine, csr, s, sites, M, X.DOUBLING, which == "real")
         assert np.abs(g - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), m.name
         exact = np.array([oracle_mod.moments(csr, s, b, [b], M)[0] for b in sites])
         assert np.abs(g - exact).max() < 2e-5, (m.name, np.abs(g - exact).max())
@@ -164,7 +170,7 @@ def test_multi_mixed_ell_widths(engine, oracle_mod, which):
     got = engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], 0)
     dbl = engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], X.DOUBLING)
     for (m, sites, M), csr, s, g, d in zip(batch, csrs, scales, got, dbl):
-        ref = single_model(engine, csr, s, sites, M, 0)
+        ref = single_model(engine, csr, s, sites, M, 0, which == "real")
         assert np.array_equal(g, ref), (m.name, np.abs(g - ref).max())
         exact = np.array([oracle_mod.moments(csr, s, b, [b], M)[0] for b in sites])
         assert np.abs(d - exact).max() < 2e-5, (m.name, np.abs(d - exact).max())
@@ -249,3 +255,32 @@ def test_multi_reports_launch_shape(engine):
         assert engine.get_option(X.OPT_GRID_SHARE) == 1
     finally:
         engine.set_option(X.OPT_TILE_ROWS, 0)
+
+
+def test_multi_real_high_moment_count_ldos_error(engine, oracle_mod):
+    """A real honeycomb ribbon (t = 2.7, not exact in FP32) at 6000 moments:
+    the uneven batch's LDOS (Jackson damping, the FP64 contraction) is within
+    1e-5 of the oracle's relative to its maximum. The batch keeps a real H in
+    FP64 (FP32 rounding of H shifts the eigenvalues by ~1e-7 of the band,
+    which at this resolution moves the LDOS by ~1e-5)."""
+    m = L.honeycomb_ribbon(30, 40, t=2.7, disorder=0.5, seed=31)
+    csr = m.csr()
+    s = m.gershgorin_bound() * 1.05
+    sites = np.array([7, m.basis_size // 2])
+    M, E = 6000, 2000
+    engine.multi_set([csr], [s])
+    for flags in (0, X.DOUBLING):
+        got = engine.multi_moments([sites], [M], flags)[0]
+        engine.lut_generate(M, E, -0.99 * s, 0.99 * s, s)
+        engine.set_damping(X.DAMP_JACKSON)
+        try:
+            for k, b in enumerate(sites):
+                exact = oracle_mod.moments(csr, s, int(b), [int(b)], M)[0]
+                l_ref = kpm.ldos_from_gf(engine.contract(exact, X.NONPRINCIPAL))
+                l_got = kpm.ldos_from_gf(engine.contract(got[k], X.NONPRINCIPAL))
+                err = np.abs(l_got - l_ref).max() / np.abs(l_ref).max()
+                print(f"flags={flags} site={b}: LDOS error {err:.2e}", flush=True)
+                assert err <= 1e-5, (flags, int(b), err)
+        finally:
+            engine.set_damping(X.DAMP_NONE)
+            engine.lut_destroy()
