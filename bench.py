@@ -349,7 +349,8 @@ def batched_comparison(eng):
         "speedup_vs_plain_fp64": sps_b / sps_p,
         "max_ldos_rel_err_vs_fp64": float(err),
         "error_sites_compared": PLAIN_CMP_SITES,
-        "batched_dtype": ("fp32 storage (H and vectors), fp64 accumulate, doubling"
+        "batched_dtype": ("fp32 vectors; real H as one fp64 value per 8-byte entry, fp64 row "
+                          "sums; fp64 moments, doubling"
                           + (", real H: two real vectors per complex column" if packed else "")),
         "spmm_block_width": B,
         "vectors_per_block": 2 * B if packed else B,
@@ -450,7 +451,8 @@ def uneven_batch(eng, graph=True, moments_div=1, tile_rows=0):
 
         "ell_slots": {str(w): int((W == w).sum()) for w in (4, 5, 6, 8) if (W == w).any()},
         "recurrence_steps": int(S.max()),
-        "dtype": ("fp32 storage (H and vectors), fp64 moments, doubling"
+        "dtype": ("fp32 vectors; real H as one fp64 value per 8-byte entry, fp64 row sums; "
+                  "fp64 moments, doubling"
                   + (", real H: two real vectors per complex column" if packed else "")),
         "timed_run": "hipGraph replay" if graph else "direct launches (--no-graph)",
         "sites_per_s": float(B.sum()) / (t_graph),
@@ -506,7 +508,8 @@ def stochastic_dos(eng):
                      f"random-phase vectors (seed {seed}), M={M}, s=Gershgorin*1.05={s:.6f} "
                      "(H and bound built on the device)"),
         "nnz": int(Z),
-        "dtype": "fp32 storage (H and vectors), fp64 moment sums, doubling",
+        "dtype": ("fp32 vectors; real H as one fp64 value per 8-byte entry, fp64 row sums; "
+                  "fp64 moment sums, doubling"),
         "seconds": ms * 1e-3,
         "wall_s": wall,
         "block_width": Bw, "spmm_shape": shape, "blocks": blocks,

@@ -35,6 +35,19 @@ namespace {
 constexpr int kWaves = kThreads / 64;
 constexpr int kDotChunk = 128;     // steps whose dot partials are kept
 
+// A real H's value in the 8 bytes of an FP32 complex entry: one FP64 value
+// (FP32 rounding of H moves its eigenvalues by ~1e-7 of the band, which
+// thousands of moments resolve). Its imaginary part is zero.
+struct RealD {
+	double x;
+	static constexpr double y = 0.0;
+};
+
+// 8-byte H entries (FP32 complex, or FP64 real): the copies v1f/v2f and
+// ell_v1f/ell_v2f.
+template <typename HT>
+constexpr bool kHt8 = std::is_same<HT, float2>::value || std::is_same<HT, RealD>::value;
+
 template <typename T> struct C2;
 template <> struct C2<float2> {
 	__device__ static double2 load(const float2 &a) {
@@ -898,10 +911,10 @@ struct BlockRun {
 	bool exact = false;   // CSR step in the reference's operation order
 };
 
-// The strip step with doubled rows in flight is built for FP32 H and
-// vectors only (the batched default).
+// The strip step with doubled rows in flight is built for FP32 vectors and
+// 8-byte H entries only (the batched default).
 template <typename VT, typename HT>
-constexpr bool kFp32Pair = std::is_same<VT, float2>::value && std::is_same<HT, float2>::value;
+constexpr bool kFp32Pair = std::is_same<VT, float2>::value && kHt8<HT>;
 // The exact step exists for FP64 H and vectors only.
 template <typename VT, typename HT>
 constexpr bool kFp64Pair = std::is_same<VT, double2>::value && std::is_same<HT, double2>::value;
@@ -927,18 +940,18 @@ int launch_step(BlockRun &r, bool first, bool dots, const VT *X, VT *Y,
                 int M, int nprev, int packed) {
 	tbtkx_context *h = r.h;
 	const HT *v = first ? h->v1.as<HT>() : h->v2.as<HT>();
-	if (std::is_same<HT, float2>::value)
+	if (kHt8<HT>)
 		v = first ? (const HT *)h->v1f.ptr : (const HT *)h->v2f.ptr;
 	const uint32_t *rp = h->rowptr.as<uint32_t>(), *cl = h->col.as<uint32_t>();
 	// FP32 H and vectors accumulate rows in FP32; anything with FP64 in it
-	// accumulates in FP64.
+	// (a real H's FP64 entries included) accumulates in FP64.
 	using AT = typename std::conditional<std::is_same<HT, float2>::value &&
 	                                         std::is_same<VT, float2>::value,
 	                                     float, double>::type;
 	if (h->ell_W > 0 && h->use_ell && !r.exact) {
 		const uint32_t *ec = h->ell_col.as<uint32_t>();
 		const HT *ev;
-		if (std::is_same<HT, float2>::value)
+		if (kHt8<HT>)
 			ev = first ? (const HT *)h->ell_v1f.ptr : (const HT *)h->ell_v2f.ptr;
 		else
 			ev = first ? (const HT *)h->ell_v1d.ptr : (const HT *)h->ell_v2d.ptr;
@@ -1509,6 +1522,8 @@ int block_entry(tbtkx_context *h, int total, const int64_t *sites, int mode,
 				rc = tune_shape<double2, double2>(h, cols, flags, &tb, &ts, &tsh, &trm);
 			else if (flags & TBTKX_H_FP64)
 				rc = tune_shape<float2, double2>(h, cols, flags, &tb, &ts, &tsh, &trm);
+			else if (h->h_real)
+				rc = tune_shape<float2, RealD>(h, cols, flags, &tb, &ts, &tsh, &trm);
 			else
 				rc = tune_shape<float2, float2>(h, cols, flags, &tb, &ts, &tsh, &trm);
 			if (rc) return rc;
@@ -1529,6 +1544,10 @@ int block_entry(tbtkx_context *h, int total, const int64_t *sites, int mode,
 	if (flags & TBTKX_H_FP64)
 		return dispatch_width<float2, double2>(h, B, total, sites, mode, seed, M,
 		                                       flags, packed, mu);
+	// A real H's 8-byte copies hold FP64 values (tbtkx_set_csr, tbtkx_set_lattice).
+	if (h->h_real)
+		return dispatch_width<float2, RealD>(h, B, total, sites, mode, seed, M, flags,
+		                                     packed, mu);
 	return dispatch_width<float2, float2>(h, B, total, sites, mode, seed, M, flags,
 	                                      packed, mu);
 }

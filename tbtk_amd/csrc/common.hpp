@@ -193,6 +193,7 @@ struct tbtkx_context {
 	int64_t mm_gen = 0;                       // bumped by every tbtkx_multi_set
 	std::vector<int64_t> mm_N, mm_e0;         // rows, first ELL entry per model
 	std::vector<int> mm_Wm;                   // ELL slots per row, per model
+	std::vector<int> mm_realm;                // per model: H real (FP64 ELL values)
 	std::vector<int64_t> mm_band;             // max |col - row| per model
 	tbtkx::DevBuf mm_ecol, mm_ev1, mm_ev2;
 	tbtkx::DevBuf mm_desc, mm_tab, mm_x, mm_y, mm_part, mm_seg, mm_D, mm_cols;

@@ -252,15 +252,24 @@ int tbtkx_set_csr(tbtkx_handle h, int64_t N, int64_t Z, const uint32_t *rowptr,
 	std::vector<double> v1(2 * Zp, 0.0), v2(2 * Zp, 0.0);
 	scale_complex(val, v1.data(), Z, 1 / scale);
 	scale_complex(v1.data(), v2.data(), Z, 2);
-	std::vector<float> v1f(2 * Zp, 0.0f), v2f(2 * Zp, 0.0f);
-	for (int64_t k = 0; k < 2 * Z; k++) {
-		v1f[k] = (float)v1[k];
-		v2f[k] = (float)v2[k];
-	}
-	std::vector<uint32_t> blocks = make_row_blocks(N, rowptr);
 	bool real = true;
 	for (int64_t k = 0; k < Z && real; k++)
 		if (val[2 * k + 1] != 0.0) real = false;
+	// The 8-byte copies of the batched paths: FP32 complex values, or for a
+	// real H one FP64 value each.
+	std::vector<float> v1f(2 * Zp, 0.0f), v2f(2 * Zp, 0.0f);
+	for (int64_t k = 0; k < Z; k++) {
+		if (real) {
+			memcpy(&v1f[2 * k], &v1[2 * k], 8);
+			memcpy(&v2f[2 * k], &v2[2 * k], 8);
+		} else {
+			for (int c = 0; c < 2; c++) {
+				v1f[2 * k + c] = (float)v1[2 * k + c];
+				v2f[2 * k + c] = (float)v2[2 * k + c];
+			}
+		}
+	}
+	std::vector<uint32_t> blocks = make_row_blocks(N, rowptr);
 
 	if ((rc = h->rowptr.reserve((N + 1) * 4))) return rc;
 	if ((rc = h->col.reserve(Zp * 4))) return rc;

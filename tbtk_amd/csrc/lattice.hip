@@ -175,11 +175,16 @@ __global__ __launch_bounds__(kThreads) void lattice_fill(
 		col[k0 + k] = cols[k];
 		v1[k0 + k] = a1[k];
 		v2[k0 + k] = a2[k];
-		v1f[k0 + k] = make_float2((float)a1[k].x, (float)a1[k].y);
-		v2f[k0 + k] = make_float2((float)a2[k].x, (float)a2[k].y);
+		// The 8-byte copies: FP32 complex, or for a real H (v1r set) one
+		// FP64 value each.
 		if (v1r) {
 			v1r[k0 + k] = a1[k].x;
 			v2r[k0 + k] = a2[k].x;
+			reinterpret_cast<double *>(v1f)[k0 + k] = a1[k].x;
+			reinterpret_cast<double *>(v2f)[k0 + k] = a2[k].x;
+		} else {
+			v1f[k0 + k] = make_float2((float)a1[k].x, (float)a1[k].y);
+			v2f[k0 + k] = make_float2((float)a2[k].x, (float)a2[k].y);
 		}
 		if ((int64_t)cols[k] == r && diag < 0) diag = k;
 	}
@@ -191,8 +196,13 @@ __global__ __launch_bounds__(kThreads) void lattice_fill(
 		ecol[e] = c;
 		e1d[e] = x1;
 		e2d[e] = x2;
-		e1f[e] = make_float2((float)x1.x, (float)x1.y);
-		e2f[e] = make_float2((float)x2.x, (float)x2.y);
+		if (v1r) {
+			reinterpret_cast<double *>(e1f)[e] = x1.x;
+			reinterpret_cast<double *>(e2f)[e] = x2.x;
+		} else {
+			e1f[e] = make_float2((float)x1.x, (float)x1.y);
+			e2f[e] = make_float2((float)x2.x, (float)x2.y);
+		}
 	};
 	const double2 zero = make_double2(0.0, 0.0);
 	if (diag >= 0) put(cols[diag], a1[diag], a2[diag]);
@@ -503,6 +513,7 @@ int tbtkx_multi_set_lattices(tbtkx_handle h, int K, const tbtkx_lattice *lats,
 	              "tbtkx_multi_set_lattices: need 1..65536 models and arrays");
 	std::vector<LatDev> g(K);
 	std::vector<int64_t> N(K);
+	std::vector<int> realm(K, 1);   // per model: every value of its H is real
 	bool real = true;
 	int64_t rows = 0;
 	for (int m = 0; m < K; m++) {
@@ -515,6 +526,7 @@ int tbtkx_multi_set_lattices(tbtkx_handle h, int K, const tbtkx_lattice *lats,
 		                       &max_row, &rm);
 		if (rc) return rc;
 		real = real && rm;
+		realm[m] = rm ? 1 : 0;
 		rows += N[m];
 	}
 	TBTKX_REQUIRE(rows * 8 < ((int64_t)1 << 40), TBTKX_ERR_ARG,
@@ -555,7 +567,7 @@ int tbtkx_multi_set_lattices(tbtkx_handle h, int K, const tbtkx_lattice *lats,
 		    g[m], h->lat_terms.as<tbtkx_lattice_term>(),
 		    g[m].has_onsite ? h->lat_onsite.as<double2>() : nullptr, N[m], 1 / scales[m], Wm[m],
 		    h->mm_ecol.as<uint32_t>() + e0, h->mm_ev1.as<float2>() + e0,
-		    h->mm_ev2.as<float2>() + e0, real ? 1 : 0);
+		    h->mm_ev2.as<float2>() + e0, realm[m]);
 		TBTKX_CHECK_HIP(hipGetLastError());
 		// The staging buffers are reused by the next model.
 		TBTKX_CHECK_HIP(hipStreamSynchronize(h->stream));
@@ -567,6 +579,7 @@ int tbtkx_multi_set_lattices(tbtkx_handle h, int K, const tbtkx_lattice *lats,
 	h->mm_K = K;
 	h->mm_W = W;
 	h->mm_real = real;
+	h->mm_realm = realm;
 	h->mm_gen++;
 	for (auto &k : h->mm_tune_key) k = -1;
 	return TBTKX_OK;

@@ -11,15 +11,15 @@
 // Storage: H in row-major ELL (W_m slots per row: the model's longest row
 // rounded up to 4, 5, 6 or 8; scaled by 1/s_m and 2/s_m exactly as the
 // single-model path scales its copies), 8 bytes per entry: FP32 complex
-// values, or, when every model of the batch is real, one FP64 value (FP32
-// rounding of H moves its eigenvalues by ~1e-7 of the band, which thousands
-// of moments resolve). FP32 complex vectors, row-major per model (N_m rows x
+// values, or, for a model whose H is real, one FP64 value (FP32 rounding
+// of H moves its eigenvalues by ~1e-7 of the band, which thousands of
+// moments resolve; a batch mixes both forms, one per model). FP32 complex vectors, row-major per model (N_m rows x
 // B_m columns). Rows are summed with the same fused multiply-add order as
 // the single-model step, in FP32 (complex H) or FP64 (real H, rounded once
 // to FP32 on store), so the recurrence vectors are the ones
-// tbtkx_moments_block produces for the same model and sites with FP32 H,
-// respectively FP64 H (TBTKX_H_FP64). Moments are FP64: gather mode reads
-// mu_n[b] = Phi_n[site_b][b] each step; doubling mode fuses
+// tbtkx_moments_block produces for the same model and sites (it keeps a
+// complex H in FP32 and a real H in FP64 the same way). Moments are FP64:
+// gather mode reads mu_n[b] = Phi_n[site_b][b] each step; doubling mode fuses
 // D0_n = <Phi_n|Phi_n> and D1_n = <Phi_n+1|Phi_n> into the step (FP64 per
 // lane, then ordered reductions) and uses mu_2n = 2 D0_n - mu_0,
 // mu_2n+1 = 2 D1_n - mu_1 (Weisse et al., Rev. Mod. Phys. 78, 275, eq. 31).
@@ -80,6 +80,7 @@ struct MDesc {
 	int wp;          // stored columns: w, or w rounded up to even for pairs
 	int pairs;       // 1: a lane moves two columns (16 B), else one
 	int W;           // ELL slots per row of this model (4, 5, 6 or 8)
+	int realh;       // the model's H is real: FP64 ELL values, FP64 row sums
 };
 
 struct GCol {
@@ -461,6 +462,15 @@ __device__ __forceinline__ void model_rows(
     int b, int64_t tr, int lane, const uint32_t *__restrict__ ecol,
     const float2 *__restrict__ ev, const float2 *__restrict__ X, float2 *__restrict__ Y,
     int packed, double (&dd)[2][4]) {
+	// REALH: every model of the batch is real. Otherwise each model takes
+	// the path of its own H (a real model's entries hold FP64 values).
+	if constexpr (!REALH) {
+		if (d.realh) {
+			model_rows<WMAX, FIRST, DOTS, true>(d, t, tend, nwx, toff_m, G, Q, qc, act, b, tr,
+			                                    lane, ecol, ev, X, Y, packed, dd);
+			return;
+		}
+	}
 #define TBTKX_MR(WW)                                                                 \
 	model_rows_w<WW, FIRST, DOTS, REALH>(d, t, tend, nwx, toff_m, G, Q, qc, act, b, tr, lane, \
 	                                     ecol, ev, X, Y, packed, dd)
@@ -750,6 +760,7 @@ int make_plan(tbtkx_context *h, const std::vector<int> &width, const std::vector
 		d.N = h->mm_N[m];
 		d.e0 = h->mm_e0[m];
 		d.W = h->mm_Wm[m];
+		d.realh = h->mm_realm[m];
 		d.w = width[m];
 		d.pairs = use_pairs(d.w, d.W) ? 1 : 0;
 		d.wp = d.pairs ? d.w + (d.w & 1) : d.w;
@@ -975,6 +986,7 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 	uint32_t maxlen = 0;
 	bool real = true;
 	std::vector<int> Wm(K, 4);   // ELL slots per row, per model
+	std::vector<int> realm(K, 1);   // per model: every value of its H is real
 	for (int m = 0; m < K; m++) {
 		const int64_t N = num_rows[m], Z = nnz[m];
 		const std::string tag = "tbtkx_multi_set: model " + std::to_string(m) + ": ";
@@ -991,7 +1003,10 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 		for (int64_t k = 0; k < Z; k++) {
 			TBTKX_REQUIRE(col[z0 + k] < (uint64_t)N, TBTKX_ERR_ARG,
 			              tag + "column index out of range");
-			if (val[2 * (z0 + k) + 1] != 0.0) real = false;
+			if (val[2 * (z0 + k) + 1] != 0.0) {
+				real = false;
+				realm[m] = 0;
+			}
 		}
 		uint32_t ml = 0;
 		for (int64_t r = 0; r < N; r++) {
@@ -1045,7 +1060,7 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 				const int64_t k = u < n ? slots[u] : -1;
 				if (k >= 0) {
 					ec[base + u] = col[z0 + k];
-					if (real) {
+					if (realm[m]) {
 						// One FP64 value in the entry's 8 bytes.
 						memcpy(&e1[2 * (base + u)], &v1[2 * k], 8);
 						memcpy(&e2[2 * (base + u)], &v2[2 * k], 8);
@@ -1084,6 +1099,7 @@ int tbtkx_multi_set(tbtkx_handle h, int K, const int64_t *num_rows,
 	h->mm_K = K;
 	h->mm_W = W;
 	h->mm_real = real;
+	h->mm_realm = realm;
 	h->mm_gen++;
 	for (auto &k : h->mm_tune_key) k = -1;
 	return TBTKX_OK;
@@ -1166,6 +1182,7 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 			d.N = h->mm_N[m];
 			d.e0 = h->mm_e0[m];
 			d.W = h->mm_Wm[m];
+		d.realh = h->mm_realm[m];
 			d.vec0 = vec;
 			d.w = width[m];
 			d.wp = width[m];
