@@ -472,19 +472,23 @@ def uneven_batch(eng, graph=True, moments_div=1, tile_rows=0):
     }
 
 
-def stochastic_dos(eng):
+def stochastic_dos(eng, moments_div=1):
     """128^3 cubic lattice (t=1, disorder U[-1,1], seed 5), R=512 random-phase
-    start vectors, M=4096, FP32 storage, FP64 moment sums, doubling."""
+    start vectors, M=4096, FP32 storage, FP64 moment sums, doubling. For
+    counter runs, moments_div > 1 divides M (per-launch figures unchanged,
+    fewer launches)."""
     spec = L.config_cubic_spec(128)
     s = eng.lattice_gershgorin(spec) * 1.05
     eng.set_lattice(spec, s)
     N, Z = eng.csr_size()
-    R, M, seed = 512, 4096, 17
+    R, M, seed = 512, max(64, 4096 // moments_div), 17
     eng.set_option(X.OPT_BLOCK_WIDTH, 0)
     eng.moments_trace(R, seed, 8)  # allocations and the shape autotuner
+    log("dos: shape chosen, timed run")
     t0 = time.perf_counter()
     mu = eng.moments_trace(R, seed, M)
     wall = time.perf_counter() - t0
+    log(f"dos: timed run done ({wall:.2f} s), FP32/FP64 error reference")
     ms, launches = eng.last_timing()
     Bw = eng.get_option(X.OPT_BLOCK_WIDTH)
     shape = {"block_width": Bw, "strip_rows": eng.get_option(X.OPT_STRIP_ROWS),
@@ -503,7 +507,9 @@ def stochastic_dos(eng):
     blocks = (R + Bw - 1) // Bw
     step_bytes = 12 * Z + 4 * N + 24 * Bw * N
     per_step = ms * 1e-3 / launches
+    extra = {"moments_divided_by": moments_div} if moments_div > 1 else {}
     return {
+        **extra,
         "workload": (f"cubic 128^3 (2,097,152 states, t=1, disorder U[-1,1] seed 5), R={R} "
                      f"random-phase vectors (seed {seed}), M={M}, s=Gershgorin*1.05={s:.6f} "
                      "(H and bound built on the device)"),
@@ -624,6 +630,9 @@ def main():
     ap.add_argument("--uneven-moments-div", type=int, default=1,
                     help="uneven part, for counter runs: divide every model's moment count "
                          "by this (at least 50 moments)")
+    ap.add_argument("--dos-moments-div", type=int, default=1,
+                    help="stochastic DOS part, for counter runs: divide M by this "
+                         "(at least 64 moments)")
     ap.add_argument("--uneven-tile-rows", type=int, default=0,
                     help="uneven part: fixed row groups per tile instead of the autotuned "
                          "launch shape")
@@ -783,7 +792,7 @@ def main():
                                            tile_rows=args.uneven_tile_rows)
     if "dos" in parts:
         log("stochastic DOS (128^3 cubic, R=512, M=4096)")
-        out["stochastic_dos"] = stochastic_dos(eng)
+        out["stochastic_dos"] = stochastic_dos(eng, moments_div=args.dos_moments_div)
     if "mfma" in parts:
         log("MFMA contraction (1e5 x 4096 x 8192)")
         out["contraction_mfma"] = contraction_mfma(eng)

@@ -49,7 +49,9 @@ enum {
 
 /* Flags for the batched recurrences. */
 enum {
-	TBTKX_H_FP64 = 1 << 0,     /* keep H in FP64 (default: FP32 complex) */
+	TBTKX_H_FP64 = 1 << 0,     /* H as FP64 complex values (default: FP32
+	                              complex values, or one FP64 value per
+	                              entry when every value of H is real) */
 	TBTKX_DOUBLING = 1 << 1,   /* diagonal moments from the doubling
 	                              identities mu_2n = 2<n|n> - mu_0,
 	                              mu_2n+1 = 2<n+1|n> - mu_1 (M/2 steps) */

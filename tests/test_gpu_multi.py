@@ -3,10 +3,9 @@
 Checks, per model of a small uneven batch:
 - gather mode is bitwise equal to the single-model batched path
   (tbtkx_moments_block, same recurrence arithmetic; mu_n is read from the
-  vectors directly): FP32 H and FP32 row sums for a batch with a complex H,
-  FP64 H and FP64 row sums (TBTKX_H_FP64 on the single-model path) for an
-  all-real batch, whose ELL entries hold one FP64 value each; FP32 vectors
-  in both;
+  vectors directly): both keep a complex H as FP32 values with FP32 row sums
+  and a real H as one FP64 value per ELL entry with FP64 row sums, per model
+  (the "complex" batch mixes both forms); FP32 vectors in both;
 - doubling mode agrees with the single-model doubling path to FP64 rounding of
   the dot-product sums (1e-12 relative);
 - both agree with the FP64 oracle (oracle/kpm_oracle.c, kpm_moments) within
@@ -57,11 +56,10 @@ def upload(engine, batch):
     return csrs, scales
 
 
-def single_model(engine, csr, s, sites, M, flags, real_batch=False):
-    """The single-model batched path with the uneven batch's arithmetic: FP64
-    H for an all-real batch, FP32 H otherwise."""
+def single_model(engine, csr, s, sites, M, flags):
+    """The single-model batched path on one model of the batch."""
     engine.set_csr(csr, s)
-    return engine.moments_block(sites, M, flags | (X.H_FP64 if real_batch else 0))
+    return engine.moments_block(sites, M, flags)
 
 
 @pytest.mark.parametrize("which", ["complex", "real"])
@@ -70,7 +68,7 @@ def test_multi_gather_bitwise_vs_block(engine, which):
     csrs, scales = upload(engine, batch)
     got = engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], 0)
     for (m, sites, M), csr, s, g in zip(batch, csrs, scales, got):
-        ref = single_model(engine, csr, s, sites, M, 0, which == "real")
+        ref = single_model(engine, csr, s, sites, M, 0)
         assert g.shape == ref.shape
         assert np.array_equal(g, ref), (m.name, m.basis_size, np.abs(g - ref).max())
 
@@ -81,7 +79,7 @@ def test_multi_doubling_vs_block_and_oracle(engine, oracle_mod, which):
     csrs, scales = upload(engine, batch)
     got = engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], X.DOUBLING)
     for (m, sites, M), csr, s, g in zip(batch, csrs, scales, got):
-        ref = single_model(engine, csr, s, sites, M, X.DOUBLING, which == "real")
+        ref = single_model(engine, csr, s, sites, M, X.DOUBLING)
         assert np.abs(g - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), m.name
         exact = np.array([oracle_mod.moments(csr, s, b, [b], M)[0] for b in sites])
         assert np.abs(g - exact).max() < 2e-5, (m.name, np.abs(g - exact).max())
@@ -170,7 +168,7 @@ def test_multi_mixed_ell_widths(engine, oracle_mod, which):
     got = engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], 0)
     dbl = engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], X.DOUBLING)
     for (m, sites, M), csr, s, g, d in zip(batch, csrs, scales, got, dbl):
-        ref = single_model(engine, csr, s, sites, M, 0, which == "real")
+        ref = single_model(engine, csr, s, sites, M, 0)
         assert np.array_equal(g, ref), (m.name, np.abs(g - ref).max())
         exact = np.array([oracle_mod.moments(csr, s, b, [b], M)[0] for b in sites])
         assert np.abs(d - exact).max() < 2e-5, (m.name, np.abs(d - exact).max())
