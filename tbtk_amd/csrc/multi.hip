@@ -568,14 +568,15 @@ __attribute__((amdgpu_waves_per_eu(WMAX <= 5 ? 4 : 1))) void multi_step(
 }
 
 // Pass 1: segment s = (model, slot_lo, slot_hi) of <= 64 slots; one
-// wavefront per (segment, which), lane = column.
+// wavefront per (segment, which), lane = column (the model's w columns).
 __global__ __launch_bounds__(kThreads) void multi_reduce_slots(
-    const int *__restrict__ seg, int nseg, const double2 *__restrict__ part,
-    double2 *__restrict__ segsum) {
+    const MDesc *__restrict__ md, const int *__restrict__ seg, int nseg,
+    const double2 *__restrict__ part, double2 *__restrict__ segsum) {
 	const int lane = threadIdx.x & 63;
 	const int gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
 	if (gw >= 2 * nseg) return;
 	const int s = gw >> 1, which = gw & 1;
+	if (lane >= md[seg[3 * s]].w) return;
 	const int lo = seg[3 * s + 1], hi = seg[3 * s + 2];
 	// Slots in ascending order; eight loads are issued before their adds.
 	double2 acc = make_double2(0.0, 0.0);
@@ -619,6 +620,7 @@ __global__ __launch_bounds__(kThreads) void multi_reduce_models(
 
 struct Phase {
 	int kact;          // active models (a prefix of the sorted order)
+	int nwg;           // workgroups of the phase's step launches
 	int n_lo, n_hi;    // steps of the phase
 	int64_t toff, xpart, pbase, pstart, mseg, seg;   // offsets into the table buffer
 	int nseg, nslots;
@@ -650,7 +652,10 @@ bool use_pairs(int w, int W) {
 // Tables of one phase: tile offsets, the byte-balanced cut of the tile list
 // into 8 XCD parts, the pieces (model x part) with the base of their dot
 // slots (model-major, so a model's slots are contiguous) and the slot groups
-// of each model for the ordered reduction.
+// of each model for the ordered reduction. nwx: wavefronts per XCD of the
+// persistent grid (p.nwg workgroups), or 0 for one tile per wavefront: the
+// grid then has as many wavefronts per XCD as the longest part has tiles,
+// and the hardware's in-order dispatch walks each part front to back.
 void plan_phase(const Plan &p, int kact, int nwx, Phase &ph, std::vector<int> &tab) {
 	std::vector<int> toff(kact + 1, 0);
 	std::vector<double> cost(kact), cum(kact + 1, 0.0);
@@ -672,6 +677,14 @@ void plan_phase(const Plan &p, int kact, int nwx, Phase &ph, std::vector<int> &t
 		int tcut = toff[k] + (int)((target - cum[k]) / per_tile);
 		tcut = std::min(std::max(tcut, toff[k]), toff[k + 1]);
 		xpart[i] = std::max(tcut, xpart[i - 1]);
+	}
+	if (nwx > 0) {
+		ph.nwg = p.nwg;
+	} else {
+		int maxpart = 1;
+		for (int x = 0; x < 8; x++) maxpart = std::max(maxpart, xpart[x + 1] - xpart[x]);
+		nwx = (maxpart + kWaves - 1) / kWaves * kWaves;
+		ph.nwg = 8 * (nwx / kWaves);
 	}
 	// Pieces and slots.
 	std::vector<int> pbase(kact * 8, -1), pstart(kact * 8, 0), mslot(kact + 1, 0);
@@ -790,8 +803,13 @@ int make_plan(tbtkx_context *h, const std::vector<int> &width, const std::vector
 	p.vec_elems = vec;
 	p.d_elems = dof;
 	p.Smax = p.S[0];
-	int rc = grid_size(h, dots, share, &p.nwg);
-	if (rc) return rc;
+	// share > 0: persistent grid of occupancy x CUs / share workgroups;
+	// share == 0: one tile per wavefront (per-phase grids).
+	p.nwg = 0;
+	if (share > 0) {
+		int rc = grid_size(h, dots, share, &p.nwg);
+		if (rc) return rc;
+	}
 	const int nwx = p.nwg / 8 * kWaves;
 	p.phases.clear();
 	p.tab.clear();
@@ -819,7 +837,7 @@ void launch_step_w(tbtkx_context *h, const Plan &p, const Phase &ph, bool first,
 	const float2 *ev = first ? h->mm_ev1.as<float2>() : h->mm_ev2.as<float2>();
 	double2 *part = h->mm_part.as<double2>();
 #define TBTKX_MS(F, D)                                                          \
-	multi_step<W, F, D, REALH><<<p.nwg, kThreads, 0, h->stream>>>(            \
+	multi_step<W, F, D, REALH><<<ph.nwg, kThreads, 0, h->stream>>>(           \
 	    md, tab + ph.toff, ph.kact, tab + ph.xpart, tab + ph.pbase,            \
 	    tab + ph.pstart, p.G, h->mm_ecol.as<uint32_t>(), ev, X, Y, part, h->mm_packed)
 	if (first && dots) TBTKX_MS(true, true);
@@ -867,6 +885,16 @@ int reserve_plan(tbtkx_context *h, const Plan &p, int ncols, int64_t mu_elems) {
 	return TBTKX_OK;
 }
 
+// The two ordered reduction passes of the doubling dots of step n.
+void launch_reduce(tbtkx_context *h, const Phase &ph, double2 *D, int n) {
+	const int *tab = h->mm_tab.as<int>();
+	multi_reduce_slots<<<(2 * ph.nseg + kWaves - 1) / kWaves, kThreads, 0, h->stream>>>(
+	    h->mm_desc.as<MDesc>(), tab + ph.seg, ph.nseg, h->mm_part.as<double2>(),
+	    h->mm_seg.as<double2>());
+	multi_reduce_models<<<(2 * ph.kact + kWaves - 1) / kWaves, kThreads, 0, h->stream>>>(
+	    h->mm_desc.as<MDesc>(), tab + ph.mseg, ph.kact, h->mm_seg.as<double2>(), D, n);
+}
+
 // Everything from the initial vectors to the last step, on h->stream.
 // Enqueue steps s_lo .. s_hi - 1 of the run (all phases' steps counted in
 // order; the vectors' roles alternate with the step's parity), with the
@@ -892,15 +920,7 @@ int enqueue_run(tbtkx_context *h, const Plan &p, bool dots, int ncols, int64_t s
 				    cols, ncols, X, D, n);
 			launch_step(h, p, ph, n == 0, dots, X, Y);
 			launches++;
-			if (dots) {
-				multi_reduce_slots<<<(2 * ph.nseg + kWaves - 1) / kWaves, kThreads, 0,
-				                     h->stream>>>(tab + ph.seg, ph.nseg,
-				                                  h->mm_part.as<double2>(),
-				                                  h->mm_seg.as<double2>());
-				multi_reduce_models<<<(2 * ph.kact + kWaves - 1) / kWaves, kThreads, 0,
-				                      h->stream>>>(h->mm_desc.as<MDesc>(), tab + ph.mseg,
-				                                   ph.kact, h->mm_seg.as<double2>(), D, n);
-			}
+			if (dots) launch_reduce(h, ph, D, n);
 			float2 *tmp = X;
 			X = Y;
 			Y = tmp;
@@ -926,16 +946,19 @@ int64_t run_steps(const Plan &p) {
 }
 
 // Autotuner: time a few steps of the first phase for each (tile groups, grid
-// share) candidate on the actual batch and keep the fastest.
+// share) candidate on the actual batch and keep the fastest: persistent grids
+// (share 1 or 2) with tiles of 2-16 row groups, and one tile per wavefront
+// (share 0) with tiles of 8-64 row groups.
 int autotune(tbtkx_context *h, const std::vector<int> &width, const std::vector<int> &S,
              bool dots, int ncols, int64_t mu_elems) {
-	static const int kG[] = {2, 4, 8, 16};
-	static const int kShare[] = {1, 2};
+	static const int kCand[][2] = {{1, 2}, {1, 4}, {1, 8}, {1, 16}, {2, 2},  {2, 4},
+	                               {2, 8}, {2, 16}, {0, 8}, {0, 16}, {0, 32}, {0, 64}};
 	constexpr int kTrial = 4;
 	float best = 0;
 	int bestG = 8, bestShare = 1;
-	for (int share : kShare) {
-		for (int G : kG) {
+	for (const auto &cand : kCand) {
+		const int share = cand[0], G = cand[1];
+		{
 			Plan p;
 			int rc = make_plan(h, width, S, dots, G, share, p);
 			if (rc) return rc;
@@ -947,6 +970,7 @@ int autotune(tbtkx_context *h, const std::vector<int> &width, const std::vector<
 			TBTKX_CHECK_HIP(hipEventRecord(h->ev_start, h->stream));
 			for (int i = 0; i < kTrial; i++) {
 				launch_step(h, p, ph, false, dots, Y, X);
+				if (dots) launch_reduce(h, ph, h->mm_D.as<double2>(), 0);
 				float2 *tmp = X;
 				X = Y;
 				Y = tmp;

@@ -26,7 +26,8 @@
  * Additions (no reference counterpart):
  *  - setUseBlockRecurrence: diagonal queries of the property extractor
  *    (LDOS, DOS) run as blocks of start vectors through one SpMM recurrence
- *    (FP32 storage, FP64 accumulation, doubling identities by default);
+ *    (FP32 vectors, a complex H in FP32 and a real H in FP64, FP64
+ *    accumulation, doubling identities by default);
  *  - calculateCoefficientsBlock, calculateTraceCoefficients,
  *    generateGreensFunctions (many coefficient vectors in one contraction);
  *  - setDamping(Lorentz | Jackson).

@@ -255,20 +255,28 @@ def test_multi_reports_launch_shape(engine):
         engine.set_option(X.OPT_TILE_ROWS, 0)
 
 
-def test_multi_real_high_moment_count_ldos_error(engine, oracle_mod):
+@pytest.mark.parametrize("path", ["uneven", "block"])
+def test_real_high_moment_count_ldos_error(engine, oracle_mod, path):
     """A real honeycomb ribbon (t = 2.7, not exact in FP32) at 6000 moments:
-    the uneven batch's LDOS (Jackson damping, the FP64 contraction) is within
-    1e-5 of the oracle's relative to its maximum. The batch keeps a real H in
-    FP64 (FP32 rounding of H shifts the eigenvalues by ~1e-7 of the band,
-    which at this resolution moves the LDOS by ~1e-5)."""
+    the LDOS (Jackson damping, the FP64 contraction) of the uneven batch and
+    of the single-model batched step is within 1e-5 of the oracle's relative
+    to its maximum, in gather and doubling mode. Both keep a real H in FP64
+    (FP32 rounding of H shifts the eigenvalues by ~1e-7 of the band, which
+    at this resolution moves the LDOS by ~1e-5)."""
     m = L.honeycomb_ribbon(30, 40, t=2.7, disorder=0.5, seed=31)
     csr = m.csr()
     s = m.gershgorin_bound() * 1.05
     sites = np.array([7, m.basis_size // 2])
     M, E = 6000, 2000
-    engine.multi_set([csr], [s])
+    if path == "uneven":
+        engine.multi_set([csr], [s])
+    else:
+        engine.set_csr(csr, s)
     for flags in (0, X.DOUBLING):
-        got = engine.multi_moments([sites], [M], flags)[0]
+        if path == "uneven":
+            got = engine.multi_moments([sites], [M], flags)[0]
+        else:
+            got = engine.moments_block(sites, M, flags)
         engine.lut_generate(M, E, -0.99 * s, 0.99 * s, s)
         engine.set_damping(X.DAMP_JACKSON)
         try:
@@ -277,8 +285,8 @@ def test_multi_real_high_moment_count_ldos_error(engine, oracle_mod):
                 l_ref = kpm.ldos_from_gf(engine.contract(exact, X.NONPRINCIPAL))
                 l_got = kpm.ldos_from_gf(engine.contract(got[k], X.NONPRINCIPAL))
                 err = np.abs(l_got - l_ref).max() / np.abs(l_ref).max()
-                print(f"flags={flags} site={b}: LDOS error {err:.2e}", flush=True)
-                assert err <= 1e-5, (flags, int(b), err)
+                print(f"{path} flags={flags} site={b}: LDOS error {err:.2e}", flush=True)
+                assert err <= 1e-5, (path, flags, int(b), err)
         finally:
             engine.set_damping(X.DAMP_NONE)
             engine.lut_destroy()
