@@ -607,11 +607,21 @@ __global__ __launch_bounds__(kThreads) void multi_reduce_models(
 	const int k = gw >> 1, which = gw & 1;
 	const MDesc d = md[k];
 	if (lane >= d.w) return;
+	// Segments in ascending order; eight loads are issued before their adds.
 	double2 acc = make_double2(0.0, 0.0);
-	for (int s = mseg[k]; s < mseg[k + 1]; s++) {
-		const double2 v = segsum[((int64_t)s * 2 + which) * kMaxWidth + lane];
-		acc.x += v.x;
-		acc.y += v.y;
+	const int lo = mseg[k], hi = mseg[k + 1];
+	for (int s0 = lo; s0 < hi; s0 += 8) {
+		double2 v[8];
+#pragma unroll
+		for (int i = 0; i < 8; i++)
+			if (s0 + i < hi) v[i] = segsum[((int64_t)(s0 + i) * 2 + which) * kMaxWidth + lane];
+#pragma unroll
+		for (int i = 0; i < 8; i++) {
+			if (s0 + i < hi) {
+				acc.x += v[i].x;
+				acc.y += v[i].y;
+			}
+		}
 	}
 	D[d.dof + ((int64_t)n * 2 + which) * d.w + lane] = acc;
 }
