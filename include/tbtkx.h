@@ -296,13 +296,13 @@ int tbtkx_multi_set_lattices(tbtkx_handle handle, int num_models,
                              const tbtkx_lattice *lattices, const double *const *onsite,
                              const double *scales);
 
-/* Diagonal moments of every uploaded model in one persistent launch per
+/* Diagonal moments of every uploaded model in one launch per
  * recurrence step: model m gets num_sites[m] (1..64) unit start vectors at
  * its local basis indices (consecutive in sites) and num_moments[m] moments.
  * out holds, model after model, num_sites[m] x num_moments[m] complex values
  * (site-major), as tbtkx_moments_block would give per model. FP32 storage,
- * FP64 moments; flags may be TBTKX_DOUBLING. The launch shape (tile size,
- * grid share) is autotuned on first use unless TBTKX_OPT_AUTOTUNE is 0 or
+ * FP64 moments; flags may be TBTKX_DOUBLING. The launch shape (tile size;
+ * a persistent grid or one tile per wavefront) is autotuned on first use unless TBTKX_OPT_AUTOTUNE is 0 or
  * TBTKX_OPT_TILE_ROWS fixes it; TBTKX_OPT_GRAPH replays the whole run as a
  * hipGraph. A model's vectors are addressed with 32-bit byte offsets: a
  * model with num_rows x (columns + 1) x 8 B >= 4 GiB is refused with
@@ -411,7 +411,9 @@ int tbtkx_set_option(tbtkx_handle handle, int option, int64_t value);
 /* Effective value of an option: TBTKX_OPT_BLOCK_WIDTH, TBTKX_OPT_STRIP_ROWS
  * and TBTKX_OPT_GRID_SHARE give what the last batched call used (the
  * autotuner's choice when they are 0; the uneven batch sets the grid share
- * too); TBTKX_OPT_TILE_ROWS, when 0, gives the row groups per tile of the
+ * too: 0 there means one tile per wavefront, a grid as large as the tile
+ * list, else a persistent grid of occupancy x CUs / share workgroups);
+ * TBTKX_OPT_TILE_ROWS, when 0, gives the row groups per tile of the
  * last uneven-batch call; TBTKX_OPT_ELL gives 1 only when the
  * ELL copy exists and is enabled; TBTKX_OPT_REAL gives 1 when the last
  * recurrence call (plain, batched or uneven batch) ran in real arithmetic. */

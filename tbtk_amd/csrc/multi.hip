@@ -1,4 +1,4 @@
-// Uneven batch: many independent models in one persistent launch per step.
+// Uneven batch: many independent models in one launch per step.
 //
 // K models, each with its own Hamiltonian H_m, scale factor s_m, B_m <= 64
 // unit start vectors (LDOS sites) and M_m moments, advance through the
@@ -37,9 +37,10 @@
 // Q = floor(64 / lpr) consecutive rows at once, lane l -> row slot
 // q = l / lpr, first column (l % lpr) times the columns per lane; lanes
 // l >= Q lpr are idle (they mirror a valid row of the same group). A tile is
-// G row groups (G Q rows). Each wavefront of the persistent grid walks a
-// contiguous range of the phase's tile list, cut so that every wavefront gets
-// the same number of algorithmic bytes.
+// G row groups (G Q rows). The phase's tile list is cut into 8 XCD parts of
+// equal algorithmic bytes; each XCD's wavefronts take the part's tiles
+// round-robin, either in a persistent grid (several tiles each) or one tile
+// each in a grid as large as the longest part.
 //
 // Dot partials: a wavefront sums its lanes' FP64 partials per column when it
 // leaves a model (lanes of one column, ascending row slot) and writes them to
@@ -496,7 +497,11 @@ __device__ __forceinline__ void model_rows(
 // (toff[k]: first tile of sorted model k) and the phase's list is cut into
 // 8 byte-balanced parts, one per XCD (xpart). Inside a part the XCD's
 // wavefronts take consecutive tiles round-robin, so the rows in flight on an
-// XCD form a narrow front. Dot partials: a wavefront flushes its sums when
+// XCD form a narrow front. The grid is persistent (wavefront wx takes tiles
+// wx, wx + nwx, ...) or has a wavefront per tile (nwx >= the longest part):
+// the hardware dispatches workgroups x, x + 8, ... to XCD x in order as
+// earlier ones finish, so the front cannot drift apart. Dot partials: a
+// wavefront flushes its sums when
 // it leaves a model, into slot pbase[k*8+x] + its rank among the
 // wavefronts that touch the model's piece of part x (pstart: the piece's
 // first tile).
