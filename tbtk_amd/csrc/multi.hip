@@ -66,6 +66,7 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kRpi = 2;          // row groups in flight per lane
 constexpr int kSegSlots = 64;    // dot slots summed per group in pass 1
 constexpr int kMaxWidth = 64;    // start vectors per model
+constexpr int64_t kMaxSlotBytes = (int64_t)1 << 30;   // dot slots of a one-tile-per-wavefront plan
 
 struct MDesc {
 	int64_t N;       // rows
@@ -977,6 +978,11 @@ int autotune(tbtkx_context *h, const std::vector<int> &width, const std::vector<
 			Plan p;
 			int rc = make_plan(h, width, S, dots, G, share, p);
 			if (rc) return rc;
+			// One dot slot per tile: skip shapes whose slots would take
+			// more than kMaxSlotBytes (very large batches of small tiles).
+			if (share == 0 && dots &&
+			    (int64_t)p.max_slots * 2 * kMaxWidth * 16 > kMaxSlotBytes)
+				continue;
 			if ((rc = reserve_plan(h, p, ncols, mu_elems))) return rc;
 			const Phase &ph = p.phases[0];
 			float2 *X = h->mm_x.as<float2>(), *Y = h->mm_y.as<float2>();
