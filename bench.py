@@ -25,7 +25,7 @@ of the same workload on this host, on one core and in one process per core
 (the throughput mode), with the GPU/CPU ratios beside it.
 
 Further objects on the same line: "uneven_batch" (64 models of uneven size
-in one persistent launch per step), "stochastic_dos" (128^3 cubic, 512
+in one launch per step), "stochastic_dos" (128^3 cubic, 512
 random-phase vectors, M=4096) and "contraction_mfma" (1e5 x 4096 x 8192
 split-BF16 Green's-function GEMM), each with its own roofline fraction and
 error against the FP64 path.
@@ -35,6 +35,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
                        [--skip-cpu-baseline] [--calibrate-copy] [--complex] [--no-palette]
                        [--no-patterns | --row-pattern-bytes] [--slot-rows R]
                        [--no-graph] [--uneven-moments-div D] [--uneven-tile-rows G]
+                       [--uneven-grid-share S] [--dos-moments-div D]
 """
 from __future__ import annotations
 
@@ -380,21 +381,24 @@ def batched_comparison(eng):
     }
 
 
-def uneven_batch(eng, graph=True, moments_div=1, tile_rows=0):
+def uneven_batch(eng, graph=True, moments_div=1, tile_rows=0, grid_share=0):
     """64 independent models of uneven size (honeycomb ribbons and square
     lattices, 20k-1M states, 1-64 sites and 500-8000 moments each, seed 4) in
-    one persistent launch per recurrence step: FP32 storage, FP64 moments,
+    one launch per recurrence step: FP32 storage, FP64 moments,
     doubling, autotuned launch shape, replayed as hipGraphs. The models
     are built on the device from their lattice descriptions
     (tbtkx_multi_set_lattices). For counter runs, moments_div > 1 divides
     every model's moment count (at least 50: the same phases, a tenth of the
     steps at 10) and tile_rows fixes the tile (row groups per wavefront
-    tile) instead of the autotuned shape."""
+    tile) instead of the autotuned shape, on the grid grid_share names (0:
+    one tile per wavefront, s > 0: persistent grid of occupancy x CUs / s
+    workgroups)."""
     t0 = time.perf_counter()
     batch = L.config_uneven_specs(seed=4)
     if moments_div > 1:
         batch = [(sp, st, max(50, M // moments_div)) for sp, st, M in batch]
     eng.set_option(X.OPT_TILE_ROWS, tile_rows)
+    eng.set_option(X.OPT_GRID_SHARE, grid_share if tile_rows > 0 else 0)
     specs = [b[0] for b in batch]
     scales = [eng.lattice_gershgorin(sp) * 1.05 for sp in specs]
     eng.multi_set_lattices(specs, scales)
@@ -451,6 +455,7 @@ def uneven_batch(eng, graph=True, moments_div=1, tile_rows=0):
             errs.append(rel_errors(l_got, l_ref))
     eng.lut_destroy()
     eng.set_option(X.OPT_TILE_ROWS, 0)
+    eng.set_option(X.OPT_GRID_SHARE, 0)
     dev_s = ms_graph * 1e-3
     extra = {"launch_shape": shape}
     if moments_div > 1:
@@ -651,6 +656,9 @@ def main():
     ap.add_argument("--uneven-tile-rows", type=int, default=0,
                     help="uneven part: fixed row groups per tile instead of the autotuned "
                          "launch shape")
+    ap.add_argument("--uneven-grid-share", type=int, default=0,
+                    help="uneven part with --uneven-tile-rows: 0 = one tile per wavefront, "
+                         "s > 0 = persistent grid of occupancy x CUs / s workgroups")
     ap.add_argument("--no-patterns", action="store_true",
                     help="slot-major plain step with per-slot offsets and codes instead of "
                          "row patterns")
@@ -801,10 +809,11 @@ def main():
         log("batched comparison (512x512 Nambu, 256 sites)")
         out["batched"] = batched_comparison(eng)
     if "uneven" in parts:
-        log("uneven batch (64 models, one persistent launch per step)")
+        log("uneven batch (64 models, one launch per step)")
         out["uneven_batch"] = uneven_batch(eng, graph=not args.no_graph,
                                            moments_div=args.uneven_moments_div,
-                                           tile_rows=args.uneven_tile_rows)
+                                           tile_rows=args.uneven_tile_rows,
+                                           grid_share=args.uneven_grid_share)
     if "dos" in parts:
         log("stochastic DOS (128^3 cubic, R=512, M=4096)")
         out["stochastic_dos"] = stochastic_dos(eng, moments_div=args.dos_moments_div)

@@ -302,9 +302,11 @@ int tbtkx_multi_set_lattices(tbtkx_handle handle, int num_models,
  * out holds, model after model, num_sites[m] x num_moments[m] complex values
  * (site-major), as tbtkx_moments_block would give per model. FP32 storage,
  * FP64 moments; flags may be TBTKX_DOUBLING. The launch shape (tile size;
- * a persistent grid or one tile per wavefront) is autotuned on first use unless TBTKX_OPT_AUTOTUNE is 0 or
- * TBTKX_OPT_TILE_ROWS fixes it; TBTKX_OPT_GRAPH replays the whole run as a
- * hipGraph. A model's vectors are addressed with 32-bit byte offsets: a
+ * a persistent grid or one tile per wavefront) is autotuned on first use
+ * unless TBTKX_OPT_AUTOTUNE is 0 or TBTKX_OPT_TILE_ROWS fixes it (the grid
+ * then follows TBTKX_OPT_GRID_SHARE: 0 = one tile per wavefront, s > 0 = a
+ * persistent grid of occupancy x CUs / s workgroups); TBTKX_OPT_GRAPH
+ * replays the whole run as a hipGraph. A model's vectors are addressed with 32-bit byte offsets: a
  * model with num_rows x (columns + 1) x 8 B >= 4 GiB is refused with
  * TBTKX_ERR_ARG (columns: num_sites, or ceil(num_sites / 2) when every
  * model's H is real). */

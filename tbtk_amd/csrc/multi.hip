@@ -1190,14 +1190,17 @@ int tbtkx_multi_moments(tbtkx_handle h, const int *num_sites, const int64_t *sit
 	if (rc) return rc;
 	const int Smax = *std::max_element(S.begin(), S.end());
 
-	// Launch shape: explicit (TBTKX_OPT_TILE_ROWS = row groups per tile) or
-	// autotuned once per (batch, mode, widths).
+	// Launch shape: explicit (TBTKX_OPT_TILE_ROWS = row groups per tile, with
+	// TBTKX_OPT_GRID_SHARE: 0 = one tile per wavefront, s > 0 = persistent
+	// grid of occupancy x CUs / s workgroups) or autotuned once per (batch,
+	// mode, widths).
 	int64_t hw = 1469598103934665603ll;
 	for (int m = 0; m < K; m++) hw = (hw ^ (width[m] * 131 + S[m])) * 1099511628211ll;
 	const int64_t key[3] = {h->mm_gen, flags + (packed << 16), hw};
 	int G = 8, share = 1;
 	if (h->tile_rows > 0) {
 		G = std::max(kRpi, (h->tile_rows + kRpi - 1) / kRpi * kRpi);
+		share = h->grid_share;
 	} else if (h->autotune && Smax > 0) {
 		if (memcmp(key, h->mm_tune_key, sizeof key) != 0) {
 			// Output sizes are needed to reserve buffers for the trial runs.

@@ -105,15 +105,21 @@ def test_multi_launch_shapes_and_graph_agree(engine, flags):
     base = engine.multi_moments(sites, Ms, flags)
     results = []
     try:
-        for tile in (2, 6, 16):
+        # fixed tiles on a persistent grid (share 1, 2) and one tile per
+        # wavefront (share 0)
+        for tile, share in ((2, 1), (6, 2), (16, 1), (2, 0), (8, 0), (32, 0)):
             engine.set_option(X.OPT_TILE_ROWS, tile)
+            engine.set_option(X.OPT_GRID_SHARE, share)
             results.append(engine.multi_moments(sites, Ms, flags))
+            assert engine.get_option(X.OPT_GRID_SHARE) == share
         engine.set_option(X.OPT_TILE_ROWS, 0)
+        engine.set_option(X.OPT_GRID_SHARE, 0)
         engine.set_option(X.OPT_GRAPH, 1)
         for _ in range(3):
             results.append(engine.multi_moments(sites, Ms, flags))
     finally:
         engine.set_option(X.OPT_TILE_ROWS, 0)
+        engine.set_option(X.OPT_GRID_SHARE, 0)
         engine.set_option(X.OPT_GRAPH, 0)
     for r in results:
         for a, b in zip(r, base):
@@ -240,19 +246,25 @@ def test_multi_graph_chunks_agree_with_direct(engine, flags):
 
 def test_multi_reports_launch_shape(engine):
     """With TBTKX_OPT_TILE_ROWS at 0 the option reads back the autotuned row
-    groups per tile of the last call, and the grid share it ran with."""
+    groups per tile of the last call, and the grid share it ran with (0: one
+    tile per wavefront); a fixed tile runs with the set grid share."""
     batch = small_batch()
     upload(engine, batch)
     engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], X.DOUBLING)
-    assert engine.get_option(X.OPT_TILE_ROWS) in (2, 4, 8, 16)
-    assert engine.get_option(X.OPT_GRID_SHARE) in (1, 2)
+    tile, share = engine.get_option(X.OPT_TILE_ROWS), engine.get_option(X.OPT_GRID_SHARE)
+    assert (share, tile) in [(s, g) for s in (1, 2) for g in (2, 4, 8, 16)] + \
+        [(0, g) for g in (8, 16, 32, 64)]
     try:
         engine.set_option(X.OPT_TILE_ROWS, 6)
         engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], X.DOUBLING)
         assert engine.get_option(X.OPT_TILE_ROWS) == 6
-        assert engine.get_option(X.OPT_GRID_SHARE) == 1
+        assert engine.get_option(X.OPT_GRID_SHARE) == 0
+        engine.set_option(X.OPT_GRID_SHARE, 2)
+        engine.multi_moments([b[1] for b in batch], [b[2] for b in batch], X.DOUBLING)
+        assert engine.get_option(X.OPT_GRID_SHARE) == 2
     finally:
         engine.set_option(X.OPT_TILE_ROWS, 0)
+        engine.set_option(X.OPT_GRID_SHARE, 0)
 
 
 @pytest.mark.parametrize("path", ["uneven", "block"])
