@@ -190,3 +190,25 @@ def test_trace_full_size_vs_oracle(engine, oracle_mod):
         err = np.abs(got - ref).max() / (R * N)
         print(f"flags={flags}: max |mu - oracle| / (R N) {err:.2e}", flush=True)
         assert err < tol, (flags, err)
+
+
+def test_uneven_full_size_vs_oracle(engine, oracle_mod):
+    """An uneven batch at the configuration's largest model size: a
+    1,000,000-state honeycomb ribbon (t = 2.7, 400 x 625 cells) beside a
+    300 x 1000 square lattice, 2 sites each, 96 and 48 moments, doubling:
+    within 2e-5 of the oracle's moments."""
+    models = [L.honeycomb_ribbon(400, 625, t=2.7, disorder=0.5, seed=41),
+              L.square(300, 1000, t=1.0, disorder=0.5, seed=42)]
+    Ms = [96, 48]
+    csrs = [m.csr() for m in models]
+    scales = [m.gershgorin_bound() * 1.05 for m in models]
+    rng = np.random.default_rng(43)
+    sites = [np.sort(rng.choice(m.basis_size, 2, replace=False)) for m in models]
+    engine.multi_set(csrs, scales)
+    got = engine.multi_moments(sites, Ms, X.DOUBLING)
+    assert models[0].basis_size == 1_000_000
+    for m, csr, s, st, M, g in zip(models, csrs, scales, sites, Ms, got):
+        exact = np.array([oracle_mod.moments(csr, s, int(b), [int(b)], M)[0] for b in st])
+        err = np.abs(g - exact).max()
+        print(f"{m.name} N={m.basis_size}: max |mu - oracle| {err:.2e}", flush=True)
+        assert err < 2e-5, (m.name, err)
