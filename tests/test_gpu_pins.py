@@ -212,3 +212,32 @@ def test_uneven_full_size_vs_oracle(engine, oracle_mod):
         err = np.abs(g - exact).max()
         print(f"{m.name} N={m.basis_size}: max |mu - oracle| {err:.2e}", flush=True)
         assert err < 2e-5, (m.name, err)
+
+
+def test_mfma_full_shape_vs_oracle(engine, oracle_mod):
+    """The contraction at the configuration's full moment and energy counts
+    (M = 4096, E = 8192, s = 10), 8 coefficient vectors of single levels at
+    seeded energies (Jackson-damped, random phases): the split-BF16 MFMA
+    contraction within 3e-5 of max|G| of the oracle's lookup contraction,
+    Retarded and NonPrincipal, with the device table and the oracle's table
+    uploaded."""
+    M, E, s = 4096, 8192, 10.0
+    rng = np.random.default_rng(21)
+    x = rng.uniform(-0.9, 0.9, 8)
+    c = np.cos(np.outer(np.arccos(x), np.arange(M))) * kpm.jackson(M)[None, :]
+    c = c * np.exp(1j * rng.uniform(0, 2 * np.pi, (8, 1)))
+    lo, hi = -0.99 * s, 0.99 * s
+    T = oracle_mod.lookup_table(M, E, lo, hi, s)
+    for table in ("device", "uploaded"):
+        if table == "device":
+            engine.lut_generate(M, E, lo, hi, s)
+        else:
+            engine.lut_upload(T)
+        engine.coeffs_upload(c)
+        for gtype in (X.RETARDED, X.NONPRINCIPAL):
+            got = engine.contract_mfma(gtype)
+            ref = np.array([oracle_mod.gf_lookup(gtype, T, row) for row in c])
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            print(f"{table} type {gtype}: {err:.2e}", flush=True)
+            assert err < 3e-5, (table, gtype, err)
+        engine.lut_destroy()
