@@ -91,6 +91,30 @@ def test_plain_full_size_bitwise(plain, oracle_mod):
     assert np.count_nonzero(ref[4]) > 0
 
 
+def test_plain_complex_full_size_bitwise(plain, oracle_mod):
+    """A complex H at full size: the 1024 x 1024 square lattice with a flux
+    of 0.01 flux quanta per plaquette (Landau-gauge Peierls phases, so more
+    than 255 distinct values and the palette off) and disorder U[-1, 1]
+    (seed 7). 32 moments from the centre equal the oracle's bit for bit, in
+    complex arithmetic on the slot-major step with full FP64 values, and on
+    the row-block step."""
+    m = L.square(1024, 1024, t=1.0, disorder=1.0, seed=7, flux=0.01)
+    csr = m.csr()
+    s, M = m.gershgorin_bound() * 1.05, 32
+    frm = 512 * 1024 + 512
+    to = [frm, frm + 1, frm + 1024, frm - 1, 0, m.basis_size - 1, frm + 5 * 1024 + 3]
+    ref = oracle_mod.moments(csr, s, frm, to, M)
+    plain.set_csr(csr, s)
+    got = plain.moments(frm, to, M)
+    assert plain.get_option(X.OPT_REAL) == 0
+    assert plain.get_option(X.OPT_PALETTE) == 0
+    assert plain.get_option(X.OPT_PLAIN_KERNEL) == 2
+    assert np.array_equal(got, ref)
+    plain.set_option(X.OPT_PLAIN_KERNEL, 0)
+    assert np.array_equal(plain.moments(frm, to, M), ref)
+    assert np.count_nonzero(ref[6]) > 0
+
+
 @pytest.fixture(scope="module")
 def chebyshev_coefficients():
     """96 Jackson-damped moment vectors (600 moments) of a Nambu model, from
